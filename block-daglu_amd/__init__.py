@@ -1,0 +1,397 @@
+"""blockdaglu_amd — MI355X-native engine for the BLOCK route of BlockDAGLU.
+
+Python mirror of the reference C++ API (blockdaglu/include/blockdaglu/solver.h:43-65)
+over the host library's C-ABI (include/bdlu_host.h):
+
+    s = Solver()
+    s.set_matrix(rows, cols, vals, dimension=-1)   # storage only
+    s.analyze()                                    # host: validate, BTF, ordering, plan upload
+    s.factor()                                     # device: refactorization (HIP, gfx950)
+    s.solve_in_place(b)                            # device: block-DAG solve (+ refinement)
+
+Every call returns a bool and sets last_error()/last_error_message() on failure,
+as the reference does. The numeric work runs only in the HIP library; when it
+cannot load or no GPU is present, factor()/analyze() fail with
+BackendUnavailable — there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_DIR = PKG_DIR / "lib"
+HOST_LIB = LIB_DIR / "libblockdaglu.so"
+GPU_LIB = LIB_DIR / "libbdlu_gpu.so"
+
+_lib = None
+
+
+class SolverError(enum.IntEnum):
+    """Same enumerators, same order as blockdaglu::SolverError (solver.h:19-31)."""
+    None_ = 0
+    NotPrepared = 1
+    InvalidDimension = 2
+    InvalidIndex = 3
+    NonFiniteValue = 4
+    InvalidRightHandSide = 5
+    BackendUnavailable = 6
+    AnalysisFailed = 7
+    FactorizationFailed = 8
+    SolveFailed = 9
+    ExportFailed = 10
+
+
+class Precision(enum.IntEnum):
+    FP64 = 0
+    FP32_IR = 1
+
+
+class _Report(C.Structure):
+    _fields_ = [("nblocks", C.c_int32), ("dense_blocks", C.c_int32), ("sparse_blocks", C.c_int32),
+                ("max_block", C.c_int32), ("fp64_fallback_blocks", C.c_int32), ("repivot_blocks", C.c_int32),
+                ("ir_iterations", C.c_int32), ("plan_reused", C.c_int32), ("rel_residual", C.c_double),
+                ("backward_error", C.c_double)]
+
+
+class GenSpec(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("n_small_blocks", C.c_int32), ("size_dist", C.c_int32),
+                ("size_param", C.c_double), ("size_lo", C.c_int32), ("size_hi", C.c_int32),
+                ("n_small_total", C.c_int64), ("n_large_blocks", C.c_int32), ("large_size_lo", C.c_int32),
+                ("large_size_hi", C.c_int32), ("large_fill", C.c_int32), ("large_fill_param", C.c_double),
+                ("extra_per_row", C.c_double), ("coupling_kind", C.c_int32), ("coupling_rate", C.c_double),
+                ("diag_factor", C.c_double), ("nrhs", C.c_int32), ("scale_exp", C.c_double),
+                ("scale_seed", C.c_uint64)]
+
+
+_P = C.c_void_p
+_I32P = C.POINTER(C.c_int32)
+_F64P = C.POINTER(C.c_double)
+
+# (name, restype, argtypes) of every symbol include/bdlu_host.h declares.
+HOST_SYMBOLS = [
+    ("bdlu_solver_create", _P, []),
+    ("bdlu_solver_destroy", None, [_P]),
+    ("bdlu_solver_set_matrix", C.c_int, [_P, C.c_int64, _I32P, _I32P, _F64P, C.c_int32]),
+    ("bdlu_solver_analyze", C.c_int, [_P]),
+    ("bdlu_solver_factor", C.c_int, [_P]),
+    ("bdlu_solver_solve", C.c_int, [_P, _F64P, C.c_int64]),
+    ("bdlu_solver_solve_multi", C.c_int, [_P, _F64P, C.c_int64, C.c_int32]),
+    ("bdlu_solver_last_error", C.c_int, [_P]),
+    ("bdlu_solver_last_error_message", C.c_char_p, [_P]),
+    ("bdlu_solver_n", C.c_int32, [_P]),
+    ("bdlu_solver_selected_mode", C.c_char_p, [_P]),
+    ("bdlu_solver_set_precision", None, [_P, C.c_int]),
+    ("bdlu_solver_report", None, [_P, C.POINTER(_Report)]),
+    ("bdlu_solver_factor_device", C.c_int, [_P, C.c_void_p]),
+    ("bdlu_solver_solve_device", C.c_int, [_P, C.c_void_p, C.c_int32, C.c_int64]),
+    ("bdlu_solver_device_context", _P, [_P]),
+    ("bdlu_solver_device_stream", _P, [_P]),
+    ("bdlu_solver_nblocks", C.c_int32, [_P]),
+    ("bdlu_solver_copy_perm", None, [_P, _I32P, _I32P, _I32P]),
+    ("bdlu_solver_export", C.c_int64, [_P, C.c_int, _I32P, _I32P, _F64P]),
+    ("bdlu_host_analyze", _P, [C.c_int64, _I32P, _I32P, _F64P, C.c_int32, C.POINTER(C.c_int)]),
+    ("bdlu_host_free", None, [_P]),
+    ("bdlu_host_info", None, [_P, _I32P, _I32P, _I32P, _I32P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _I32P]),
+    ("bdlu_host_copy_perm", None, [_P, _I32P, _I32P, _I32P]),
+    ("bdlu_host_block_kinds", None, [_P, _I32P, _I32P]),
+    ("bdlu_host_sparse_stats", C.c_int, [_P, C.c_int32, _I32P, _F64P]),
+    ("bdlu_host_sparse_order", C.c_int, [_P, C.c_int32, _I32P, _I32P]),
+    ("bdlu_host_signatures", None, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    ("bdlu_gen_config_spec", None, [C.c_int, C.c_int, C.POINTER(GenSpec)]),
+    ("bdlu_gen_create", _P, [C.POINTER(GenSpec)]),
+    ("bdlu_gen_info", None, [_P, _I32P, C.POINTER(C.c_int64), _I32P]),
+    ("bdlu_gen_copy", None, [_P, _I32P, _I32P, _F64P, _F64P]),
+    ("bdlu_gen_free", None, [_P]),
+    ("bdlu_gen_perturb", None, [_F64P, C.c_int64, C.c_uint64, C.c_double]),
+]
+
+# Symbols of include/bdlu_gpu.h (device engine).
+GPU_SYMBOLS = [
+    ("bdlu_gpu_device_count", C.c_int, [_I32P]),
+    ("bdlu_gpu_create", C.c_int, [C.c_int, C.c_uint64, C.POINTER(_P)]),
+    ("bdlu_gpu_destroy", None, [_P]),
+    ("bdlu_gpu_last_error", C.c_char_p, [_P]),
+    ("bdlu_gpu_plan_upload", C.c_int, [_P, _P, C.POINTER(C.c_uint64)]),
+    ("bdlu_gpu_plan_release", C.c_int, [_P, C.c_uint64]),
+    ("bdlu_gpu_plan_bytes", C.c_int, [_P, C.c_uint64, C.POINTER(C.c_uint64)]),
+    ("bdlu_gpu_refactor", C.c_int, [_P, C.c_uint64, C.c_void_p, C.c_int, C.c_int, _P]),
+    ("bdlu_gpu_solve", C.c_int, [_P, C.c_uint64, C.c_void_p, C.c_int32, C.c_int64, C.c_int, _P, _P]),
+    ("bdlu_gpu_plan_values", C.c_int, [_P, C.c_uint64, _F64P]),
+    ("bdlu_gpu_stream", _P, [_P]),
+    ("bdlu_gpu_profile", C.c_int, [_P, C.c_int]),
+    ("bdlu_gpu_profile_read", C.c_int, [_P, C.c_int, _F64P, C.POINTER(C.c_int64), _F64P, _F64P]),
+    ("bdlu_gpu_debug_dense_lu", C.c_int, [_P, C.c_int32, _F64P, C.c_int]),
+]
+
+PROF_TAIL_GEMM, PROF_TAIL_PANEL, PROF_DENSE_LU, PROF_SPARSE, PROF_SOLVE = range(5)
+
+
+def lib() -> C.CDLL:
+    """Loads the host library (and, through it, the HIP engine). Raises if absent."""
+    global _lib
+    if _lib is None:
+        if not HOST_LIB.exists() or not GPU_LIB.exists():
+            raise RuntimeError(f"blockdaglu_amd native libraries are not built ({LIB_DIR}); run build()")
+        C.CDLL(str(GPU_LIB), mode=C.RTLD_GLOBAL)
+        L = C.CDLL(str(HOST_LIB))
+        for name, res, args in HOST_SYMBOLS:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        G = C.CDLL(str(GPU_LIB))
+        for name, res, args in GPU_SYMBOLS:
+            f = getattr(G, name)
+            f.restype = res
+            f.argtypes = args
+        L.gpu = G
+        _lib = L
+    return _lib
+
+
+def build(oracle: bool = False) -> None:
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("_bdlu_build", PKG_DIR / "build.py")
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    mod.build(oracle=oracle)
+
+
+def gpu_available() -> bool:
+    n = C.c_int32(0)
+    return lib().gpu.bdlu_gpu_device_count(C.byref(n)) == 0 and n.value > 0
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+class Solver:
+    """blockdaglu::Solver over the C-ABI (reference solver.h:43-65)."""
+
+    def __init__(self):
+        self._L = lib()
+        self._h = self._L.bdlu_solver_create()
+        if not self._h:
+            raise MemoryError("bdlu_solver_create failed")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.bdlu_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_matrix(self, rows, cols=None, vals=None, dimension: int = -1) -> bool:
+        """set_matrix(coo_triples) or set_matrix(rows, cols, vals); storage only."""
+        if cols is None:
+            t = list(rows)
+            rows = [x[0] for x in t]
+            cols = [x[1] for x in t]
+            vals = [x[2] for x in t]
+        r, c, v = _i32(rows), _i32(cols), _f64(vals)
+        if not (len(r) == len(c) == len(v)):
+            raise ValueError("rows/cols/vals lengths differ")
+        return bool(self._L.bdlu_solver_set_matrix(self._h, len(r), _ptr(r, _I32P), _ptr(c, _I32P),
+                                                   _ptr(v, _F64P), int(dimension)))
+
+    def analyze(self) -> bool:
+        return bool(self._L.bdlu_solver_analyze(self._h))
+
+    def factor(self) -> bool:
+        return bool(self._L.bdlu_solver_factor(self._h))
+
+    def solve_in_place(self, b: np.ndarray) -> bool:
+        if not (isinstance(b, np.ndarray) and b.dtype == np.float64 and b.flags.c_contiguous):
+            raise TypeError("b must be a C-contiguous float64 numpy array (overwritten with x)")
+        return bool(self._L.bdlu_solver_solve(self._h, _ptr(b, _F64P), b.size))
+
+    def solve_multi_in_place(self, B: np.ndarray, nrhs: int) -> bool:
+        """B: n*nrhs float64, column-major (Fortran order or a flat column stack)."""
+        if not (isinstance(B, np.ndarray) and B.dtype == np.float64 and
+                (B.flags.f_contiguous or B.ndim == 1)):
+            raise TypeError("B must be a float64 Fortran-ordered array")
+        return bool(self._L.bdlu_solver_solve_multi(self._h, _ptr(B, _F64P), B.size // nrhs, nrhs))
+
+    def factor_device(self, d_values_ptr: int) -> bool:
+        return bool(self._L.bdlu_solver_factor_device(self._h, C.c_void_p(d_values_ptr)))
+
+    def solve_device(self, d_B_ptr: int, nrhs: int, ldb: int) -> bool:
+        return bool(self._L.bdlu_solver_solve_device(self._h, C.c_void_p(d_B_ptr), nrhs, ldb))
+
+    def device_stream(self) -> int:
+        return self._L.bdlu_solver_device_stream(self._h) or 0
+
+    def device_context(self) -> int:
+        return self._L.bdlu_solver_device_context(self._h) or 0
+
+    def last_error(self) -> SolverError:
+        return SolverError(self._L.bdlu_solver_last_error(self._h))
+
+    def last_error_message(self) -> str:
+        return self._L.bdlu_solver_last_error_message(self._h).decode()
+
+    def n(self) -> int:
+        return self._L.bdlu_solver_n(self._h)
+
+    def selected_mode(self) -> str:
+        return self._L.bdlu_solver_selected_mode(self._h).decode()
+
+    def set_precision(self, p: Precision) -> None:
+        self._L.bdlu_solver_set_precision(self._h, int(p))
+
+    def report(self) -> dict:
+        r = _Report()
+        self._L.bdlu_solver_report(self._h, C.byref(r))
+        return {k: getattr(r, k) for k, _ in _Report._fields_}
+
+    def permutations(self):
+        nb = self._L.bdlu_solver_nblocks(self._h)
+        n = self.n()
+        P = np.zeros(max(n, 0), np.int32)
+        Q = np.zeros(max(n, 0), np.int32)
+        R = np.zeros(nb + 1, np.int32)
+        self._L.bdlu_solver_copy_perm(self._h, _ptr(P, _I32P), _ptr(Q, _I32P), _ptr(R, _I32P))
+        return P, Q, R
+
+    def _export(self, which):
+        cnt = self._L.bdlu_solver_export(self._h, which, None, None, None)
+        if cnt < 0:
+            return []
+        r = np.zeros(cnt, np.int32)
+        c = np.zeros(cnt, np.int32)
+        v = np.zeros(cnt, np.float64)
+        self._L.bdlu_solver_export(self._h, which, _ptr(r, _I32P), _ptr(c, _I32P), _ptr(v, _F64P))
+        return list(zip(r.tolist(), c.tolist(), v.tolist()))
+
+    def get_rearranged_matrix(self):
+        return self._export(0)
+
+    def getP(self):
+        return self._export(1)
+
+    def getQ(self):
+        return self._export(2)
+
+    def getL(self):
+        return self._export(3)
+
+    def getU(self):
+        return self._export(4)
+
+
+class HostAnalysis:
+    """Read-only view of the host analysis (canonicalize, BTF, merge, per-block symbolic). CPU only."""
+
+    def __init__(self, rows, cols, vals, dimension: int = -1):
+        L = lib()
+        r, c, v = _i32(rows), _i32(cols), _f64(vals)
+        err = C.c_int(0)
+        self._L = L
+        self._h = L.bdlu_host_analyze(len(r), _ptr(r, _I32P), _ptr(c, _I32P), _ptr(v, _F64P), dimension,
+                                      C.byref(err))
+        self.error = SolverError(err.value)
+        if not self._h:
+            return
+        n, nbr, nb, nm, nl = (C.c_int32() for _ in range(5))
+        nnz, noff = C.c_int64(), C.c_int64()
+        L.bdlu_host_info(self._h, C.byref(n), C.byref(nbr), C.byref(nb), C.byref(nm), C.byref(nnz), C.byref(noff),
+                         C.byref(nl))
+        self.n, self.nblocks_raw, self.nblocks, self.nmatch = n.value, nbr.value, nb.value, nm.value
+        self.nnz, self.nnz_off, self.nlevels = nnz.value, noff.value, nl.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._L.bdlu_host_free(self._h)
+            self._h = None
+
+    def permutations(self):
+        P = np.zeros(self.n, np.int32)
+        Q = np.zeros(self.n, np.int32)
+        R = np.zeros(self.nblocks + 1, np.int32)
+        self._L.bdlu_host_copy_perm(self._h, _ptr(P, _I32P), _ptr(Q, _I32P), _ptr(R, _I32P))
+        return P, Q, R
+
+    def block_kinds(self):
+        k = np.zeros(self.nblocks, np.int32)
+        s = np.zeros(self.nblocks, np.int32)
+        self._L.bdlu_host_block_kinds(self._h, _ptr(k, _I32P), _ptr(s, _I32P))
+        return k, s
+
+    def sparse_stats(self, b):
+        st = np.zeros(6, np.int32)
+        fs = np.zeros(4, np.float64)
+        if not self._L.bdlu_host_sparse_stats(self._h, b, _ptr(st, _I32P), _ptr(fs, _F64P)):
+            return None
+        keys = ["s", "k0", "m", "levels", "offdiag_pivots", "scale"]
+        out = dict(zip(keys, st.tolist()))
+        out.update(dict(zip(["nnz_l", "nnz_u", "updates", "flops"], fs.tolist())))
+        return out
+
+    def sparse_order(self, b):
+        st = self.sparse_stats(b)
+        q = np.zeros(st["s"], np.int32)
+        prow = np.zeros(max(st["k0"], 1), np.int32)
+        self._L.bdlu_host_sparse_order(self._h, b, _ptr(q, _I32P), _ptr(prow, _I32P))
+        return q, prow[: st["k0"]]
+
+    def signatures(self):
+        a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self._L.bdlu_host_signatures(self._h, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
+
+
+def config_spec(cfg: int, pattern: int = 0) -> GenSpec:
+    sp = GenSpec()
+    lib().bdlu_gen_config_spec(cfg, pattern, C.byref(sp))
+    return sp
+
+
+def generate(spec_or_cfg, pattern: int = 0, **overrides):
+    """Seeded workload -> (n, rows, cols, vals, rhs[n, nrhs] Fortran order)."""
+    L = lib()
+    sp = config_spec(spec_or_cfg, pattern) if isinstance(spec_or_cfg, int) else spec_or_cfg
+    for k, v in overrides.items():
+        setattr(sp, k, v)
+    h = L.bdlu_gen_create(C.byref(sp))
+    if not h:
+        raise RuntimeError("generator failed")
+    try:
+        n, nnz, nrhs = C.c_int32(), C.c_int64(), C.c_int32()
+        L.bdlu_gen_info(h, C.byref(n), C.byref(nnz), C.byref(nrhs))
+        r = np.zeros(nnz.value, np.int32)
+        c = np.zeros(nnz.value, np.int32)
+        v = np.zeros(nnz.value, np.float64)
+        b = np.zeros((n.value, nrhs.value), np.float64, order="F")
+        L.bdlu_gen_copy(h, _ptr(r, _I32P), _ptr(c, _I32P), _ptr(v, _F64P), _ptr(b, _F64P))
+    finally:
+        L.bdlu_gen_free(h)
+    return n.value, r, c, v, b
+
+
+def perturb(vals: np.ndarray, seed: int, rel: float = 0.05) -> np.ndarray:
+    out = _f64(vals).copy()
+    lib().bdlu_gen_perturb(_ptr(out, _F64P), out.size, seed, rel)
+    return out
+
+
+def canonical_order(rows, cols) -> np.ndarray:
+    """Permutation that sorts COO entries by (row, col) — the canonical order of
+    a duplicate-free input (reference src/solver.cpp:692)."""
+    return np.lexsort((np.asarray(cols), np.asarray(rows)))

@@ -1,0 +1,863 @@
+// C-ABI of the device engine (declared in include/bdlu_gpu.h): plan upload,
+// refactorization, block-DAG solve and FP64 refinement on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../../include/bdlu_gpu.h"
+#include "kernels.h"
+
+namespace {
+
+struct HipError {
+  std::string msg;
+};
+
+#define HIPCHECK(expr)                                                                   \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) throw HipError{std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+struct Alloc {
+  std::vector<void*> ptrs;
+  uint64_t bytes = 0;
+  template <typename T>
+  T* get(std::size_t count) {
+    if (count == 0) count = 1;
+    void* p = nullptr;
+    HIPCHECK(hipMalloc(&p, count * sizeof(T)));
+    ptrs.push_back(p);
+    bytes += count * sizeof(T);
+    return static_cast<T*>(p);
+  }
+  template <typename T>
+  T* copy(const T* host, std::size_t count, hipStream_t st) {
+    T* d = get<T>(count);
+    if (count && host) HIPCHECK(hipMemcpyAsync(d, host, count * sizeof(T), hipMemcpyHostToDevice, st));
+    return d;
+  }
+  template <typename T>
+  T* copy(const std::vector<T>& v, hipStream_t st) {
+    return copy(v.data(), v.size(), st);
+  }
+  void release() {
+    for (void* p : ptrs) (void)hipFree(p);
+    ptrs.clear();
+    bytes = 0;
+  }
+};
+
+struct SparseDev {
+  int32_t block = 0, start = 0, s = 0, k0 = 0, m = 0, scale = 0;
+  double tol = 0.001;
+  bdk::SpView view{};
+  std::vector<int32_t> lev_ptr;  // refactor levels (host)
+  int32_t* lev_col = nullptr;
+  int32_t* rowperm = nullptr;
+  int32_t* q = nullptr;
+  // solve structures
+  std::vector<int32_t> fl_ptr, bl_ptr;  // level pointers (host)
+  int32_t *fl_rows = nullptr, *tl_rows = nullptr, *bl_rows = nullptr;
+  int32_t *lr_ptr = nullptr, *lr_col = nullptr, *lr_vidx = nullptr;
+  int32_t *ur_ptr = nullptr, *ur_col = nullptr, *ur_vidx = nullptr;
+  int64_t nL = 0, nU = 0;
+  double *Lx64 = nullptr, *Ux64 = nullptr, *Ud64 = nullptr, *S64 = nullptr;
+  float *Lx32 = nullptr, *Ux32 = nullptr, *Ud32 = nullptr, *S32 = nullptr;
+  double* rs = nullptr;
+  double* y = nullptr;
+  int32_t *flags_f = nullptr, *flags_b = nullptr;
+  int32_t* status = nullptr;  // device int
+  int prec = 0;
+};
+
+struct Phase {
+  bool sparse = false;
+  int32_t index = 0;   // sparse block index or segment id
+  int32_t ntasks = 0;
+  int32_t* tasks = nullptr;
+};
+
+struct Plan {
+  Alloc mem;
+  int32_t n = 0, nblocks = 0;
+  int64_t nnz = 0, arena_len = 0;
+  int64_t* vmap = nullptr;
+  double* arena = nullptr;
+  double* vals = nullptr;
+  int64_t* csr_ptr = nullptr;
+  int32_t* csr_col = nullptr;
+  int32_t *P = nullptr, *Q = nullptr, *R = nullptr;
+  int32_t *off_ptr = nullptr, *off_col = nullptr;
+  double* off_val = nullptr;
+  int32_t *dep_ptr = nullptr, *dep_blk = nullptr, *seg_of = nullptr;
+  // dense blocks
+  int32_t ndense = 0;
+  std::vector<int32_t> h_dense_block, h_dense_size, h_dense_start;
+  std::vector<int64_t> h_dense_val0, h_dense_lu0;
+  int32_t *d_size = nullptr, *d_start = nullptr, *d_block = nullptr;
+  int64_t *d_val0 = nullptr, *d_lu0 = nullptr;
+  int64_t* lu0_of_block = nullptr;
+  int64_t lu_len = 0;
+  double* lu64 = nullptr;
+  float* lu32 = nullptr;
+  int32_t* piv = nullptr;
+  int32_t* block_status = nullptr;  // per block (dense status)
+  int32_t* prec_of_block = nullptr; // per block: 0 fp64 factors, 1 fp32
+  std::vector<int32_t> h_prec;
+  // sparse
+  std::vector<SparseDev> sp;
+  std::vector<int32_t> sp_of_block;
+  void* xscratch = nullptr;
+  int32_t nslots = 0;
+  // solve
+  std::vector<Phase> phases;
+  int32_t* done = nullptr;
+  int32_t* tickets = nullptr;
+  int32_t nticket = 0;
+  int32_t* err = nullptr;
+  double *b = nullptr, *x = nullptr, *r = nullptr, *d = nullptr, *bperm = nullptr, *xperm = nullptr;
+  double* partials = nullptr;
+  double* norms = nullptr;  // [0..4] residual norms, [8] ||A||_inf
+  int32_t epoch = 0;
+  bool factored = false;
+};
+
+}  // namespace
+
+struct bdlu_ctx {
+  bool prof = false;
+  double prof_ms[BDLU_PROF_KINDS] = {0};
+  int64_t prof_n[BDLU_PROF_KINDS] = {0};
+  double prof_flops[BDLU_PROF_KINDS] = {0};
+  double prof_bytes[BDLU_PROF_KINDS] = {0};
+  int device = 0;
+  uint64_t budget = 0;
+  uint64_t used = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  std::string last_error;
+  uint64_t next_id = 1;
+  std::map<uint64_t, std::unique_ptr<Plan>> plans;
+};
+
+namespace {
+
+// Event pairs recorded on the engine stream while profiling.
+struct ProfEvents {
+  std::vector<hipEvent_t> ev[BDLU_PROF_KINDS];
+  std::vector<double> gemm_flops, gemm_bytes;
+  void mark(int kind, hipStream_t st) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) == hipSuccess) {
+      (void)hipEventRecord(e, st);
+      ev[kind].push_back(e);
+    }
+  }
+  // Call after the stream is synchronized.
+  void collect(bdlu_ctx* c) {
+    for (int k = 0; k < BDLU_PROF_KINDS; ++k) {
+      for (std::size_t i = 0; i + 1 < ev[k].size(); i += 2) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ev[k][i], ev[k][i + 1]) == hipSuccess) {
+          c->prof_ms[k] += ms;
+          c->prof_n[k] += 1;
+        }
+      }
+      for (hipEvent_t e : ev[k]) (void)hipEventDestroy(e);
+      ev[k].clear();
+    }
+    for (double f : gemm_flops) c->prof_flops[BDLU_PROF_TAIL_GEMM] += f;
+    for (double b : gemm_bytes) c->prof_bytes[BDLU_PROF_TAIL_GEMM] += b;
+    gemm_flops.clear();
+    gemm_bytes.clear();
+  }
+};
+
+int set_err(bdlu_ctx* c, int code, const std::string& m) {
+  if (c) c->last_error = m;
+  return code;
+}
+
+// Dense triangular-solve structures of one sparse block, from its patterns.
+void build_sparse_solve(const bdlu_sparse_desc& d, std::vector<int32_t>& rowperm, std::vector<int32_t>& pos,
+                        std::vector<int32_t>& lr_ptr, std::vector<int32_t>& lr_col, std::vector<int32_t>& lr_vidx,
+                        std::vector<int32_t>& fl_ptr, std::vector<int32_t>& fl_rows,
+                        std::vector<int32_t>& ur_ptr, std::vector<int32_t>& ur_col,
+                        std::vector<int32_t>& ur_vidx, std::vector<int32_t>& bl_ptr,
+                        std::vector<int32_t>& bl_rows) {
+  const int32_t s = d.s, k0 = d.k0, m = d.m;
+  rowperm.resize(s);
+  pos.assign(s, -1);
+  for (int32_t k = 0; k < k0; ++k) rowperm[k] = d.prow[k];
+  for (int32_t t = 0; t < m; ++t) rowperm[k0 + t] = d.trow[d.tperm[t]];
+  for (int32_t k = 0; k < s; ++k) pos[rowperm[k]] = k;
+  // L by position rows: entries (column k', value index p).
+  std::vector<int32_t> cnt(s + 1, 0);
+  const int32_t nL = d.l_ptr[k0];
+  for (int32_t p = 0; p < nL; ++p) cnt[pos[d.l_row[p]] + 1]++;
+  for (int32_t i = 0; i < s; ++i) cnt[i + 1] += cnt[i];
+  lr_ptr = cnt;
+  lr_col.resize(nL);
+  lr_vidx.resize(nL);
+  {
+    std::vector<int32_t> next(cnt.begin(), cnt.end() - 1);
+    for (int32_t kc = 0; kc < k0; ++kc)
+      for (int32_t p = d.l_ptr[kc]; p < d.l_ptr[kc + 1]; ++p) {
+        const int32_t w = next[pos[d.l_row[p]]]++;
+        lr_col[w] = kc;
+        lr_vidx[w] = p;
+      }
+  }
+  // Forward levels over positions < k0.
+  std::vector<int32_t> lev(k0, 0);
+  int32_t maxl = -1;
+  for (int32_t k = 0; k < k0; ++k) {
+    int32_t l = 0;
+    for (int32_t p = lr_ptr[k]; p < lr_ptr[k + 1]; ++p) l = std::max(l, lev[lr_col[p]] + 1);
+    lev[k] = l;
+    maxl = std::max(maxl, l);
+  }
+  fl_ptr.assign(maxl + 2, 0);
+  for (int32_t k = 0; k < k0; ++k) fl_ptr[lev[k] + 1]++;
+  for (int32_t l = 0; l <= maxl; ++l) fl_ptr[l + 1] += fl_ptr[l];
+  fl_rows.resize(k0);
+  {
+    std::vector<int32_t> next(fl_ptr.begin(), fl_ptr.end() - 1);
+    for (int32_t k = 0; k < k0; ++k) fl_rows[next[lev[k]]++] = k;
+  }
+  // U rows k < k0: entries (column position j, value index p).
+  const int32_t nU = d.u_ptr[s];
+  std::vector<int32_t> uc(k0 + 1, 0);
+  for (int32_t p = 0; p < nU; ++p) uc[d.u_idx[p] + 1]++;
+  for (int32_t k = 0; k < k0; ++k) uc[k + 1] += uc[k];
+  ur_ptr = uc;
+  ur_col.resize(nU);
+  ur_vidx.resize(nU);
+  {
+    std::vector<int32_t> next(uc.begin(), uc.end() - 1);
+    for (int32_t j = 0; j < s; ++j)
+      for (int32_t p = d.u_ptr[j]; p < d.u_ptr[j + 1]; ++p) {
+        const int32_t w = next[d.u_idx[p]]++;
+        ur_col[w] = j;
+        ur_vidx[w] = p;
+      }
+  }
+  std::vector<int32_t> blev(k0, 0);
+  maxl = -1;
+  for (int32_t k = k0 - 1; k >= 0; --k) {
+    int32_t l = 0;
+    for (int32_t p = ur_ptr[k]; p < ur_ptr[k + 1]; ++p)
+      if (ur_col[p] < k0) l = std::max(l, blev[ur_col[p]] + 1);
+    blev[k] = l;
+    maxl = std::max(maxl, l);
+  }
+  bl_ptr.assign(maxl + 2, 0);
+  for (int32_t k = 0; k < k0; ++k) bl_ptr[blev[k] + 1]++;
+  for (int32_t l = 0; l <= maxl; ++l) bl_ptr[l + 1] += bl_ptr[l];
+  bl_rows.resize(k0);
+  {
+    std::vector<int32_t> next(bl_ptr.begin(), bl_ptr.end() - 1);
+    for (int32_t k = 0; k < k0; ++k) bl_rows[next[blev[k]]++] = k;
+  }
+}
+
+std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
+  auto pl = std::make_unique<Plan>();
+  Plan& P = *pl;
+  hipStream_t st = c->stream;
+  Alloc& A = P.mem;
+  P.n = d.n;
+  P.nnz = d.nnz;
+  P.nblocks = d.nblocks;
+  P.arena_len = d.arena_len;
+  const int32_t n = d.n, nb = d.nblocks;
+  P.vmap = A.copy(d.vmap, d.nnz, st);
+  P.arena = A.get<double>(d.arena_len);
+  HIPCHECK(hipMemsetAsync(P.arena, 0, sizeof(double) * std::max<int64_t>(d.arena_len, 1), st));
+  P.vals = A.get<double>(d.nnz);
+  P.csr_ptr = A.copy(d.csr_ptr, n + 1, st);
+  P.csr_col = A.copy(d.csr_col, d.nnz, st);
+  P.P = A.copy(d.P, n, st);
+  P.Q = A.copy(d.Q, n, st);
+  P.R = A.copy(d.R, nb + 1, st);
+  P.off_ptr = A.copy(d.off_ptr, n + 1, st);
+  const int64_t noff = d.off_ptr[n];
+  P.off_col = A.copy(d.off_col, noff, st);
+  P.off_val = P.arena + d.off_val0;
+  P.dep_ptr = A.copy(d.dep_ptr, nb + 1, st);
+  P.dep_blk = A.copy(d.dep_blk, d.dep_ptr[nb], st);
+
+  // Dense blocks.
+  P.ndense = d.ndense;
+  P.h_dense_block.assign(d.dense_block, d.dense_block + d.ndense);
+  P.h_dense_val0.assign(d.dense_val0, d.dense_val0 + d.ndense);
+  std::vector<int64_t> lu0_of_block(nb, 0);
+  int64_t lu = 0;
+  for (int32_t t = 0; t < d.ndense; ++t) {
+    const int32_t b = d.dense_block[t];
+    const int32_t s = d.R[b + 1] - d.R[b];
+    if (s < 1 || s > 64) throw HipError{"dense block larger than 64"};
+    P.h_dense_size.push_back(s);
+    P.h_dense_start.push_back(d.R[b]);
+    P.h_dense_lu0.push_back(lu);
+    lu0_of_block[b] = lu;
+    lu += static_cast<int64_t>(s) * s;
+  }
+  P.lu_len = lu;
+  P.d_size = A.copy(P.h_dense_size, st);
+  P.d_start = A.copy(P.h_dense_start, st);
+  P.d_block = A.copy(P.h_dense_block, st);
+  P.d_val0 = A.copy(P.h_dense_val0, st);
+  P.d_lu0 = A.copy(P.h_dense_lu0, st);
+  P.lu0_of_block = A.copy(lu0_of_block, st);
+  P.lu64 = A.get<double>(lu);
+  P.lu32 = A.get<float>(lu);
+  P.piv = A.get<int32_t>(n);
+  P.block_status = A.get<int32_t>(nb);
+  P.prec_of_block = A.get<int32_t>(nb);
+  P.h_prec.assign(nb, 0);
+
+  // Sparse blocks.
+  P.sp_of_block.assign(nb, -1);
+  int64_t max_s = 0;
+  int32_t max_width = 1;
+  for (int32_t i = 0; i < d.nsparse; ++i) {
+    const bdlu_sparse_desc& sd = d.sparse[i];
+    SparseDev S;
+    S.block = sd.block;
+    S.start = d.R[sd.block];
+    S.s = sd.s;
+    S.k0 = sd.k0;
+    S.m = sd.m;
+    S.scale = sd.scale;
+    S.tol = sd.tol;
+    if (sd.s != d.R[sd.block + 1] - d.R[sd.block] || sd.k0 + sd.m != sd.s) throw HipError{"sparse block sizes"};
+    P.sp_of_block[sd.block] = i;
+    max_s = std::max<int64_t>(max_s, sd.s);
+    for (int32_t l = 0; l < sd.nlev; ++l) max_width = std::max(max_width, sd.lev_ptr[l + 1] - sd.lev_ptr[l]);
+    max_width = std::max(max_width, sd.m);
+    const int32_t ne = sd.a_ptr[sd.s];
+    S.nL = sd.l_ptr[sd.k0];
+    S.nU = sd.u_ptr[sd.s];
+    // Entries by local row (row scaling in a fixed order).
+    std::vector<int32_t> rs_ptr(sd.s + 1, 0), rs_idx(ne);
+    for (int32_t p = 0; p < ne; ++p) rs_ptr[sd.a_row[p] + 1]++;
+    for (int32_t r = 0; r < sd.s; ++r) rs_ptr[r + 1] += rs_ptr[r];
+    {
+      std::vector<int32_t> next(rs_ptr.begin(), rs_ptr.end() - 1);
+      for (int32_t p = 0; p < ne; ++p) rs_idx[next[sd.a_row[p]]++] = p;
+    }
+    std::vector<int32_t> tpos(sd.m);
+    for (int32_t t = 0; t < sd.m; ++t) tpos[sd.tperm[t]] = t;
+    std::vector<int32_t> rowperm, pos, lr_ptr, lr_col, lr_vidx, fl_ptr, fl_rows, ur_ptr, ur_col, ur_vidx, bl_ptr,
+        bl_rows;
+    build_sparse_solve(sd, rowperm, pos, lr_ptr, lr_col, lr_vidx, fl_ptr, fl_rows, ur_ptr, ur_col, ur_vidx, bl_ptr,
+                       bl_rows);
+    bdk::SpView& v = S.view;
+    v.s = sd.s;
+    v.k0 = sd.k0;
+    v.m = sd.m;
+    v.scale = sd.scale;
+    v.inv_tol = sd.tol > 0 ? 1.0 / sd.tol : 1e300;
+    v.a_val0 = sd.a_val0;
+    v.a_ptr = A.copy(sd.a_ptr, sd.s + 1, st);
+    v.a_row = A.copy(sd.a_row, ne, st);
+    v.prow = A.copy(sd.prow, sd.k0, st);
+    v.l_ptr = A.copy(sd.l_ptr, sd.k0 + 1, st);
+    v.l_row = A.copy(sd.l_row, S.nL, st);
+    v.u_ptr = A.copy(sd.u_ptr, sd.s + 1, st);
+    v.u_idx = A.copy(sd.u_idx, S.nU, st);
+    v.trow = A.copy(sd.trow, sd.m, st);
+    v.tpos = A.copy(tpos, st);
+    v.rs_ptr = A.copy(rs_ptr, st);
+    v.rs_idx = A.copy(rs_idx, st);
+    S.lev_ptr.assign(sd.lev_ptr, sd.lev_ptr + sd.nlev + 1);
+    S.lev_col = A.copy(sd.lev_col, sd.k0, st);
+    S.rowperm = A.copy(rowperm, st);
+    S.q = A.copy(sd.q, sd.s, st);
+    S.fl_ptr = fl_ptr;
+    S.bl_ptr = bl_ptr;
+    S.fl_rows = A.copy(fl_rows, st);
+    std::vector<int32_t> tl_rows(sd.m);
+    for (int32_t t = 0; t < sd.m; ++t) tl_rows[t] = sd.k0 + t;
+    S.tl_rows = A.copy(tl_rows, st);
+    S.bl_rows = A.copy(bl_rows, st);
+    S.lr_ptr = A.copy(lr_ptr, st);
+    S.lr_col = A.copy(lr_col, st);
+    S.lr_vidx = A.copy(lr_vidx, st);
+    S.ur_ptr = A.copy(ur_ptr, st);
+    S.ur_col = A.copy(ur_col, st);
+    S.ur_vidx = A.copy(ur_vidx, st);
+    S.Lx64 = A.get<double>(S.nL);
+    S.Ux64 = A.get<double>(S.nU);
+    S.Ud64 = A.get<double>(sd.k0);
+    S.S64 = A.get<double>(static_cast<std::size_t>(sd.m) * sd.m);
+    S.Lx32 = A.get<float>(S.nL);
+    S.Ux32 = A.get<float>(S.nU);
+    S.Ud32 = A.get<float>(sd.k0);
+    S.S32 = A.get<float>(static_cast<std::size_t>(sd.m) * sd.m);
+    S.rs = A.get<double>(sd.s);
+    S.y = A.get<double>(sd.s);
+    const int32_t np = (sd.m + 63) / 64;
+    S.flags_f = A.get<int32_t>(np);
+    S.flags_b = A.get<int32_t>(np);
+    HIPCHECK(hipMemsetAsync(S.flags_f, 0, sizeof(int32_t) * std::max(np, 1), st));
+    HIPCHECK(hipMemsetAsync(S.flags_b, 0, sizeof(int32_t) * std::max(np, 1), st));
+    S.status = A.get<int32_t>(1);
+    P.sp.push_back(std::move(S));
+  }
+  if (!P.sp.empty()) {
+    P.nslots = static_cast<int32_t>(std::min<int64_t>(1024, max_width));
+    const std::size_t xbytes = sizeof(double) * static_cast<std::size_t>(P.nslots) * max_s;
+    P.xscratch = A.get<char>(xbytes);
+    HIPCHECK(hipMemsetAsync(P.xscratch, 0, xbytes, st));
+  }
+
+  // Solve phases: blocks in descending order; runs of dense blocks form one
+  // persistent-kernel segment, each sparse block is its own phase.
+  std::vector<int32_t> seg_of(nb, -1);
+  std::vector<int32_t> cur;
+  auto flush = [&]() {
+    if (cur.empty()) return;
+    Phase ph;
+    ph.sparse = false;
+    ph.index = static_cast<int32_t>(P.phases.size());
+    for (int32_t b : cur) seg_of[b] = ph.index;
+    ph.ntasks = static_cast<int32_t>(cur.size());
+    ph.tasks = A.copy(cur, st);
+    P.phases.push_back(ph);
+    cur.clear();
+  };
+  for (int32_t b = nb - 1; b >= 0; --b) {
+    if (P.sp_of_block[b] >= 0) {
+      flush();
+      Phase ph;
+      ph.sparse = true;
+      ph.index = P.sp_of_block[b];
+      P.phases.push_back(ph);
+    } else {
+      cur.push_back(b);
+    }
+  }
+  flush();
+  P.seg_of = A.copy(seg_of, st);
+  P.done = A.get<int32_t>(nb);
+  HIPCHECK(hipMemsetAsync(P.done, 0, sizeof(int32_t) * std::max(nb, 1), st));
+  P.nticket = static_cast<int32_t>(P.phases.size()) + 2 * d.nsparse;
+  P.tickets = A.get<int32_t>(P.nticket);
+  P.err = A.get<int32_t>(1);
+  P.b = A.get<double>(n);
+  P.x = A.get<double>(n);
+  P.r = A.get<double>(n);
+  P.d = A.get<double>(n);
+  P.bperm = A.get<double>(n);
+  P.xperm = A.get<double>(n);
+  P.partials = A.get<double>(512 * 5);
+  P.norms = A.get<double>(16);
+  HIPCHECK(hipStreamSynchronize(st));
+  return pl;
+}
+
+// Solve A x = b for the resident factors; b and x are device vectors of length n.
+void block_solve(Plan& P, hipStream_t st, const double* b, double* x) {
+  P.epoch += 1;
+  const int32_t epoch = P.epoch;
+  bdk::launch_fill_i32(st, P.tickets, P.nticket, 0);
+  bdk::launch_permute(st, P.n, P.P, b, P.bperm);
+  int32_t tix = 0;
+  for (const Phase& ph : P.phases) {
+    if (!ph.sparse) {
+      // Each dense block's factors are read in the precision it was factored in.
+      bdk::launch_dense_seg_solve(st, ph.ntasks, ph.tasks, P.R, P.lu0_of_block, P.lu64, P.lu32, P.prec_of_block,
+                                  P.piv, P.off_ptr, P.off_col, P.off_val, P.dep_ptr, P.dep_blk, P.seg_of, ph.index,
+                                  P.done, epoch, P.tickets + tix, P.bperm, P.xperm, P.err);
+      ++tix;
+      continue;
+    }
+    SparseDev& S = P.sp[ph.index];
+    ++tix;
+    bdk::launch_sp_gather(st, S.s, S.start, S.rowperm, S.scale ? S.rs : nullptr, P.off_ptr, P.off_col, P.off_val,
+                          P.bperm, P.xperm, S.y);
+    auto run = [&](auto* Lx, auto* Ux, auto* Ud, auto* Sm) {
+      using T = std::remove_pointer_t<decltype(Lx)>;
+      for (std::size_t l = 0; l + 1 < S.fl_ptr.size(); ++l)
+        bdk::launch_sp_rows<T>(st, S.fl_ptr[l + 1] - S.fl_ptr[l], S.fl_rows + S.fl_ptr[l], S.lr_ptr, S.lr_col,
+                               S.lr_vidx, Lx, nullptr, S.y);
+      bdk::launch_sp_rows<T>(st, S.m, S.tl_rows, S.lr_ptr, S.lr_col, S.lr_vidx, Lx, nullptr, S.y);
+      bdk::launch_tail_trsv<T>(st, S.m, Sm, S.m, S.y + S.k0, false, S.flags_f, epoch, P.tickets + tix, P.err);
+      bdk::launch_tail_trsv<T>(st, S.m, Sm, S.m, S.y + S.k0, true, S.flags_b, epoch, P.tickets + tix + 1, P.err);
+      for (std::size_t l = 0; l + 1 < S.bl_ptr.size(); ++l)
+        bdk::launch_sp_rows<T>(st, S.bl_ptr[l + 1] - S.bl_ptr[l], S.bl_rows + S.bl_ptr[l], S.ur_ptr, S.ur_col,
+                               S.ur_vidx, Ux, Ud, S.y);
+    };
+    if (S.prec == 0) run(S.Lx64, S.Ux64, S.Ud64, S.S64);
+    else run(S.Lx32, S.Ux32, S.Ud32, S.S32);
+    tix += 2;
+    bdk::launch_sp_scatter_x(st, S.s, S.start, S.q, S.y, P.xperm);
+  }
+  bdk::launch_unpermute(st, P.n, P.Q, P.xperm, x);
+}
+
+template <typename T>
+void factor_sparse(Plan& P, SparseDev& S, hipStream_t st, T* Lx, T* Ux, T* Ud, T* Sm, ProfEvents* pe) {
+  bdk::launch_fill_i32(st, S.status, 1, 0);
+  if (pe) pe->mark(BDLU_PROF_SPARSE, st);
+  if (S.scale) bdk::launch_rowscale(st, S.view, P.arena, S.rs);
+  T* xs = static_cast<T*>(P.xscratch);
+  for (std::size_t l = 0; l + 1 < S.lev_ptr.size(); ++l)
+    bdk::launch_sp_cols<T>(st, S.view, P.arena, S.rs, S.lev_ptr[l + 1] - S.lev_ptr[l], S.lev_col + S.lev_ptr[l], xs,
+                           P.nslots, Lx, Ux, Ud, S.status);
+  bdk::launch_sp_tail<T>(st, S.view, P.arena, S.rs, xs, P.nslots, Lx, Ux, Sm, S.m);
+  if (pe) pe->mark(BDLU_PROF_SPARSE, st);
+  bdk::TailTimer tt;
+  if (pe) {
+    tt.gemm_events = &pe->ev[BDLU_PROF_TAIL_GEMM];
+    tt.gemm_flops = &pe->gemm_flops;
+    tt.gemm_bytes = &pe->gemm_bytes;
+    tt.panel_events = &pe->ev[BDLU_PROF_TAIL_PANEL];
+  }
+  bdk::launch_tail_lu<T>(st, S.m, Sm, S.m, S.view.inv_tol, S.status, pe ? &tt : nullptr);
+}
+
+}  // namespace
+
+extern "C" {
+
+int bdlu_gpu_device_count(int* count) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  if (count) *count = c;
+  return c > 0 ? BDLU_OK : BDLU_E_BACKEND;
+}
+
+int bdlu_gpu_create(int device, uint64_t hbm_budget_bytes, bdlu_ctx** out) {
+  if (!out) return BDLU_E_INVALID;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return BDLU_E_BACKEND;
+  if (device < 0 || device >= count) return BDLU_E_INVALID;
+  auto* c = new (std::nothrow) bdlu_ctx;
+  if (!c) return BDLU_E_BACKEND;
+  c->device = device;
+  c->budget = hbm_budget_bytes;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return BDLU_E_BACKEND;
+  }
+  *out = c;
+  return BDLU_OK;
+}
+
+void bdlu_gpu_destroy(bdlu_ctx* c) {
+  if (!c) return;
+  {
+    std::lock_guard<std::mutex> g(c->mu);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& kv : c->plans) kv.second->mem.release();
+    c->plans.clear();
+    (void)hipStreamDestroy(c->stream);
+  }
+  delete c;
+}
+
+const char* bdlu_gpu_last_error(bdlu_ctx* c) { return c ? c->last_error.c_str() : "null context"; }
+
+int bdlu_gpu_plan_upload(bdlu_ctx* c, const bdlu_plan_desc* d, uint64_t* plan_id) {
+  if (!c || !d || !plan_id) return BDLU_E_INVALID;
+  std::lock_guard<std::mutex> g(c->mu);
+  if (d->n <= 0 || d->nblocks <= 0 || d->nnz < 0) return set_err(c, BDLU_E_ANALYSIS, "empty plan");
+  try {
+    HIPCHECK(hipSetDevice(c->device));
+    std::unique_ptr<Plan> pl = upload(c, *d);
+    if (c->budget && c->used + pl->mem.bytes > c->budget) {
+      pl->mem.release();
+      return set_err(c, BDLU_E_BUDGET, "plan exceeds the HBM budget");
+    }
+    c->used += pl->mem.bytes;
+    const uint64_t id = c->next_id++;
+    c->plans[id] = std::move(pl);
+    *plan_id = id;
+    return BDLU_OK;
+  } catch (const HipError& e) {
+    return set_err(c, BDLU_E_BACKEND, e.msg);
+  } catch (const std::exception& e) {
+    return set_err(c, BDLU_E_BACKEND, e.what());
+  }
+}
+
+int bdlu_gpu_plan_release(bdlu_ctx* c, uint64_t id) {
+  if (!c) return BDLU_E_INVALID;
+  std::lock_guard<std::mutex> g(c->mu);
+  auto it = c->plans.find(id);
+  if (it == c->plans.end()) return set_err(c, BDLU_E_INVALID, "unknown plan");
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  c->used -= it->second->mem.bytes;
+  it->second->mem.release();
+  c->plans.erase(it);
+  return BDLU_OK;
+}
+
+int bdlu_gpu_plan_bytes(bdlu_ctx* c, uint64_t id, uint64_t* bytes) {
+  if (!c || !bytes) return BDLU_E_INVALID;
+  std::lock_guard<std::mutex> g(c->mu);
+  auto it = c->plans.find(id);
+  if (it == c->plans.end()) return set_err(c, BDLU_E_INVALID, "unknown plan");
+  *bytes = it->second->mem.bytes;
+  return BDLU_OK;
+}
+
+int bdlu_gpu_refactor(bdlu_ctx* c, uint64_t id, const double* values, int where, int precision,
+                      bdlu_factor_stats* stats) {
+  if (!c || !values) return BDLU_E_INVALID;
+  std::lock_guard<std::mutex> g(c->mu);
+  auto it = c->plans.find(id);
+  if (it == c->plans.end()) return set_err(c, BDLU_E_INVALID, "unknown plan");
+  Plan& P = *it->second;
+  hipStream_t st = c->stream;
+  bdlu_factor_stats fs{0, 0, 0, -1, 0.0};
+  try {
+    HIPCHECK(hipSetDevice(c->device));
+    HIPCHECK(hipMemcpyAsync(P.vals, values, sizeof(double) * P.nnz,
+                            where == BDLU_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+    bdk::launch_scatter(st, P.nnz, P.vals, P.vmap, P.arena);
+    bdk::launch_anorm(st, P.n, P.csr_ptr, P.vals, P.partials, P.norms + 8);
+    const bool fp32 = precision == BDLU_FP32;
+    ProfEvents pev;
+    ProfEvents* pe = c->prof ? &pev : nullptr;
+    // Dense blocks (dynamic partial pivoting, reference semantics).
+    if (pe) pe->mark(BDLU_PROF_DENSE_LU, st);
+    if (fp32)
+      bdk::launch_dense_lu<float>(st, P.ndense, P.d_size, P.d_start, P.d_val0, P.d_lu0, P.arena, P.lu32, P.piv,
+                                  P.block_status, P.d_block);
+    else
+      bdk::launch_dense_lu<double>(st, P.ndense, P.d_size, P.d_start, P.d_val0, P.d_lu0, P.arena, P.lu64, P.piv,
+                                   P.block_status, P.d_block);
+    if (pe) pe->mark(BDLU_PROF_DENSE_LU, st);
+    for (SparseDev& S : P.sp) {
+      S.prec = fp32 ? 1 : 0;
+      if (fp32) factor_sparse<float>(P, S, st, S.Lx32, S.Ux32, S.Ud32, S.S32, pe);
+      else factor_sparse<double>(P, S, st, S.Lx64, S.Ux64, S.Ud64, S.S64, pe);
+    }
+    // Collect statuses.
+    std::vector<int32_t> bst(P.nblocks, 0);
+    if (P.ndense) HIPCHECK(hipMemcpyAsync(bst.data(), P.block_status, sizeof(int32_t) * P.nblocks,
+                                          hipMemcpyDeviceToHost, st));
+    std::vector<int32_t> sst(P.sp.size(), 0);
+    for (std::size_t i = 0; i < P.sp.size(); ++i)
+      HIPCHECK(hipMemcpyAsync(&sst[i], P.sp[i].status, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if (pe) pe->collect(c);
+    std::fill(P.h_prec.begin(), P.h_prec.end(), fp32 ? 1 : 0);
+    // FP32 dense blocks that failed the small-pivot check: redo in FP64.
+    std::vector<int32_t> redo;
+    for (int32_t t = 0; t < P.ndense; ++t) {
+      const int32_t b = P.h_dense_block[t];
+      if (bst[b] & bdk::kStZeroPivot) {
+        fs.failed_blocks++;
+        if (fs.first_bad_block < 0) fs.first_bad_block = b;
+      } else if (fp32 && (bst[b] & bdk::kStCheck)) {
+        redo.push_back(t);
+      }
+    }
+    if (!redo.empty()) {
+      std::vector<int32_t> sz, sta, blk;
+      std::vector<int64_t> v0, l0;
+      for (int32_t t : redo) {
+        sz.push_back(P.h_dense_size[t]);
+        sta.push_back(P.h_dense_start[t]);
+        blk.push_back(P.h_dense_block[t]);
+        v0.push_back(P.h_dense_val0[t]);
+        l0.push_back(P.h_dense_lu0[t]);
+        P.h_prec[P.h_dense_block[t]] = 0;
+      }
+      Alloc tmp;
+      int32_t* dsz = tmp.copy(sz, st);
+      int32_t* dst_ = tmp.copy(sta, st);
+      int32_t* dbl = tmp.copy(blk, st);
+      int64_t* dv0 = tmp.copy(v0, st);
+      int64_t* dl0 = tmp.copy(l0, st);
+      bdk::launch_dense_lu<double>(st, static_cast<int32_t>(redo.size()), dsz, dst_, dv0, dl0, P.arena, P.lu64,
+                                   P.piv, P.block_status, dbl);
+      HIPCHECK(hipStreamSynchronize(st));
+      tmp.release();
+      fs.fp64_fallback_blocks += static_cast<int32_t>(redo.size());
+    }
+    for (std::size_t i = 0; i < P.sp.size(); ++i) {
+      SparseDev& S = P.sp[i];
+      int32_t s0 = sst[i];
+      if (fp32 && s0 != 0) {
+        // FP32 refactor failed its check: redo this block in FP64.
+        factor_sparse<double>(P, S, st, S.Lx64, S.Ux64, S.Ud64, S.S64, nullptr);
+        S.prec = 0;
+        HIPCHECK(hipMemcpyAsync(&s0, S.status, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        fs.fp64_fallback_blocks++;
+        P.h_prec[S.block] = 0;
+      }
+      if (s0 & (bdk::kStCheck | bdk::kStZeroPivot | bdk::kStNonFinite)) {
+        fs.repivot_blocks++;
+        if (fs.first_bad_block < 0) fs.first_bad_block = S.block;
+      }
+    }
+    HIPCHECK(hipMemcpyAsync(P.prec_of_block, P.h_prec.data(), sizeof(int32_t) * P.nblocks, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    P.factored = fs.failed_blocks == 0 && fs.repivot_blocks == 0;
+    if (stats) *stats = fs;
+    if (fs.failed_blocks) return set_err(c, BDLU_E_FACTOR, "block factorization failed (zero pivot)");
+    if (fs.repivot_blocks) return set_err(c, BDLU_E_REPIVOT, "fixed pivot order failed the pivot check");
+    return BDLU_OK;
+  } catch (const HipError& e) {
+    return set_err(c, BDLU_E_BACKEND, e.msg);
+  }
+}
+
+int bdlu_gpu_solve(bdlu_ctx* c, uint64_t id, double* B, int32_t nrhs, int64_t ldb, int where, const bdlu_ir_opts* ir,
+                   bdlu_solve_stats* stats) {
+  if (!c || !B || nrhs < 1) return BDLU_E_INVALID;
+  std::lock_guard<std::mutex> g(c->mu);
+  auto it = c->plans.find(id);
+  if (it == c->plans.end()) return set_err(c, BDLU_E_INVALID, "unknown plan");
+  Plan& P = *it->second;
+  if (!P.factored) return set_err(c, BDLU_E_SOLVE, "plan has no valid factors");
+  if (ldb < P.n) return set_err(c, BDLU_E_INVALID, "ldb < n");
+  hipStream_t st = c->stream;
+  const int32_t max_iters = ir ? ir->max_iters : 0;
+  const bool want_stats = max_iters > 0 || (ir && ir->stop_rule < 0);
+  bdlu_solve_stats ss{0, -1.0, -1.0};
+  try {
+    HIPCHECK(hipSetDevice(c->device));
+    const hipMemcpyKind in = where == BDLU_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipMemcpyKind out = where == BDLU_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    bdk::launch_fill_i32(st, P.err, 1, 0);
+    for (int32_t k = 0; k < nrhs; ++k) {
+      double* col = B + static_cast<int64_t>(k) * ldb;
+      HIPCHECK(hipMemcpyAsync(P.b, col, sizeof(double) * P.n, in, st));
+      ProfEvents pev;
+      if (c->prof) pev.mark(BDLU_PROF_SOLVE, st);
+      block_solve(P, st, P.b, P.x);
+      if (c->prof) {
+        pev.mark(BDLU_PROF_SOLVE, st);
+        HIPCHECK(hipStreamSynchronize(st));
+        pev.collect(c);
+      }
+      int32_t iters = 0;
+      double h[16];
+      auto eval = [&]() {
+        bdk::launch_residual(st, P.n, P.csr_ptr, P.csr_col, P.vals, P.x, P.b, P.r);
+        bdk::launch_norms(st, P.n, P.r, P.x, P.b, P.partials, P.norms);
+        HIPCHECK(hipMemcpyAsync(h, P.norms, sizeof(double) * 9, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+      };
+      if (want_stats) {
+        eval();
+        for (int it2 = 0; it2 < max_iters; ++it2) {
+          const double bn2 = std::sqrt(h[3]);
+          const double rel = bn2 > 0 ? std::sqrt(h[0]) / bn2 : 0.0;
+          const double berr_den = h[8] * h[2] + h[4];
+          const double berr = berr_den > 0 ? h[1] / berr_den : 0.0;
+          const bool stop = (ir->stop_rule == 1) ? (berr <= ir->tol) : (rel < ir->tol);
+          if (stop || bn2 == 0.0) break;
+          block_solve(P, st, P.r, P.d);
+          bdk::launch_axpy(st, P.n, P.d, P.x);
+          ++iters;
+          eval();
+        }
+        const double bn2 = std::sqrt(h[3]);
+        ss.rel_residual = bn2 > 0 ? std::sqrt(h[0]) / bn2 : 0.0;
+        const double den = h[8] * h[2] + h[4];
+        ss.backward_error = den > 0 ? h[1] / den : 0.0;
+        ss.iterations = std::max(ss.iterations, iters);
+      }
+      HIPCHECK(hipMemcpyAsync(col, P.x, sizeof(double) * P.n, out, st));
+    }
+    int32_t herr = 0;
+    HIPCHECK(hipMemcpyAsync(&herr, P.err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if (stats) *stats = ss;
+    if (herr) return set_err(c, BDLU_E_SOLVE, "solve wait bound exceeded");
+    return BDLU_OK;
+  } catch (const HipError& e) {
+    return set_err(c, BDLU_E_BACKEND, e.msg);
+  }
+}
+
+int bdlu_gpu_plan_values(bdlu_ctx* c, uint64_t id, double* host_values) {
+  if (!c || !host_values) return BDLU_E_INVALID;
+  std::lock_guard<std::mutex> g(c->mu);
+  auto it = c->plans.find(id);
+  if (it == c->plans.end()) return set_err(c, BDLU_E_INVALID, "unknown plan");
+  Plan& P = *it->second;
+  if (hipSetDevice(c->device) != hipSuccess ||
+      hipMemcpyAsync(host_values, P.vals, sizeof(double) * P.nnz, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return set_err(c, BDLU_E_BACKEND, "value copy failed");
+  return BDLU_OK;
+}
+
+void* bdlu_gpu_stream(bdlu_ctx* c) { return c ? static_cast<void*>(c->stream) : nullptr; }
+
+int bdlu_gpu_profile(bdlu_ctx* c, int enable) {
+  if (!c) return BDLU_E_INVALID;
+  std::lock_guard<std::mutex> g(c->mu);
+  c->prof = enable != 0;
+  for (int k = 0; k < BDLU_PROF_KINDS; ++k) {
+    c->prof_ms[k] = c->prof_flops[k] = c->prof_bytes[k] = 0.0;
+    c->prof_n[k] = 0;
+  }
+  return BDLU_OK;
+}
+
+int bdlu_gpu_profile_read(bdlu_ctx* c, int kind, double* total_ms, int64_t* launches, double* flops, double* bytes) {
+  if (!c || kind < 0 || kind >= BDLU_PROF_KINDS) return BDLU_E_INVALID;
+  std::lock_guard<std::mutex> g(c->mu);
+  if (total_ms) *total_ms = c->prof_ms[kind];
+  if (launches) *launches = c->prof_n[kind];
+  if (flops) *flops = c->prof_flops[kind];
+  if (bytes) *bytes = c->prof_bytes[kind];
+  return BDLU_OK;
+}
+
+int bdlu_gpu_debug_dense_lu(bdlu_ctx* c, int32_t m, double* A, int precision) {
+  if (!c || !A || m <= 0) return BDLU_E_INVALID;
+  std::lock_guard<std::mutex> g(c->mu);
+  hipStream_t st = c->stream;
+  Alloc tmp;
+  try {
+    HIPCHECK(hipSetDevice(c->device));
+    const std::size_t mm = static_cast<std::size_t>(m) * m;
+    int32_t* status = tmp.get<int32_t>(1);
+    bdk::launch_fill_i32(st, status, 1, 0);
+    if (precision == BDLU_FP32) {
+      std::vector<float> h(mm);
+      for (std::size_t i = 0; i < mm; ++i) h[i] = static_cast<float>(A[i]);
+      float* d = tmp.copy(h, st);
+      bdk::launch_tail_lu<float>(st, m, d, m, 1e300, status);
+      HIPCHECK(hipMemcpyAsync(h.data(), d, sizeof(float) * mm, hipMemcpyDeviceToHost, st));
+      HIPCHECK(hipStreamSynchronize(st));
+      for (std::size_t i = 0; i < mm; ++i) A[i] = h[i];
+    } else {
+      double* d = tmp.copy(A, mm, st);
+      bdk::launch_tail_lu<double>(st, m, d, m, 1e300, status);
+      HIPCHECK(hipMemcpyAsync(A, d, sizeof(double) * mm, hipMemcpyDeviceToHost, st));
+      HIPCHECK(hipStreamSynchronize(st));
+    }
+    tmp.release();
+    return BDLU_OK;
+  } catch (const HipError& e) {
+    tmp.release();
+    return set_err(c, BDLU_E_BACKEND, e.msg);
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,94 @@
+#pragma once
+// Device kernels of the BLOCK-route engine (gfx950) and their launchers.
+// Launchers enqueue on the given stream and never synchronize.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+namespace bdk {
+
+// Per-sparse-block device view (pointers into device memory).
+struct SpView {
+  int32_t s, k0, m, scale;
+  double inv_tol;          // 1/tol: |L| above this fails the fixed-pivot check
+  int64_t a_val0;
+  const int32_t* a_ptr;    // s+1
+  const int32_t* a_row;
+  const int32_t* prow;     // k0
+  const int32_t* l_ptr;    // k0+1
+  const int32_t* l_row;
+  const int32_t* u_ptr;    // s+1
+  const int32_t* u_idx;
+  const int32_t* trow;     // m
+  const int32_t* tpos;     // m: tail-local row -> tail position
+  const int32_t* rs_ptr;   // s+1: block entries by local row (for the row scaling)
+  const int32_t* rs_idx;
+};
+
+// Status bits written by the factor kernels into a per-block int.
+enum : int32_t { kStZeroPivot = 1, kStCheck = 2, kStNonFinite = 4 };
+
+void launch_scatter(hipStream_t st, int64_t nnz, const double* vals, const int64_t* vmap, double* arena);
+void launch_fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v);
+void launch_fill_f64(hipStream_t st, double* p, int64_t n, double v);
+
+template <typename T>
+void launch_dense_lu(hipStream_t st, int32_t nd, const int32_t* sizes, const int32_t* starts, const int64_t* val0,
+                     const int64_t* lu0, const double* arena, T* lu, int32_t* piv, int32_t* status,
+                     const int32_t* block_of);
+
+void launch_rowscale(hipStream_t st, const SpView& v, const double* arena, double* rs);
+template <typename T>
+void launch_sp_cols(hipStream_t st, const SpView& v, const double* arena, const double* rs, int32_t ncols,
+                    const int32_t* cols, T* xs, int32_t nslots, T* Lx, T* Ux, T* Ud, int32_t* status);
+template <typename T>
+void launch_sp_tail(hipStream_t st, const SpView& v, const double* arena, const double* rs, T* xs, int32_t nslots,
+                    const T* Lx, T* Ux, T* S, int64_t lds);
+// Optional per-launch event timing of the tail kernels (profiling runs only).
+struct TailTimer {
+  std::vector<hipEvent_t>* gemm_events = nullptr;   // pairs (start, stop)
+  std::vector<double>* gemm_flops = nullptr;
+  std::vector<double>* gemm_bytes = nullptr;
+  std::vector<hipEvent_t>* panel_events = nullptr;  // pairs around diag + trsm
+};
+
+// No-pivot blocked LU of the m x m column-major S (the tail after its rows were
+// placed in the fixed pivot order). Check: |L| <= inv_tol and nonzero pivots.
+template <typename T>
+void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double inv_tol, int32_t* status,
+                    TailTimer* timer = nullptr);
+
+// ---- solve ----
+void launch_permute(hipStream_t st, int32_t n, const int32_t* P, const double* src, double* dst);    // dst[k]=src[P[k]]
+void launch_unpermute(hipStream_t st, int32_t n, const int32_t* Q, const double* src, double* dst);  // dst[Q[k]]=src[k]
+void launch_dense_seg_solve(hipStream_t st, int32_t ntasks, const int32_t* tasks, const int32_t* R,
+                            const int64_t* lu0_of_block, const double* lu64, const float* lu32,
+                            const int32_t* prec_of_block, const int32_t* piv, const int32_t* off_ptr,
+                            const int32_t* off_col, const double* off_val, const int32_t* dep_ptr,
+                            const int32_t* dep_blk, const int32_t* seg_of, int32_t seg, int32_t* done, int32_t epoch,
+                            int32_t* ticket, const double* bperm, double* xperm, int32_t* err);
+// Sparse block solve pieces; y is a length-s work vector in position order.
+void launch_sp_gather(hipStream_t st, int32_t s, int32_t start, const int32_t* rowperm, const double* rs,
+                      const int32_t* off_ptr, const int32_t* off_col, const double* off_val, const double* bperm,
+                      const double* xperm, double* y);
+template <typename T>
+void launch_sp_rows(hipStream_t st, int32_t nrows, const int32_t* rows, const int32_t* ptr, const int32_t* col,
+                    const int32_t* vidx, const T* vals, const T* diag, double* y);
+template <typename T>
+void launch_tail_trsv(hipStream_t st, int32_t m, const T* S, int64_t lds, double* y, bool upper, int32_t* flags,
+                      int32_t epoch, int32_t* ticket, int32_t* err);
+void launch_sp_scatter_x(hipStream_t st, int32_t s, int32_t start, const int32_t* q, const double* x, double* xperm);
+
+// ---- residual / norms / refinement ----
+void launch_residual(hipStream_t st, int32_t n, const int64_t* csr_ptr, const int32_t* csr_col, const double* vals,
+                     const double* x, const double* b, double* r);
+// out[0]=sum r^2, out[1]=max|r|, out[2]=max|x|, out[3]=sum b^2, out[4]=max|b|
+void launch_norms(hipStream_t st, int32_t n, const double* r, const double* x, const double* b, double* partials,
+                  double* out);
+// out[0] = max_i sum_j |a_ij|
+void launch_anorm(hipStream_t st, int32_t n, const int64_t* csr_ptr, const double* vals, double* partials,
+                  double* out);
+void launch_axpy(hipStream_t st, int32_t n, const double* d, double* x);  // x += d
+
+}  // namespace bdk

@@ -1,0 +1,844 @@
+// HIP kernels for gfx950 (CDNA4, 64-wide wavefronts, MFMA, 160 KB LDS/CU).
+//
+// Built with -ffp-contract=off: every multiply-subtract below rounds twice,
+// in the same order as the reference's loops (ubtf:1465-1494 for the dense
+// LU), which keeps the FP64 dense factors bitwise equal to the CPU restatement
+// and the pivot sequence identical. The tail GEMM uses MFMA (fused, its own
+// accumulation order); only its results are tolerance-checked.
+#include "kernels.h"
+
+#include <hip/hip_runtime.h>
+
+namespace bdk {
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kLdsPad = 65;  // odd leading dimension for 64x64 tiles in LDS
+
+__device__ __forceinline__ void wave_sync_mem() {
+  // Orders this wave's global-memory accesses: every lane's earlier loads and
+  // stores complete before any later one issues (all lanes share the CU's L1).
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <typename T>
+__device__ __forceinline__ T tabs(T x) {
+  return x < T(0) ? -x : x;
+}
+template <typename T>
+__device__ __forceinline__ T tmax(T a, T b) {
+  return a < b ? b : a;
+}
+
+__device__ __forceinline__ int32_t load_acquire(const int32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_release(int32_t* p, int32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bounded spin: returns false (and the caller records an error) if the flag
+// never reaches `epoch`; valid schedules never come near the bound.
+__device__ __forceinline__ bool wait_flag(const int32_t* flag, int32_t epoch) {
+  for (int64_t it = 0; it < (int64_t(1) << 26); ++it) {
+    if (load_acquire(flag) == epoch) return true;
+    __builtin_amdgcn_s_sleep(2);
+  }
+  return false;
+}
+
+inline int grid_for(int64_t n, int block, int cap = 65535) {
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return static_cast<int>(g);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// value routing
+__global__ void k_scatter(int64_t nnz, const double* __restrict__ vals, const int64_t* __restrict__ vmap,
+                          double* __restrict__ arena) {
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < nnz; e += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t d = vmap[e];
+    if (d >= 0) arena[d] = vals[e];
+  }
+}
+void launch_scatter(hipStream_t st, int64_t nnz, const double* vals, const int64_t* vmap, double* arena) {
+  if (nnz <= 0) return;
+  hipLaunchKernelGGL(k_scatter, dim3(grid_for(nnz, 256, 8192)), dim3(256), 0, st, nnz, vals, vmap, arena);
+}
+
+template <typename T>
+__global__ void k_fill(T* p, int64_t n, T v) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) p[i] = v;
+}
+void launch_fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v) {
+  if (n > 0) hipLaunchKernelGGL(k_fill<int32_t>, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, p, n, v);
+}
+void launch_fill_f64(hipStream_t st, double* p, int64_t n, double v) {
+  if (n > 0) hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, p, n, v);
+}
+
+// ---------------------------------------------------------------------------
+// K1: batched dense LU with partial pivoting, one wavefront per block (s <= 64).
+// Same arithmetic as the reference dense branch (ubtf:1465-1494): pivot = first
+// row with the largest |a_ik| (ties -> smaller row), fail iff that is 0, full
+// row swap, l_ik = a_ik / a_kk, a_ij -= l_ik * a_kj.
+template <typename T>
+__global__ __launch_bounds__(64) void k_dense_lu(int32_t nd, const int32_t* __restrict__ sizes,
+                                                 const int32_t* __restrict__ starts, const int64_t* __restrict__ val0,
+                                                 const int64_t* __restrict__ lu0, const double* __restrict__ arena,
+                                                 T* __restrict__ lu, int32_t* __restrict__ piv,
+                                                 int32_t* __restrict__ status, const int32_t* __restrict__ block_of) {
+  __shared__ T a[64 * kLdsPad];
+  const int lane = threadIdx.x;
+  for (int32_t t = blockIdx.x; t < nd; t += gridDim.x) {
+    const int s = sizes[t];
+    const double* src = arena + val0[t];
+    for (int idx = lane; idx < s * s; idx += kWave) {
+      const int j = idx / s, i = idx - j * s;
+      a[j * kLdsPad + i] = static_cast<T>(src[idx]);
+    }
+    __syncthreads();
+    int32_t mypiv = lane;
+    int32_t st = 0;
+    T amax_in = T(0);
+    if (sizeof(T) == 4) {
+      for (int idx = lane; idx < s * s; idx += kWave) {
+        const int j = idx / s, i = idx - j * s;
+        amax_in = tmax(amax_in, tabs(a[j * kLdsPad + i]));
+      }
+      for (int off = 32; off > 0; off >>= 1) amax_in = tmax(amax_in, __shfl_xor(amax_in, off));
+    }
+    T umin = T(0);
+    for (int k = 0; k < s; ++k) {
+      T v = (lane >= k && lane < s) ? tabs(a[k * kLdsPad + lane]) : T(-1);
+      int idx = lane;
+      for (int off = 32; off > 0; off >>= 1) {
+        const T ov = __shfl_xor(v, off);
+        const int oi = __shfl_xor(idx, off);
+        if (ov > v || (ov == v && oi < idx)) {
+          v = ov;
+          idx = oi;
+        }
+      }
+      if (!(v > T(0))) {  // best == 0 (or NaN): the reference marks the block failed
+        st |= kStZeroPivot;
+        break;
+      }
+      if (k == 0 || v < umin) umin = v;
+      const int p = idx;
+      if (lane == k) mypiv = p;
+      if (p != k) {
+        if (lane < s) {
+          const T x = a[lane * kLdsPad + k];
+          a[lane * kLdsPad + k] = a[lane * kLdsPad + p];
+          a[lane * kLdsPad + p] = x;
+        }
+        __syncthreads();
+      }
+      const T akk = a[k * kLdsPad + k];
+      if (lane > k && lane < s) {
+        const T l = a[k * kLdsPad + lane] / akk;
+        a[k * kLdsPad + lane] = l;
+        for (int j = k + 1; j < s; ++j) a[j * kLdsPad + lane] = a[j * kLdsPad + lane] - l * a[j * kLdsPad + k];
+      }
+      __syncthreads();
+    }
+    if (sizeof(T) == 4 && st == 0) {
+      // FP32 small-pivot check: a pivot far below the block's scale means FP32
+      // factors cannot drive FP64 refinement; the block is redone in FP64.
+      if (umin < amax_in * T(7.62939453125e-06)) st |= kStCheck;
+    }
+    T* dst = lu + lu0[t];
+    for (int idx = lane; idx < s * s; idx += kWave) {
+      const int j = idx / s, i = idx - j * s;
+      dst[idx] = a[j * kLdsPad + i];
+    }
+    if (lane < s) piv[starts[t] + lane] = mypiv;
+    if (lane == 0) status[block_of[t]] = st;
+    __syncthreads();
+  }
+}
+
+template <typename T>
+void launch_dense_lu(hipStream_t st, int32_t nd, const int32_t* sizes, const int32_t* starts, const int64_t* val0,
+                     const int64_t* lu0, const double* arena, T* lu, int32_t* piv, int32_t* status,
+                     const int32_t* block_of) {
+  if (nd <= 0) return;
+  hipLaunchKernelGGL(k_dense_lu<T>, dim3(grid_for(nd, 1, 16384)), dim3(64), 0, st, nd, sizes, starts, val0, lu0,
+                     arena, lu, piv, status, block_of);
+}
+template void launch_dense_lu<double>(hipStream_t, int32_t, const int32_t*, const int32_t*, const int64_t*,
+                                      const int64_t*, const double*, double*, int32_t*, int32_t*, const int32_t*);
+template void launch_dense_lu<float>(hipStream_t, int32_t, const int32_t*, const int32_t*, const int64_t*,
+                                     const int64_t*, const double*, float*, int32_t*, int32_t*, const int32_t*);
+
+// ---------------------------------------------------------------------------
+// Sparse blocks: KLU scale=1 row sums, in a fixed order per row.
+__global__ void k_rowscale(SpView v, const double* __restrict__ arena, double* __restrict__ rs) {
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < v.s; i += gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int32_t p = v.rs_ptr[i]; p < v.rs_ptr[i + 1]; ++p) acc += fabs(arena[v.a_val0 + v.rs_idx[p]]);
+    rs[i] = acc == 0.0 ? 1.0 : acc;
+  }
+}
+void launch_rowscale(hipStream_t st, const SpView& v, const double* arena, double* rs) {
+  hipLaunchKernelGGL(k_rowscale, dim3(grid_for(v.s, 256, 1024)), dim3(256), 0, st, v, arena, rs);
+}
+
+// Left-looking refactorization of the sparse leading columns on the fixed
+// pattern, one wavefront per column, one level per launch. x is the wave's
+// dense scratch column (length s, all zero between columns). Updates follow
+// ascending pivot position, the order the host analysis used.
+template <typename T>
+__global__ __launch_bounds__(64) void k_sp_cols(SpView v, const double* __restrict__ arena,
+                                                const double* __restrict__ rs, int32_t ncols,
+                                                const int32_t* __restrict__ cols, T* __restrict__ xs,
+                                                T* __restrict__ Lx, T* __restrict__ Ux, T* __restrict__ Ud,
+                                                int32_t* __restrict__ status) {
+  const int lane = threadIdx.x;
+  T* x = xs + int64_t(blockIdx.x) * v.s;
+  int32_t st = 0;
+  T lmax_all = T(0);
+  for (int32_t c = blockIdx.x; c < ncols; c += gridDim.x) {
+    const int32_t j = cols[c];
+    for (int32_t p = v.a_ptr[j] + lane; p < v.a_ptr[j + 1]; p += kWave) {
+      const int32_t r = v.a_row[p];
+      const double a = arena[v.a_val0 + p];
+      x[r] = static_cast<T>(v.scale ? a / rs[r] : a);
+    }
+    wave_sync_mem();
+    for (int32_t up = v.u_ptr[j]; up < v.u_ptr[j + 1]; ++up) {
+      const int32_t k = v.u_idx[up];
+      const T xk = x[v.prow[k]];
+      if (lane == 0) Ux[up] = xk;
+      for (int32_t p = v.l_ptr[k] + lane; p < v.l_ptr[k + 1]; p += kWave) {
+        const int32_t r = v.l_row[p];
+        x[r] = x[r] - Lx[p] * xk;
+      }
+      wave_sync_mem();
+    }
+    const T piv = x[v.prow[j]];
+    if (lane == 0) Ud[j] = piv;
+    if (piv == T(0)) st |= kStZeroPivot;
+    for (int32_t p = v.l_ptr[j] + lane; p < v.l_ptr[j + 1]; p += kWave) {
+      const int32_t r = v.l_row[p];
+      const T l = x[r] / piv;
+      Lx[p] = l;
+      const T al = tabs(l);
+      if (!(al <= lmax_all)) lmax_all = al;  // NaN propagates as "large"
+      x[r] = T(0);
+    }
+    for (int32_t up = v.u_ptr[j] + lane; up < v.u_ptr[j + 1]; up += kWave) x[v.prow[v.u_idx[up]]] = T(0);
+    wave_sync_mem();
+    if (lane == 0) x[v.prow[j]] = T(0);
+    wave_sync_mem();
+  }
+  if (!(lmax_all <= static_cast<T>(v.inv_tol))) st |= kStCheck;
+  for (int off = 32; off > 0; off >>= 1) st |= __shfl_xor(st, off);
+  if (lane == 0 && st) atomicOr(status, st);
+}
+
+template <typename T>
+void launch_sp_cols(hipStream_t st, const SpView& v, const double* arena, const double* rs, int32_t ncols,
+                    const int32_t* cols, T* xs, int32_t nslots, T* Lx, T* Ux, T* Ud, int32_t* status) {
+  if (ncols <= 0) return;
+  const int g = ncols < nslots ? ncols : nslots;
+  hipLaunchKernelGGL(k_sp_cols<T>, dim3(g), dim3(64), 0, st, v, arena, rs, ncols, cols, xs, Lx, Ux, Ud, status);
+}
+template void launch_sp_cols<double>(hipStream_t, const SpView&, const double*, const double*, int32_t,
+                                     const int32_t*, double*, int32_t, double*, double*, double*, int32_t*);
+template void launch_sp_cols<float>(hipStream_t, const SpView&, const double*, const double*, int32_t,
+                                    const int32_t*, float*, int32_t, float*, float*, float*, int32_t*);
+
+// Tail columns: U(0:k0, j) and the Schur column, written straight into the
+// tail's fixed pivot-row order (column-major, leading dimension lds).
+template <typename T>
+__global__ __launch_bounds__(64) void k_sp_tail(SpView v, const double* __restrict__ arena,
+                                                const double* __restrict__ rs, T* __restrict__ xs,
+                                                const T* __restrict__ Lx, T* __restrict__ Ux, T* __restrict__ S,
+                                                int64_t lds) {
+  const int lane = threadIdx.x;
+  T* x = xs + int64_t(blockIdx.x) * v.s;
+  for (int32_t t = blockIdx.x; t < v.m; t += gridDim.x) {
+    const int32_t j = v.k0 + t;
+    for (int32_t p = v.a_ptr[j] + lane; p < v.a_ptr[j + 1]; p += kWave) {
+      const int32_t r = v.a_row[p];
+      const double a = arena[v.a_val0 + p];
+      x[r] = static_cast<T>(v.scale ? a / rs[r] : a);
+    }
+    wave_sync_mem();
+    for (int32_t up = v.u_ptr[j]; up < v.u_ptr[j + 1]; ++up) {
+      const int32_t k = v.u_idx[up];
+      const T xk = x[v.prow[k]];
+      if (lane == 0) Ux[up] = xk;
+      for (int32_t p = v.l_ptr[k] + lane; p < v.l_ptr[k + 1]; p += kWave) {
+        const int32_t r = v.l_row[p];
+        x[r] = x[r] - Lx[p] * xk;
+      }
+      wave_sync_mem();
+    }
+    T* col = S + int64_t(t) * lds;
+    for (int32_t r = lane; r < v.m; r += kWave) {
+      const int32_t row = v.trow[r];
+      col[v.tpos[r]] = x[row];
+      x[row] = T(0);
+    }
+    for (int32_t up = v.u_ptr[j] + lane; up < v.u_ptr[j + 1]; up += kWave) x[v.prow[v.u_idx[up]]] = T(0);
+    wave_sync_mem();
+  }
+}
+template <typename T>
+void launch_sp_tail(hipStream_t st, const SpView& v, const double* arena, const double* rs, T* xs, int32_t nslots,
+                    const T* Lx, T* Ux, T* S, int64_t lds) {
+  if (v.m <= 0) return;
+  const int g = v.m < nslots ? v.m : nslots;
+  hipLaunchKernelGGL(k_sp_tail<T>, dim3(g), dim3(64), 0, st, v, arena, rs, xs, Lx, Ux, S, lds);
+}
+template void launch_sp_tail<double>(hipStream_t, const SpView&, const double*, const double*, double*, int32_t,
+                                     const double*, double*, double*, int64_t);
+template void launch_sp_tail<float>(hipStream_t, const SpView&, const double*, const double*, float*, int32_t,
+                                    const float*, float*, float*, int64_t);
+
+// ---------------------------------------------------------------------------
+// Dense tail: no-pivot blocked right-looking LU, panel width 64.
+constexpr int kNb = 64;
+
+// Diagonal 64x64 block, one wavefront, in LDS.
+template <typename T>
+__global__ __launch_bounds__(64) void k_diag_lu(int32_t m, T* __restrict__ S, int64_t lds, int32_t p, int32_t b,
+                                                double inv_tol, int32_t* __restrict__ status) {
+  __shared__ T a[kNb * kLdsPad];
+  const int lane = threadIdx.x;
+  for (int j = 0; j < b; ++j)
+    if (lane < b) a[j * kLdsPad + lane] = S[int64_t(p + j) * lds + p + lane];
+  __syncthreads();
+  int32_t st = 0;
+  T lmax = T(0);
+  for (int k = 0; k < b; ++k) {
+    const T akk = a[k * kLdsPad + k];
+    if (akk == T(0)) st |= kStZeroPivot;
+    if (lane > k && lane < b) {
+      const T l = a[k * kLdsPad + lane] / akk;
+      a[k * kLdsPad + lane] = l;
+      const T al = tabs(l);
+      if (!(al <= lmax)) lmax = al;
+      for (int j = k + 1; j < b; ++j) a[j * kLdsPad + lane] = a[j * kLdsPad + lane] - l * a[j * kLdsPad + k];
+    }
+    __syncthreads();
+  }
+  for (int j = 0; j < b; ++j)
+    if (lane < b) S[int64_t(p + j) * lds + p + lane] = a[j * kLdsPad + lane];
+  if (!(lmax <= static_cast<T>(inv_tol))) st |= kStCheck;
+  for (int off = 32; off > 0; off >>= 1) st |= __shfl_xor(st, off);
+  if (lane == 0 && st) atomicOr(status, st);
+}
+
+// L21 = A21 * U11^{-1}: one wavefront per 64 rows, a lane per row; U11 and
+// the lane's finished row entries live in LDS (column j of the row solve reads
+// only the same lane's earlier entries, so no barrier is needed inside).
+template <typename T>
+__global__ __launch_bounds__(64) void k_trsm_l(int32_t m, T* __restrict__ S, int64_t lds, int32_t p, int32_t b,
+                                               double inv_tol, int32_t* __restrict__ status) {
+  __shared__ T u[kNb * kNb];   // u[j*64 + k] = U(k, j)
+  __shared__ T xr[kNb * kNb];  // xr[k*64 + lane] = L(row, k)
+  const int lane = threadIdx.x;
+  for (int idx = lane; idx < kNb * kNb; idx += kWave) {
+    const int j = idx / kNb, k = idx - j * kNb;
+    T val = (j == k) ? T(1) : T(0);
+    if (j < b && k < b && k <= j) val = S[int64_t(p + j) * lds + p + k];
+    u[idx] = val;
+  }
+  __syncthreads();
+  const int32_t row = p + b + blockIdx.x * kWave + lane;
+  const bool live = row < m;
+  T lmax = T(0);
+  for (int j = 0; j < b; ++j) {
+    T acc = live ? S[int64_t(p + j) * lds + row] : T(0);
+    for (int k = 0; k < j; ++k) acc = acc - xr[k * kNb + lane] * u[j * kNb + k];
+    const T x = acc / u[j * kNb + j];
+    xr[j * kNb + lane] = x;
+    if (live) S[int64_t(p + j) * lds + row] = x;
+    const T ax = tabs(x);
+    if (live && !(ax <= lmax)) lmax = ax;
+  }
+  int32_t st = !(lmax <= static_cast<T>(inv_tol)) ? kStCheck : 0;
+  for (int off = 32; off > 0; off >>= 1) st |= __shfl_xor(st, off);
+  if (lane == 0 && st) atomicOr(status, st);
+}
+
+// U12 = L11^{-1} * A12: one wavefront per column, lanes over rows.
+template <typename T>
+__global__ __launch_bounds__(64) void k_trsm_u(int32_t m, T* __restrict__ S, int64_t lds, int32_t p, int32_t b) {
+  const int lane = threadIdx.x;
+  const int32_t col = p + b + blockIdx.x;
+  if (col >= m) return;
+  T* c = S + int64_t(col) * lds + p;
+  T acc = (lane < b) ? c[lane] : T(0);
+  for (int k = 0; k < b; ++k) {
+    const T xk = __shfl(acc, k);
+    if (lane > k && lane < b) acc = acc - S[int64_t(p + k) * lds + p + lane] * xk;
+  }
+  if (lane < b) c[lane] = acc;
+}
+
+// Trailing update C -= A * B on MFMA. 64x64 C tile per 256-thread workgroup,
+// each wavefront a 32x32 quarter as 2x2 MFMA 16x16 tiles; A/B panel tiles
+// staged in LDS; k padded to a multiple of 4 with zeros.
+template <typename T>
+struct MfmaOps;
+template <>
+struct MfmaOps<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  __device__ static v4 mma(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+};
+template <>
+struct MfmaOps<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  __device__ static v4 mma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_update(int32_t m, T* __restrict__ S, int64_t lds, int32_t p,
+                                                     int32_t b) {
+  using V = typename MfmaOps<T>::v4;
+  // Leading dimension 64 keeps both tiles within 64 KB of LDS for FP64.
+  constexpr int LD = kNb;
+  __shared__ T As[kNb * LD];  // As[k*LD + i] = A(i, k)
+  __shared__ T Bs[kNb * LD];  // Bs[k*LD + j] = B(k, j)
+  const int32_t base = p + b;
+  const int32_t M = m - base;
+  const int32_t i0 = blockIdx.x * kNb, j0 = blockIdx.y * kNb;
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < kNb * kNb; idx += 256) {
+    const int k = idx / kNb, i = idx - k * kNb;
+    T av = T(0);
+    if (k < b && i0 + i < M) av = S[int64_t(p + k) * lds + base + i0 + i];
+    As[k * LD + i] = av;
+  }
+  for (int idx = tid; idx < kNb * kNb; idx += 256) {
+    const int j = idx / kNb, k = idx - j * kNb;
+    T bv = T(0);
+    if (k < b && j0 + j < M) bv = S[int64_t(base + j0 + j) * lds + p + k];
+    Bs[k * LD + j] = bv;
+  }
+  __syncthreads();
+  const int wave = tid / kWave, lane = tid - wave * kWave;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  V acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[a][c] = V{T(0), T(0), T(0), T(0)};
+  const int kend = (b + 3) & ~3;
+  for (int kk = 0; kk < kend; kk += 4) {
+    T av[2], bv[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) av[a] = As[(kk + lk) * LD + wr + a * 16 + li];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) bv[c] = Bs[(kk + lk) * LD + wc + c * 16 + li];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) acc[a][c] = MfmaOps<T>::mma(av[a], bv[c], acc[a][c]);
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int32_t i = i0 + wr + a * 16 + lk * 4 + r;
+        const int32_t j = j0 + wc + c * 16 + li;
+        if (i < M && j < M) {
+          T* cp = S + int64_t(base + j) * lds + base + i;
+          *cp = *cp - acc[a][c][r];
+        }
+      }
+}
+
+namespace {
+hipEvent_t timer_event(std::vector<hipEvent_t>* pool, hipStream_t st) {
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  (void)hipEventRecord(e, st);
+  pool->push_back(e);
+  return e;
+}
+}  // namespace
+
+template <typename T>
+void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double inv_tol, int32_t* status,
+                    TailTimer* timer) {
+  for (int32_t p = 0; p < m; p += kNb) {
+    const int32_t b = (m - p) < kNb ? (m - p) : kNb;
+    const int32_t rest = m - p - b;
+    if (timer && timer->panel_events) timer_event(timer->panel_events, st);
+    hipLaunchKernelGGL(k_diag_lu<T>, dim3(1), dim3(64), 0, st, m, S, lds, p, b, inv_tol, status);
+    if (rest > 0) {
+      hipLaunchKernelGGL(k_trsm_l<T>, dim3(grid_for(rest, kWave)), dim3(kWave), 0, st, m, S, lds, p, b, inv_tol,
+                         status);
+      hipLaunchKernelGGL(k_trsm_u<T>, dim3(rest), dim3(64), 0, st, m, S, lds, p, b);
+    }
+    if (timer && timer->panel_events) timer_event(timer->panel_events, st);
+    if (rest <= 0) break;
+    const int tiles = (rest + kNb - 1) / kNb;
+    if (timer && timer->gemm_events) timer_event(timer->gemm_events, st);
+    hipLaunchKernelGGL(k_gemm_update<T>, dim3(tiles, tiles), dim3(256), 0, st, m, S, lds, p, b);
+    if (timer && timer->gemm_events) {
+      timer_event(timer->gemm_events, st);
+      const double r = rest;
+      timer->gemm_flops->push_back(2.0 * r * r * b);
+      timer->gemm_bytes->push_back(sizeof(T) * (2.0 * r * r + 2.0 * r * b));
+    }
+  }
+}
+template void launch_tail_lu<double>(hipStream_t, int32_t, double*, int64_t, double, int32_t*, TailTimer*);
+template void launch_tail_lu<float>(hipStream_t, int32_t, float*, int64_t, double, int32_t*, TailTimer*);
+
+// ---------------------------------------------------------------------------
+// Solve.
+__global__ void k_permute(int32_t n, const int32_t* __restrict__ P, const double* __restrict__ src,
+                          double* __restrict__ dst) {
+  for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) dst[k] = src[P[k]];
+}
+__global__ void k_unpermute(int32_t n, const int32_t* __restrict__ Q, const double* __restrict__ src,
+                            double* __restrict__ dst) {
+  for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) dst[Q[k]] = src[k];
+}
+void launch_permute(hipStream_t st, int32_t n, const int32_t* P, const double* src, double* dst) {
+  hipLaunchKernelGGL(k_permute, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, n, P, src, dst);
+}
+void launch_unpermute(hipStream_t st, int32_t n, const int32_t* Q, const double* src, double* dst) {
+  hipLaunchKernelGGL(k_unpermute, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, n, Q, src, dst);
+}
+
+// Dense-block DAG solve of one segment: a persistent wavefront per task, tasks
+// claimed in order from a ticket counter (descending block id = a topological
+// order of the block DAG), so every dependency a wave waits for has already
+// been claimed by a running wave. rhs = bperm - A_off * xperm (ubtf:2269-2286),
+// then pivot swaps, unit-L forward and U backward (ubtf:2071-2097).
+__global__ __launch_bounds__(64) void k_dense_seg_solve(
+    int32_t ntasks, const int32_t* __restrict__ tasks, const int32_t* __restrict__ R,
+    const int64_t* __restrict__ lu0_of_block, const double* __restrict__ lu64, const float* __restrict__ lu32,
+    const int32_t* __restrict__ prec_of_block, const int32_t* __restrict__ piv, const int32_t* __restrict__ off_ptr,
+    const int32_t* __restrict__ off_col, const double* __restrict__ off_val, const int32_t* __restrict__ dep_ptr,
+    const int32_t* __restrict__ dep_blk, const int32_t* __restrict__ seg_of, int32_t seg, int32_t* __restrict__ done,
+    int32_t epoch, int32_t* __restrict__ ticket, const double* __restrict__ bperm, double* __restrict__ xperm,
+    int32_t* __restrict__ err) {
+  __shared__ double y[64];
+  __shared__ int32_t tk;
+  const int lane = threadIdx.x;
+  while (true) {
+    if (lane == 0) tk = atomicAdd(ticket, 1);
+    __syncthreads();
+    const int32_t t = tk;
+    __syncthreads();
+    if (t >= ntasks) break;
+    const int32_t b = tasks[t];
+    // Wait for dependencies inside this segment (others finished earlier).
+    bool ok = true;
+    for (int32_t d = dep_ptr[b] + lane; d < dep_ptr[b + 1]; d += kWave) {
+      const int32_t c = dep_blk[d];
+      if (seg_of[c] == seg && !wait_flag(done + c, epoch)) ok = false;
+    }
+    if (!ok) atomicOr(err, 1);
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const int32_t r0 = R[b];
+    const int s = R[b + 1] - r0;
+    double acc = 0.0;
+    if (lane < s) {
+      const int32_t r = r0 + lane;
+      acc = bperm[r];
+      for (int32_t p = off_ptr[r]; p < off_ptr[r + 1]; ++p) acc = acc - off_val[p] * xperm[off_col[p]];
+      y[lane] = acc;
+    }
+    __syncthreads();
+    if (lane == 0) {
+      for (int k = 0; k < s; ++k) {
+        const int32_t pk = piv[r0 + k];
+        if (pk != k) {
+          const double tmp = y[k];
+          y[k] = y[pk];
+          y[pk] = tmp;
+        }
+      }
+    }
+    __syncthreads();
+    acc = lane < s ? y[lane] : 0.0;
+    const bool single = prec_of_block[b] != 0;
+    const double* f64 = lu64 + lu0_of_block[b];
+    const float* f32 = lu32 + lu0_of_block[b];
+    auto f = [&](int idx) -> double { return single ? static_cast<double>(f32[idx]) : f64[idx]; };
+    for (int j = 0; j < s; ++j) {
+      const double xj = __shfl(acc, j);
+      if (lane > j && lane < s) acc = acc - f(j * s + lane) * xj;
+    }
+    for (int j = s - 1; j >= 0; --j) {
+      if (lane == j) acc = acc / f(j * s + j);
+      const double xj = __shfl(acc, j);
+      if (lane < j) acc = acc - f(j * s + lane) * xj;
+    }
+    if (lane < s) xperm[r0 + lane] = acc;
+    __syncthreads();
+    if (lane == 0) store_release(done + b, epoch);
+  }
+}
+
+void launch_dense_seg_solve(hipStream_t st, int32_t ntasks, const int32_t* tasks, const int32_t* R,
+                            const int64_t* lu0_of_block, const double* lu64, const float* lu32,
+                            const int32_t* prec_of_block, const int32_t* piv, const int32_t* off_ptr,
+                            const int32_t* off_col, const double* off_val, const int32_t* dep_ptr,
+                            const int32_t* dep_blk, const int32_t* seg_of, int32_t seg, int32_t* done, int32_t epoch,
+                            int32_t* ticket, const double* bperm, double* xperm, int32_t* err) {
+  if (ntasks <= 0) return;
+  const int g = ntasks < 2048 ? ntasks : 2048;
+  hipLaunchKernelGGL(k_dense_seg_solve, dim3(g), dim3(64), 0, st, ntasks, tasks, R, lu0_of_block, lu64, lu32,
+                     prec_of_block, piv, off_ptr, off_col, off_val, dep_ptr, dep_blk, seg_of, seg, done, epoch,
+                     ticket, bperm, xperm, err);
+}
+
+// Sparse block: y[k] = (bperm[r] - A_off(r,:) xperm) / Rs(row), r = start + rowperm[k].
+__global__ void k_sp_gather(int32_t s, int32_t start, const int32_t* __restrict__ rowperm,
+                            const double* __restrict__ rs, const int32_t* __restrict__ off_ptr,
+                            const int32_t* __restrict__ off_col, const double* __restrict__ off_val,
+                            const double* __restrict__ bperm, const double* __restrict__ xperm,
+                            double* __restrict__ y) {
+  for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < s; k += gridDim.x * blockDim.x) {
+    const int32_t lr = rowperm[k];
+    const int32_t r = start + lr;
+    double acc = bperm[r];
+    for (int32_t p = off_ptr[r]; p < off_ptr[r + 1]; ++p) acc = acc - off_val[p] * xperm[off_col[p]];
+    y[k] = rs ? acc / rs[lr] : acc;
+  }
+}
+void launch_sp_gather(hipStream_t st, int32_t s, int32_t start, const int32_t* rowperm, const double* rs,
+                      const int32_t* off_ptr, const int32_t* off_col, const double* off_val, const double* bperm,
+                      const double* xperm, double* y) {
+  hipLaunchKernelGGL(k_sp_gather, dim3(grid_for(s, 256, 4096)), dim3(256), 0, st, s, start, rowperm, rs, off_ptr,
+                     off_col, off_val, bperm, xperm, y);
+}
+
+// Row-oriented sparse triangular step over one level: for each listed row k,
+// y[k] = (y[k] - sum vals[vidx[p]] * y[col[p]]) [/ diag[k]]. A wavefront per row.
+template <typename T>
+__global__ __launch_bounds__(64) void k_sp_rows(int32_t nrows, const int32_t* __restrict__ rows,
+                                                const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                                                const int32_t* __restrict__ vidx, const T* __restrict__ vals,
+                                                const T* __restrict__ diag, double* __restrict__ y) {
+  const int lane = threadIdx.x;
+  for (int32_t t = blockIdx.x; t < nrows; t += gridDim.x) {
+    const int32_t k = rows[t];
+    double acc = 0.0;
+    for (int32_t p = ptr[k] + lane; p < ptr[k + 1]; p += kWave)
+      acc += static_cast<double>(vals[vidx[p]]) * y[col[p]];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) {
+      double v = y[k] - acc;
+      if (diag) v = v / static_cast<double>(diag[k]);
+      y[k] = v;
+    }
+  }
+}
+template <typename T>
+void launch_sp_rows(hipStream_t st, int32_t nrows, const int32_t* rows, const int32_t* ptr, const int32_t* col,
+                    const int32_t* vidx, const T* vals, const T* diag, double* y) {
+  if (nrows <= 0) return;
+  hipLaunchKernelGGL(k_sp_rows<T>, dim3(grid_for(nrows, 1, 8192)), dim3(64), 0, st, nrows, rows, ptr, col, vidx,
+                     vals, diag, y);
+}
+template void launch_sp_rows<double>(hipStream_t, int32_t, const int32_t*, const int32_t*, const int32_t*,
+                                     const int32_t*, const double*, const double*, double*);
+template void launch_sp_rows<float>(hipStream_t, int32_t, const int32_t*, const int32_t*, const int32_t*,
+                                    const int32_t*, const float*, const float*, double*);
+
+// Dense tail triangular solve, sync-free over 64-row panels: a wavefront per
+// panel, panels claimed in dependency order from a ticket, each waits for the
+// flags of the panels it reads. Lower: unit diagonal, forward. Upper: backward.
+template <typename T>
+__global__ __launch_bounds__(64) void k_tail_trsv(int32_t m, const T* __restrict__ S, int64_t lds,
+                                                  double* __restrict__ y, int upper, int32_t* __restrict__ flags,
+                                                  int32_t epoch, int32_t* __restrict__ ticket,
+                                                  int32_t* __restrict__ err) {
+  __shared__ int32_t tk;
+  const int lane = threadIdx.x;
+  const int32_t np = (m + kNb - 1) / kNb;
+  if (lane == 0) tk = atomicAdd(ticket, 1);
+  __syncthreads();
+  const int32_t t = tk;
+  if (t >= np) return;
+  const int32_t w = upper ? (np - 1 - t) : t;
+  const int32_t r0 = w * kNb;
+  const int32_t rows = (m - r0) < kNb ? (m - r0) : kNb;
+  const int32_t row = r0 + lane;
+  double acc = lane < rows ? y[row] : 0.0;
+  bool ok = true;
+  for (int32_t q = 0; q < np - 1; ++q) {
+    int32_t j;
+    if (upper) {
+      j = np - 1 - q;
+      if (j <= w) break;
+    } else {
+      j = q;
+      if (j >= w) break;
+    }
+    if (lane == 0 && !wait_flag(flags + j, epoch)) ok = false;
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const int32_t c0 = j * kNb;
+    const int32_t cols = (m - c0) < kNb ? (m - c0) : kNb;
+    const double xv = lane < cols ? y[c0 + lane] : 0.0;
+    for (int c = 0; c < cols; ++c) {
+      const double xc = __shfl(xv, c);  // all lanes active: the source lane is never masked off
+      if (lane < rows) acc = acc - static_cast<double>(S[int64_t(c0 + c) * lds + row]) * xc;
+    }
+  }
+  if (!upper) {
+    for (int c = 0; c < rows; ++c) {
+      const double xc = __shfl(acc, c);
+      if (lane > c && lane < rows) acc = acc - static_cast<double>(S[int64_t(r0 + c) * lds + row]) * xc;
+    }
+  } else {
+    for (int c = rows - 1; c >= 0; --c) {
+      if (lane == c) acc = acc / static_cast<double>(S[int64_t(r0 + c) * lds + r0 + c]);
+      const double xc = __shfl(acc, c);
+      if (lane < c) acc = acc - static_cast<double>(S[int64_t(r0 + c) * lds + row]) * xc;
+    }
+  }
+  if (lane < rows) y[row] = acc;
+  if (!ok && lane == 0) atomicOr(err, 2);
+  __syncthreads();
+  if (lane == 0) store_release(flags + w, epoch);
+}
+template <typename T>
+void launch_tail_trsv(hipStream_t st, int32_t m, const T* S, int64_t lds, double* y, bool upper, int32_t* flags,
+                      int32_t epoch, int32_t* ticket, int32_t* err) {
+  if (m <= 0) return;
+  const int np = (m + kNb - 1) / kNb;
+  hipLaunchKernelGGL(k_tail_trsv<T>, dim3(np), dim3(64), 0, st, m, S, lds, y, upper ? 1 : 0, flags, epoch, ticket,
+                     err);
+}
+template void launch_tail_trsv<double>(hipStream_t, int32_t, const double*, int64_t, double*, bool, int32_t*,
+                                       int32_t, int32_t*, int32_t*);
+template void launch_tail_trsv<float>(hipStream_t, int32_t, const float*, int64_t, double*, bool, int32_t*, int32_t,
+                                      int32_t*, int32_t*);
+
+__global__ void k_sp_scatter_x(int32_t s, int32_t start, const int32_t* __restrict__ q, const double* __restrict__ x,
+                               double* __restrict__ xperm) {
+  for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < s; k += gridDim.x * blockDim.x)
+    xperm[start + q[k]] = x[k];
+}
+void launch_sp_scatter_x(hipStream_t st, int32_t s, int32_t start, const int32_t* q, const double* x, double* xperm) {
+  hipLaunchKernelGGL(k_sp_scatter_x, dim3(grid_for(s, 256, 4096)), dim3(256), 0, st, s, start, q, x, xperm);
+}
+
+// ---------------------------------------------------------------------------
+// Residual r = b - A x over the canonical matrix, row by row in canonical
+// order (the reference's refinement loop, src/solver.cpp:1497-1506).
+__global__ void k_residual(int32_t n, const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                           const double* __restrict__ vals, const double* __restrict__ x,
+                           const double* __restrict__ b, double* __restrict__ r) {
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double acc = b[i];
+    for (int64_t p = ptr[i]; p < ptr[i + 1]; ++p) acc = acc - vals[p] * x[col[p]];
+    r[i] = acc;
+  }
+}
+void launch_residual(hipStream_t st, int32_t n, const int64_t* csr_ptr, const int32_t* csr_col, const double* vals,
+                     const double* x, const double* b, double* r) {
+  hipLaunchKernelGGL(k_residual, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, n, csr_ptr, csr_col, vals, x, b, r);
+}
+
+// Deterministic reductions: fixed grid, fixed per-block order, then one block.
+constexpr int kRedBlocks = 512;
+__global__ __launch_bounds__(256) void k_norms_partial(int32_t n, const double* __restrict__ r,
+                                                       const double* __restrict__ x, const double* __restrict__ b,
+                                                       double* __restrict__ part) {
+  __shared__ double sh[5][256];
+  double v[5] = {0, 0, 0, 0, 0};
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double ri = r[i], xi = x[i], bi = b[i];
+    v[0] += ri * ri;
+    v[1] = fmax(v[1], fabs(ri));
+    v[2] = fmax(v[2], fabs(xi));
+    v[3] += bi * bi;
+    v[4] = fmax(v[4], fabs(bi));
+  }
+  for (int q = 0; q < 5; ++q) sh[q][threadIdx.x] = v[q];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + h];
+      sh[1][threadIdx.x] = fmax(sh[1][threadIdx.x], sh[1][threadIdx.x + h]);
+      sh[2][threadIdx.x] = fmax(sh[2][threadIdx.x], sh[2][threadIdx.x + h]);
+      sh[3][threadIdx.x] += sh[3][threadIdx.x + h];
+      sh[4][threadIdx.x] = fmax(sh[4][threadIdx.x], sh[4][threadIdx.x + h]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 5; ++q) part[blockIdx.x * 5 + q] = sh[q][0];
+}
+__global__ __launch_bounds__(256) void k_norms_final(int32_t nb, const double* __restrict__ part,
+                                                     double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double v[5] = {0, 0, 0, 0, 0};
+  for (int32_t i = 0; i < nb; ++i) {
+    v[0] += part[i * 5 + 0];
+    v[1] = fmax(v[1], part[i * 5 + 1]);
+    v[2] = fmax(v[2], part[i * 5 + 2]);
+    v[3] += part[i * 5 + 3];
+    v[4] = fmax(v[4], part[i * 5 + 4]);
+  }
+  for (int q = 0; q < 5; ++q) out[q] = v[q];
+}
+void launch_norms(hipStream_t st, int32_t n, const double* r, const double* x, const double* b, double* partials,
+                  double* out) {
+  hipLaunchKernelGGL(k_norms_partial, dim3(kRedBlocks), dim3(256), 0, st, n, r, x, b, partials);
+  hipLaunchKernelGGL(k_norms_final, dim3(1), dim3(64), 0, st, kRedBlocks, partials, out);
+}
+
+__global__ __launch_bounds__(256) void k_anorm_partial(int32_t n, const int64_t* __restrict__ ptr,
+                                                       const double* __restrict__ vals, double* __restrict__ part) {
+  __shared__ double sh[256];
+  double v = 0.0;
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int64_t p = ptr[i]; p < ptr[i + 1]; ++p) s += fabs(vals[p]);
+    v = fmax(v, s);
+  }
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + h]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+__global__ void k_max_final(int32_t nb, const double* __restrict__ part, double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double v = 0.0;
+  for (int32_t i = 0; i < nb; ++i) v = fmax(v, part[i]);
+  out[0] = v;
+}
+void launch_anorm(hipStream_t st, int32_t n, const int64_t* csr_ptr, const double* vals, double* partials,
+                  double* out) {
+  hipLaunchKernelGGL(k_anorm_partial, dim3(kRedBlocks), dim3(256), 0, st, n, csr_ptr, vals, partials);
+  hipLaunchKernelGGL(k_max_final, dim3(1), dim3(64), 0, st, kRedBlocks, partials, out);
+}
+
+__global__ void k_axpy(int32_t n, const double* __restrict__ d, double* __restrict__ x) {
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = x[i] + d[i];
+}
+void launch_axpy(hipStream_t st, int32_t n, const double* d, double* x) {
+  hipLaunchKernelGGL(k_axpy, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, n, d, x);
+}
+
+}  // namespace bdk

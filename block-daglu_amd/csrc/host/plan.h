@@ -1,0 +1,54 @@
+#pragma once
+// Device plan assembly: turns the host analysis of one sparsity pattern into
+// the plain arrays of bdlu_plan_desc (include/bdlu_gpu.h).
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "../../../include/bdlu_gpu.h"
+#include "host.h"
+#include "symbolic.h"
+
+namespace bdlu {
+
+struct HostPlan {
+  int32_t n = 0;
+  int64_t nnz = 0;
+  Decomposition dec;
+  int32_t dense_max = 64;
+  std::vector<int32_t> kind;            // per block: 0 dense, 1 sparse
+  std::vector<int32_t> sparse_blocks;   // block ids of sparse blocks
+  std::vector<SparseBlockPlan> sparse;  // parallel to sparse_blocks
+  // desc arrays
+  std::vector<int64_t> vmap;
+  int64_t arena_len = 0;
+  std::vector<int64_t> csr_ptr;
+  std::vector<int32_t> csr_col;
+  std::vector<int32_t> dep_ptr, dep_blk;
+  std::vector<int32_t> dense_block;
+  std::vector<int64_t> dense_val0;
+  int64_t off_val0 = 0;
+  std::vector<int64_t> sparse_val0;
+  std::vector<bdlu_sparse_desc> sdesc;
+  bdlu_plan_desc desc{};
+  // statistics
+  int32_t max_block = 0;
+  double flops_sparse = 0.0, flops_dense = 0.0;
+  int64_t nnz_l = 0, nnz_u = 0;
+  double analysis_seconds = 0.0;
+};
+
+struct PlanStatus {
+  bool ok = true;
+  int32_t bad_block = -1;  // sparse block whose analysis failed (singular)
+};
+
+// Builds the full host plan for a canonical matrix. Throws std::runtime_error
+// on decomposition failure; returns ok=false if a sparse block is singular.
+PlanStatus build_host_plan(const Canonical& a, HostPlan& out);
+// Fills out.desc (pointers into out's own vectors).
+void finalize_desc(HostPlan& p);
+
+int32_t dense_block_max_from_env();
+
+}  // namespace bdlu

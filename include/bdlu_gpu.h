@@ -1,0 +1,160 @@
+#pragma once
+// C-ABI of the MI355X numeric engine (libbdlu_gpu.so, HIP for gfx950).
+//
+// It replaces the numeric half of the reference BLOCK route, the internal seam
+// declared at blockdaglu/src/ubtf_block_solver.h:145-152:
+//   factor_blocks_parallel(dec, threads)  -> bdlu_gpu_refactor
+//        (called at src/solver.cpp:1252; dense branch ubtf:1441-1498, KLU
+//         branch ubtf:1749-1825)
+//   free_factors(fac)                     -> bdlu_gpu_plan_release
+//        (called at src/solver.cpp:728, :756, :1245, :1262)
+//   solve_ubtf(dec, fac, b, threads)      -> bdlu_gpu_solve
+//        (called at src/solver.cpp:1481 and :1529; body ubtf:2220-2555; the
+//         refinement loop of src/solver.cpp:1486-1536 runs inside when asked)
+// build_decomposition_btf (ubtf:570) stays on the host; its output, plus the
+// per-block symbolic analysis, crosses this boundary once per sparsity
+// pattern as a plan (plain arrays, borrowed for the duration of the call).
+//
+// Conventions: every function is extern "C", never throws, returns 0 on
+// success or a status code below, and records a message for
+// bdlu_gpu_last_error. A context owns one HIP stream; calls on one context are
+// serialized by an internal mutex and complete before they return.
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bdlu_ctx bdlu_ctx;
+
+// Status codes (the nonzero ones equal blockdaglu::SolverError values where one
+// applies, so the host maps them 1:1).
+enum {
+  BDLU_OK = 0,
+  BDLU_E_INVALID = 5,             // bad argument / size (InvalidRightHandSide role)
+  BDLU_E_BACKEND = 6,             // no device, HIP error, out of the HBM budget
+  BDLU_E_ANALYSIS = 7,            // inconsistent plan
+  BDLU_E_FACTOR = 8,              // a block has an exactly-zero pivot
+  BDLU_E_SOLVE = 9,               // solve failed (zero diagonal, non-finite result)
+  BDLU_E_REPIVOT = 20,            // a fixed-pivot check failed: re-analyse with the current values
+  BDLU_E_BUDGET = 21,             // plan does not fit the context's HBM budget
+};
+
+enum { BDLU_MEM_HOST = 0, BDLU_MEM_DEVICE = 1 };
+enum { BDLU_FP64 = 0, BDLU_FP32 = 1 };
+
+// One diagonal block larger than the dense threshold (positions are the pivot
+// order: position k < k0 pivots on local row prow[k] in column q[k]; positions
+// k0 + t pivot on local row trow[tperm[t]] in column q[k0 + t]).
+typedef struct bdlu_sparse_desc {
+  int32_t block;
+  int32_t s, k0, m;
+  int32_t scale;          // 1: rows scaled by the sum of |a_ij| inside the block (KLU scale=1)
+  double tol;             // pivot tolerance of the check (KLU tol)
+  int64_t a_val0;         // value-arena offset of the block's entries, in a_ptr order
+  const int32_t* a_ptr;   // s+1: entries of column position j
+  const int32_t* a_row;   // local row of each entry
+  const int32_t* q;       // s: position -> local column
+  const int32_t* prow;    // k0
+  const int32_t* l_ptr;   // k0+1
+  const int32_t* l_row;   // local rows of L(:,k), pivot row excluded
+  const int32_t* u_ptr;   // s+1
+  const int32_t* u_idx;   // positions k < min(j, k0), ascending per column
+  const int32_t* lev_ptr; // nlev+1
+  const int32_t* lev_col; // k0: sparse columns grouped by refactor level
+  int32_t nlev;
+  const int32_t* trow;    // m: tail-local row -> local row
+  const int32_t* tperm;   // m: tail position -> tail-local row
+} bdlu_sparse_desc;
+
+typedef struct bdlu_plan_desc {
+  int32_t n;
+  int64_t nnz;              // canonical entries = length of the values array of refactor
+  const int64_t* vmap;      // nnz: value-arena slot of each canonical entry, -1 if dropped
+  int64_t arena_len;        // doubles
+  const int64_t* csr_ptr;   // n+1: canonical entries by original row (canonical order is row-major)
+  const int32_t* csr_col;   // nnz: original column
+  const int32_t* P;         // n: permuted row k -> original row
+  const int32_t* Q;         // n: permuted col k -> original col
+  int32_t nblocks;
+  const int32_t* R;         // nblocks+1
+  const int32_t* off_ptr;   // n+1: strictly-upper inter-block entries by permuted row
+  const int32_t* off_col;   // permuted column
+  int64_t off_val0;         // arena offset of the off-diagonal values (CSR order)
+  const int32_t* dep_ptr;   // nblocks+1: blocks each block's rows couple to (all > b)
+  const int32_t* dep_blk;
+  int32_t ndense;
+  const int32_t* dense_block;  // block id of each dense block
+  const int64_t* dense_val0;   // arena offset of its s*s column-major values
+  int32_t nsparse;
+  const bdlu_sparse_desc* sparse;
+} bdlu_plan_desc;
+
+typedef struct bdlu_factor_stats {
+  int32_t failed_blocks;        // exactly-zero pivot (dense: partial pivoting found none)
+  int32_t repivot_blocks;       // fixed-pivot check failed in FP64
+  int32_t fp64_fallback_blocks; // FP32 check failed, block refactored in FP64
+  int32_t first_bad_block;      // a block id needing re-analysis, or -1
+  double max_abs_l;             // largest |L| seen by the checks (sparse blocks)
+} bdlu_factor_stats;
+
+typedef struct bdlu_ir_opts {
+  int32_t max_iters;        // 0: plain solve. >0: refinement steps at most
+  int32_t stop_rule;        // 0: reference rule ||r||_2/||b||_2 < tol (src/solver.cpp:1508-1521)
+                            // 1: normwise backward error <= tol
+  double tol;
+} bdlu_ir_opts;
+
+typedef struct bdlu_solve_stats {
+  int32_t iterations;
+  double rel_residual;      // ||b - A x||_2 / ||b||_2 (FP64)
+  double backward_error;    // ||r||_inf / (||A||_inf ||x||_inf + ||b||_inf)
+} bdlu_solve_stats;
+
+int bdlu_gpu_device_count(int* count);
+int bdlu_gpu_create(int device, uint64_t hbm_budget_bytes, bdlu_ctx** out);
+void bdlu_gpu_destroy(bdlu_ctx* ctx);
+const char* bdlu_gpu_last_error(bdlu_ctx* ctx);
+
+int bdlu_gpu_plan_upload(bdlu_ctx* ctx, const bdlu_plan_desc* plan, uint64_t* plan_id);
+int bdlu_gpu_plan_release(bdlu_ctx* ctx, uint64_t plan_id);
+int bdlu_gpu_plan_bytes(bdlu_ctx* ctx, uint64_t plan_id, uint64_t* bytes);
+
+// New values for the plan's pattern, canonical order. precision: BDLU_FP64 or
+// BDLU_FP32 (FP32 blocks that fail the pivot check are refactored in FP64).
+int bdlu_gpu_refactor(bdlu_ctx* ctx, uint64_t plan_id, const double* values, int where, int precision,
+                      bdlu_factor_stats* stats);
+// B: n x nrhs column-major (leading dimension ldb), overwritten by X.
+int bdlu_gpu_solve(bdlu_ctx* ctx, uint64_t plan_id, double* B, int32_t nrhs, int64_t ldb, int where,
+                   const bdlu_ir_opts* ir, bdlu_solve_stats* stats);
+
+// Copies the plan's current canonical-order values (last refactor) to host memory.
+int bdlu_gpu_plan_values(bdlu_ctx* ctx, uint64_t plan_id, double* host_values);
+
+// The context's HIP stream (a hipStream_t), so callers can order or time their
+// own work around engine calls.
+void* bdlu_gpu_stream(bdlu_ctx* ctx);
+
+// Per-launch timing with HIP events on the engine stream, by kernel family.
+// Enabling resets the counters. flops/bytes are algorithmic (from the plan's
+// shapes), accumulated over the timed launches.
+enum {
+  BDLU_PROF_TAIL_GEMM = 0,   // trailing MFMA update of the dense Schur tails
+  BDLU_PROF_TAIL_PANEL = 1,  // diagonal LU + both triangular solves of a tail panel
+  BDLU_PROF_DENSE_LU = 2,    // batched dense LU of the blocks <= 64
+  BDLU_PROF_SPARSE = 3,      // sparse leading columns + Schur columns
+  BDLU_PROF_SOLVE = 4,       // one block-DAG solve
+  BDLU_PROF_KINDS = 5
+};
+int bdlu_gpu_profile(bdlu_ctx* ctx, int enable);
+int bdlu_gpu_profile_read(bdlu_ctx* ctx, int kind, double* total_ms, int64_t* launches, double* flops,
+                          double* bytes);
+
+// Test hook: no-pivot dense LU of an m x m column-major matrix on the device
+// through the same kernels the tail factorization uses; A is overwritten by L\U.
+int bdlu_gpu_debug_dense_lu(bdlu_ctx* ctx, int32_t m, double* A, int precision);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,130 @@
+#pragma once
+// C-ABI of the host side (libblockdaglu_host.so): the blockdaglu::Solver
+// pipeline over plain pointers, read-only queries of the host analysis, and the
+// seeded workload generator. All functions are extern "C", never throw, and
+// report failure through return values.
+//
+// Reference interfaces replaced or mirrored:
+//   bdlu_solver_*   blockdaglu::Solver (blockdaglu/include/blockdaglu/solver.h:43-65);
+//                   the reference exposes it to the organizer tool through
+//                   CreateAlgorithm/DestroyAlgorithm (src/exports.cpp:14-24) and
+//                   the ALInterface stage mapping (src/UserAlgorithmWrapper.cpp:8-22).
+//   bdlu_host_*     build_decomposition_btf + per-block KLU symbolic analysis
+//                   (src/ubtf_block_solver.h:131-136, ubtf_block_solver.cpp:1749-1825),
+//                   as read-only queries so callers can check permutations.
+//   bdlu_gen_*      no reference counterpart: the seeded synthetic inputs the
+//                   benchmarks and tests run on.
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+// ---- Solver --------------------------------------------------------------------
+// Error codes equal the blockdaglu::SolverError enumerators (solver.h:19-31).
+enum {
+  BDLU_ERR_NONE = 0,
+  BDLU_ERR_NOT_PREPARED = 1,
+  BDLU_ERR_INVALID_DIMENSION = 2,
+  BDLU_ERR_INVALID_INDEX = 3,
+  BDLU_ERR_NON_FINITE = 4,
+  BDLU_ERR_INVALID_RHS = 5,
+  BDLU_ERR_BACKEND_UNAVAILABLE = 6,
+  BDLU_ERR_ANALYSIS_FAILED = 7,
+  BDLU_ERR_FACTORIZATION_FAILED = 8,
+  BDLU_ERR_SOLVE_FAILED = 9,
+  BDLU_ERR_EXPORT_FAILED = 10,
+};
+
+typedef struct bdlu_report {
+  int32_t nblocks, dense_blocks, sparse_blocks, max_block;
+  int32_t fp64_fallback_blocks, repivot_blocks, ir_iterations;
+  int32_t plan_reused;
+  double rel_residual, backward_error;
+} bdlu_report;
+
+void* bdlu_solver_create(void);
+void bdlu_solver_destroy(void* s);
+// Returns 1 on success, 0 on failure (then see last_error / last_error_message).
+int bdlu_solver_set_matrix(void* s, int64_t nnz, const int32_t* rows, const int32_t* cols, const double* vals,
+                           int32_t dimension /* -1: infer */);
+int bdlu_solver_analyze(void* s);
+int bdlu_solver_factor(void* s);
+int bdlu_solver_solve(void* s, double* b, int64_t len);
+int bdlu_solver_solve_multi(void* s, double* B, int64_t n, int32_t nrhs);
+int bdlu_solver_last_error(void* s);
+const char* bdlu_solver_last_error_message(void* s);
+int32_t bdlu_solver_n(void* s);
+const char* bdlu_solver_selected_mode(void* s);
+void bdlu_solver_set_precision(void* s, int precision /* 0 FP64, 1 FP32+IR */);
+void bdlu_solver_report(void* s, bdlu_report* out);
+// Device-resident variants (additive): values in canonical order and B in device
+// memory (HBM); same return convention.
+int bdlu_solver_factor_device(void* s, const double* d_values);
+int bdlu_solver_solve_device(void* s, double* d_B, int32_t nrhs, int64_t ldb);
+// Engine context (bdlu_ctx*, see bdlu_gpu.h) and its HIP stream.
+void* bdlu_solver_device_context(void* s);
+void* bdlu_solver_device_stream(void* s);
+// Structural permutations of the last analyze(); arrays of n (n+1 / nblocks+1 for R).
+int32_t bdlu_solver_nblocks(void* s);
+void bdlu_solver_copy_perm(void* s, int32_t* P, int32_t* Q, int32_t* R);
+// Exports (reference Solver::getP/getQ/getL/getU/get_rearranged_matrix). Two-call
+// protocol: pass nullptr arrays to get the entry count, then arrays of that size.
+// which: 0 rearranged, 1 P, 2 Q, 3 L, 4 U. Returns entry count or -1 on failure.
+int64_t bdlu_solver_export(void* s, int which, int32_t* rows, int32_t* cols, double* vals);
+
+// ---- host analysis queries (CPU only) ---------------------------------------
+void* bdlu_host_analyze(int64_t nnz, const int32_t* rows, const int32_t* cols, const double* vals,
+                        int32_t dimension, int* error_code);
+void bdlu_host_free(void* h);
+// n, raw/merged block counts, nmatch, nnz (canonical), off-diagonal nnz, levels.
+void bdlu_host_info(void* h, int32_t* n, int32_t* nblocks_raw, int32_t* nblocks, int32_t* nmatch, int64_t* nnz,
+                    int64_t* nnz_off, int32_t* nlevels);
+void bdlu_host_copy_perm(void* h, int32_t* P, int32_t* Q, int32_t* R);
+// Per block: 0 dense, 1 sparse; size.
+void bdlu_host_block_kinds(void* h, int32_t* kind, int32_t* size);
+// Sparse block statistics for block b (returns 0 if b is not a sparse block):
+// stats = {s, k0, m, levels, offdiag_pivots, scale}, fstats = {nnz_l, nnz_u, updates, flops}.
+int bdlu_host_sparse_stats(void* h, int32_t b, int32_t* stats6, double* fstats4);
+// Column order (s) and sparse-part pivot rows (k0) of sparse block b (local indices).
+int bdlu_host_sparse_order(void* h, int32_t b, int32_t* q, int32_t* prow);
+// Signatures (reference src/solver.cpp:90-136) of the canonical matrix.
+void bdlu_host_signatures(void* h, uint64_t* sig, uint64_t* sig2, uint64_t* value_sig);
+
+// ---- workload generator -------------------------------------------------------
+enum { BDLU_GEN_GEOMETRIC = 0, BDLU_GEN_UNIFORM = 1, BDLU_GEN_LOGUNIFORM = 2 };
+// large_fill kinds; BDLU_GEN_BERNOULLI also names Bernoulli(rate) coupling.
+enum { BDLU_GEN_POISSON4 = 0, BDLU_GEN_ZIPF = 1, BDLU_GEN_BERNOULLI = 2 };
+enum { BDLU_GEN_POISSON = 0 /* coupling: Poisson(rate) entries per row */ };
+
+typedef struct bdlu_gen_spec {
+  uint64_t seed;
+  int32_t n_small_blocks;
+  int32_t size_dist;      // BDLU_GEN_GEOMETRIC (size_param = p) | UNIFORM (size_lo..size_param) | LOGUNIFORM (exp(U(0, ln size_param)))
+  double size_param;
+  int32_t size_lo, size_hi;  // clamp
+  int64_t n_small_total;  // > 0: sizes adjusted by +-1 to sum to this
+  int32_t n_large_blocks;
+  int32_t large_size_lo, large_size_hi;
+  int32_t large_fill;     // BDLU_GEN_ZIPF | BDLU_GEN_BERNOULLI (large_fill_param) | BDLU_GEN_POISSON4
+  double large_fill_param;
+  double extra_per_row;   // Poisson mean of in-block extra entries per row
+  int32_t coupling_kind;  // BDLU_GEN_POISSON | BDLU_GEN_BERNOULLI
+  double coupling_rate;
+  double diag_factor;
+  int32_t nrhs;
+  double scale_exp;       // > 0: A <- D_r A D_c with d ~ 10^U(-scale_exp, scale_exp)
+  uint64_t scale_seed;
+} bdlu_gen_spec;
+
+void bdlu_gen_config_spec(int cfg, int pattern, bdlu_gen_spec* spec);
+void* bdlu_gen_create(const bdlu_gen_spec* spec);
+void bdlu_gen_info(const void* h, int32_t* n, int64_t* nnz, int32_t* nrhs);
+void bdlu_gen_copy(const void* h, int32_t* rows, int32_t* cols, double* vals, double* rhs);
+void bdlu_gen_free(void* h);
+void bdlu_gen_perturb(double* vals, int64_t nnz, uint64_t seed, double rel);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,61 @@
+"""Shared fixtures. The package directory is `block-daglu_amd/` (not an importable
+identifier), so it is loaded by path and registered as `blockdaglu_amd`."""
+from __future__ import annotations
+
+import importlib.util
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+
+def load_package():
+    if "blockdaglu_amd" in sys.modules:
+        return sys.modules["blockdaglu_amd"]
+    spec = importlib.util.spec_from_file_location("blockdaglu_amd", ROOT / "block-daglu_amd" / "__init__.py")
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["blockdaglu_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs on the GPU box)")
+
+
+@pytest.fixture(scope="session")
+def bd():
+    return load_package()
+
+
+@pytest.fixture(scope="session")
+def orc():
+    from oracle import oracle as o
+    return o
+
+
+@pytest.fixture(scope="session")
+def gpu(bd):
+    if not bd.gpu_available():
+        pytest.skip("no GPU visible")
+    return bd
+
+
+def rel_residual(n, rows, cols, vals, x, b):
+    """||b - A x||_2 / ||b||_2 over COO (reference apps/benchmark/main.cpp:55-80)."""
+    y = np.zeros(n)
+    np.add.at(y, rows, vals * x[cols])
+    return float(np.linalg.norm(b - y) / np.linalg.norm(b))
+
+
+def backward_error(n, rows, cols, vals, x, b):
+    y = np.zeros(n)
+    np.add.at(y, rows, vals * x[cols])
+    rowsum = np.zeros(n)
+    np.add.at(rowsum, rows, np.abs(vals))
+    return float(np.max(np.abs(b - y)) / (rowsum.max() * np.max(np.abs(x)) + np.max(np.abs(b))))
