@@ -1,0 +1,247 @@
+#!/usr/bin/env python3
+"""Refactor+solve throughput of the MI355X BLOCK engine.
+
+Workload (one GPU): the circuit-like system — n = 200,000, ~1.29M nonzeros,
+one 20,000 power-law block plus 9,000 blocks of size 1-40, coupling 0.5/row,
+hidden by random row/column permutations (seeded generator). A step is one
+refactorization of the resident pattern with new values (10 pre-generated
++-5% perturbations, cycled, already in HBM) followed by one solve of a 1-RHS
+system (RHS in HBM). Reported:
+  value            FP32 factors + FP64 refinement to ||b-Ax||/||b|| < 1e-12
+  fp64             the plain FP64 GPU path (no refinement, the reference's default)
+  latency p50/p99  per step, host clock around the synchronous calls
+  roofline         the dominant kernel (dense-tail trailing update), per-launch
+                   algorithmic bytes/flops over its average HIP-event duration
+  cpu_baseline     the CPU restatement of the reference BLOCK route (oracle/),
+                   one request = set_matrix -> analyze -> factor -> solve
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "refactor+solve/s, p50/p99 latency; FP32+IR vs FP64 GPU at rel-resid≤1e-12"
+HBM_PEAK_GBS = 8000.0
+MFMA_PEAK = {"f64": 78.6, "f32": 157.3}  # TFLOP/s dense
+
+
+def load_package():
+    import importlib.util
+    if "blockdaglu_amd" in sys.modules:
+        return sys.modules["blockdaglu_amd"]
+    spec = importlib.util.spec_from_file_location("blockdaglu_amd", ROOT / "block-daglu_amd" / "__init__.py")
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["blockdaglu_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def rel_residual(n, r, c, v, x, b):
+    y = np.zeros(n)
+    np.add.at(y, r, v * x[c])
+    return float(np.linalg.norm(b - y) / np.linalg.norm(b))
+
+
+def run_gpu(bd, torch, n, r, c, v_sets, b, precision, steps, warmup, profile):
+    """Returns (seconds per step list, final x of the last step, report, prof)."""
+    L = bd.lib()
+    s = bd.Solver()
+    s.set_precision(bd.Precision(precision))
+    assert s.set_matrix(r, c, v_sets[0], n), s.last_error_message()
+    t0 = time.time()
+    assert s.analyze(), s.last_error_message()
+    log(f"analyze (plan build + upload) {time.time() - t0:.2f}s precision={precision}")
+    assert s.factor(), s.last_error_message()
+    order = bd.canonical_order(r, c)
+    dev = torch.device("cuda:0")
+    vals_dev = torch.tensor(np.stack([vs[order] for vs in v_sets]), dtype=torch.float64, device=dev)
+    b_dev = torch.tensor(b, dtype=torch.float64, device=dev)
+    x_dev = torch.empty_like(b_dev)
+    torch.cuda.synchronize()
+    ext = torch.cuda.ExternalStream(s.device_stream())
+    ctx = C.c_void_p(s.device_context())
+
+    def step(i):
+        with torch.cuda.stream(ext):
+            x_dev.copy_(b_dev)
+        ok = s.factor_device(vals_dev[i % len(v_sets)].data_ptr())
+        assert ok, s.last_error_message()
+        ok = s.solve_device(x_dev.data_ptr(), 1, n)
+        assert ok, s.last_error_message()
+
+    for i in range(warmup):
+        t = time.perf_counter()
+        step(i)
+        log(f"  warmup step {i}: {1e3 * (time.perf_counter() - t):.2f} ms")
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    times = []
+    ev0.record(ext)
+    t_start = time.perf_counter()
+    for i in range(steps):
+        t = time.perf_counter()
+        step(i)
+        times.append(time.perf_counter() - t)
+    ev1.record(ext)
+    ev1.synchronize()
+    wall = time.perf_counter() - t_start
+    gpu_ms = ev0.elapsed_time(ev1)
+    rep = s.report()
+    log(f"  precision={precision} refine={os.environ.get('BLOCKDAGLU_REFINE_ITERS', '0')} steps={steps} "
+        f"wall={wall:.3f}s gpu={gpu_ms:.1f}ms p50={1e3 * float(np.median(times)):.2f}ms "
+        f"max={1e3 * max(times):.2f}ms ir_iters={rep['ir_iterations']} repivots={rep['repivot_blocks']} "
+        f"fp64_fallbacks={rep['fp64_fallback_blocks']} rr_engine={rep['rel_residual']:.2e}")
+    x = x_dev.cpu().numpy()
+    last_vals = v_sets[(steps - 1) % len(v_sets)]
+    prof = None
+    if profile:
+        L.gpu.bdlu_gpu_profile(ctx, 1)
+        for i in range(steps):
+            step(i)
+        prof = {}
+        for kind, name in ((bd.PROF_TAIL_GEMM, "tail_gemm"), (bd.PROF_TAIL_PANEL, "tail_panel"),
+                           (bd.PROF_DENSE_LU, "dense_lu"), (bd.PROF_SPARSE, "sparse_cols"),
+                           (bd.PROF_SOLVE, "solve")):
+            ms, cnt, fl, by = C.c_double(), C.c_int64(), C.c_double(), C.c_double()
+            L.gpu.bdlu_gpu_profile_read(ctx, kind, C.byref(ms), C.byref(cnt), C.byref(fl), C.byref(by))
+            prof[name] = {"ms": ms.value, "launches": cnt.value, "flops": fl.value, "bytes": by.value}
+        L.gpu.bdlu_gpu_profile(ctx, 0)
+    s.close()
+    return times, wall, gpu_ms, x, last_vals, rep, prof
+
+
+def cpu_baseline(bd, n, r, c, v, b, budget_s):
+    """The oracle (CPU restatement of the reference BLOCK route) on one request
+    of the same workload: set_matrix -> analyze -> factor -> solve, reference
+    thread policy (src/solver.cpp:366-390)."""
+    from oracle import oracle as orc
+    t0 = time.perf_counter()
+    o = orc.OracleSolver(r, c, v)
+    assert o.analyze() == 0
+    t1 = time.perf_counter()
+    assert o.factor() == 0
+    t2 = time.perf_counter()
+    rc, x = o.solve(b)
+    t3 = time.perf_counter()
+    assert rc == 0
+    info = o.info()
+    total = t3 - t0
+    return {"value": 1.0 / total, "unit": "refactor+solve/s", "cores": info["threads"], "kind": "port",
+            "sample": f"1 request (analyze {t1 - t0:.1f}s + factor {t2 - t1:.1f}s + solve {t3 - t2:.2f}s) "
+                      f"of the same n={n} workload; factor threads={info['threads']} (reference policy)",
+            "rel_residual": rel_residual(n, r, c, v, x, b), "seconds": total}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=1, help="workload index (1: circuit-like n=200,000)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build's path has no multi-GPU mode (see the sharding note "
+              "in the design document); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    bd = load_package()
+    if not bd.gpu_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    t = time.time()
+    n, r, c, v, B = bd.generate(args.config)
+    b = B[:, 0].copy()
+    v_sets = [v] + [bd.perturb(v, seed=1000 + i) for i in range(1, 10)]
+    log(f"workload n={n} nnz={len(r)} generated in {time.time() - t:.1f}s")
+
+    # Plain FP64 (the reference default, no refinement), FP64 + FP64 refinement to
+    # ||r||/||b|| < 1e-12 (REFINE_ITERS semantics), and FP32 factors + FP64 refinement.
+    os.environ.pop("BLOCKDAGLU_REFINE_ITERS", None)
+    res64p = run_gpu(bd, torch, n, r, c, v_sets, b, 0, args.steps, args.warmup, False)
+    os.environ["BLOCKDAGLU_REFINE_ITERS"] = "10"
+    res64 = run_gpu(bd, torch, n, r, c, v_sets, b, 0, args.steps, args.warmup, False)
+    os.environ.pop("BLOCKDAGLU_REFINE_ITERS", None)
+    res32 = run_gpu(bd, torch, n, r, c, v_sets, b, 1, args.steps, args.warmup, not args.no_profile)
+
+    def summarize(res):
+        times, wall, gpu_ms, x, last_vals, rep, prof = res
+        ts = np.array(times)
+        return {"value": args.steps / wall, "ms_per_step": 1e3 * wall / args.steps,
+                "gpu_ms_per_step": gpu_ms / args.steps,
+                "p50_ms": float(np.percentile(ts, 50) * 1e3), "p99_ms": float(np.percentile(ts, 99) * 1e3),
+                "rel_residual": rel_residual(n, r, c, last_vals, x, b),
+                "ir_iterations": rep["ir_iterations"], "fp64_fallback_blocks": rep["fp64_fallback_blocks"]}
+
+    s64p, s64, s32 = summarize(res64p), summarize(res64), summarize(res32)
+    prof = res32[6]
+    roof = None
+    if prof and prof["tail_gemm"]["launches"] > 0:
+        g = prof["tail_gemm"]
+        avg_s = g["ms"] / g["launches"] / 1e3
+        flops = g["flops"] / g["launches"]
+        byts = g["bytes"] / g["launches"]
+        tf = flops / avg_s / 1e12
+        gbs = byts / avg_s / 1e9
+        intensity = flops / byts
+        ridge = MFMA_PEAK["f32"] * 1e12 / (HBM_PEAK_GBS * 1e9)
+        if intensity < ridge:
+            roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
+        else:
+            roof = {"bound": "mfma", "achieved": tf, "peak": MFMA_PEAK["f32"], "unit": "TFLOP/s",
+                    "frac": tf / MFMA_PEAK["f32"]}
+        roof.update({"traffic": None, "kernel": "k_gemm_update<float>", "avg_launch_us": avg_s * 1e6,
+                     "launches_per_step": g["launches"] / args.steps, "algorithmic_bytes_per_launch": byts,
+                     "algorithmic_flops_per_launch": flops, "tflops": tf, "gbs": gbs})
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle) ...")
+        cpu = cpu_baseline(bd, n, r, c, v, b, 60)
+    out = {
+        "metric": METRIC,
+        "value": s32["value"],
+        "unit": "refactor+solve/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": s32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 factors + f64 refinement",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": f"circuit-like n={n} nnz={len(r)}: one 20000 power-law block + 9000 blocks "
+                               f"of size 1-40, 10 refactors at +-5%, 1 RHS", "parallelism": "single GPU"},
+        "p50_ms": s32["p50_ms"], "p99_ms": s32["p99_ms"], "rel_residual": s32["rel_residual"],
+        "ir_iterations": s32["ir_iterations"], "fp64_fallback_blocks": s32["fp64_fallback_blocks"],
+        "fp64": {"value": s64["value"], "ms_per_step": s64["ms_per_step"], "p50_ms": s64["p50_ms"],
+                 "p99_ms": s64["p99_ms"], "rel_residual": s64["rel_residual"], "ir_iterations": s64["ir_iterations"],
+                 "dtype": "f64 + f64 refinement"},
+        "fp64_plain": {"value": s64p["value"], "ms_per_step": s64p["ms_per_step"], "p50_ms": s64p["p50_ms"],
+                       "p99_ms": s64p["p99_ms"], "rel_residual": s64p["rel_residual"], "dtype": "f64"},
+        "fp32ir_over_fp64": s32["value"] / s64["value"],
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernel_profile": prof,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

