@@ -49,13 +49,23 @@ __device__ __forceinline__ int32_t uniform_from_lane0(int32_t v) {
   return __builtin_amdgcn_readfirstlane(__shfl(v, 0));
 }
 
+__device__ __forceinline__ int32_t load_relaxed(const int32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Bounded spin by the whole wavefront on one flag (every lane loads it, lane
-// 0's value decides, so the loop exit is uniform). Returns false if the flag
-// never reaches `epoch`; valid schedules never come near the bound.
+// 0's value decides, so the loop exit is uniform). The spin uses relaxed
+// coherent loads; one acquire fence follows once the flag is seen, so the
+// caches are invalidated once per dependency rather than once per poll.
+// Returns false if the flag never reaches `epoch`; valid schedules never come
+// near the bound.
 __device__ __forceinline__ bool wait_flag_uniform(const int32_t* flag, int32_t epoch) {
   for (int32_t it = 0; it < (1 << 24); ++it) {
-    if (uniform_from_lane0(load_acquire(flag)) == epoch) return true;
-    __builtin_amdgcn_s_sleep(2);
+    if (uniform_from_lane0(load_relaxed(flag)) == epoch) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      return true;
+    }
+    __builtin_amdgcn_s_sleep(4);
   }
   return false;
 }
@@ -550,6 +560,13 @@ void launch_unpermute(hipStream_t st, int32_t n, const int32_t* Q, const double*
 // order of the block DAG), so every dependency a wave waits for has already
 // been claimed by a running wave. rhs = bperm - A_off * xperm (ubtf:2269-2286),
 // then pivot swaps, unit-L forward and U backward (ubtf:2071-2097).
+// Dense-block DAG solve of one segment: a persistent grid of wavefronts claims
+// tasks in order from a ticket counter (descending block id = a topological
+// order of the block DAG), so every dependency a wave waits for has already
+// been claimed by a running wave. rhs = bperm - A_off * xperm (ubtf:2269-2286),
+// then pivot swaps, unit-L forward and U backward (ubtf:2071-2097).
+// Only the ticket claim is a lane-0 region; everything that ends an iteration
+// runs on all lanes, so the loop cannot be split across lanes.
 __global__ __launch_bounds__(64) void k_dense_seg_solve(
     int32_t ntasks, const int32_t* __restrict__ tasks, const int32_t* __restrict__ R,
     const int64_t* __restrict__ lu0_of_block, const double* __restrict__ lu64, const float* __restrict__ lu32,
@@ -558,59 +575,53 @@ __global__ __launch_bounds__(64) void k_dense_seg_solve(
     const int32_t* __restrict__ dep_blk, const int32_t* __restrict__ seg_of, int32_t seg, int32_t* __restrict__ done,
     int32_t epoch, int32_t* __restrict__ ticket, const double* __restrict__ bperm, double* __restrict__ xperm,
     int32_t* __restrict__ err) {
-  __shared__ double y[64];
   const int lane = threadIdx.x;
-  int32_t t0 = 0;
-  if (lane == 0) t0 = atomicAdd(ticket, 1);
-  const int32_t t = uniform_from_lane0(t0);
-  if (t >= ntasks) return;
-  const int32_t b = tasks[t];
-  // Wait for dependencies inside this segment (others finished in earlier launches).
-  bool ok = true;
-  for (int32_t d = dep_ptr[b]; d < dep_ptr[b + 1]; ++d) {
-    const int32_t c = dep_blk[d];
-    if (seg_of[c] == seg && !wait_flag_uniform(done + c, epoch)) ok = false;
-  }
-  if (!ok && lane == 0) atomicOr(err, 1);
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  const int32_t r0 = R[b];
-  const int s = R[b + 1] - r0;
-  double acc = 0.0;
-  if (lane < s) {
-    const int32_t r = r0 + lane;
-    acc = bperm[r];
-    for (int32_t p = off_ptr[r]; p < off_ptr[r + 1]; ++p) acc = acc - off_val[p] * xperm[off_col[p]];
-    y[lane] = acc;
-  }
-  __syncthreads();
-  if (lane == 0) {
-    for (int k = 0; k < s; ++k) {
-      const int32_t pk = piv[r0 + k];
-      if (pk != k) {
-        const double tmp = y[k];
-        y[k] = y[pk];
-        y[pk] = tmp;
-      }
+  for (;;) {
+    int32_t t0 = 0;
+    if (lane == 0) t0 = atomicAdd(ticket, 1);
+    const int32_t t = uniform_from_lane0(t0);
+    if (t >= ntasks) break;
+    const int32_t b = tasks[t];
+    // Wait for dependencies inside this segment (others finished in earlier launches).
+    int32_t bad = 0;
+    for (int32_t d = dep_ptr[b]; d < dep_ptr[b + 1]; ++d) {
+      const int32_t c = dep_blk[d];
+      if (seg_of[c] == seg && !wait_flag_uniform(done + c, epoch)) bad = 1;
     }
+    if (bad) atomicOr(err, 1);  // every lane ORs the same bit
+    const int32_t r0 = R[b];
+    const int s = R[b + 1] - r0;
+    double acc = 0.0;
+    if (lane < s) {
+      const int32_t r = r0 + lane;
+      acc = bperm[r];
+      for (int32_t p = off_ptr[r]; p < off_ptr[r + 1]; ++p) acc = acc - off_val[p] * xperm[off_col[p]];
+    }
+    // Row swaps (piv[k] <-> k for k = 0..s-1, in order) as one gather: lane i
+    // follows the swaps backwards to find which row lands at position i.
+    int32_t src = lane;
+    for (int k = s - 1; k >= 0; --k) {
+      const int32_t pk = piv[r0 + k];
+      src = (src == k) ? pk : ((src == pk) ? k : src);
+    }
+    acc = __shfl(acc, src < 64 ? src : 0);
+    if (lane >= s) acc = 0.0;
+    const bool single = prec_of_block[b] != 0;
+    const double* f64 = lu64 + lu0_of_block[b];
+    const float* f32 = lu32 + lu0_of_block[b];
+    auto f = [&](int idx) -> double { return single ? static_cast<double>(f32[idx]) : f64[idx]; };
+    for (int j = 0; j < s; ++j) {
+      const double xj = __shfl(acc, j);
+      if (lane > j && lane < s) acc = acc - f(j * s + lane) * xj;
+    }
+    for (int j = s - 1; j >= 0; --j) {
+      if (lane == j) acc = acc / f(j * s + j);
+      const double xj = __shfl(acc, j);
+      if (lane < j) acc = acc - f(j * s + lane) * xj;
+    }
+    if (lane < s) xperm[r0 + lane] = acc;
+    store_release(done + b, epoch);  // every lane stores the same value
   }
-  __syncthreads();
-  acc = lane < s ? y[lane] : 0.0;
-  const bool single = prec_of_block[b] != 0;
-  const double* f64 = lu64 + lu0_of_block[b];
-  const float* f32 = lu32 + lu0_of_block[b];
-  auto f = [&](int idx) -> double { return single ? static_cast<double>(f32[idx]) : f64[idx]; };
-  for (int j = 0; j < s; ++j) {
-    const double xj = __shfl(acc, j);
-    if (lane > j && lane < s) acc = acc - f(j * s + lane) * xj;
-  }
-  for (int j = s - 1; j >= 0; --j) {
-    if (lane == j) acc = acc / f(j * s + j);
-    const double xj = __shfl(acc, j);
-    if (lane < j) acc = acc - f(j * s + lane) * xj;
-  }
-  if (lane < s) xperm[r0 + lane] = acc;
-  __syncthreads();
-  if (lane == 0) store_release(done + b, epoch);
 }
 
 void launch_dense_seg_solve(hipStream_t st, int32_t ntasks, const int32_t* tasks, const int32_t* R,
@@ -620,7 +631,8 @@ void launch_dense_seg_solve(hipStream_t st, int32_t ntasks, const int32_t* tasks
                             const int32_t* dep_blk, const int32_t* seg_of, int32_t seg, int32_t* done, int32_t epoch,
                             int32_t* ticket, const double* bperm, double* xperm, int32_t* err) {
   if (ntasks <= 0) return;
-  hipLaunchKernelGGL(k_dense_seg_solve, dim3(ntasks), dim3(64), 0, st, ntasks, tasks, R, lu0_of_block, lu64, lu32,
+  const int g = ntasks < 256 ? ntasks : 256;
+  hipLaunchKernelGGL(k_dense_seg_solve, dim3(g), dim3(64), 0, st, ntasks, tasks, R, lu0_of_block, lu64, lu32,
                      prec_of_block, piv, off_ptr, off_col, off_val, dep_ptr, dep_blk, seg_of, seg, done, epoch,
                      ticket, bperm, xperm, err);
 }
