@@ -75,6 +75,8 @@ struct SparseDev {
   float *Lx32 = nullptr, *Ux32 = nullptr, *Ud32 = nullptr, *S32 = nullptr;
   double* rs = nullptr;
   double* y = nullptr;
+  double* inv64 = nullptr;  // 2*64*64 diagonal-block inverses (tail LU scratch)
+  float* inv32 = nullptr;
   int32_t *flags_f = nullptr, *flags_b = nullptr;
   int32_t* status = nullptr;  // device int
   int prec = 0;
@@ -385,6 +387,7 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
     v.tpos = A.copy(tpos, st);
     v.rs_ptr = A.copy(rs_ptr, st);
     v.rs_idx = A.copy(rs_idx, st);
+    v.rpos = A.copy(pos, st);
     S.lev_ptr.assign(sd.lev_ptr, sd.lev_ptr + sd.nlev + 1);
     S.lev_col = A.copy(sd.lev_col, sd.k0, st);
     S.rowperm = A.copy(rowperm, st);
@@ -412,6 +415,8 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
     S.S32 = A.get<float>(static_cast<std::size_t>(sd.m) * sd.m);
     S.rs = A.get<double>(sd.s);
     S.y = A.get<double>(sd.s);
+    S.inv64 = A.get<double>(2 * 64 * 64);
+    S.inv32 = A.get<float>(2 * 64 * 64);
     const int32_t np = (sd.m + 63) / 64;
     S.flags_f = A.get<int32_t>(np);
     S.flags_b = A.get<int32_t>(np);
@@ -524,6 +529,8 @@ void factor_sparse(Plan& P, SparseDev& S, hipStream_t st, T* Lx, T* Ux, T* Ud, T
   for (std::size_t l = 0; l + 1 < S.lev_ptr.size(); ++l)
     bdk::launch_sp_cols<T>(st, view, P.arena, S.rs, S.lev_ptr[l + 1] - S.lev_ptr[l], S.lev_col + S.lev_ptr[l], xs,
                            P.nslots, Lx, Ux, Ud, S.status);
+  if (sizeof(T) == 8) bdk::launch_fill_f64(st, reinterpret_cast<double*>(Sm), int64_t(S.m) * S.m, 0.0);
+  else bdk::launch_fill_f32(st, reinterpret_cast<float*>(Sm), int64_t(S.m) * S.m, 0.0f);
   bdk::launch_sp_tail<T>(st, view, P.arena, S.rs, xs, P.nslots, Lx, Ux, Sm, S.m);
   if (pe) pe->mark(BDLU_PROF_SPARSE, st);
   bdk::TailTimer tt;
@@ -533,7 +540,8 @@ void factor_sparse(Plan& P, SparseDev& S, hipStream_t st, T* Lx, T* Ux, T* Ud, T
     tt.gemm_bytes = &pe->gemm_bytes;
     tt.panel_events = &pe->ev[BDLU_PROF_TAIL_PANEL];
   }
-  bdk::launch_tail_lu<T>(st, S.m, Sm, S.m, growth_limit, S.status, pe ? &tt : nullptr);
+  T* inv = sizeof(T) == 8 ? reinterpret_cast<T*>(S.inv64) : reinterpret_cast<T*>(S.inv32);
+  bdk::launch_tail_lu<T>(st, S.m, Sm, S.m, growth_limit, S.status, inv, pe ? &tt : nullptr);
 }
 
 }  // namespace
@@ -852,13 +860,15 @@ int bdlu_gpu_debug_dense_lu(bdlu_ctx* c, int32_t m, double* A, int precision) {
       std::vector<float> h(mm);
       for (std::size_t i = 0; i < mm; ++i) h[i] = static_cast<float>(A[i]);
       float* d = tmp.copy(h, st);
-      bdk::launch_tail_lu<float>(st, m, d, m, 1e300, status);
+      float* inv = tmp.get<float>(2 * 64 * 64);
+      bdk::launch_tail_lu<float>(st, m, d, m, 1e300, status, inv);
       HIPCHECK(hipMemcpyAsync(h.data(), d, sizeof(float) * mm, hipMemcpyDeviceToHost, st));
       HIPCHECK(hipStreamSynchronize(st));
       for (std::size_t i = 0; i < mm; ++i) A[i] = h[i];
     } else {
       double* d = tmp.copy(A, mm, st);
-      bdk::launch_tail_lu<double>(st, m, d, m, 1e300, status);
+      double* inv = tmp.get<double>(2 * 64 * 64);
+      bdk::launch_tail_lu<double>(st, m, d, m, 1e300, status, inv);
       HIPCHECK(hipMemcpyAsync(A, d, sizeof(double) * mm, hipMemcpyDeviceToHost, st));
       HIPCHECK(hipStreamSynchronize(st));
     }

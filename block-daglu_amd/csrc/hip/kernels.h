@@ -24,6 +24,7 @@ struct SpView {
   const int32_t* tpos;     // m: tail-local row -> tail position
   const int32_t* rs_ptr;   // s+1: block entries by local row (for the row scaling)
   const int32_t* rs_idx;
+  const int32_t* rpos;     // s: local row -> pivot position
 };
 
 // Status bits written by the factor kernels into a per-block int.
@@ -32,6 +33,7 @@ enum : int32_t { kStZeroPivot = 1, kStCheck = 2, kStNonFinite = 4 };
 void launch_scatter(hipStream_t st, int64_t nnz, const double* vals, const int64_t* vmap, double* arena);
 void launch_fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v);
 void launch_fill_f64(hipStream_t st, double* p, int64_t n, double v);
+void launch_fill_f32(hipStream_t st, float* p, int64_t n, float v);
 
 template <typename T>
 void launch_dense_lu(hipStream_t st, int32_t nd, const int32_t* sizes, const int32_t* starts, const int64_t* val0,
@@ -54,9 +56,10 @@ struct TailTimer {
 };
 
 // No-pivot blocked LU of the m x m column-major S (the tail after its rows were
-// placed in the fixed pivot order). Check: |L| <= inv_tol and nonzero pivots.
+// placed in the fixed pivot order). Check: |L| <= limit and nonzero pivots.
+// inv: 2*64*64 scratch for the diagonal blocks' inverses.
 template <typename T>
-void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double inv_tol, int32_t* status,
+void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, int32_t* status, T* inv,
                     TailTimer* timer = nullptr);
 
 // ---- solve ----

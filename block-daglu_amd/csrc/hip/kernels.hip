@@ -103,6 +103,9 @@ void launch_fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v) {
 void launch_fill_f64(hipStream_t st, double* p, int64_t n, double v) {
   if (n > 0) hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, p, n, v);
 }
+void launch_fill_f32(hipStream_t st, float* p, int64_t n, float v) {
+  if (n > 0) hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, p, n, v);
+}
 
 // ---------------------------------------------------------------------------
 // K1: batched dense LU with partial pivoting, one wavefront per block (s <= 64).
@@ -283,8 +286,11 @@ template void launch_sp_cols<double>(hipStream_t, const SpView&, const double*, 
 template void launch_sp_cols<float>(hipStream_t, const SpView&, const double*, const double*, int32_t,
                                     const int32_t*, float*, int32_t, float*, float*, float*, int32_t*);
 
-// Tail columns: U(0:k0, j) and the Schur column, written straight into the
-// tail's fixed pivot-row order (column-major, leading dimension lds).
+// Tail columns: U(0:k0, j) and the Schur column S(:, t) = A(tail, j) -
+// L(tail, 0:k0) U(0:k0, j). S arrives zero-filled, in the tail's fixed
+// pivot-row order (column-major, leading dimension lds); rows pivotal in the
+// sparse part live in the wave's scratch x, tail rows are updated in place in S
+// (rpos maps a local row to its pivot position).
 template <typename T>
 __global__ __launch_bounds__(64) void k_sp_tail(SpView v, const double* __restrict__ arena,
                                                 const double* __restrict__ rs, T* __restrict__ xs,
@@ -294,10 +300,14 @@ __global__ __launch_bounds__(64) void k_sp_tail(SpView v, const double* __restri
   T* x = xs + int64_t(blockIdx.x) * v.s;
   for (int32_t t = blockIdx.x; t < v.m; t += gridDim.x) {
     const int32_t j = v.k0 + t;
+    T* col = S + int64_t(t) * lds;
     for (int32_t p = v.a_ptr[j] + lane; p < v.a_ptr[j + 1]; p += kWave) {
       const int32_t r = v.a_row[p];
       const double a = arena[v.a_val0 + p];
-      x[r] = static_cast<T>(v.scale ? a / rs[r] : a);
+      const T val = static_cast<T>(v.scale ? a / rs[r] : a);
+      const int32_t q = v.rpos[r];
+      if (q < v.k0) x[r] = val;
+      else col[q - v.k0] = val;
     }
     wave_sync_mem();
     for (int32_t up = v.u_ptr[j]; up < v.u_ptr[j + 1]; ++up) {
@@ -305,15 +315,12 @@ __global__ __launch_bounds__(64) void k_sp_tail(SpView v, const double* __restri
       const T xk = x[v.prow[k]];
       for (int32_t p = v.l_ptr[k] + lane; p < v.l_ptr[k + 1]; p += kWave) {
         const int32_t r = v.l_row[p];
-        x[r] = x[r] - Lx[p] * xk;
+        const T d = Lx[p] * xk;
+        const int32_t q = v.rpos[r];
+        if (q < v.k0) x[r] = x[r] - d;
+        else col[q - v.k0] = col[q - v.k0] - d;
       }
       wave_sync_mem();
-    }
-    T* col = S + int64_t(t) * lds;
-    for (int32_t r = lane; r < v.m; r += kWave) {
-      const int32_t row = v.trow[r];
-      col[v.tpos[r]] = x[row];
-      x[row] = T(0);
     }
     for (int32_t up = v.u_ptr[j] + lane; up < v.u_ptr[j + 1]; up += kWave) {
       const int32_t r = v.prow[v.u_idx[up]];
@@ -336,96 +343,89 @@ template void launch_sp_tail<float>(hipStream_t, const SpView&, const double*, c
                                     const float*, float*, float*, int64_t);
 
 // ---------------------------------------------------------------------------
-// Dense tail: no-pivot blocked right-looking LU, panel width 64.
+// Dense tail: no-pivot blocked right-looking LU, panel width 64. Per panel:
+//   1. k_diag_lu_inv: LU of the 64x64 diagonal block in registers (a lane per
+//      column), then L11^{-1} and U11^{-1};
+//   2. k_panel_gemm: L21 = A21 U11^{-1} and U12 = L11^{-1} A12 as MFMA tile
+//      products (one launch, both jobs), with the |L| growth check;
+//   3. k_panel_gemm: trailing update A22 -= L21 U12 on MFMA.
 constexpr int kNb = 64;
 
-// Diagonal 64x64 block, one wavefront, in LDS.
+__device__ __forceinline__ double readlane_t(double v, int src) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(u)), src);
+  const uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(u >> 32)), src);
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+__device__ __forceinline__ float readlane_t(float v, int src) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), src));
+}
+
+// inv[0..4095]: L11^{-1} row-major (unit lower); inv[4096..8191]: U11^{-1}
+// row-major (upper). Rows/columns at or past b are an identity pad.
 template <typename T>
-__global__ __launch_bounds__(64) void k_diag_lu(int32_t m, T* __restrict__ S, int64_t lds, int32_t p, int32_t b,
-                                                double inv_tol, int32_t* __restrict__ status) {
-  __shared__ T a[kNb * kLdsPad];
+__global__ __launch_bounds__(64) void k_diag_lu_inv(T* __restrict__ S, int64_t lds, int32_t p, int32_t b,
+                                                    double limit, int32_t* __restrict__ status, T* __restrict__ inv) {
+  __shared__ T f[kNb * kNb];   // f[i*64 + j] = LU(i, j)
+  __shared__ T xs[kNb * kNb];  // xs[i*64 + lane]: the lane's inverse column
   const int lane = threadIdx.x;
-  for (int j = 0; j < b; ++j)
-    if (lane < b) a[j * kLdsPad + lane] = S[int64_t(p + j) * lds + p + lane];
-  __syncthreads();
+  T col[kNb];
+#pragma unroll
+  for (int i = 0; i < kNb; ++i)
+    col[i] = (lane < b && i < b) ? S[int64_t(p + lane) * lds + p + i] : (i == lane ? T(1) : T(0));
   int32_t st = 0;
   T lmax = T(0);
-  for (int k = 0; k < b; ++k) {
-    const T akk = a[k * kLdsPad + k];
+#pragma unroll
+  for (int k = 0; k < kNb; ++k) {
+    const T akk = readlane_t(col[k], k);
     if (akk == T(0)) st |= kStZeroPivot;
-    if (lane > k && lane < b) {
-      const T l = a[k * kLdsPad + lane] / akk;
-      a[k * kLdsPad + lane] = l;
-      const T al = tabs(l);
-      if (!(al <= lmax)) lmax = al;
-      for (int j = k + 1; j < b; ++j) a[j * kLdsPad + lane] = a[j * kLdsPad + lane] - l * a[j * kLdsPad + k];
+    if (lane == k) {
+#pragma unroll
+      for (int i = k + 1; i < kNb; ++i) {
+        col[i] = col[i] / akk;
+        const T al = tabs(col[i]);
+        if (!(al <= lmax)) lmax = al;
+      }
     }
-    __syncthreads();
+#pragma unroll
+    for (int i = k + 1; i < kNb; ++i) {
+      const T li = readlane_t(col[i], k);
+      if (lane > k) col[i] = col[i] - li * col[k];
+    }
   }
-  for (int j = 0; j < b; ++j)
-    if (lane < b) S[int64_t(p + j) * lds + p + lane] = a[j * kLdsPad + lane];
-  if (!(lmax <= static_cast<T>(inv_tol))) st |= kStCheck;
-  for (int off = 32; off > 0; off >>= 1) st |= __shfl_xor(st, off);
-  if (lane == 0 && st) atomicOr(status, st);
-}
-
-// L21 = A21 * U11^{-1}: one wavefront per 64 rows, a lane per row; U11 and
-// the lane's finished row entries live in LDS (column j of the row solve reads
-// only the same lane's earlier entries, so no barrier is needed inside).
-template <typename T>
-__global__ __launch_bounds__(64) void k_trsm_l(int32_t m, T* __restrict__ S, int64_t lds, int32_t p, int32_t b,
-                                               double inv_tol, int32_t* __restrict__ status) {
-  __shared__ T u[kNb * kNb];   // u[j*64 + k] = U(k, j)
-  __shared__ T xr[kNb * kNb];  // xr[k*64 + lane] = L(row, k)
-  const int lane = threadIdx.x;
-  for (int idx = lane; idx < kNb * kNb; idx += kWave) {
-    const int j = idx / kNb, k = idx - j * kNb;
-    T val = (j == k) ? T(1) : T(0);
-    if (j < b && k < b && k <= j) val = S[int64_t(p + j) * lds + p + k];
-    u[idx] = val;
+#pragma unroll
+  for (int i = 0; i < kNb; ++i) {
+    f[i * kNb + lane] = col[i];
+    if (lane < b && i < b) S[int64_t(p + lane) * lds + p + i] = col[i];
   }
   __syncthreads();
-  const int32_t row = p + b + blockIdx.x * kWave + lane;
-  const bool live = row < m;
-  T lmax = T(0);
-  for (int j = 0; j < b; ++j) {
-    T acc = live ? S[int64_t(p + j) * lds + row] : T(0);
-    for (int k = 0; k < j; ++k) acc = acc - xr[k * kNb + lane] * u[j * kNb + k];
-    const T x = acc / u[j * kNb + j];
-    xr[j * kNb + lane] = x;
-    if (live) S[int64_t(p + j) * lds + row] = x;
-    const T ax = tabs(x);
-    if (live && !(ax <= lmax)) lmax = ax;
+  // L11^{-1}: lane j forms column j (x_i = delta_ij - sum_{k<i} L_ik x_k).
+  for (int i = 0; i < kNb; ++i) {
+    T acc = (i == lane) ? T(1) : T(0);
+    for (int k = 0; k < i; ++k) acc = acc - f[i * kNb + k] * xs[k * kNb + lane];
+    xs[i * kNb + lane] = acc;
+    inv[i * kNb + lane] = acc;
   }
-  int32_t st = !(lmax <= static_cast<T>(inv_tol)) ? kStCheck : 0;
+  __syncthreads();
+  // U11^{-1}: lane j forms column j (x_i = (delta_ij - sum_{k>i} U_ik x_k) / U_ii).
+  for (int i = kNb - 1; i >= 0; --i) {
+    T acc = (i == lane) ? T(1) : T(0);
+    for (int k = i + 1; k < kNb; ++k) acc = acc - f[i * kNb + k] * xs[k * kNb + lane];
+    acc = acc / f[i * kNb + i];
+    xs[i * kNb + lane] = acc;
+    inv[kNb * kNb + i * kNb + lane] = acc;
+  }
+  if (!(lmax <= static_cast<T>(limit))) st |= kStCheck;
   for (int off = 32; off > 0; off >>= 1) st |= __shfl_xor(st, off);
   if (lane == 0 && st) atomicOr(status, st);
 }
 
-// U12 = L11^{-1} * A12: one wavefront per column, lanes over rows.
-template <typename T>
-__global__ __launch_bounds__(64) void k_trsm_u(int32_t m, T* __restrict__ S, int64_t lds, int32_t p, int32_t b) {
-  const int lane = threadIdx.x;
-  const int32_t col = p + b + blockIdx.x;
-  if (col >= m) return;
-  T* c = S + int64_t(col) * lds + p;
-  T acc = (lane < b) ? c[lane] : T(0);
-  for (int k = 0; k < b; ++k) {
-    const T xk = __shfl(acc, k);
-    if (lane > k && lane < b) acc = acc - S[int64_t(p + k) * lds + p + lane] * xk;
-  }
-  if (lane < b) c[lane] = acc;
-}
-
-// Trailing update C -= A * B on MFMA. 64x64 C tile per 256-thread workgroup,
-// each wavefront a 32x32 quarter as 2x2 MFMA 16x16 tiles; A/B panel tiles
-// staged in LDS; k padded to a multiple of 4 with zeros.
-template <typename T>
-struct MfmaOps;
 // Inputs of both forms: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16].
 // Output element r of lane l is D[row(l / 16, r)][l % 16]; the f64 form
 // interleaves its rows (measured on gfx950: row = 4 r + l/16), the f32 form
 // keeps them in blocks of four (row = 4 (l/16) + r).
+template <typename T>
+struct MfmaOps;
 template <>
 struct MfmaOps<double> {
   typedef double v4 __attribute__((ext_vector_type(4)));
@@ -439,29 +439,57 @@ struct MfmaOps<float> {
   __device__ static int row(int lk, int r) { return 4 * lk + r; }
 };
 
+// One tile product job: C(MxN) op= A(MxK) B(KxN), K <= 64, element (i,j) of X
+// at X[i*x_r + j*x_c]. op: 0 C -= AB, 1 C = AB, 2 C = AB and check |C| <= limit.
 template <typename T>
-__global__ __launch_bounds__(256) void k_gemm_update(int32_t m, T* __restrict__ S, int64_t lds, int32_t p,
-                                                     int32_t b) {
+struct TileJob {
+  int32_t M, N, K, op, tiles_m, tiles_n;
+  const T* A;
+  int64_t a_r, a_c;
+  const T* B;
+  int64_t b_r, b_c;
+  T* C;
+  int64_t c_r, c_c;
+};
+
+// 64x64 output tile per 256-thread workgroup (each wavefront a 32x32 quarter,
+// 2x2 MFMA 16x16 tiles); A and B tiles staged whole in LDS before any store,
+// so C may alias A (tile covers all K columns of its rows) or B (all K rows of
+// its columns). Workgroups below j0.tiles_m*j0.tiles_n run job 0, the rest job 1.
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_gemm(TileJob<T> j0, TileJob<T> j1, double limit,
+                                                    int32_t* __restrict__ status) {
   using V = typename MfmaOps<T>::v4;
-  // Leading dimension 64 keeps both tiles within 64 KB of LDS for FP64.
-  constexpr int LD = kNb;
-  __shared__ T As[kNb * LD];  // As[k*LD + i] = A(i, k)
-  __shared__ T Bs[kNb * LD];  // Bs[k*LD + j] = B(k, j)
-  const int32_t base = p + b;
-  const int32_t M = m - base;
-  const int32_t i0 = blockIdx.x * kNb, j0 = blockIdx.y * kNb;
+  __shared__ T As[kNb * kNb];  // As[k*64 + i] = A(i0 + i, k)
+  __shared__ T Bs[kNb * kNb];  // Bs[k*64 + j] = B(k, j0 + j)
+  const int32_t n0 = j0.tiles_m * j0.tiles_n;
+  const bool second = static_cast<int32_t>(blockIdx.x) >= n0;
+  const TileJob<T>& J = second ? j1 : j0;
+  const int32_t tid_blk = second ? static_cast<int32_t>(blockIdx.x) - n0 : static_cast<int32_t>(blockIdx.x);
+  const int32_t i0 = (tid_blk % J.tiles_m) * kNb, c0 = (tid_blk / J.tiles_m) * kNb;
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < kNb * kNb; idx += 256) {
-    const int k = idx / kNb, i = idx - k * kNb;
-    T av = T(0);
-    if (k < b && i0 + i < M) av = S[int64_t(p + k) * lds + base + i0 + i];
-    As[k * LD + i] = av;
+  // Loads: the unit-stride index runs across threads.
+  if (J.a_r == 1) {
+    for (int idx = tid; idx < kNb * kNb; idx += 256) {
+      const int k = idx / kNb, i = idx - k * kNb;
+      As[k * kNb + i] = (k < J.K && i0 + i < J.M) ? J.A[(i0 + i) + int64_t(k) * J.a_c] : T(0);
+    }
+  } else {
+    for (int idx = tid; idx < kNb * kNb; idx += 256) {
+      const int i = idx / kNb, k = idx - i * kNb;
+      As[k * kNb + i] = (k < J.K && i0 + i < J.M) ? J.A[int64_t(i0 + i) * J.a_r + k * J.a_c] : T(0);
+    }
   }
-  for (int idx = tid; idx < kNb * kNb; idx += 256) {
-    const int j = idx / kNb, k = idx - j * kNb;
-    T bv = T(0);
-    if (k < b && j0 + j < M) bv = S[int64_t(base + j0 + j) * lds + p + k];
-    Bs[k * LD + j] = bv;
+  if (J.b_r == 1) {
+    for (int idx = tid; idx < kNb * kNb; idx += 256) {
+      const int j = idx / kNb, k = idx - j * kNb;
+      Bs[k * kNb + j] = (k < J.K && c0 + j < J.N) ? J.B[k + int64_t(c0 + j) * J.b_c] : T(0);
+    }
+  } else {
+    for (int idx = tid; idx < kNb * kNb; idx += 256) {
+      const int k = idx / kNb, j = idx - k * kNb;
+      Bs[k * kNb + j] = (k < J.K && c0 + j < J.N) ? J.B[int64_t(k) * J.b_r + (c0 + j) * J.b_c] : T(0);
+    }
   }
   __syncthreads();
   const int wave = tid / kWave, lane = tid - wave * kWave;
@@ -472,18 +500,19 @@ __global__ __launch_bounds__(256) void k_gemm_update(int32_t m, T* __restrict__ 
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int c = 0; c < 2; ++c) acc[a][c] = V{T(0), T(0), T(0), T(0)};
-  const int kend = (b + 3) & ~3;
+  const int kend = (J.K + 3) & ~3;
   for (int kk = 0; kk < kend; kk += 4) {
     T av[2], bv[2];
 #pragma unroll
-    for (int a = 0; a < 2; ++a) av[a] = As[(kk + lk) * LD + wr + a * 16 + li];
+    for (int a = 0; a < 2; ++a) av[a] = As[(kk + lk) * kNb + wr + a * 16 + li];
 #pragma unroll
-    for (int c = 0; c < 2; ++c) bv[c] = Bs[(kk + lk) * LD + wc + c * 16 + li];
+    for (int c = 0; c < 2; ++c) bv[c] = Bs[(kk + lk) * kNb + wc + c * 16 + li];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int c = 0; c < 2; ++c) acc[a][c] = MfmaOps<T>::mma(av[a], bv[c], acc[a][c]);
   }
+  T lmax = T(0);
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -491,12 +520,24 @@ __global__ __launch_bounds__(256) void k_gemm_update(int32_t m, T* __restrict__ 
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int32_t i = i0 + wr + a * 16 + MfmaOps<T>::row(lk, r);
-        const int32_t j = j0 + wc + c * 16 + li;
-        if (i < M && j < M) {
-          T* cp = S + int64_t(base + j) * lds + base + i;
-          *cp = *cp - acc[a][c][r];
+        const int32_t j = c0 + wc + c * 16 + li;
+        if (i < J.M && j < J.N) {
+          T* cp = J.C + int64_t(i) * J.c_r + int64_t(j) * J.c_c;
+          const T v = acc[a][c][r];
+          if (J.op == 0) {
+            *cp = *cp - v;
+          } else {
+            *cp = v;
+            const T av2 = tabs(v);
+            if (!(av2 <= lmax)) lmax = av2;
+          }
         }
       }
+  if (J.op == 2) {
+    int32_t st = !(lmax <= static_cast<T>(limit)) ? kStCheck : 0;
+    for (int off = 32; off > 0; off >>= 1) st |= __shfl_xor(st, off);
+    if (lane == 0 && st) atomicOr(status, st);
+  }
 }
 
 namespace {
@@ -510,23 +551,28 @@ hipEvent_t timer_event(std::vector<hipEvent_t>* pool, hipStream_t st) {
 }  // namespace
 
 template <typename T>
-void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double inv_tol, int32_t* status,
+void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, int32_t* status, T* inv,
                     TailTimer* timer) {
   for (int32_t p = 0; p < m; p += kNb) {
     const int32_t b = (m - p) < kNb ? (m - p) : kNb;
     const int32_t rest = m - p - b;
     if (timer && timer->panel_events) timer_event(timer->panel_events, st);
-    hipLaunchKernelGGL(k_diag_lu<T>, dim3(1), dim3(64), 0, st, m, S, lds, p, b, inv_tol, status);
+    hipLaunchKernelGGL(k_diag_lu_inv<T>, dim3(1), dim3(64), 0, st, S, lds, p, b, limit, status, inv);
     if (rest > 0) {
-      hipLaunchKernelGGL(k_trsm_l<T>, dim3(grid_for(rest, kWave)), dim3(kWave), 0, st, m, S, lds, p, b, inv_tol,
-                         status);
-      hipLaunchKernelGGL(k_trsm_u<T>, dim3(rest), dim3(64), 0, st, m, S, lds, p, b);
+      const int32_t tr = (rest + kNb - 1) / kNb;
+      TileJob<T> l21{rest, b, b, 2, tr, 1, S + int64_t(p) * lds + p + b, 1, lds, inv + kNb * kNb, kNb, 1,
+                     S + int64_t(p) * lds + p + b, 1, lds};
+      TileJob<T> u12{b, rest, b, 1, 1, tr, inv, kNb, 1, S + int64_t(p + b) * lds + p, 1, lds,
+                     S + int64_t(p + b) * lds + p, 1, lds};
+      hipLaunchKernelGGL(k_panel_gemm<T>, dim3(2 * tr), dim3(256), 0, st, l21, u12, limit, status);
     }
     if (timer && timer->panel_events) timer_event(timer->panel_events, st);
     if (rest <= 0) break;
-    const int tiles = (rest + kNb - 1) / kNb;
+    const int32_t tr = (rest + kNb - 1) / kNb;
+    TileJob<T> upd{rest, rest, b, 0, tr, tr, S + int64_t(p) * lds + p + b, 1, lds,
+                   S + int64_t(p + b) * lds + p, 1, lds, S + int64_t(p + b) * lds + p + b, 1, lds};
     if (timer && timer->gemm_events) timer_event(timer->gemm_events, st);
-    hipLaunchKernelGGL(k_gemm_update<T>, dim3(tiles, tiles), dim3(256), 0, st, m, S, lds, p, b);
+    hipLaunchKernelGGL(k_panel_gemm<T>, dim3(tr * tr), dim3(256), 0, st, upd, upd, limit, status);
     if (timer && timer->gemm_events) {
       timer_event(timer->gemm_events, st);
       const double r = rest;
@@ -535,8 +581,8 @@ void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double inv_tol
     }
   }
 }
-template void launch_tail_lu<double>(hipStream_t, int32_t, double*, int64_t, double, int32_t*, TailTimer*);
-template void launch_tail_lu<float>(hipStream_t, int32_t, float*, int64_t, double, int32_t*, TailTimer*);
+template void launch_tail_lu<double>(hipStream_t, int32_t, double*, int64_t, double, int32_t*, double*, TailTimer*);
+template void launch_tail_lu<float>(hipStream_t, int32_t, float*, int64_t, double, int32_t*, float*, TailTimer*);
 
 // ---------------------------------------------------------------------------
 // Solve.
@@ -575,6 +621,7 @@ __global__ __launch_bounds__(64) void k_dense_seg_solve(
     const int32_t* __restrict__ dep_blk, const int32_t* __restrict__ seg_of, int32_t seg, int32_t* __restrict__ done,
     int32_t epoch, int32_t* __restrict__ ticket, const double* __restrict__ bperm, double* __restrict__ xperm,
     int32_t* __restrict__ err) {
+  __shared__ double fs[64 * 64];
   const int lane = threadIdx.x;
   for (;;) {
     int32_t t0 = 0;
@@ -606,19 +653,23 @@ __global__ __launch_bounds__(64) void k_dense_seg_solve(
     }
     acc = __shfl(acc, src < 64 ? src : 0);
     if (lane >= s) acc = 0.0;
+    // The block's factors go to LDS first (independent coalesced loads), so the
+    // substitution chain below never waits on global memory.
     const bool single = prec_of_block[b] != 0;
     const double* f64 = lu64 + lu0_of_block[b];
     const float* f32 = lu32 + lu0_of_block[b];
-    auto f = [&](int idx) -> double { return single ? static_cast<double>(f32[idx]) : f64[idx]; };
+    for (int idx = lane; idx < s * s; idx += kWave) fs[idx] = single ? static_cast<double>(f32[idx]) : f64[idx];
+    __syncthreads();
     for (int j = 0; j < s; ++j) {
       const double xj = __shfl(acc, j);
-      if (lane > j && lane < s) acc = acc - f(j * s + lane) * xj;
+      if (lane > j && lane < s) acc = acc - fs[j * s + lane] * xj;
     }
     for (int j = s - 1; j >= 0; --j) {
-      if (lane == j) acc = acc / f(j * s + j);
+      if (lane == j) acc = acc / fs[j * s + j];
       const double xj = __shfl(acc, j);
-      if (lane < j) acc = acc - f(j * s + lane) * xj;
+      if (lane < j) acc = acc - fs[j * s + lane] * xj;
     }
+    __syncthreads();
     if (lane < s) xperm[r0 + lane] = acc;
     store_release(done + b, epoch);  // every lane stores the same value
   }
@@ -694,6 +745,9 @@ template void launch_sp_rows<float>(hipStream_t, int32_t, const int32_t*, const 
 // Dense tail triangular solve, sync-free over 64-row panels: a wavefront per
 // panel, panels claimed in dependency order from a ticket, each waits for the
 // flags of the panels it reads. Lower: unit diagonal, forward. Upper: backward.
+// Each 64x64 tile of S is loaded into registers (independent loads) before the
+// wave waits or runs the sequential part, so no step of the substitution
+// chain waits on memory.
 template <typename T>
 __global__ __launch_bounds__(64) void k_tail_trsv(int32_t m, const T* __restrict__ S, int64_t lds,
                                                   double* __restrict__ y, int upper, int32_t* __restrict__ flags,
@@ -709,8 +763,10 @@ __global__ __launch_bounds__(64) void k_tail_trsv(int32_t m, const T* __restrict
   const int32_t r0 = w * kNb;
   const int32_t rows = (m - r0) < kNb ? (m - r0) : kNb;
   const int32_t row = r0 + lane;
-  double acc = lane < rows ? y[row] : 0.0;
+  const bool live = lane < rows;
+  double acc = live ? y[row] : 0.0;
   bool ok = true;
+  T v[kNb];
   for (int32_t q = 0; q < np - 1; ++q) {
     int32_t j;
     if (upper) {
@@ -720,32 +776,36 @@ __global__ __launch_bounds__(64) void k_tail_trsv(int32_t m, const T* __restrict
       j = q;
       if (j >= w) break;
     }
-    if (!wait_flag_uniform(flags + j, epoch)) ok = false;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     const int32_t c0 = j * kNb;
     const int32_t cols = (m - c0) < kNb ? (m - c0) : kNb;
+#pragma unroll
+    for (int c = 0; c < kNb; ++c) v[c] = (live && c < cols) ? S[int64_t(c0 + c) * lds + row] : T(0);
+    if (!wait_flag_uniform(flags + j, epoch)) ok = false;
     const double xv = lane < cols ? y[c0 + lane] : 0.0;
-    for (int c = 0; c < cols; ++c) {
-      const double xc = __shfl(xv, c);  // all lanes active: the source lane is never masked off
-      if (lane < rows) acc = acc - static_cast<double>(S[int64_t(c0 + c) * lds + row]) * xc;
-    }
+#pragma unroll
+    for (int c = 0; c < kNb; ++c) acc = acc - static_cast<double>(v[c]) * __shfl(xv, c);
   }
+#pragma unroll
+  for (int c = 0; c < kNb; ++c) v[c] = (live && c < rows) ? S[int64_t(r0 + c) * lds + row] : T(0);
   if (!upper) {
-    for (int c = 0; c < rows; ++c) {
+#pragma unroll
+    for (int c = 0; c < kNb; ++c) {
       const double xc = __shfl(acc, c);
-      if (lane > c && lane < rows) acc = acc - static_cast<double>(S[int64_t(r0 + c) * lds + row]) * xc;
+      if (lane > c) acc = acc - static_cast<double>(v[c]) * xc;
     }
   } else {
-    for (int c = rows - 1; c >= 0; --c) {
-      if (lane == c) acc = acc / static_cast<double>(S[int64_t(r0 + c) * lds + r0 + c]);
-      const double xc = __shfl(acc, c);
-      if (lane < c) acc = acc - static_cast<double>(S[int64_t(r0 + c) * lds + row]) * xc;
+#pragma unroll
+    for (int c = kNb - 1; c >= 0; --c) {
+      if (c < rows) {
+        if (lane == c) acc = acc / static_cast<double>(v[c]);
+        const double xc = __shfl(acc, c);
+        if (lane < c) acc = acc - static_cast<double>(v[c]) * xc;
+      }
     }
   }
-  if (lane < rows) y[row] = acc;
+  if (live) y[row] = acc;
   if (!ok && lane == 0) atomicOr(err, 2);
-  __syncthreads();
-  if (lane == 0) store_release(flags + w, epoch);
+  store_release(flags + w, epoch);  // every lane stores the same value
 }
 template <typename T>
 void launch_tail_trsv(hipStream_t st, int32_t m, const T* S, int64_t lds, double* y, bool upper, int32_t* flags,

@@ -128,7 +128,7 @@ bool evict_one() {
 
 uint64_t gpu_config_sig() {
   uint64_t h = bdlu::cache_config_sig(1);
-  static const char* const kKeys[] = {"BLOCKDAGLU_GPU_TAIL_DENSITY", "BLOCKDAGLU_GPU_TAIL_MIN",
+  static const char* const kKeys[] = {"BLOCKDAGLU_GPU_TAIL_DENSITY", "BLOCKDAGLU_GPU_TAIL_MIN", "BLOCKDAGLU_GPU_TAIL_TOL",
                                       "USERALGO_BLOCK_KLU_TOL", "USERALGO_BLOCK_KLU_BIG_N"};
   for (const char* k : kKeys) {
     const char* v = blockdaglu_getenv_compat(k);

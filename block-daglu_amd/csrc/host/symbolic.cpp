@@ -18,6 +18,10 @@ SymbolicOptions symbolic_options_from_env() {
     const double d = std::atof(v);
     if (d >= 0.0) o.tol = std::min(1.0, d);
   }
+  if (const char* v = std::getenv("BLOCKDAGLU_GPU_TAIL_TOL")) {
+    const double d = std::atof(v);
+    if (d > 0.0) o.tail_tol = std::min(1.0, d);
+  }
   if (const char* v = std::getenv("BLOCKDAGLU_GPU_TAIL_DENSITY")) {
     const double d = std::atof(v);
     if (d > 0.0) o.tail_density = d;
@@ -250,7 +254,7 @@ bool analyze_sparse_block(int32_t s, const std::vector<int32_t>& rows, const std
     }
     for (int32_t k : reach) x[out.prow[k]] = 0.0;
   }
-  if (out.m > 0 && !dense_threshold_lu(out.m, S.data(), out.tdiag, opt.tol, out.tperm)) return false;
+  if (out.m > 0 && !dense_threshold_lu(out.m, S.data(), out.tdiag, opt.tail_tol, out.tperm)) return false;
 
   // Refactor levels of the sparse columns.
   std::vector<int32_t> lev(k0, 0);

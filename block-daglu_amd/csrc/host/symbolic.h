@@ -41,7 +41,11 @@ struct SparseBlockPlan {
 };
 
 struct SymbolicOptions {
-  double tol = 0.001;            // KLU default pivot tolerance
+  double tol = 0.001;            // KLU default pivot tolerance (sparse leading columns)
+  // Pivot tolerance inside the dense Schur tail. KLU's diagonal preference
+  // protects sparsity; the tail is dense, so it only costs stability there.
+  // 1.0 = partial pivoting (the diagonal wins only ties), which bounds |L| by 1.
+  double tail_tol = 1.0;
   double tail_density = 0.2;     // switch to the dense tail when |L(:,j)| > density*(s-j)
   int32_t tail_min = 32;         // ... and at least this many columns remain
   int32_t scale_min_n = 4096;    // KLU scale=1 for blocks at least this large
