@@ -112,7 +112,7 @@ struct Plan {
   int64_t lu_len = 0;
   double* lu64 = nullptr;
   float* lu32 = nullptr;
-  int32_t* piv = nullptr;
+  int32_t* piv = nullptr;  // per row: block-local source row of the pivot gather
   int32_t* block_status = nullptr;  // per block (dense status)
   int32_t* prec_of_block = nullptr; // per block: 0 fp64 factors, 1 fp32
   std::vector<int32_t> h_prec;

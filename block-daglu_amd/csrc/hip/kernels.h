@@ -67,7 +67,7 @@ void launch_permute(hipStream_t st, int32_t n, const int32_t* P, const double* s
 void launch_unpermute(hipStream_t st, int32_t n, const int32_t* Q, const double* src, double* dst);  // dst[Q[k]]=src[k]
 void launch_dense_seg_solve(hipStream_t st, int32_t ntasks, const int32_t* tasks, const int32_t* R,
                             const int64_t* lu0_of_block, const double* lu64, const float* lu32,
-                            const int32_t* prec_of_block, const int32_t* piv, const int32_t* off_ptr,
+                            const int32_t* prec_of_block, const int32_t* gsrc, const int32_t* off_ptr,
                             const int32_t* off_col, const double* off_val, const int32_t* dep_ptr,
                             const int32_t* dep_blk, const int32_t* seg_of, int32_t seg, int32_t* done, int32_t epoch,
                             int32_t* ticket, const double* bperm, double* xperm, int32_t* err);

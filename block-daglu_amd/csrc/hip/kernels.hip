@@ -31,6 +31,8 @@ template <typename T>
 __device__ __forceinline__ T tmax(T a, T b) {
   return a < b ? b : a;
 }
+__device__ __forceinline__ float tfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double tfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 __device__ __forceinline__ int32_t load_acquire(const int32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
@@ -111,7 +113,8 @@ void launch_fill_f32(hipStream_t st, float* p, int64_t n, float v) {
 // K1: batched dense LU with partial pivoting, one wavefront per block (s <= 64).
 // Same arithmetic as the reference dense branch (ubtf:1465-1494): pivot = first
 // row with the largest |a_ik| (ties -> smaller row), fail iff that is 0, full
-// row swap, l_ik = a_ik / a_kk, a_ij -= l_ik * a_kj.
+// row swap, l_ik = a_ik / a_kk, a_ij -= l_ik * a_kj. piv out: for each row
+// position of the block, the block-local row that the swaps bring there.
 template <typename T>
 __global__ __launch_bounds__(64) void k_dense_lu(int32_t nd, const int32_t* __restrict__ sizes,
                                                  const int32_t* __restrict__ starts, const int64_t* __restrict__ val0,
@@ -122,7 +125,7 @@ __global__ __launch_bounds__(64) void k_dense_lu(int32_t nd, const int32_t* __re
   const int lane = threadIdx.x;
   const int32_t t = blockIdx.x;
   if (t >= nd) return;
-  const int s = sizes[t];
+  const int s = __builtin_amdgcn_readfirstlane(sizes[t]);
   const double* src = arena + val0[t];
   for (int idx = lane; idx < s * s; idx += kWave) {
     const int j = idx / s, i = idx - j * s;
@@ -185,7 +188,14 @@ __global__ __launch_bounds__(64) void k_dense_lu(int32_t nd, const int32_t* __re
     const int j = idx / s, i = idx - j * s;
     dst[idx] = a[j * kLdsPad + i];
   }
-  if (lane < s) piv[starts[t] + lane] = mypiv;
+  // Gather form of the row swaps (row piv_k <-> row k for k = 0..s-1, in order):
+  // lane i follows the swaps backwards to the original row landing at i.
+  int32_t srcrow = lane;
+  for (int k = s - 1; k >= 0; --k) {
+    const int32_t pk = __builtin_amdgcn_readlane(mypiv, k);
+    srcrow = (srcrow == k) ? pk : ((srcrow == pk) ? k : srcrow);
+  }
+  if (lane < s) piv[starts[t] + lane] = srcrow;
   if (lane == 0) status[block_of[t]] = st;
 }
 
@@ -344,80 +354,166 @@ template void launch_sp_tail<float>(hipStream_t, const SpView&, const double*, c
 
 // ---------------------------------------------------------------------------
 // Dense tail: no-pivot blocked right-looking LU, panel width 64. Per panel:
-//   1. k_diag_lu_inv: LU of the 64x64 diagonal block in registers (a lane per
-//      column), then L11^{-1} and U11^{-1};
+//   1. k_diag_lu_inv: LU of the 64x64 diagonal block (one 1024-thread
+//      workgroup, register tiles, one barrier per step), then L11^{-1} and
+//      U11^{-1};
 //   2. k_panel_gemm: L21 = A21 U11^{-1} and U12 = L11^{-1} A12 as MFMA tile
 //      products (one launch, both jobs), with the |L| growth check;
 //   3. k_panel_gemm: trailing update A22 -= L21 U12 on MFMA.
 constexpr int kNb = 64;
 
-__device__ __forceinline__ double readlane_t(double v, int src) {
-  const uint64_t u = __builtin_bit_cast(uint64_t, v);
-  const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(u)), src);
-  const uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(u >> 32)), src);
-  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
-}
-__device__ __forceinline__ float readlane_t(float v, int src) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), src));
-}
-
 // inv[0..4095]: L11^{-1} row-major (unit lower); inv[4096..8191]: U11^{-1}
 // row-major (upper). Rows/columns at or past b are an identity pad.
+//
+// 1024 threads: thread (w = wave, j = lane) owns column j of rows w + 16r,
+// r = 0..3, in registers. A wave issues at most one instruction per four
+// cycles, so many waves with short per-step streams set the step time.
+// Right-looking LU with one barrier per step: at step k every thread reads
+// pivot row k and pivot column k from LDS (final since step k-1, all reads
+// issued before any use) and the owners of row k+1 / column k+1 publish them
+// for step k+1. Row tests depend on the wave index and the step only, so they
+// are scalar. The two triangular inverses then run interleaved (L11^{-1}
+// forward, U11^{-1} backward), each step publishing one finished row through a
+// two-slot ring.
+constexpr int kDiagWaves = 16;
+constexpr int kDiagRows = kNb / kDiagWaves;
+
 template <typename T>
-__global__ __launch_bounds__(64) void k_diag_lu_inv(T* __restrict__ S, int64_t lds, int32_t p, int32_t b,
-                                                    double limit, int32_t* __restrict__ status, T* __restrict__ inv) {
-  __shared__ T f[kNb * kNb];   // f[i*64 + j] = LU(i, j)
-  __shared__ T xs[kNb * kNb];  // xs[i*64 + lane]: the lane's inverse column
-  const int lane = threadIdx.x;
-  T col[kNb];
+__global__ __launch_bounds__(kDiagWaves * kNb) void k_diag_lu_inv(T* __restrict__ S, int64_t lds, int32_t p,
+                                                                  int32_t b, double limit,
+                                                                  int32_t* __restrict__ status, T* __restrict__ inv) {
+  __shared__ T f[kNb * (kNb + 1)];  // f[i*65 + j] = LU(i, j)
+  __shared__ T xr[2][kNb], yr[2][kNb];
+  __shared__ int32_t st_sh;
+  constexpr int LD = kNb + 1;
+  const int j = threadIdx.x & (kNb - 1);
+  const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  constexpr int kThreads = kDiagWaves * kNb;
+  if (threadIdx.x == 0) st_sh = 0;
+  {
+    T v[kNb * kNb / kThreads];
 #pragma unroll
-  for (int i = 0; i < kNb; ++i)
-    col[i] = (lane < b && i < b) ? S[int64_t(p + lane) * lds + p + i] : (i == lane ? T(1) : T(0));
+    for (int u = 0; u < kNb * kNb / kThreads; ++u) {
+      const int c = u * kDiagWaves + w, r = j;  // element (r, c): coalesced down column c
+      v[u] = (r < b && c < b) ? S[int64_t(p + c) * lds + p + r] : (r == c ? T(1) : T(0));
+    }
+#pragma unroll
+    for (int u = 0; u < kNb * kNb / kThreads; ++u) f[j * LD + u * kDiagWaves + w] = v[u];
+  }
+  __syncthreads();
+  T a[kDiagRows];
+#pragma unroll
+  for (int r = 0; r < kDiagRows; ++r) a[r] = f[(w + kDiagWaves * r) * LD + j];
   int32_t st = 0;
-  T lmax = T(0);
-#pragma unroll
   for (int k = 0; k < kNb; ++k) {
-    const T akk = readlane_t(col[k], k);
-    if (akk == T(0)) st |= kStZeroPivot;
-    if (lane == k) {
+    const T akk = f[k * LD + k];
+    const T ukj_raw = f[k * LD + j];
+    T cv[kDiagRows];
 #pragma unroll
-      for (int i = k + 1; i < kNb; ++i) {
-        col[i] = col[i] / akk;
-        const T al = tabs(col[i]);
-        if (!(al <= lmax)) lmax = al;
+    for (int r = 0; r < kDiagRows; ++r) cv[r] = f[(w + kDiagWaves * r) * LD + k];
+    if (akk == T(0)) st |= kStZeroPivot;
+    const T rp = T(1) / akk;
+    // Columns left of k keep their values (a - lik * 0), column k takes l_ik.
+    const T ukj = (j > k) ? ukj_raw : T(0);
+    if (w + kDiagWaves * (kDiagRows - 1) > k) {  // this wave still has active rows
+#pragma unroll
+      for (int r = 0; r < kDiagRows; ++r) {
+        if (w + kDiagWaves * r > k) {
+          const T lik = cv[r] * rp;
+          a[r] = (j == k) ? lik : tfma(-lik, ukj, a[r]);
+        }
       }
     }
+    // Publish row k+1 and column k+1. The column owner writes all its rows:
+    // entries above row k+1 are not read again before the final publish.
+    const int kn = k + 1;
+    if (kn < kNb) {
+      // Register a[kn / kDiagWaves] of the owner wave, picked by scalar branches (a
+      // value select over the register array makes the compiler index it
+      // dynamically, i.e. move it to scratch memory).
+      if ((kn & (kDiagWaves - 1)) == w) {
 #pragma unroll
-    for (int i = k + 1; i < kNb; ++i) {
-      const T li = readlane_t(col[i], k);
-      if (lane > k) col[i] = col[i] - li * col[k];
+        for (int r = 0; r < kDiagRows; ++r)
+          if (r == kn / kDiagWaves && j >= kn) f[kn * LD + j] = a[r];
+      }
+      if (j == kn) {
+#pragma unroll
+        for (int r = 0; r < kDiagRows; ++r) f[(w + kDiagWaves * r) * LD + kn] = a[r];
+      }
     }
+    __syncthreads();
+  }
+  // |L| check on the finished factor (a NaN fails it).
+  T lmax = T(0);
+#pragma unroll
+  for (int r = 0; r < kDiagRows; ++r) {
+    const T al = (j < w + kDiagWaves * r) ? tabs(a[r]) : T(0);
+    lmax = (al <= lmax || lmax != lmax) ? lmax : al;
+  }
+  // Publish the full LU (L below the diagonal) for the inverse sweeps and the
+  // coalesced write-back.
+#pragma unroll
+  for (int r = 0; r < kDiagRows; ++r) f[(w + kDiagWaves * r) * LD + j] = a[r];
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kNb * kNb / kThreads; ++u) {
+    const int c = u * kDiagWaves + w, r = j;
+    if (r < b && c < b) S[int64_t(p + c) * lds + p + r] = f[r * LD + c];
+  }
+  // X = L11^{-1} (forward: X[i] -= L[i][k] X[k] for i > k) and
+  // Y = U11^{-1} (backward: Y[k] /= U[k][k]; Y[i] -= U[i][k] Y[k] for i < k).
+  // Registers keep Y undivided; a row is divided when it is published and the
+  // registers once at the end (the same operation on the same values).
+  T x[kDiagRows], y[kDiagRows];
+#pragma unroll
+  for (int r = 0; r < kDiagRows; ++r) {
+    const int i = w + kDiagWaves * r;
+    x[r] = (i == j) ? T(1) : T(0);
+    y[r] = (i == j) ? T(1) : T(0);
+    if (i == 0) xr[0][j] = x[r];
+    if (i == kNb - 1) yr[(kNb - 1) & 1][j] = y[r] / f[i * LD + i];
+  }
+  __syncthreads();
+  for (int s = 0; s < kNb - 1; ++s) {
+    const int kx = s, ky = kNb - 1 - s;
+    const T xk = xr[kx & 1][j];
+    const T yk = yr[ky & 1][j];
+    T lv[kDiagRows], uv[kDiagRows];
+#pragma unroll
+    for (int r = 0; r < kDiagRows; ++r) {
+      lv[r] = f[(w + kDiagWaves * r) * LD + kx];
+      uv[r] = f[(w + kDiagWaves * r) * LD + ky];
+    }
+#pragma unroll
+    for (int r = 0; r < kDiagRows; ++r) {
+      const int i = w + kDiagWaves * r;
+      if (i > kx) x[r] = tfma(-lv[r], xk, x[r]);
+      if (i < ky) y[r] = tfma(-uv[r], yk, y[r]);
+    }
+    const int xn = kx + 1, yn = ky - 1;
+    if ((xn & (kDiagWaves - 1)) == w) {
+#pragma unroll
+      for (int r = 0; r < kDiagRows; ++r)
+        if (r == xn / kDiagWaves) xr[xn & 1][j] = x[r];
+    }
+    if ((yn & (kDiagWaves - 1)) == w) {
+      const T d = f[yn * LD + yn];
+#pragma unroll
+      for (int r = 0; r < kDiagRows; ++r)
+        if (r == yn / kDiagWaves) yr[yn & 1][j] = y[r] / d;
+    }
+    __syncthreads();
   }
 #pragma unroll
-  for (int i = 0; i < kNb; ++i) {
-    f[i * kNb + lane] = col[i];
-    if (lane < b && i < b) S[int64_t(p + lane) * lds + p + i] = col[i];
-  }
-  __syncthreads();
-  // L11^{-1}: lane j forms column j (x_i = delta_ij - sum_{k<i} L_ik x_k).
-  for (int i = 0; i < kNb; ++i) {
-    T acc = (i == lane) ? T(1) : T(0);
-    for (int k = 0; k < i; ++k) acc = acc - f[i * kNb + k] * xs[k * kNb + lane];
-    xs[i * kNb + lane] = acc;
-    inv[i * kNb + lane] = acc;
-  }
-  __syncthreads();
-  // U11^{-1}: lane j forms column j (x_i = (delta_ij - sum_{k>i} U_ik x_k) / U_ii).
-  for (int i = kNb - 1; i >= 0; --i) {
-    T acc = (i == lane) ? T(1) : T(0);
-    for (int k = i + 1; k < kNb; ++k) acc = acc - f[i * kNb + k] * xs[k * kNb + lane];
-    acc = acc / f[i * kNb + i];
-    xs[i * kNb + lane] = acc;
-    inv[kNb * kNb + i * kNb + lane] = acc;
+  for (int r = 0; r < kDiagRows; ++r) {
+    const int i = w + kDiagWaves * r;
+    inv[i * kNb + j] = x[r];
+    inv[kNb * kNb + i * kNb + j] = y[r] / f[i * LD + i];
   }
   if (!(lmax <= static_cast<T>(limit))) st |= kStCheck;
-  for (int off = 32; off > 0; off >>= 1) st |= __shfl_xor(st, off);
-  if (lane == 0 && st) atomicOr(status, st);
+  if (st) atomicOr(&st_sh, st);
+  __syncthreads();
+  if (threadIdx.x == 0 && st_sh) atomicOr(status, st_sh);
 }
 
 // Inputs of both forms: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16].
@@ -557,7 +653,7 @@ void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, 
     const int32_t b = (m - p) < kNb ? (m - p) : kNb;
     const int32_t rest = m - p - b;
     if (timer && timer->panel_events) timer_event(timer->panel_events, st);
-    hipLaunchKernelGGL(k_diag_lu_inv<T>, dim3(1), dim3(64), 0, st, S, lds, p, b, limit, status, inv);
+    hipLaunchKernelGGL(k_diag_lu_inv<T>, dim3(1), dim3(kDiagWaves * kNb), 0, st, S, lds, p, b, limit, status, inv);
     if (rest > 0) {
       const int32_t tr = (rest + kNb - 1) / kNb;
       TileJob<T> l21{rest, b, b, 2, tr, 1, S + int64_t(p) * lds + p + b, 1, lds, inv + kNb * kNb, kNb, 1,
@@ -601,90 +697,129 @@ void launch_unpermute(hipStream_t st, int32_t n, const int32_t* Q, const double*
   hipLaunchKernelGGL(k_unpermute, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, n, Q, src, dst);
 }
 
-// Dense-block DAG solve of one segment: a persistent wavefront per task, tasks
-// claimed in order from a ticket counter (descending block id = a topological
-// order of the block DAG), so every dependency a wave waits for has already
-// been claimed by a running wave. rhs = bperm - A_off * xperm (ubtf:2269-2286),
-// then pivot swaps, unit-L forward and U backward (ubtf:2071-2097).
-// Dense-block DAG solve of one segment: a persistent grid of wavefronts claims
-// tasks in order from a ticket counter (descending block id = a topological
-// order of the block DAG), so every dependency a wave waits for has already
-// been claimed by a running wave. rhs = bperm - A_off * xperm (ubtf:2269-2286),
-// then pivot swaps, unit-L forward and U backward (ubtf:2071-2097).
-// Only the ticket claim is a lane-0 region; everything that ends an iteration
-// runs on all lanes, so the loop cannot be split across lanes.
+__device__ __forceinline__ double readlane_f64(double v, int src) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(u)), src);
+  const uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(u >> 32)), src);
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// Stages an s x s column-major factor into LDS with eight loads in flight per
+// lane (the copy is off the dependency chain only if it does not serialize).
+template <typename T>
+__device__ __forceinline__ void stage_factor(double* __restrict__ fs, const T* __restrict__ f, int ss, int lane) {
+  for (int base = 0; base < ss; base += 8 * kWave) {
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int idx = base + u * kWave + lane;
+      v[u] = idx < ss ? f[idx] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int idx = base + u * kWave + lane;
+      if (idx < ss) fs[idx] = static_cast<double>(v[u]);
+    }
+  }
+}
+
+// Persistent solve of one segment of dense diagonal blocks (a segment is a
+// maximal run of dense blocks between sparse blocks in the solve order). Each
+// workgroup (one wavefront) claims block tickets in order (descending block id,
+// a topological order of the block DAG) and solves block b:
+// rhs = bperm - A_off * xperm (ubtf:2269-2286), then the pivot gather, unit-L
+// forward and U backward substitution (ubtf:2071-2097). Everything
+// that does not depend on other blocks' x (factor to LDS, row gather, the first
+// off-diagonal entries, the next ticket) is issued before waiting, so a block
+// on the critical path pays only for the x loads and the substitution.
+// Deadlock-free with the early claim: the lowest unfinished ticket is always
+// the current block of some workgroup (a claimed, unstarted ticket is higher
+// than its holder's current one), and all its dependencies hold lower tickets.
 __global__ __launch_bounds__(64) void k_dense_seg_solve(
     int32_t ntasks, const int32_t* __restrict__ tasks, const int32_t* __restrict__ R,
     const int64_t* __restrict__ lu0_of_block, const double* __restrict__ lu64, const float* __restrict__ lu32,
-    const int32_t* __restrict__ prec_of_block, const int32_t* __restrict__ piv, const int32_t* __restrict__ off_ptr,
+    const int32_t* __restrict__ prec_of_block, const int32_t* __restrict__ gsrc, const int32_t* __restrict__ off_ptr,
     const int32_t* __restrict__ off_col, const double* __restrict__ off_val, const int32_t* __restrict__ dep_ptr,
     const int32_t* __restrict__ dep_blk, const int32_t* __restrict__ seg_of, int32_t seg, int32_t* __restrict__ done,
     int32_t epoch, int32_t* __restrict__ ticket, const double* __restrict__ bperm, double* __restrict__ xperm,
     int32_t* __restrict__ err) {
   __shared__ double fs[64 * 64];
+  constexpr int kPre = 4;
   const int lane = threadIdx.x;
-  for (;;) {
-    int32_t t0 = 0;
-    if (lane == 0) t0 = atomicAdd(ticket, 1);
-    const int32_t t = uniform_from_lane0(t0);
-    if (t >= ntasks) break;
-    const int32_t b = tasks[t];
-    // Wait for dependencies inside this segment (others finished in earlier launches).
+  int32_t tk = 0;
+  if (lane == 0) tk = atomicAdd(ticket, 1);
+  int32_t t = uniform_from_lane0(tk);
+  while (t < ntasks) {
+    int32_t tn = 0;
+    if (lane == 0) tn = atomicAdd(ticket, 1);
+    const int32_t b = __builtin_amdgcn_readfirstlane(tasks[t]);
+    const int32_t r0 = __builtin_amdgcn_readfirstlane(R[b]);
+    const int s = __builtin_amdgcn_readfirstlane(R[b + 1]) - r0;
+    const bool act = lane < s;
+    const int32_t r = r0 + lane;
+    // --- independent of other blocks ---
+    const int32_t src = act ? gsrc[r] : 0;
+    const double bv = act ? bperm[r] : 0.0;
+    const int32_t p0 = act ? off_ptr[r] : 0, p1 = act ? off_ptr[r + 1] : 0;
+    int32_t oc[kPre];
+    double ov[kPre];
+#pragma unroll
+    for (int u = 0; u < kPre; ++u) {
+      const bool ok = p0 + u < p1;
+      oc[u] = ok ? off_col[p0 + u] : -1;
+      ov[u] = ok ? off_val[p0 + u] : 0.0;
+    }
+    if (__builtin_amdgcn_readfirstlane(prec_of_block[b]) != 0)
+      stage_factor(fs, lu32 + lu0_of_block[b], s * s, lane);
+    else
+      stage_factor(fs, lu64 + lu0_of_block[b], s * s, lane);
+    __syncthreads();
+    const double rd = act ? 1.0 / fs[lane * s + lane] : 1.0;
+    // --- dependencies inside this segment (others finished in earlier launches) ---
     int32_t bad = 0;
-    for (int32_t d = dep_ptr[b]; d < dep_ptr[b + 1]; ++d) {
-      const int32_t c = dep_blk[d];
+    const int32_t d1 = __builtin_amdgcn_readfirstlane(dep_ptr[b + 1]);
+    for (int32_t d = __builtin_amdgcn_readfirstlane(dep_ptr[b]); d < d1; ++d) {
+      const int32_t c = __builtin_amdgcn_readfirstlane(dep_blk[d]);
       if (seg_of[c] == seg && !wait_flag_uniform(done + c, epoch)) bad = 1;
     }
     if (bad) atomicOr(err, 1);  // every lane ORs the same bit
-    const int32_t r0 = R[b];
-    const int s = R[b + 1] - r0;
-    double acc = 0.0;
-    if (lane < s) {
-      const int32_t r = r0 + lane;
-      acc = bperm[r];
-      for (int32_t p = off_ptr[r]; p < off_ptr[r + 1]; ++p) acc = acc - off_val[p] * xperm[off_col[p]];
-    }
-    // Row swaps (piv[k] <-> k for k = 0..s-1, in order) as one gather: lane i
-    // follows the swaps backwards to find which row lands at position i.
-    int32_t src = lane;
-    for (int k = s - 1; k >= 0; --k) {
-      const int32_t pk = piv[r0 + k];
-      src = (src == k) ? pk : ((src == pk) ? k : src);
-    }
-    acc = __shfl(acc, src < 64 ? src : 0);
-    if (lane >= s) acc = 0.0;
-    // The block's factors go to LDS first (independent coalesced loads), so the
-    // substitution chain below never waits on global memory.
-    const bool single = prec_of_block[b] != 0;
-    const double* f64 = lu64 + lu0_of_block[b];
-    const float* f32 = lu32 + lu0_of_block[b];
-    for (int idx = lane; idx < s * s; idx += kWave) fs[idx] = single ? static_cast<double>(f32[idx]) : f64[idx];
-    __syncthreads();
+    double xv[kPre];
+#pragma unroll
+    for (int u = 0; u < kPre; ++u) xv[u] = oc[u] >= 0 ? xperm[oc[u]] : 0.0;
+    double acc = bv;
+#pragma unroll
+    for (int u = 0; u < kPre; ++u)
+      if (oc[u] >= 0) acc = acc - ov[u] * xv[u];
+    for (int32_t p = p0 + kPre; p < p1; ++p) acc = acc - off_val[p] * xperm[off_col[p]];
+    // Row gather (the block's pivoting), then unit-lower and upper substitution.
+    acc = __shfl(acc, src);
+    if (!act) acc = 0.0;
     for (int j = 0; j < s; ++j) {
-      const double xj = __shfl(acc, j);
-      if (lane > j && lane < s) acc = acc - fs[j * s + lane] * xj;
+      const double xj = readlane_f64(acc, j);
+      if (lane > j && act) acc = acc - fs[j * s + lane] * xj;
     }
     for (int j = s - 1; j >= 0; --j) {
-      if (lane == j) acc = acc / fs[j * s + j];
-      const double xj = __shfl(acc, j);
+      if (lane == j) acc = acc * rd;
+      const double xj = readlane_f64(acc, j);
       if (lane < j) acc = acc - fs[j * s + lane] * xj;
     }
-    __syncthreads();
-    if (lane < s) xperm[r0 + lane] = acc;
+    if (act) xperm[r] = acc;
     store_release(done + b, epoch);  // every lane stores the same value
+    __syncthreads();                 // fs is restaged by the next block
+    t = uniform_from_lane0(tn);
   }
 }
 
 void launch_dense_seg_solve(hipStream_t st, int32_t ntasks, const int32_t* tasks, const int32_t* R,
                             const int64_t* lu0_of_block, const double* lu64, const float* lu32,
-                            const int32_t* prec_of_block, const int32_t* piv, const int32_t* off_ptr,
+                            const int32_t* prec_of_block, const int32_t* gsrc, const int32_t* off_ptr,
                             const int32_t* off_col, const double* off_val, const int32_t* dep_ptr,
                             const int32_t* dep_blk, const int32_t* seg_of, int32_t seg, int32_t* done, int32_t epoch,
                             int32_t* ticket, const double* bperm, double* xperm, int32_t* err) {
   if (ntasks <= 0) return;
   const int g = ntasks < 256 ? ntasks : 256;
   hipLaunchKernelGGL(k_dense_seg_solve, dim3(g), dim3(64), 0, st, ntasks, tasks, R, lu0_of_block, lu64, lu32,
-                     prec_of_block, piv, off_ptr, off_col, off_val, dep_ptr, dep_blk, seg_of, seg, done, epoch,
+                     prec_of_block, gsrc, off_ptr, off_col, off_val, dep_ptr, dep_blk, seg_of, seg, done, epoch,
                      ticket, bperm, xperm, err);
 }
 
