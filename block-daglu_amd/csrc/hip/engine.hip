@@ -77,7 +77,7 @@ struct SparseDev {
   double* y = nullptr;
   double* inv64 = nullptr;  // 2*64*64 diagonal-block inverses (tail LU scratch)
   float* inv32 = nullptr;
-  int32_t *flags_f = nullptr, *flags_b = nullptr;
+  double* ty = nullptr;  // m: tail forward-solve output (published values)
   int32_t* status = nullptr;  // device int
   int prec = 0;
 };
@@ -101,7 +101,6 @@ struct Plan {
   int32_t *P = nullptr, *Q = nullptr, *R = nullptr;
   int32_t *off_ptr = nullptr, *off_col = nullptr;
   double* off_val = nullptr;
-  int32_t *dep_ptr = nullptr, *dep_blk = nullptr, *seg_of = nullptr;
   // dense blocks
   int32_t ndense = 0;
   std::vector<int32_t> h_dense_block, h_dense_size, h_dense_start;
@@ -123,14 +122,12 @@ struct Plan {
   int32_t nslots = 0;
   // solve
   std::vector<Phase> phases;
-  int32_t* done = nullptr;
   int32_t* tickets = nullptr;
   int32_t nticket = 0;
   int32_t* err = nullptr;
   double *b = nullptr, *x = nullptr, *r = nullptr, *d = nullptr, *bperm = nullptr, *xperm = nullptr;
   double* partials = nullptr;
   double* norms = nullptr;  // [0..4] residual norms, [8] ||A||_inf
-  int32_t epoch = 0;
   bool factored = false;
 };
 
@@ -300,8 +297,6 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
   const int64_t noff = d.off_ptr[n];
   P.off_col = A.copy(d.off_col, noff, st);
   P.off_val = P.arena + d.off_val0;
-  P.dep_ptr = A.copy(d.dep_ptr, nb + 1, st);
-  P.dep_blk = A.copy(d.dep_blk, d.dep_ptr[nb], st);
 
   // Dense blocks.
   P.ndense = d.ndense;
@@ -417,11 +412,7 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
     S.y = A.get<double>(sd.s);
     S.inv64 = A.get<double>(2 * 64 * 64);
     S.inv32 = A.get<float>(2 * 64 * 64);
-    const int32_t np = (sd.m + 63) / 64;
-    S.flags_f = A.get<int32_t>(np);
-    S.flags_b = A.get<int32_t>(np);
-    HIPCHECK(hipMemsetAsync(S.flags_f, 0, sizeof(int32_t) * std::max(np, 1), st));
-    HIPCHECK(hipMemsetAsync(S.flags_b, 0, sizeof(int32_t) * std::max(np, 1), st));
+    S.ty = A.get<double>(std::max(sd.m, 1));
     S.status = A.get<int32_t>(1);
     P.sp.push_back(std::move(S));
   }
@@ -434,14 +425,12 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
 
   // Solve phases: blocks in descending order; runs of dense blocks form one
   // persistent-kernel segment, each sparse block is its own phase.
-  std::vector<int32_t> seg_of(nb, -1);
   std::vector<int32_t> cur;
   auto flush = [&]() {
     if (cur.empty()) return;
     Phase ph;
     ph.sparse = false;
     ph.index = static_cast<int32_t>(P.phases.size());
-    for (int32_t b : cur) seg_of[b] = ph.index;
     ph.ntasks = static_cast<int32_t>(cur.size());
     ph.tasks = A.copy(cur, st);
     P.phases.push_back(ph);
@@ -459,9 +448,6 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
     }
   }
   flush();
-  P.seg_of = A.copy(seg_of, st);
-  P.done = A.get<int32_t>(nb);
-  HIPCHECK(hipMemsetAsync(P.done, 0, sizeof(int32_t) * std::max(nb, 1), st));
   P.nticket = static_cast<int32_t>(P.phases.size()) + 2 * d.nsparse;
   P.tickets = A.get<int32_t>(P.nticket);
   P.err = A.get<int32_t>(1);
@@ -479,17 +465,15 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
 
 // Solve A x = b for the resident factors; b and x are device vectors of length n.
 void block_solve(Plan& P, hipStream_t st, const double* b, double* x) {
-  P.epoch += 1;
-  const int32_t epoch = P.epoch;
   bdk::launch_fill_i32(st, P.tickets, P.nticket, 0);
+  bdk::launch_fill_sentinel(st, P.xperm, P.n);
   bdk::launch_permute(st, P.n, P.P, b, P.bperm);
   int32_t tix = 0;
   for (const Phase& ph : P.phases) {
     if (!ph.sparse) {
       // Each dense block's factors are read in the precision it was factored in.
       bdk::launch_dense_seg_solve(st, ph.ntasks, ph.tasks, P.R, P.lu0_of_block, P.lu64, P.lu32, P.prec_of_block,
-                                  P.piv, P.off_ptr, P.off_col, P.off_val, P.dep_ptr, P.dep_blk, P.seg_of, ph.index,
-                                  P.done, epoch, P.tickets + tix, P.bperm, P.xperm, P.err);
+                                  P.piv, P.off_ptr, P.off_col, P.off_val, P.tickets + tix, P.bperm, P.xperm, P.err);
       ++tix;
       continue;
     }
@@ -503,8 +487,11 @@ void block_solve(Plan& P, hipStream_t st, const double* b, double* x) {
         bdk::launch_sp_rows<T>(st, S.fl_ptr[l + 1] - S.fl_ptr[l], S.fl_rows + S.fl_ptr[l], S.lr_ptr, S.lr_col,
                                S.lr_vidx, Lx, nullptr, S.y);
       bdk::launch_sp_rows<T>(st, S.m, S.tl_rows, S.lr_ptr, S.lr_col, S.lr_vidx, Lx, nullptr, S.y);
-      bdk::launch_tail_trsv<T>(st, S.m, Sm, S.m, S.y + S.k0, false, S.flags_f, epoch, P.tickets + tix, P.err);
-      bdk::launch_tail_trsv<T>(st, S.m, Sm, S.m, S.y + S.k0, true, S.flags_b, epoch, P.tickets + tix + 1, P.err);
+      // Tail: forward into ty, backward back into y (each output starts unpublished).
+      bdk::launch_fill_sentinel(st, S.ty, S.m);
+      bdk::launch_tail_trsv<T>(st, S.m, Sm, S.m, S.y + S.k0, S.ty, false, P.tickets + tix, P.err);
+      bdk::launch_fill_sentinel(st, S.y + S.k0, S.m);
+      bdk::launch_tail_trsv<T>(st, S.m, Sm, S.m, S.ty, S.y + S.k0, true, P.tickets + tix + 1, P.err);
       for (std::size_t l = 0; l + 1 < S.bl_ptr.size(); ++l)
         bdk::launch_sp_rows<T>(st, S.bl_ptr[l + 1] - S.bl_ptr[l], S.bl_rows + S.bl_ptr[l], S.ur_ptr, S.ur_col,
                                S.ur_vidx, Ux, Ud, S.y);

@@ -34,6 +34,8 @@ void launch_scatter(hipStream_t st, int64_t nnz, const double* vals, const int64
 void launch_fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v);
 void launch_fill_f64(hipStream_t st, double* p, int64_t n, double v);
 void launch_fill_f32(hipStream_t st, float* p, int64_t n, float v);
+// Fills with the solve's "not yet published" pattern (see kernels.hip).
+void launch_fill_sentinel(hipStream_t st, double* p, int64_t n);
 
 template <typename T>
 void launch_dense_lu(hipStream_t st, int32_t nd, const int32_t* sizes, const int32_t* starts, const int64_t* val0,
@@ -65,12 +67,13 @@ void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, 
 // ---- solve ----
 void launch_permute(hipStream_t st, int32_t n, const int32_t* P, const double* src, double* dst);    // dst[k]=src[P[k]]
 void launch_unpermute(hipStream_t st, int32_t n, const int32_t* Q, const double* src, double* dst);  // dst[Q[k]]=src[k]
+// Dense-block solve of one segment (persistent, sync-free). ticket starts at 0;
+// xperm entries of this segment's blocks start as the unpublished pattern.
 void launch_dense_seg_solve(hipStream_t st, int32_t ntasks, const int32_t* tasks, const int32_t* R,
                             const int64_t* lu0_of_block, const double* lu64, const float* lu32,
                             const int32_t* prec_of_block, const int32_t* gsrc, const int32_t* off_ptr,
-                            const int32_t* off_col, const double* off_val, const int32_t* dep_ptr,
-                            const int32_t* dep_blk, const int32_t* seg_of, int32_t seg, int32_t* done, int32_t epoch,
-                            int32_t* ticket, const double* bperm, double* xperm, int32_t* err);
+                            const int32_t* off_col, const double* off_val, int32_t* ticket, const double* bperm,
+                            double* xperm, int32_t* err);
 // Sparse block solve pieces; y is a length-s work vector in position order.
 void launch_sp_gather(hipStream_t st, int32_t s, int32_t start, const int32_t* rowperm, const double* rs,
                       const int32_t* off_ptr, const int32_t* off_col, const double* off_val, const double* bperm,
@@ -78,9 +81,11 @@ void launch_sp_gather(hipStream_t st, int32_t s, int32_t start, const int32_t* r
 template <typename T>
 void launch_sp_rows(hipStream_t st, int32_t nrows, const int32_t* rows, const int32_t* ptr, const int32_t* col,
                     const int32_t* vidx, const T* vals, const T* diag, double* y);
+// Dense tail triangular solve over 64-row bands. ticket starts at 0; yout
+// starts as the unpublished pattern and does not alias yin.
 template <typename T>
-void launch_tail_trsv(hipStream_t st, int32_t m, const T* S, int64_t lds, double* y, bool upper, int32_t* flags,
-                      int32_t epoch, int32_t* ticket, int32_t* err);
+void launch_tail_trsv(hipStream_t st, int32_t m, const T* S, int64_t lds, const double* yin, double* yout, bool upper,
+                      int32_t* ticket, int32_t* err);
 void launch_sp_scatter_x(hipStream_t st, int32_t s, int32_t start, const int32_t* q, const double* x, double* xperm);
 
 // ---- residual / norms / refinement ----

@@ -72,6 +72,33 @@ __device__ __forceinline__ bool wait_flag_uniform(const int32_t* flag, int32_t e
   return false;
 }
 
+// Published-value protocol of the solves: an output vector starts filled with
+// kXSentinel (a signaling-NaN pattern, which no arithmetic produces: results
+// are quiet NaNs), each value is written once with a relaxed agent-scope
+// atomic store, and a consumer polls the very values it needs until they
+// differ from the sentinel. The value is its own flag, so no fences, flags or
+// dependency lists sit on the critical path.
+constexpr uint64_t kXSentinel = 0x7FF4DEADBEEF0001ull;
+
+__device__ __forceinline__ uint64_t ld_x_relaxed(const double* p) {
+  return __hip_atomic_load(reinterpret_cast<const uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_x_relaxed(double* p, double v) {
+  __hip_atomic_store(reinterpret_cast<uint64_t*>(p), __builtin_bit_cast(uint64_t, v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+// Per-lane bounded wait for a published value (first poll already done: u).
+// Valid schedules never come near the bound; expiry sets `late`.
+__device__ __forceinline__ double finish_wait_x(const double* p, uint64_t u, bool& late) {
+  for (int32_t it = 0; u == kXSentinel && it < (1 << 22); ++it) {
+    __builtin_amdgcn_s_sleep(2);
+    u = ld_x_relaxed(p);
+  }
+  if (u == kXSentinel) late = true;
+  return __builtin_bit_cast(double, u);
+}
+__device__ __forceinline__ double wait_x(const double* p, bool& late) { return finish_wait_x(p, ld_x_relaxed(p), late); }
+
 inline int grid_for(int64_t n, int block, int cap = 65535) {
   int64_t g = (n + block - 1) / block;
   if (g < 1) g = 1;
@@ -107,6 +134,11 @@ void launch_fill_f64(hipStream_t st, double* p, int64_t n, double v) {
 }
 void launch_fill_f32(hipStream_t st, float* p, int64_t n, float v) {
   if (n > 0) hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, p, n, v);
+}
+void launch_fill_sentinel(hipStream_t st, double* p, int64_t n) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_fill<uint64_t>, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st,
+                       reinterpret_cast<uint64_t*>(p), n, kXSentinel);
 }
 
 // ---------------------------------------------------------------------------
@@ -535,8 +567,9 @@ struct MfmaOps<float> {
   __device__ static int row(int lk, int r) { return 4 * lk + r; }
 };
 
-// One tile product job: C(MxN) op= A(MxK) B(KxN), K <= 64, element (i,j) of X
-// at X[i*x_r + j*x_c]. op: 0 C -= AB, 1 C = AB, 2 C = AB and check |C| <= limit.
+// One tile product job: C(MxN) op= A(MxK) B(KxN), K <= 64, element (i,j) of A
+// at A[i*a_r + j*a_c] (B likewise); C is column-major (c_r must be 1, column
+// stride c_c). op: 0 C -= AB, 1 C = AB, 2 C = AB and check |C| <= limit.
 template <typename T>
 struct TileJob {
   int32_t M, N, K, op, tiles_m, tiles_n;
@@ -548,86 +581,121 @@ struct TileJob {
   int64_t c_r, c_c;
 };
 
-// 64x64 output tile per 256-thread workgroup (each wavefront a 32x32 quarter,
-// 2x2 MFMA 16x16 tiles); A and B tiles staged whole in LDS before any store,
-// so C may alias A (tile covers all K columns of its rows) or B (all K rows of
-// its columns). Workgroups below j0.tiles_m*j0.tiles_n run job 0, the rest job 1.
+// 128x128 output tile per 256-thread workgroup (each wavefront a 64x64
+// quarter, 4x4 MFMA 16x16 blocks), K <= 64 staged whole in LDS (zero padded),
+// so C may alias A (a tile covers all K columns of its rows) or B (all K rows
+// of its columns). Each MFMA forms a block of C^T (B^T as the row operand,
+// A^T as the column operand): the output's fast lane index then runs down a
+// column of C, so every C access is a 16-element contiguous run. For op 0 the
+// accumulators start as C and B is negated, so the MFMA chain yields C - AB.
+// All global loads are unconditional with clamped indices (a conditional load
+// turns into a branch that waits for it). Workgroups below
+// j0.tiles_m*j0.tiles_n run job 0, the rest job 1.
+constexpr int kGT = 128;       // output tile edge
+constexpr int kGLd = kGT + 1;  // LDS row length (padded against bank conflicts)
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel_gemm(TileJob<T> j0, TileJob<T> j1, double limit,
                                                     int32_t* __restrict__ status) {
   using V = typename MfmaOps<T>::v4;
-  __shared__ T As[kNb * kNb];  // As[k*64 + i] = A(i0 + i, k)
-  __shared__ T Bs[kNb * kNb];  // Bs[k*64 + j] = B(k, j0 + j)
+  __shared__ T As[kNb * kGLd];  // As[k*kGLd + i] = A(i0 + i, k)
+  __shared__ T Bs[kNb * kGLd];  // Bs[k*kGLd + j] = +-B(k, c0 + j)
   const int32_t n0 = j0.tiles_m * j0.tiles_n;
   const bool second = static_cast<int32_t>(blockIdx.x) >= n0;
   const TileJob<T>& J = second ? j1 : j0;
-  const int32_t tid_blk = second ? static_cast<int32_t>(blockIdx.x) - n0 : static_cast<int32_t>(blockIdx.x);
-  const int32_t i0 = (tid_blk % J.tiles_m) * kNb, c0 = (tid_blk / J.tiles_m) * kNb;
+  const int32_t tb = second ? static_cast<int32_t>(blockIdx.x) - n0 : static_cast<int32_t>(blockIdx.x);
+  const int32_t i0 = (tb % J.tiles_m) * kGT, c0 = (tb / J.tiles_m) * kGT;
   const int tid = threadIdx.x;
-  // Loads: the unit-stride index runs across threads.
-  if (J.a_r == 1) {
-    for (int idx = tid; idx < kNb * kNb; idx += 256) {
-      const int k = idx / kNb, i = idx - k * kNb;
-      As[k * kNb + i] = (k < J.K && i0 + i < J.M) ? J.A[(i0 + i) + int64_t(k) * J.a_c] : T(0);
-    }
+  const int wave = tid >> 6, lane = tid & 63;
+  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+  const int li = lane & 15, lk = lane >> 4;
+  const int32_t M = J.M, N = J.N, K = J.K;
+  V acc[4][4];  // acc[c][a]: C^T block (j-block c, i-block a)
+  if (J.op == 0) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int32_t i = min(i0 + wr + a * 16 + li, M - 1);
+          const int32_t j = min(c0 + wc + c * 16 + MfmaOps<T>::row(lk, r), N - 1);
+          acc[c][a][r] = J.C[i + int64_t(j) * J.c_c];
+        }
   } else {
-    for (int idx = tid; idx < kNb * kNb; idx += 256) {
-      const int i = idx / kNb, k = idx - i * kNb;
-      As[k * kNb + i] = (k < J.K && i0 + i < J.M) ? J.A[int64_t(i0 + i) * J.a_r + k * J.a_c] : T(0);
-    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int a = 0; a < 4; ++a) acc[c][a] = V{T(0), T(0), T(0), T(0)};
   }
-  if (J.b_r == 1) {
-    for (int idx = tid; idx < kNb * kNb; idx += 256) {
-      const int j = idx / kNb, k = idx - j * kNb;
-      Bs[k * kNb + j] = (k < J.K && c0 + j < J.N) ? J.B[k + int64_t(c0 + j) * J.b_c] : T(0);
+  // A tile: 128 x 64, B tile: 64 x 128, 32 elements each per thread, loaded in
+  // two rounds of 16 with the unit-stride index running across threads.
+  const T sgn = J.op == 0 ? T(-1) : T(1);
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    T v[16];
+    int ii[16], kk[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int e = (half * 16 + u) * 256 + tid;
+      if (J.a_r == 1) {
+        ii[u] = e & (kGT - 1);
+        kk[u] = e >> 7;
+      } else {
+        kk[u] = e & (kNb - 1);
+        ii[u] = e >> 6;
+      }
+      const int32_t gi = min(i0 + ii[u], M - 1), gk = min(kk[u], K - 1);
+      v[u] = J.A[int64_t(gi) * J.a_r + int64_t(gk) * J.a_c];
     }
-  } else {
-    for (int idx = tid; idx < kNb * kNb; idx += 256) {
-      const int k = idx / kNb, j = idx - k * kNb;
-      Bs[k * kNb + j] = (k < J.K && c0 + j < J.N) ? J.B[int64_t(k) * J.b_r + (c0 + j) * J.b_c] : T(0);
+#pragma unroll
+    for (int u = 0; u < 16; ++u) As[kk[u] * kGLd + ii[u]] = (i0 + ii[u] < M && kk[u] < K) ? v[u] : T(0);
+  }
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    T v[16];
+    int jj[16], kk[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int e = (half * 16 + u) * 256 + tid;
+      if (J.b_r == 1) {
+        kk[u] = e & (kNb - 1);
+        jj[u] = e >> 6;
+      } else {
+        jj[u] = e & (kGT - 1);
+        kk[u] = e >> 7;
+      }
+      const int32_t gj = min(c0 + jj[u], N - 1), gk = min(kk[u], K - 1);
+      v[u] = J.B[int64_t(gk) * J.b_r + int64_t(gj) * J.b_c];
     }
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      Bs[kk[u] * kGLd + jj[u]] = (c0 + jj[u] < N && kk[u] < K) ? sgn * v[u] : T(0);
   }
   __syncthreads();
-  const int wave = tid / kWave, lane = tid - wave * kWave;
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-  const int li = lane & 15, lk = lane >> 4;
-  V acc[2][2];
+  for (int k = 0; k < kNb; k += 4) {
+    T af[4], bf[4];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < 4; ++a) af[a] = As[(k + lk) * kGLd + wr + a * 16 + li];
 #pragma unroll
-    for (int c = 0; c < 2; ++c) acc[a][c] = V{T(0), T(0), T(0), T(0)};
-  const int kend = (J.K + 3) & ~3;
-  for (int kk = 0; kk < kend; kk += 4) {
-    T av[2], bv[2];
+    for (int c = 0; c < 4; ++c) bf[c] = Bs[(k + lk) * kGLd + wc + c * 16 + li];
 #pragma unroll
-    for (int a = 0; a < 2; ++a) av[a] = As[(kk + lk) * kNb + wr + a * 16 + li];
+    for (int c = 0; c < 4; ++c)
 #pragma unroll
-    for (int c = 0; c < 2; ++c) bv[c] = Bs[(kk + lk) * kNb + wc + c * 16 + li];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int c = 0; c < 2; ++c) acc[a][c] = MfmaOps<T>::mma(av[a], bv[c], acc[a][c]);
+      for (int a = 0; a < 4; ++a) acc[c][a] = MfmaOps<T>::mma(bf[c], af[a], acc[c][a]);
   }
   T lmax = T(0);
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int c = 0; c < 4; ++c)
 #pragma unroll
-    for (int c = 0; c < 2; ++c)
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int32_t i = i0 + wr + a * 16 + MfmaOps<T>::row(lk, r);
-        const int32_t j = c0 + wc + c * 16 + li;
-        if (i < J.M && j < J.N) {
-          T* cp = J.C + int64_t(i) * J.c_r + int64_t(j) * J.c_c;
-          const T v = acc[a][c][r];
-          if (J.op == 0) {
-            *cp = *cp - v;
-          } else {
-            *cp = v;
-            const T av2 = tabs(v);
-            if (!(av2 <= lmax)) lmax = av2;
-          }
-        }
+        const int32_t i = i0 + wr + a * 16 + li;
+        const int32_t j = c0 + wc + c * 16 + MfmaOps<T>::row(lk, r);
+        const T v = acc[c][a][r];
+        if (i < M && j < N) J.C[i + int64_t(j) * J.c_c] = v;
+        const T av = (i < M && j < N) ? tabs(v) : T(0);
+        lmax = (av <= lmax || lmax != lmax) ? lmax : av;
       }
   if (J.op == 2) {
     int32_t st = !(lmax <= static_cast<T>(limit)) ? kStCheck : 0;
@@ -654,8 +722,8 @@ void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, 
     const int32_t rest = m - p - b;
     if (timer && timer->panel_events) timer_event(timer->panel_events, st);
     hipLaunchKernelGGL(k_diag_lu_inv<T>, dim3(1), dim3(kDiagWaves * kNb), 0, st, S, lds, p, b, limit, status, inv);
+    const int32_t tr = (rest + kGT - 1) / kGT;
     if (rest > 0) {
-      const int32_t tr = (rest + kNb - 1) / kNb;
       TileJob<T> l21{rest, b, b, 2, tr, 1, S + int64_t(p) * lds + p + b, 1, lds, inv + kNb * kNb, kNb, 1,
                      S + int64_t(p) * lds + p + b, 1, lds};
       TileJob<T> u12{b, rest, b, 1, 1, tr, inv, kNb, 1, S + int64_t(p + b) * lds + p, 1, lds,
@@ -664,7 +732,6 @@ void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, 
     }
     if (timer && timer->panel_events) timer_event(timer->panel_events, st);
     if (rest <= 0) break;
-    const int32_t tr = (rest + kNb - 1) / kNb;
     TileJob<T> upd{rest, rest, b, 0, tr, tr, S + int64_t(p) * lds + p + b, 1, lds,
                    S + int64_t(p + b) * lds + p, 1, lds, S + int64_t(p + b) * lds + p + b, 1, lds};
     if (timer && timer->gemm_events) timer_event(timer->gemm_events, st);
@@ -705,7 +772,9 @@ __device__ __forceinline__ double readlane_f64(double v, int src) {
 }
 
 // Stages an s x s column-major factor into LDS with eight loads in flight per
-// lane (the copy is off the dependency chain only if it does not serialize).
+// lane (the copy is off the dependency chain only if it does not serialize:
+// loads are unconditional with a clamped index, since a conditional load
+// becomes a branch that waits for it).
 template <typename T>
 __device__ __forceinline__ void stage_factor(double* __restrict__ fs, const T* __restrict__ f, int ss, int lane) {
   for (int base = 0; base < ss; base += 8 * kWave) {
@@ -713,7 +782,7 @@ __device__ __forceinline__ void stage_factor(double* __restrict__ fs, const T* _
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int idx = base + u * kWave + lane;
-      v[u] = idx < ss ? f[idx] : T(0);
+      v[u] = f[idx < ss ? idx : ss - 1];
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
@@ -728,27 +797,26 @@ __device__ __forceinline__ void stage_factor(double* __restrict__ fs, const T* _
 // workgroup (one wavefront) claims block tickets in order (descending block id,
 // a topological order of the block DAG) and solves block b:
 // rhs = bperm - A_off * xperm (ubtf:2269-2286), then the pivot gather, unit-L
-// forward and U backward substitution (ubtf:2071-2097). Everything
-// that does not depend on other blocks' x (factor to LDS, row gather, the first
-// off-diagonal entries, the next ticket) is issued before waiting, so a block
-// on the critical path pays only for the x loads and the substitution.
-// Deadlock-free with the early claim: the lowest unfinished ticket is always
-// the current block of some workgroup (a claimed, unstarted ticket is higher
-// than its holder's current one), and all its dependencies hold lower tickets.
+// forward and U backward substitution (ubtf:2071-2097). x values are published
+// values (kXSentinel protocol): everything that does not need other blocks' x
+// (factor to LDS, row gather, the first off-diagonal entries, the next ticket)
+// is issued first, then the block polls exactly the x entries its rows use.
+// Deadlock-free: the lowest unfinished ticket is always the current block of
+// some workgroup (a claimed, unstarted ticket is higher than its holder's
+// current one), and every x entry it reads belongs to a lower ticket.
 __global__ __launch_bounds__(64) void k_dense_seg_solve(
     int32_t ntasks, const int32_t* __restrict__ tasks, const int32_t* __restrict__ R,
     const int64_t* __restrict__ lu0_of_block, const double* __restrict__ lu64, const float* __restrict__ lu32,
     const int32_t* __restrict__ prec_of_block, const int32_t* __restrict__ gsrc, const int32_t* __restrict__ off_ptr,
-    const int32_t* __restrict__ off_col, const double* __restrict__ off_val, const int32_t* __restrict__ dep_ptr,
-    const int32_t* __restrict__ dep_blk, const int32_t* __restrict__ seg_of, int32_t seg, int32_t* __restrict__ done,
-    int32_t epoch, int32_t* __restrict__ ticket, const double* __restrict__ bperm, double* __restrict__ xperm,
-    int32_t* __restrict__ err) {
+    const int32_t* __restrict__ off_col, const double* __restrict__ off_val, int32_t* __restrict__ ticket,
+    const double* __restrict__ bperm, double* __restrict__ xperm, int32_t* __restrict__ err) {
   __shared__ double fs[64 * 64];
   constexpr int kPre = 4;
   const int lane = threadIdx.x;
   int32_t tk = 0;
   if (lane == 0) tk = atomicAdd(ticket, 1);
   int32_t t = uniform_from_lane0(tk);
+  bool late = false;
   while (t < ntasks) {
     int32_t tn = 0;
     if (lane == 0) tn = atomicAdd(ticket, 1);
@@ -775,22 +843,15 @@ __global__ __launch_bounds__(64) void k_dense_seg_solve(
       stage_factor(fs, lu64 + lu0_of_block[b], s * s, lane);
     __syncthreads();
     const double rd = act ? 1.0 / fs[lane * s + lane] : 1.0;
-    // --- dependencies inside this segment (others finished in earlier launches) ---
-    int32_t bad = 0;
-    const int32_t d1 = __builtin_amdgcn_readfirstlane(dep_ptr[b + 1]);
-    for (int32_t d = __builtin_amdgcn_readfirstlane(dep_ptr[b]); d < d1; ++d) {
-      const int32_t c = __builtin_amdgcn_readfirstlane(dep_blk[d]);
-      if (seg_of[c] == seg && !wait_flag_uniform(done + c, epoch)) bad = 1;
-    }
-    if (bad) atomicOr(err, 1);  // every lane ORs the same bit
-    double xv[kPre];
+    // --- published x of other blocks: all first polls in flight together ---
+    uint64_t xu[kPre];
 #pragma unroll
-    for (int u = 0; u < kPre; ++u) xv[u] = oc[u] >= 0 ? xperm[oc[u]] : 0.0;
+    for (int u = 0; u < kPre; ++u) xu[u] = oc[u] >= 0 ? ld_x_relaxed(xperm + oc[u]) : 0ull;
     double acc = bv;
 #pragma unroll
     for (int u = 0; u < kPre; ++u)
-      if (oc[u] >= 0) acc = acc - ov[u] * xv[u];
-    for (int32_t p = p0 + kPre; p < p1; ++p) acc = acc - off_val[p] * xperm[off_col[p]];
+      if (oc[u] >= 0) acc = acc - ov[u] * finish_wait_x(xperm + oc[u], xu[u], late);
+    for (int32_t p = p0 + kPre; p < p1; ++p) acc = acc - off_val[p] * wait_x(xperm + off_col[p], late);
     // Row gather (the block's pivoting), then unit-lower and upper substitution.
     acc = __shfl(acc, src);
     if (!act) acc = 0.0;
@@ -803,24 +864,22 @@ __global__ __launch_bounds__(64) void k_dense_seg_solve(
       const double xj = readlane_f64(acc, j);
       if (lane < j) acc = acc - fs[j * s + lane] * xj;
     }
-    if (act) xperm[r] = acc;
-    store_release(done + b, epoch);  // every lane stores the same value
-    __syncthreads();                 // fs is restaged by the next block
+    if (act) st_x_relaxed(xperm + r, acc);
+    __syncthreads();  // fs is restaged by the next block
     t = uniform_from_lane0(tn);
   }
+  if (late) atomicOr(err, 1);
 }
 
 void launch_dense_seg_solve(hipStream_t st, int32_t ntasks, const int32_t* tasks, const int32_t* R,
                             const int64_t* lu0_of_block, const double* lu64, const float* lu32,
                             const int32_t* prec_of_block, const int32_t* gsrc, const int32_t* off_ptr,
-                            const int32_t* off_col, const double* off_val, const int32_t* dep_ptr,
-                            const int32_t* dep_blk, const int32_t* seg_of, int32_t seg, int32_t* done, int32_t epoch,
-                            int32_t* ticket, const double* bperm, double* xperm, int32_t* err) {
+                            const int32_t* off_col, const double* off_val, int32_t* ticket, const double* bperm,
+                            double* xperm, int32_t* err) {
   if (ntasks <= 0) return;
   const int g = ntasks < 256 ? ntasks : 256;
   hipLaunchKernelGGL(k_dense_seg_solve, dim3(g), dim3(64), 0, st, ntasks, tasks, R, lu0_of_block, lu64, lu32,
-                     prec_of_block, gsrc, off_ptr, off_col, off_val, dep_ptr, dep_blk, seg_of, seg, done, epoch,
-                     ticket, bperm, xperm, err);
+                     prec_of_block, gsrc, off_ptr, off_col, off_val, ticket, bperm, xperm, err);
 }
 
 // Sparse block: y[k] = (bperm[r] - A_off(r,:) xperm) / Rs(row), r = start + rowperm[k].
@@ -877,17 +936,17 @@ template void launch_sp_rows<double>(hipStream_t, int32_t, const int32_t*, const
 template void launch_sp_rows<float>(hipStream_t, int32_t, const int32_t*, const int32_t*, const int32_t*,
                                     const int32_t*, const float*, const float*, double*);
 
-// Dense tail triangular solve, sync-free over 64-row panels: a wavefront per
-// panel, panels claimed in dependency order from a ticket, each waits for the
-// flags of the panels it reads. Lower: unit diagonal, forward. Upper: backward.
-// Each 64x64 tile of S is loaded into registers (independent loads) before the
-// wave waits or runs the sequential part, so no step of the substitution
-// chain waits on memory.
+// Dense tail triangular solve (unit-lower forward or upper backward) on the
+// column-major m x m factor, one wavefront per 64-row band, bands claimed in
+// dependency order by ticket. A band accumulates the contributions of earlier
+// bands as their values are published (kXSentinel protocol on yout), with the
+// band's factor tile loaded before the poll, then solves its diagonal tile by
+// substitution and publishes its 64 values. yin and yout must not alias; yout
+// starts filled with kXSentinel.
 template <typename T>
 __global__ __launch_bounds__(64) void k_tail_trsv(int32_t m, const T* __restrict__ S, int64_t lds,
-                                                  double* __restrict__ y, int upper, int32_t* __restrict__ flags,
-                                                  int32_t epoch, int32_t* __restrict__ ticket,
-                                                  int32_t* __restrict__ err) {
+                                                  const double* __restrict__ yin, double* __restrict__ yout, int upper,
+                                                  int32_t* __restrict__ ticket, int32_t* __restrict__ err) {
   const int lane = threadIdx.x;
   const int32_t np = (m + kNb - 1) / kNb;
   int32_t t0 = 0;
@@ -899,8 +958,12 @@ __global__ __launch_bounds__(64) void k_tail_trsv(int32_t m, const T* __restrict
   const int32_t rows = (m - r0) < kNb ? (m - r0) : kNb;
   const int32_t row = r0 + lane;
   const bool live = lane < rows;
-  double acc = live ? y[row] : 0.0;
-  bool ok = true;
+  double acc = live ? yin[row] : 0.0;
+  bool late = false;
+  // Tile loads are unconditional with clamped indices, so all 64 issue back to
+  // back (a per-element condition becomes a branch that waits on each load).
+  // Lanes past the band never store; columns past the matrix meet xv = 0.
+  const int32_t rowc = live ? row : r0;
   T v[kNb];
   for (int32_t q = 0; q < np - 1; ++q) {
     int32_t j;
@@ -914,18 +977,17 @@ __global__ __launch_bounds__(64) void k_tail_trsv(int32_t m, const T* __restrict
     const int32_t c0 = j * kNb;
     const int32_t cols = (m - c0) < kNb ? (m - c0) : kNb;
 #pragma unroll
-    for (int c = 0; c < kNb; ++c) v[c] = (live && c < cols) ? S[int64_t(c0 + c) * lds + row] : T(0);
-    if (!wait_flag_uniform(flags + j, epoch)) ok = false;
-    const double xv = lane < cols ? y[c0 + lane] : 0.0;
+    for (int c = 0; c < kNb; ++c) v[c] = S[int64_t(c0 + (c < cols ? c : cols - 1)) * lds + rowc];
+    const double xv = lane < cols ? wait_x(yout + c0 + lane, late) : 0.0;
 #pragma unroll
-    for (int c = 0; c < kNb; ++c) acc = acc - static_cast<double>(v[c]) * __shfl(xv, c);
+    for (int c = 0; c < kNb; ++c) acc = acc - static_cast<double>(v[c]) * readlane_f64(xv, c);
   }
 #pragma unroll
-  for (int c = 0; c < kNb; ++c) v[c] = (live && c < rows) ? S[int64_t(r0 + c) * lds + row] : T(0);
+  for (int c = 0; c < kNb; ++c) v[c] = S[int64_t(r0 + (c < rows ? c : rows - 1)) * lds + rowc];
   if (!upper) {
 #pragma unroll
     for (int c = 0; c < kNb; ++c) {
-      const double xc = __shfl(acc, c);
+      const double xc = readlane_f64(acc, c);
       if (lane > c) acc = acc - static_cast<double>(v[c]) * xc;
     }
   } else {
@@ -933,26 +995,24 @@ __global__ __launch_bounds__(64) void k_tail_trsv(int32_t m, const T* __restrict
     for (int c = kNb - 1; c >= 0; --c) {
       if (c < rows) {
         if (lane == c) acc = acc / static_cast<double>(v[c]);
-        const double xc = __shfl(acc, c);
+        const double xc = readlane_f64(acc, c);
         if (lane < c) acc = acc - static_cast<double>(v[c]) * xc;
       }
     }
   }
-  if (live) y[row] = acc;
-  if (!ok && lane == 0) atomicOr(err, 2);
-  store_release(flags + w, epoch);  // every lane stores the same value
+  if (live) st_x_relaxed(yout + row, acc);
+  if (late) atomicOr(err, 2);
 }
 template <typename T>
-void launch_tail_trsv(hipStream_t st, int32_t m, const T* S, int64_t lds, double* y, bool upper, int32_t* flags,
-                      int32_t epoch, int32_t* ticket, int32_t* err) {
+void launch_tail_trsv(hipStream_t st, int32_t m, const T* S, int64_t lds, const double* yin, double* yout, bool upper,
+                      int32_t* ticket, int32_t* err) {
   if (m <= 0) return;
   const int np = (m + kNb - 1) / kNb;
-  hipLaunchKernelGGL(k_tail_trsv<T>, dim3(np), dim3(64), 0, st, m, S, lds, y, upper ? 1 : 0, flags, epoch, ticket,
-                     err);
+  hipLaunchKernelGGL(k_tail_trsv<T>, dim3(np), dim3(64), 0, st, m, S, lds, yin, yout, upper ? 1 : 0, ticket, err);
 }
-template void launch_tail_trsv<double>(hipStream_t, int32_t, const double*, int64_t, double*, bool, int32_t*,
-                                       int32_t, int32_t*, int32_t*);
-template void launch_tail_trsv<float>(hipStream_t, int32_t, const float*, int64_t, double*, bool, int32_t*, int32_t,
+template void launch_tail_trsv<double>(hipStream_t, int32_t, const double*, int64_t, const double*, double*, bool,
+                                       int32_t*, int32_t*);
+template void launch_tail_trsv<float>(hipStream_t, int32_t, const float*, int64_t, const double*, double*, bool,
                                       int32_t*, int32_t*);
 
 __global__ void k_sp_scatter_x(int32_t s, int32_t start, const int32_t* __restrict__ q, const double* __restrict__ x,
