@@ -794,8 +794,8 @@ __device__ __forceinline__ void stage_factor(double* __restrict__ fs, const T* _
 
 // Persistent solve of one segment of dense diagonal blocks (a segment is a
 // maximal run of dense blocks between sparse blocks in the solve order). Each
-// workgroup (one wavefront) claims block tickets in order (descending block id,
-// a topological order of the block DAG) and solves block b:
+// workgroup (one wavefront) claims block tickets in order (a topological order
+// of the block DAG: solve level, then descending block id) and solves block b:
 // rhs = bperm - A_off * xperm (ubtf:2269-2286), then the pivot gather, unit-L
 // forward and U backward substitution (ubtf:2071-2097). x values are published
 // values (kXSentinel protocol): everything that does not need other blocks' x
@@ -877,7 +877,7 @@ void launch_dense_seg_solve(hipStream_t st, int32_t ntasks, const int32_t* tasks
                             const int32_t* off_col, const double* off_val, int32_t* ticket, const double* bperm,
                             double* xperm, int32_t* err) {
   if (ntasks <= 0) return;
-  const int g = ntasks < 256 ? ntasks : 256;
+  const int g = ntasks < 1024 ? ntasks : 1024;  // up to four single-wave workgroups per CU
   hipLaunchKernelGGL(k_dense_seg_solve, dim3(g), dim3(64), 0, st, ntasks, tasks, R, lu0_of_block, lu64, lu32,
                      prec_of_block, gsrc, off_ptr, off_col, off_val, ticket, bperm, xperm, err);
 }
