@@ -721,7 +721,7 @@ void oracle_perm(void* h, int32_t* P, int32_t* Q, int32_t* R) {
 // Dense block b: row-major LU (size^2) and pivots; returns size or -1.
 int32_t oracle_dense_factor(void* h, int32_t b, double* lu, int32_t* piv) {
   const auto* s = static_cast<oracle::Solver*>(h);
-  if (b < 0 || b >= s->nblocks || !s->fac[b].dense) return -1;
+  if (b < 0 || b >= static_cast<int32_t>(s->fac.size()) || !s->fac[b].dense) return -1;  // not factored yet
   const auto& f = s->fac[b];
   if (lu) std::copy(f.lu.begin(), f.lu.end(), lu);
   if (piv) std::copy(f.piv.begin(), f.piv.end(), piv);
@@ -730,7 +730,7 @@ int32_t oracle_dense_factor(void* h, int32_t b, double* lu, int32_t* piv) {
 // Sparse (KLU-like) block b: column order q (size) and pivot rows (size).
 int32_t oracle_sparse_order(void* h, int32_t b, int32_t* q, int32_t* prow) {
   const auto* s = static_cast<oracle::Solver*>(h);
-  if (b < 0 || b >= s->nblocks || s->fac[b].dense || s->fac[b].q.empty()) return -1;
+  if (b < 0 || b >= static_cast<int32_t>(s->fac.size()) || s->fac[b].dense || s->fac[b].q.empty()) return -1;
   const auto& f = s->fac[b];
   if (q) std::copy(f.q.begin(), f.q.end(), q);
   if (prow) std::copy(f.prow.begin(), f.prow.end(), prow);

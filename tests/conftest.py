@@ -59,3 +59,30 @@ def backward_error(n, rows, cols, vals, x, b):
     rowsum = np.zeros(n)
     np.add.at(rowsum, rows, np.abs(vals))
     return float(np.max(np.abs(b - y)) / (rowsum.max() * np.max(np.abs(x)) + np.max(np.abs(b))))
+
+
+def check_btf(n, r, c, P, Q, R, nmatch, g, k, merged=False):
+    """Checks a block-triangular ordering against a btf.npz case: structural
+    rank, the unique fine blocks (as row and column sets; with merged=True a
+    block may be a union of whole fine blocks) and the upper block triangular
+    form (no entry below the block diagonal)."""
+    assert nmatch == int(g[f"rank{k}"])
+    assert sorted(P.tolist()) == list(range(n)) and sorted(Q.tolist()) == list(range(n))
+    invP = np.empty(n, np.int64)
+    invP[P] = np.arange(n)
+    invQ = np.empty(n, np.int64)
+    invQ[Q] = np.arange(n)
+    blk = np.searchsorted(R, np.arange(n), side="right") - 1
+    if f"brow{k}" in g:
+        brow, bcol = g[f"brow{k}"], g[f"bcol{k}"]
+        if not merged:
+            assert len(R) - 1 == len(np.unique(brow))
+        for b in range(len(R) - 1):
+            rows = P[R[b]:R[b + 1]]
+            cols = Q[R[b]:R[b + 1]]
+            rid, cid = np.unique(brow[rows]), np.unique(bcol[cols])
+            assert np.array_equal(rid, cid), f"block {b}: rows and columns come from different fine blocks"
+            if not merged:
+                assert len(rid) == 1, f"block {b} mixes fine blocks {rid}"
+            assert sum(np.count_nonzero(brow == i) for i in rid) == len(rows), f"block {b} splits a fine block"
+        assert np.all(blk[invP[r]] <= blk[invQ[c]]), "entry below the block diagonal"

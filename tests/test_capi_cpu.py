@@ -1,0 +1,92 @@
+"""The C-ABI boundary on the CPU: both libraries load and export every function
+declared in include/*.h, and the Solver's host-side contract (validation error
+codes, state errors, loud failure without a GPU) matches the reference's
+include/blockdaglu/solver.h and apps/tests/main.cpp. No device compute."""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _declared(header: Path):
+    text = header.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    names = set(re.findall(r"\b(bdlu_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_headers_declare_functions():
+    assert len(_declared(ROOT / "include" / "bdlu_gpu.h")) >= 10
+    assert len(_declared(ROOT / "include" / "bdlu_host.h")) >= 10
+
+
+def test_libraries_export_every_declared_symbol(bd):
+    L = bd.lib()  # libblockdaglu.so (host) with libbdlu_gpu.so loaded as L.gpu
+    missing = []
+    for name in _declared(ROOT / "include" / "bdlu_gpu.h"):
+        if not hasattr(L.gpu, name):
+            missing.append(("libbdlu_gpu.so", name))
+    for name in _declared(ROOT / "include" / "bdlu_host.h"):
+        if not hasattr(L, name):
+            missing.append(("libblockdaglu.so", name))
+    assert not missing, missing
+
+
+def test_native_libraries_are_in_tree(bd):
+    L = bd.lib()
+    for lib in (L, L.gpu):
+        path = Path(lib._name).resolve()
+        assert path.parent == (ROOT / "block-daglu_amd" / "lib").resolve(), path
+
+
+@pytest.mark.parametrize("entries,dim,err", [
+    ([(-1, 0, 1.0)], -1, "InvalidIndex"),             # tests:94-97
+    ([(0, 0, float("inf"))], -1, "NonFiniteValue"),   # tests:99-102
+    ([(0, 0, float("nan"))], -1, "NonFiniteValue"),
+    ([(2, 2, 1.0)], 2, "InvalidIndex"),               # tests:104-107
+    ([(0, -3, 1.0)], -1, "InvalidIndex"),
+])
+def test_validation_errors_at_analyze(bd, entries, dim, err):
+    # set_matrix stays storage-only; analyze validates (src/solver.cpp:642-722).
+    s = bd.Solver()
+    assert s.set_matrix(entries, dimension=dim)
+    assert not s.analyze()
+    assert s.last_error() == bd.SolverError[err]
+    assert s.last_error_message()
+
+
+def test_state_errors(bd):
+    s = bd.Solver()
+    assert not s.factor()
+    assert s.last_error() == bd.SolverError.NotPrepared
+    x = np.ones(3)
+    assert not s.solve_in_place(x)
+    assert s.last_error() == bd.SolverError.NotPrepared
+    assert s.selected_mode() == "BLOCK"
+
+
+def test_no_gpu_fails_loudly(bd):
+    """Without a visible GPU the BLOCK engine reports BackendUnavailable; there is
+    no CPU fallback path in the product."""
+    if bd.gpu_available():
+        pytest.skip("a GPU is visible")
+    s = bd.Solver()
+    assert s.set_matrix([(0, 0, 2.0), (1, 1, 3.0)])
+    assert not s.analyze()
+    assert s.last_error() == bd.SolverError.BackendUnavailable
+
+
+def test_error_enum_matches_reference_order(bd):
+    # include/blockdaglu/solver.h:19-31 of the reference: same names, same order
+    ref = ["None", "NotPrepared", "InvalidDimension", "InvalidIndex", "NonFiniteValue", "InvalidRightHandSide",
+           "BackendUnavailable", "AnalysisFailed", "FactorizationFailed", "SolveFailed", "ExportFailed"]
+    names = [e.name.rstrip("_") for e in sorted(bd.SolverError, key=int)]
+    assert names == ref
+    assert [int(e) for e in sorted(bd.SolverError, key=int)] == list(range(len(ref)))
