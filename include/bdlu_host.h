@@ -1,5 +1,5 @@
 #pragma once
-// C-ABI of the host side (libblockdaglu_host.so): the blockdaglu::Solver
+// C-ABI of the host side (libblockdaglu.so): the blockdaglu::Solver
 // pipeline over plain pointers, read-only queries of the host analysis, and the
 // seeded workload generator. All functions are extern "C", never throw, and
 // report failure through return values.
