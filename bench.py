@@ -32,7 +32,58 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "refactor+solve/s, p50/p99 latency; FP32+IR vs FP64 GPU at rel-resid≤1e-12"
 HBM_PEAK_GBS = 8000.0
+COPY_BYTES = 1 << 30  # the plain copy that calibrates the HBM byte counters
 MFMA_PEAK = {"f64": 78.6, "f32": 157.3}  # TFLOP/s dense
+
+
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_tail_update_circuit200k.json"
+ROOFLINE_KERNEL = "k_tail_update<float>"
+
+
+def _db_view(db, name):
+    return name if any(r[0] == name for r in db.execute("select name from sqlite_master")) else None
+
+
+def profile_summary(db_path: str) -> str:
+    """Markdown table of a rocprofv3 --kernel-trace database: per kernel, calls and
+    GPU time (all dispatches of the run)."""
+    import sqlite3
+    db = sqlite3.connect(db_path)
+    rows = list(db.execute(
+        "select s.kernel_name, count(*), sum(d.end - d.start), min(d.end - d.start), max(d.end - d.start) "
+        "from rocpd_kernel_dispatch d join rocpd_info_kernel_symbol s on d.kernel_id = s.id "
+        "group by s.kernel_name order by sum(d.end - d.start) desc"))
+    total = sum(r[2] for r in rows) or 1
+    out = ["| kernel | calls | total ms | avg us | min us | max us | share |", "|---|---:|---:|---:|---:|---:|---:|"]
+    for name, cnt, tot, mn, mx in rows:
+        short = name.split("(")[0].replace("void ", "")
+        out.append(f"| `{short}` | {cnt} | {tot / 1e6:.2f} | {tot / cnt / 1e3:.2f} | {mn / 1e3:.2f} | {mx / 1e3:.2f} | "
+                   f"{100.0 * tot / total:.1f}% |")
+    return "\n".join(out)
+
+
+def traffic_from_counters(fetch_db: str, write_db: str) -> dict:
+    """HBM bytes per dispatch of the roofline kernel from two rocprofv3 --pmc runs
+    (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by what the same
+    counter reads for the plain 1 GiB copy that --copy-calibration runs."""
+    import sqlite3
+    out = {}
+    for which, path in (("FETCH_SIZE", fetch_db), ("WRITE_SIZE", write_db)):
+        db = sqlite3.connect(path)
+        # the calibration copies are the run's first three copy dispatches
+        cal = [v for (v,) in db.execute(
+            "select value from counters_collection where kernel_name like '%copyBuffer%' and counter_name = ? "
+            "order by dispatch_id limit 3", (which,))]
+        if not cal:
+            raise RuntimeError(f"no {COPY_BYTES}-byte copy dispatch in {path} (run bench.py with --copy-calibration)")
+        factor = COPY_BYTES / (float(np.median(cal)) * 1024.0)
+        vals = [v for (name, v) in db.execute(
+            "select kernel_name, value from counters_collection where counter_name = ?", (which,))
+            if "k_tail_update<float>" in name]
+        out[which] = {"copy_counter_kib": float(np.median(cal)), "factor": factor, "dispatches": len(vals),
+                      "mean_kib": float(np.mean(vals)), "mean_bytes_scaled": float(np.mean(vals)) * 1024.0 * factor}
+    out["bytes_per_launch"] = out["FETCH_SIZE"]["mean_bytes_scaled"] + out["WRITE_SIZE"]["mean_bytes_scaled"]
+    return out
 
 
 def load_package():
@@ -155,7 +206,23 @@ def main():
     ap.add_argument("--config", type=int, default=1, help="workload index (1: circuit-like n=200,000)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--profile-summary", metavar="DB", help="print the kernel table of a rocprofv3 database and exit")
+    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
+                    help="compute HBM bytes per launch of the roofline kernel from two --pmc runs, write "
+                         "profiles/traffic_tail_update_circuit200k.json and exit")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 1 GiB device-to-device copy (the known-size reference kernel for "
+                         "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs)")
     args = ap.parse_args()
+    if args.profile_summary:
+        print(profile_summary(args.profile_summary))
+        return
+    if args.traffic:
+        t = traffic_from_counters(*args.traffic)
+        t["kernel"] = ROOFLINE_KERNEL
+        TRAFFIC_FILE.write_text(json.dumps(t, indent=1) + "\n")
+        print(json.dumps(t, indent=1))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build's path has no multi-GPU mode (see the sharding note "
               "in the design document); run with --gpus 1", file=sys.stderr)
@@ -165,6 +232,14 @@ def main():
     if not bd.gpu_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_calibration:
+        x = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
+        log(f"copy calibration: 3 x {COPY_BYTES} B device-to-device")
     t = time.time()
     n, r, c, v, B = bd.generate(args.config)
     b = B[:, 0].copy()
@@ -206,7 +281,10 @@ def main():
         else:
             roof = {"bound": "mfma", "achieved": tf, "peak": MFMA_PEAK["f32"], "unit": "TFLOP/s",
                     "frac": tf / MFMA_PEAK["f32"]}
-        roof.update({"traffic": None, "kernel": "k_gemm_update<float>", "avg_launch_us": avg_s * 1e6,
+        traffic = None
+        if TRAFFIC_FILE.exists():
+            traffic = json.loads(TRAFFIC_FILE.read_text())["bytes_per_launch"]
+        roof.update({"traffic": traffic, "kernel": ROOFLINE_KERNEL, "avg_launch_us": avg_s * 1e6,
                      "launches_per_step": g["launches"] / args.steps, "algorithmic_bytes_per_launch": byts,
                      "algorithmic_flops_per_launch": flops, "tflops": tf, "gbs": gbs})
     cpu = None

@@ -389,9 +389,9 @@ template void launch_sp_tail<float>(hipStream_t, const SpView&, const double*, c
 //   1. k_diag_lu_inv: LU of the 64x64 diagonal block (one 1024-thread
 //      workgroup, register tiles, one barrier per step), then L11^{-1} and
 //      U11^{-1};
-//   2. k_panel_gemm: L21 = A21 U11^{-1} and U12 = L11^{-1} A12 as MFMA tile
+//   2. k_tail_panel: L21 = A21 U11^{-1} and U12 = L11^{-1} A12 as MFMA tile
 //      products (one launch, both jobs), with the |L| growth check;
-//   3. k_panel_gemm: trailing update A22 -= L21 U12 on MFMA.
+//   3. k_tail_update: trailing update A22 -= L21 U12 on MFMA.
 constexpr int kNb = 64;
 
 // inv[0..4095]: L11^{-1} row-major (unit lower); inv[4096..8191]: U11^{-1}
@@ -593,9 +593,9 @@ struct TileJob {
 // j0.tiles_m*j0.tiles_n run job 0, the rest job 1.
 constexpr int kGT = 128;       // output tile edge
 constexpr int kGLd = kGT + 1;  // LDS row length (padded against bank conflicts)
-template <typename T>
-__global__ __launch_bounds__(256) void k_panel_gemm(TileJob<T> j0, TileJob<T> j1, double limit,
-                                                    int32_t* __restrict__ status) {
+template <typename T, bool kUpdate>
+__device__ __forceinline__ void tile_gemm(const TileJob<T>& j0, const TileJob<T>& j1, double limit,
+                                          int32_t* __restrict__ status) {
   using V = typename MfmaOps<T>::v4;
   __shared__ T As[kNb * kGLd];  // As[k*kGLd + i] = A(i0 + i, k)
   __shared__ T Bs[kNb * kGLd];  // Bs[k*kGLd + j] = +-B(k, c0 + j)
@@ -610,7 +610,7 @@ __global__ __launch_bounds__(256) void k_panel_gemm(TileJob<T> j0, TileJob<T> j1
   const int li = lane & 15, lk = lane >> 4;
   const int32_t M = J.M, N = J.N, K = J.K;
   V acc[4][4];  // acc[c][a]: C^T block (j-block c, i-block a)
-  if (J.op == 0) {
+  if (kUpdate) {
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
@@ -629,7 +629,7 @@ __global__ __launch_bounds__(256) void k_panel_gemm(TileJob<T> j0, TileJob<T> j1
   }
   // A tile: 128 x 64, B tile: 64 x 128, 32 elements each per thread, loaded in
   // two rounds of 16 with the unit-stride index running across threads.
-  const T sgn = J.op == 0 ? T(-1) : T(1);
+  const T sgn = kUpdate ? T(-1) : T(1);
 #pragma unroll
   for (int half = 0; half < 2; ++half) {
     T v[16];
@@ -697,11 +697,22 @@ __global__ __launch_bounds__(256) void k_panel_gemm(TileJob<T> j0, TileJob<T> j1
         const T av = (i < M && j < N) ? tabs(v) : T(0);
         lmax = (av <= lmax || lmax != lmax) ? lmax : av;
       }
-  if (J.op == 2) {
+  if (!kUpdate && J.op == 2) {
     int32_t st = !(lmax <= static_cast<T>(limit)) ? kStCheck : 0;
     for (int off = 32; off > 0; off >>= 1) st |= __shfl_xor(st, off);
     if (lane == 0 && st) atomicOr(status, st);
   }
+}
+// Trailing update (op 0 only) and the panel pair (ops 1 and 2): two kernels so
+// profiles and counters separate them.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tail_update(TileJob<T> j, double limit, int32_t* __restrict__ status) {
+  tile_gemm<T, true>(j, j, limit, status);
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_tail_panel(TileJob<T> j0, TileJob<T> j1, double limit,
+                                                    int32_t* __restrict__ status) {
+  tile_gemm<T, false>(j0, j1, limit, status);
 }
 
 namespace {
@@ -728,14 +739,14 @@ void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, 
                      S + int64_t(p) * lds + p + b, 1, lds};
       TileJob<T> u12{b, rest, b, 1, 1, tr, inv, kNb, 1, S + int64_t(p + b) * lds + p, 1, lds,
                      S + int64_t(p + b) * lds + p, 1, lds};
-      hipLaunchKernelGGL(k_panel_gemm<T>, dim3(2 * tr), dim3(256), 0, st, l21, u12, limit, status);
+      hipLaunchKernelGGL(k_tail_panel<T>, dim3(2 * tr), dim3(256), 0, st, l21, u12, limit, status);
     }
     if (timer && timer->panel_events) timer_event(timer->panel_events, st);
     if (rest <= 0) break;
     TileJob<T> upd{rest, rest, b, 0, tr, tr, S + int64_t(p) * lds + p + b, 1, lds,
                    S + int64_t(p + b) * lds + p, 1, lds, S + int64_t(p + b) * lds + p + b, 1, lds};
     if (timer && timer->gemm_events) timer_event(timer->gemm_events, st);
-    hipLaunchKernelGGL(k_panel_gemm<T>, dim3(tr * tr), dim3(256), 0, st, upd, upd, limit, status);
+    hipLaunchKernelGGL(k_tail_update<T>, dim3(tr * tr), dim3(256), 0, st, upd, limit, status);
     if (timer && timer->gemm_events) {
       timer_event(timer->gemm_events, st);
       const double r = rest;
@@ -843,6 +854,12 @@ __global__ __launch_bounds__(64) void k_dense_seg_solve(
       stage_factor(fs, lu64 + lu0_of_block[b], s * s, lane);
     __syncthreads();
     const double rd = act ? 1.0 / fs[lane * s + lane] : 1.0;
+    // Each lane's row of L\U into registers (clamped, unconditional LDS reads):
+    // the substitution chain below then runs on registers and readlane only.
+    double row[64];
+    const int li = lane < s ? lane : s - 1;
+#pragma unroll
+    for (int j = 0; j < 64; ++j) row[j] = fs[(j < s ? j : s - 1) * s + li];
     // --- published x of other blocks: all first polls in flight together ---
     uint64_t xu[kPre];
 #pragma unroll
@@ -855,14 +872,22 @@ __global__ __launch_bounds__(64) void k_dense_seg_solve(
     // Row gather (the block's pivoting), then unit-lower and upper substitution.
     acc = __shfl(acc, src);
     if (!act) acc = 0.0;
-    for (int j = 0; j < s; ++j) {
-      const double xj = readlane_f64(acc, j);
-      if (lane > j && act) acc = acc - fs[j * s + lane] * xj;
+#pragma unroll
+    for (int j = 0; j < 64; ++j) {
+      if (j < s) {
+        const double xj = readlane_f64(acc, j);
+        const double t = __builtin_fma(-row[j], xj, acc);
+        acc = (lane > j && act) ? t : acc;
+      }
     }
-    for (int j = s - 1; j >= 0; --j) {
-      if (lane == j) acc = acc * rd;
-      const double xj = readlane_f64(acc, j);
-      if (lane < j) acc = acc - fs[j * s + lane] * xj;
+#pragma unroll
+    for (int j = 63; j >= 0; --j) {
+      if (j < s) {
+        acc = (lane == j) ? acc * rd : acc;
+        const double xj = readlane_f64(acc, j);
+        const double t = __builtin_fma(-row[j], xj, acc);
+        acc = (lane < j) ? t : acc;
+      }
     }
     if (act) st_x_relaxed(xperm + r, acc);
     __syncthreads();  // fs is restaged by the next block
