@@ -198,12 +198,107 @@ def cpu_baseline(bd, n, r, c, v, b, budget_s):
             "rel_residual": rel_residual(n, r, c, v, x, b), "seconds": total}
 
 
+def bench_serving(bd, args):
+    """Serving stream: 16 seeded patterns (config 4), 2000 requests with Poisson
+    arrivals at a fixed rate (0.8 x the measured FP64 capacity, the same rate for
+    both precisions), each a +-5% value perturbation of its pattern with 1-8
+    right-hand sides; 4 slots, 64 GB HBM budget."""
+    import threading
+    rng = np.random.default_rng(4)
+    pats = []
+    t = time.time()
+    for p in range(16):
+        n, r, c, v, _ = bd.generate(4, pattern=p)
+        vals = [bd.perturb(v, seed=40_000 + 16 * p + i) for i in range(6)]
+        rhs = np.asfortranarray(rng.uniform(-1, 1, (n, 8)))
+        pats.append((n, r, c, v, vals, rhs))
+    log(f"serving: 16 patterns n={[p[0] for p in pats]} generated in {time.time() - t:.1f}s")
+    nreq = args.requests
+    req_pat = rng.integers(0, 16, nreq)
+    req_val = rng.integers(0, 6, nreq)
+    req_k = rng.integers(1, 9, nreq)
+    gaps = rng.exponential(1.0, nreq)  # scaled by 1/lambda below
+    out = {}
+    lam = None
+    for prec in (bd.Precision.FP64, bd.Precision.FP32_IR):
+        srv = bd.Server(slots=args.slots, hbm_budget_gb=64.0, precision=prec)
+        t = time.time()
+        pid = [srv.add_pattern(p[1], p[2], p[3], p[0]) for p in pats]
+        t_an = time.time() - t
+        # warm every slot on every pattern (plan upload, first factor)
+        ids = [srv.submit(pid[p], pats[p][4][0], pats[p][5][:, :1]) for p in range(16) for _ in range(args.slots)]
+        for i in ids:
+            srv.wait(i)
+        # capacity: closed loop, all requests queued at once
+        ncap = min(nreq, 400)
+        t = time.perf_counter()
+        ids = [srv.submit(pid[req_pat[i]], pats[req_pat[i]][4][req_val[i]], pats[req_pat[i]][5][:, :req_k[i]])
+               for i in range(ncap)]
+        for i in ids:
+            srv.wait(i)
+        cap = ncap / (time.perf_counter() - t)
+        if lam is None:
+            lam = 0.8 * cap
+        log(f"  {prec.name}: analysis {t_an:.1f}s, closed-loop capacity {cap:.1f} req/s; open loop at {lam:.1f} req/s")
+        # open loop
+        arrivals = np.cumsum(gaps / lam)
+        ids = [None] * nreq
+
+        def feeder():
+            t0 = time.perf_counter()
+            for i in range(nreq):
+                d = arrivals[i] - (time.perf_counter() - t0)
+                if d > 0:
+                    time.sleep(d)
+                p = req_pat[i]
+                ids[i] = srv.submit(pid[p], pats[p][4][req_val[i]], pats[p][5][:, :req_k[i]])
+
+        th = threading.Thread(target=feeder)
+        t = time.perf_counter()
+        th.start()
+        res = []
+        for i in range(nreq):
+            while ids[i] is None:
+                time.sleep(0.0002)
+            X, rr = srv.wait(ids[i])
+            res.append(rr)
+            if (i + 1) % 500 == 0:
+                log(f"    {i + 1}/{nreq} done")
+        th.join()
+        wall = time.perf_counter() - t
+        lat = np.array([r["t_done_ms"] - r["t_submit_ms"] for r in res])
+        errs = [r for r in res if r["error"] != bd.SolverError.None_]
+        out[prec.name] = {
+            "capacity_req_per_s": cap, "offered_req_per_s": lam, "served_req_per_s": nreq / wall,
+            "solves_per_s": float(req_k.sum()) / wall, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "max_rel_residual": max(r["rel_residual"] for r in res),
+            "max_backward_error": max(r["backward_error"] for r in res),
+            "max_ir_iterations": max(r["ir_iterations"] for r in res),
+            "fp64_fallback_blocks": int(sum(r["fp64_fallback_blocks"] for r in res)),
+            "repivots": int(sum(r["repivots"] for r in res)), "errors": len(errs), "analysis_s": t_an}
+        log(f"  {prec.name}: p50 {out[prec.name]['p50_ms']:.2f} ms p99 {out[prec.name]['p99_ms']:.2f} ms "
+            f"served {out[prec.name]['served_req_per_s']:.1f} req/s errors {len(errs)}")
+        srv.close()
+    line = {"metric": "served refactor+solve/s, p50/p99 latency (serving stream, 16 patterns)",
+            "value": out["FP32_IR"]["capacity_req_per_s"], "unit": "refactor+solve/s (closed-loop capacity)",
+            "n_gpus": 1, "higher_is_better": True, "dtype": "f32 factors + f64 refinement",
+            "data": "synthetic (seeded generator)",
+            "config": {"workload": f"serving: 16 patterns n=1e3..1e5, {nreq} Poisson requests at "
+                                   f"{lam:.1f}/s, 1-8 RHS, {args.slots} slots, 64 GB budget"},
+            "fp32ir": out["FP32_IR"], "fp64": out["FP64"],
+            "fp32ir_over_fp64_capacity": out["FP32_IR"]["capacity_req_per_s"] / out["FP64"]["capacity_req_per_s"]}
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", type=int, default=1, help="workload index (1: circuit-like n=200,000)")
+    ap.add_argument("--config", type=int, default=1,
+                    help="workload: 1 circuit-like n=200,000 (default); 4 serving stream")
+    ap.add_argument("--requests", type=int, default=2000, help="serving stream length (config 4)")
+    ap.add_argument("--slots", type=int, default=4, help="serving slots (config 4)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--profile-summary", metavar="DB", help="print the kernel table of a rocprofv3 database and exit")
@@ -240,6 +335,9 @@ def main():
         torch.cuda.synchronize()
         del x, y
         log(f"copy calibration: 3 x {COPY_BYTES} B device-to-device")
+    if args.config == 4:
+        bench_serving(bd, args)
+        return
     t = time.time()
     n, r, c, v, B = bd.generate(args.config)
     b = B[:, 0].copy()

@@ -68,6 +68,13 @@ class GenSpec(C.Structure):
                 ("scale_seed", C.c_uint64)]
 
 
+class _ServeResult(C.Structure):
+    _fields_ = [("error", C.c_int32), ("slot", C.c_int32), ("ir_iterations", C.c_int32),
+                ("fp64_fallback_blocks", C.c_int32), ("repivots", C.c_int32), ("rel_residual", C.c_double),
+                ("backward_error", C.c_double), ("t_submit_ms", C.c_double), ("t_start_ms", C.c_double),
+                ("t_done_ms", C.c_double)]
+
+
 _P = C.c_void_p
 _I32P = C.POINTER(C.c_int32)
 _F64P = C.POINTER(C.c_double)
@@ -94,6 +101,12 @@ HOST_SYMBOLS = [
     ("bdlu_solver_nblocks", C.c_int32, [_P]),
     ("bdlu_solver_copy_perm", None, [_P, _I32P, _I32P, _I32P]),
     ("bdlu_solver_export", C.c_int64, [_P, C.c_int, _I32P, _I32P, _F64P]),
+    ("bdlu_serve_create", _P, [C.c_int, C.c_int, C.c_uint64, C.c_int]),
+    ("bdlu_serve_destroy", None, [_P]),
+    ("bdlu_serve_add_pattern", C.c_int32, [_P, C.c_int64, _I32P, _I32P, _F64P, C.c_int32]),
+    ("bdlu_serve_pattern_n", C.c_int32, [_P, C.c_int32]),
+    ("bdlu_serve_submit", C.c_int64, [_P, C.c_int32, _F64P, C.c_int64, _F64P, C.c_int32]),
+    ("bdlu_serve_wait", C.c_int, [_P, C.c_int64, _F64P, C.POINTER(_ServeResult)]),
     ("bdlu_host_analyze", _P, [C.c_int64, _I32P, _I32P, _F64P, C.c_int32, C.POINTER(C.c_int)]),
     ("bdlu_host_free", None, [_P]),
     ("bdlu_host_info", None, [_P, _I32P, _I32P, _I32P, _I32P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _I32P]),
@@ -294,6 +307,60 @@ class Solver:
 
     def getU(self):
         return self._export(4)
+
+
+class Server:
+    """Serving loop (include/blockdaglu/serve.h): registered patterns, plans
+    resident per slot, requests admitted in arrival order as slots free up, each
+    slot on its own stream so in-flight requests overlap on the GPU."""
+
+    def __init__(self, slots: int = 4, hbm_budget_gb: float = 64.0, precision: Precision = Precision.FP64,
+                 device: int = 0):
+        self._L = lib()
+        self._h = self._L.bdlu_serve_create(device, slots, int(hbm_budget_gb * (1 << 30)), int(precision))
+        if not self._h:
+            raise RuntimeError("bdlu_serve_create failed")
+        self._n = {}
+        self._nnz = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.bdlu_serve_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def add_pattern(self, rows, cols, vals, dimension: int = -1) -> int:
+        r, c, v = _i32(rows), _i32(cols), _f64(vals)
+        pid = self._L.bdlu_serve_add_pattern(self._h, len(r), _ptr(r, _I32P), _ptr(c, _I32P), _ptr(v, _F64P),
+                                            dimension)
+        if pid < 0:
+            raise RuntimeError(f"add_pattern failed: {SolverError(-pid).name}")
+        self._n[pid] = self._L.bdlu_serve_pattern_n(self._h, pid)
+        self._nnz[pid] = len(r)
+        return pid
+
+    def submit(self, pattern: int, vals, B) -> int:
+        v = _f64(vals)
+        b = np.asfortranarray(B, dtype=np.float64)
+        k = 1 if b.ndim == 1 else b.shape[1]
+        rid = self._L.bdlu_serve_submit(self._h, pattern, _ptr(v, _F64P), len(v), _ptr(b, _F64P), k)
+        if rid < 0:
+            raise RuntimeError(f"submit failed: {SolverError(-rid).name}")
+        self._pending = getattr(self, "_pending", {})
+        self._pending[rid] = (self._n[pattern], k, b.ndim)
+        return rid
+
+    def wait(self, request: int):
+        """-> (X, result dict). X has the shape of the submitted B."""
+        n, k, nd = self._pending.pop(request)
+        X = np.zeros((n, k), np.float64, order="F")
+        res = _ServeResult()
+        self._L.bdlu_serve_wait(self._h, request, _ptr(X, _F64P), C.byref(res))
+        out = {f: getattr(res, f) for f, _ in _ServeResult._fields_}
+        out["error"] = SolverError(out["error"])
+        return (X[:, 0] if nd == 1 else X), out
 
 
 class HostAnalysis:

@@ -785,10 +785,11 @@ int bdlu_gpu_solve(bdlu_ctx* c, uint64_t id, double* B, int32_t nrhs, int64_t ld
           ++iters;
           eval();
         }
+        // Worst right-hand side of the call.
         const double bn2 = std::sqrt(h[3]);
-        ss.rel_residual = bn2 > 0 ? std::sqrt(h[0]) / bn2 : 0.0;
+        ss.rel_residual = std::max(ss.rel_residual, bn2 > 0 ? std::sqrt(h[0]) / bn2 : 0.0);
         const double den = h[8] * h[2] + h[4];
-        ss.backward_error = den > 0 ? h[1] / den : 0.0;
+        ss.backward_error = std::max(ss.backward_error, den > 0 ? h[1] / den : 0.0);
         ss.iterations = std::max(ss.iterations, iters);
       }
       HIPCHECK(hipMemcpyAsync(col, P.x, sizeof(double) * P.n, out, st));

@@ -5,12 +5,25 @@
 #include <vector>
 
 #include "../../../include/bdlu_host.h"
+#include "blockdaglu/serve.h"
 #include "blockdaglu/solver.h"
 #include "host.h"
 #include "plan.h"
 
 using blockdaglu::CooMatrix;
 using blockdaglu::Solver;
+
+namespace blockdaglu {
+class ServeEngine;
+struct ServeResult;
+ServeEngine* serve_engine_create(int device, int slots, uint64_t budget, int precision);
+void serve_engine_destroy(ServeEngine* e);
+int32_t serve_engine_add_pattern(ServeEngine* e, const CooMatrix& m, int32_t dim);
+int32_t serve_engine_pattern_n(ServeEngine* e, int32_t p);
+int64_t serve_engine_submit(ServeEngine* e, int32_t p, const double* v, int64_t nv, const double* B, int32_t k);
+bool serve_engine_wait(ServeEngine* e, int64_t id, double* X, ServeResult& res);
+SolverError serve_engine_last_error(ServeEngine* e);
+}  // namespace blockdaglu
 
 extern "C" {
 
@@ -234,6 +247,70 @@ void bdlu_host_signatures(void* hp, uint64_t* sig, uint64_t* sig2, uint64_t* val
   if (sig) *sig = bdlu::matrix_sig(h->canon.n, h->canon);
   if (sig2) *sig2 = bdlu::matrix_sig2(h->canon.n, h->canon);
   if (value_sig) *value_sig = bdlu::matrix_value_sig(h->canon);
+}
+
+}  // extern "C"
+
+// ---- serving loop ------------------------------------------------------------
+extern "C" {
+
+void* bdlu_serve_create(int device, int slots, uint64_t hbm_budget_bytes, int precision) {
+  try {
+    return blockdaglu::serve_engine_create(device, slots, hbm_budget_bytes, precision);
+  } catch (...) {
+    return nullptr;
+  }
+}
+
+void bdlu_serve_destroy(void* s) { blockdaglu::serve_engine_destroy(static_cast<blockdaglu::ServeEngine*>(s)); }
+
+int32_t bdlu_serve_add_pattern(void* s, int64_t nnz, const int32_t* rows, const int32_t* cols, const double* vals,
+                               int32_t dimension) {
+  if (!s || nnz < 0 || (nnz > 0 && (!rows || !cols || !vals))) return -static_cast<int32_t>(BDLU_ERR_INVALID_INDEX);
+  auto* e = static_cast<blockdaglu::ServeEngine*>(s);
+  try {
+    CooMatrix m;
+    m.reserve(static_cast<std::size_t>(nnz));
+    for (int64_t k = 0; k < nnz; ++k) m.emplace_back(rows[k], cols[k], vals[k]);
+    const int32_t id = blockdaglu::serve_engine_add_pattern(e, m, dimension);
+    return id >= 0 ? id : -static_cast<int32_t>(blockdaglu::serve_engine_last_error(e));
+  } catch (...) {
+    return -static_cast<int32_t>(BDLU_ERR_ANALYSIS_FAILED);
+  }
+}
+
+int32_t bdlu_serve_pattern_n(void* s, int32_t pattern) {
+  return s ? blockdaglu::serve_engine_pattern_n(static_cast<blockdaglu::ServeEngine*>(s), pattern) : -1;
+}
+
+int64_t bdlu_serve_submit(void* s, int32_t pattern, const double* vals, int64_t nnz, const double* B, int32_t nrhs) {
+  if (!s || !vals || !B) return -static_cast<int64_t>(BDLU_ERR_INVALID_RHS);
+  auto* e = static_cast<blockdaglu::ServeEngine*>(s);
+  try {
+    const int64_t id = blockdaglu::serve_engine_submit(e, pattern, vals, nnz, B, nrhs);
+    return id >= 0 ? id : -static_cast<int64_t>(blockdaglu::serve_engine_last_error(e));
+  } catch (...) {
+    return -static_cast<int64_t>(BDLU_ERR_BACKEND_UNAVAILABLE);
+  }
+}
+
+int bdlu_serve_wait(void* s, int64_t request, double* X, bdlu_serve_result* out) {
+  if (!s) return BDLU_ERR_INVALID_INDEX;
+  blockdaglu::ServeResult r;
+  blockdaglu::serve_engine_wait(static_cast<blockdaglu::ServeEngine*>(s), request, X, r);
+  if (out) {
+    out->error = static_cast<int32_t>(r.error);
+    out->slot = r.slot;
+    out->ir_iterations = r.ir_iterations;
+    out->fp64_fallback_blocks = r.fp64_fallback_blocks;
+    out->repivots = r.repivots;
+    out->rel_residual = r.rel_residual;
+    out->backward_error = r.backward_error;
+    out->t_submit_ms = r.t_submit_ms;
+    out->t_start_ms = r.t_start_ms;
+    out->t_done_ms = r.t_done_ms;
+  }
+  return static_cast<int>(r.error);
 }
 
 }  // extern "C"

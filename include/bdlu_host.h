@@ -74,6 +74,33 @@ void bdlu_solver_copy_perm(void* s, int32_t* P, int32_t* Q, int32_t* R);
 // which: 0 rearranged, 1 P, 2 Q, 3 L, 4 U. Returns entry count or -1 on failure.
 int64_t bdlu_solver_export(void* s, int which, int32_t* rows, int32_t* cols, double* vals);
 
+// ---- serving loop (blockdaglu/serve.h) -----------------------------------------
+// No counterpart in the reference: for callers that serve many refactor+solve
+// requests on registered patterns (Newton-like loops, circuit simulators). Slots
+// run concurrently on the device, each with its own stream and resident plans;
+// requests are admitted in arrival order as slots free up, within an HBM budget.
+typedef struct bdlu_serve_result {
+  int32_t error;                 // SolverError value (0 = success)
+  int32_t slot;
+  int32_t ir_iterations;
+  int32_t fp64_fallback_blocks;
+  int32_t repivots;
+  double rel_residual;           // worst right-hand side of the request
+  double backward_error;         // worst right-hand side of the request
+  double t_submit_ms, t_start_ms, t_done_ms;  // server clock
+} bdlu_serve_result;
+void* bdlu_serve_create(int device, int slots, uint64_t hbm_budget_bytes, int precision /* 0 FP64, 1 FP32+IR */);
+void bdlu_serve_destroy(void* s);
+// Returns the pattern id (>= 0) or -(SolverError).
+int32_t bdlu_serve_add_pattern(void* s, int64_t nnz, const int32_t* rows, const int32_t* cols, const double* vals,
+                               int32_t dimension);
+int32_t bdlu_serve_pattern_n(void* s, int32_t pattern);
+// values: nnz entries in the registration's entry order; B: n x nrhs column-major.
+// Returns a request id (>= 0) or -(SolverError).
+int64_t bdlu_serve_submit(void* s, int32_t pattern, const double* vals, int64_t nnz, const double* B, int32_t nrhs);
+// Blocks until done; X (n x nrhs) receives the solutions. Returns a SolverError value.
+int bdlu_serve_wait(void* s, int64_t request, double* X, bdlu_serve_result* out);
+
 // ---- host analysis queries (CPU only) ---------------------------------------
 void* bdlu_host_analyze(int64_t nnz, const int32_t* rows, const int32_t* cols, const double* vals,
                         int32_t dimension, int* error_code);

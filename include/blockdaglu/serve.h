@@ -1,0 +1,63 @@
+#pragma once
+// Serving loop for the BLOCK engine (additive API; the reference has no
+// counterpart). Sparsity patterns are registered once: their host analysis
+// (canonical order, UBTF decomposition, per-block symbolic analysis) is kept,
+// and each slot keeps the device plan (symbolic arrays + factor storage)
+// resident in HBM. Requests (new values for a registered pattern plus k
+// right-hand sides) are admitted as slots free up: every slot owns an engine
+// context and stream, so the work of requests that arrive while others run
+// overlaps on the GPU (continuous batching), and resident plans are held
+// within an HBM budget split across slots (least recently used plans are
+// evicted). A request's result does not depend on what else is in flight: each
+// runs on its own buffers with deterministic kernels.
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "blockdaglu/solver.h"
+
+namespace blockdaglu {
+
+struct ServeResult {
+  SolverError error = SolverError::None;
+  int32_t slot = -1;
+  int32_t ir_iterations = 0;
+  int32_t fp64_fallback_blocks = 0;
+  int32_t repivots = 0;
+  double rel_residual = -1.0;    // max over the request's right-hand sides
+  double backward_error = -1.0;  // max over the request's right-hand sides
+  double t_submit_ms = 0.0;      // server clock (ms since the server started)
+  double t_start_ms = 0.0;
+  double t_done_ms = 0.0;
+};
+
+class ServeEngine;
+
+class Server {
+ public:
+  // slots: concurrent requests on the device. hbm_budget_bytes: resident plans
+  // of all slots together. precision: Fp64 (plain) or Fp32Ir.
+  Server(int device, int slots, uint64_t hbm_budget_bytes, Precision precision);
+  ~Server();
+  Server(const Server&) = delete;
+  Server& operator=(const Server&) = delete;
+
+  // Registers a pattern (entries in any order, duplicates allowed; the values
+  // choose the first pivots of blocks larger than 64). Returns the pattern id
+  // or -1 (see last_error()).
+  int32_t add_pattern(const CooMatrix& m, int32_t dimension = -1);
+  int32_t pattern_n(int32_t pattern) const;
+
+  // Queues a request: values in the registration's entry order, B column-major
+  // n x nrhs. Returns a request id (>= 0) or -1.
+  int64_t submit(int32_t pattern, const std::vector<double>& values, const std::vector<double>& B, int32_t nrhs);
+  // Blocks until the request is done; X gets n x nrhs (column-major).
+  bool wait(int64_t request, std::vector<double>& X, ServeResult& result);
+
+  SolverError last_error() const;
+
+ private:
+  std::unique_ptr<ServeEngine> engine_;
+};
+
+}  // namespace blockdaglu

@@ -1,0 +1,102 @@
+"""Serving loop on the GPU (include/blockdaglu/serve.h): requests on registered
+patterns run concurrently in slots; each result must equal the same request run
+alone (tolerance: bitwise, since every kernel is deterministic and a request
+never shares buffers), and meet the accuracy of its precision."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_residual
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _patterns(bd, count=3):
+    out = []
+    for p in range(count):
+        sp = bd.config_spec(1)
+        sp.seed = 500 + p
+        sp.n_small_blocks = 60 + 20 * p
+        sp.n_small_total = sp.n_small_blocks * 20
+        sp.large_size_lo = sp.large_size_hi = 300 + 250 * p
+        n, r, c, v, _ = bd.generate(sp)
+        out.append((n, r, c, v))
+    return out
+
+
+def _requests(bd, pats, count, seed):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(count):
+        p = int(rng.integers(len(pats)))
+        n, r, c, v = pats[p]
+        vals = bd.perturb(v, seed=10_000 * seed + i)
+        k = int(rng.integers(1, 5))
+        B = np.asfortranarray(rng.uniform(-1, 1, (n, k)))
+        reqs.append((p, vals, B))
+    return reqs
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_serve_alone_equals_mix(gpu, precision):
+    bd = gpu
+    prec = bd.Precision(precision)
+    _say(f"case: serve alone vs mix precision={prec.name}")
+    pats = _patterns(bd)
+    reqs = _requests(bd, pats, 24, seed=3 + precision)
+    mix = bd.Server(slots=4, hbm_budget_gb=8.0, precision=prec)
+    pid = [mix.add_pattern(r, c, v, n) for (n, r, c, v) in pats]
+    ids = [mix.submit(pid[p], vals, B) for (p, vals, B) in reqs]
+    mixed = [mix.wait(i) for i in ids]
+    mix.close()
+    slots = {res["slot"] for _, res in mixed}
+    spans = sorted((res["t_start_ms"], res["t_done_ms"]) for _, res in mixed)
+    overlapped = any(b[0] < a[1] for a, b in zip(spans[:-1], spans[1:]))
+    _say(f"  slots used={sorted(slots)} overlapped={overlapped}")
+    assert len(slots) > 1 and overlapped
+
+    alone = bd.Server(slots=1, hbm_budget_gb=8.0, precision=prec)
+    pid1 = [alone.add_pattern(r, c, v, n) for (n, r, c, v) in pats]
+    worst_rr = 0.0
+    for (p, vals, B), (Xm, rm) in zip(reqs, mixed):
+        Xa, ra = alone.wait(alone.submit(pid1[p], vals, B))
+        assert rm["error"] == bd.SolverError.None_ and ra["error"] == bd.SolverError.None_
+        assert np.array_equal(Xa, Xm), "result differs between alone and in the mix"
+        n, r, c, _ = pats[p]
+        for j in range(B.shape[1]):
+            worst_rr = max(worst_rr, rel_residual(n, r, c, vals, Xm[:, j], B[:, j]))
+        assert rm["ir_iterations"] == ra["ir_iterations"]
+    alone.close()
+    _say(f"  worst rel residual {worst_rr:.2e}")
+    if precision == 1:
+        assert worst_rr < 1e-12
+    else:
+        assert worst_rr < 1e-9
+
+
+def test_serve_budget_eviction(gpu):
+    """A budget that holds one plan per slot still serves every pattern (plans are
+    evicted and re-uploaded), with the same results."""
+    bd = gpu
+    _say("case: serve with a tight HBM budget")
+    pats = _patterns(bd)
+    reqs = _requests(bd, pats, 9, seed=11)
+    big = bd.Server(slots=1, hbm_budget_gb=8.0)
+    pb = [big.add_pattern(r, c, v, n) for (n, r, c, v) in pats]
+    ref = [big.wait(big.submit(pb[p], vals, B))[0] for (p, vals, B) in reqs]
+    big.close()
+    tight = bd.Server(slots=1, hbm_budget_gb=0.02)
+    pt = [tight.add_pattern(r, c, v, n) for (n, r, c, v) in pats]
+    for (p, vals, B), x in zip(reqs, ref):
+        X, res = tight.wait(tight.submit(pt[p], vals, B))
+        assert res["error"] == bd.SolverError.None_, res
+        assert np.array_equal(X, x)
+    tight.close()
