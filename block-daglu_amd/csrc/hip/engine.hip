@@ -16,6 +16,11 @@
 #include "../../../include/bdlu_gpu.h"
 #include "kernels.h"
 
+// Right-hand sides solved together per pass (work sets are n x kMaxRhs), and the
+// slot of ||A||_inf after the per-column norms.
+constexpr int32_t kMaxRhs = 8;
+constexpr int kAnormSlot = 9 * kMaxRhs;
+
 namespace {
 
 struct HipError {
@@ -127,7 +132,7 @@ struct Plan {
   int32_t* err = nullptr;
   double *b = nullptr, *x = nullptr, *r = nullptr, *d = nullptr, *bperm = nullptr, *xperm = nullptr;
   double* partials = nullptr;
-  double* norms = nullptr;  // [0..4] residual norms, [8] ||A||_inf
+  double* norms = nullptr;  // per column c: [c*9 + 0..4] residual norms; [kAnormSlot] ||A||_inf
   bool factored = false;
 };
 
@@ -409,10 +414,10 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
     S.Ud32 = A.get<float>(sd.k0);
     S.S32 = A.get<float>(static_cast<std::size_t>(sd.m) * sd.m);
     S.rs = A.get<double>(sd.s);
-    S.y = A.get<double>(sd.s);
+    S.y = A.get<double>(static_cast<std::size_t>(sd.s) * kMaxRhs);
     S.inv64 = A.get<double>(2 * 64 * 64);
     S.inv32 = A.get<float>(2 * 64 * 64);
-    S.ty = A.get<double>(std::max(sd.m, 1));
+    S.ty = A.get<double>(static_cast<std::size_t>(std::max(sd.m, 1)) * kMaxRhs);
     S.status = A.get<int32_t>(1);
     P.sp.push_back(std::move(S));
   }
@@ -458,57 +463,62 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
   P.nticket = static_cast<int32_t>(P.phases.size()) + 2 * d.nsparse;
   P.tickets = A.get<int32_t>(P.nticket);
   P.err = A.get<int32_t>(1);
-  P.b = A.get<double>(n);
-  P.x = A.get<double>(n);
-  P.r = A.get<double>(n);
-  P.d = A.get<double>(n);
-  P.bperm = A.get<double>(n);
-  P.xperm = A.get<double>(n);
-  P.partials = A.get<double>(512 * 5);
-  P.norms = A.get<double>(16);
+  const std::size_t nk = static_cast<std::size_t>(n) * kMaxRhs;
+  P.b = A.get<double>(nk);
+  P.x = A.get<double>(nk);
+  P.r = A.get<double>(nk);
+  P.d = A.get<double>(nk);
+  P.bperm = A.get<double>(nk);
+  P.xperm = A.get<double>(nk);
+  P.partials = A.get<double>(512 * 5 * kMaxRhs);
+  P.norms = A.get<double>(kAnormSlot + 8);
   HIPCHECK(hipStreamSynchronize(st));
   return pl;
 }
 
 // Solve A x = b for the resident factors; b and x are device vectors of length n.
-void block_solve(Plan& P, hipStream_t st, const double* b, double* x) {
+// Solves kt columns at once: b and x are n x kt sets (column stride n).
+void block_solve(Plan& P, hipStream_t st, const double* b, double* x, int32_t kt) {
+  const int64_t ld = P.n;
   bdk::launch_fill_i32(st, P.tickets, P.nticket, 0);
-  bdk::launch_fill_sentinel(st, P.xperm, P.n);
-  bdk::launch_permute(st, P.n, P.P, b, P.bperm);
+  bdk::launch_fill_sentinel(st, P.xperm, int64_t(P.n) * kt);
+  bdk::launch_permute(st, P.n, kt, P.P, b, ld, P.bperm, ld);
   int32_t tix = 0;
   for (const Phase& ph : P.phases) {
     if (!ph.sparse) {
       // Each dense block's factors are read in the precision it was factored in.
-      bdk::launch_dense_seg_solve(st, ph.ntasks, ph.tasks, P.R, P.lu0_of_block, P.lu64, P.lu32, P.prec_of_block,
-                                  P.piv, P.off_ptr, P.off_col, P.off_val, P.tickets + tix, P.bperm, P.xperm, P.err);
+      bdk::launch_dense_seg_solve(st, kt, ph.ntasks, ph.tasks, P.R, P.lu0_of_block, P.lu64, P.lu32, P.prec_of_block,
+                                  P.piv, P.off_ptr, P.off_col, P.off_val, P.tickets + tix, P.bperm, P.xperm, ld,
+                                  P.err);
       ++tix;
       continue;
     }
     SparseDev& S = P.sp[ph.index];
     ++tix;
-    bdk::launch_sp_gather(st, S.s, S.start, S.rowperm, S.scale ? S.rs : nullptr, P.off_ptr, P.off_col, P.off_val,
-                          P.bperm, P.xperm, S.y);
+    const int64_t ldy = S.s, ldt = std::max(S.m, 1);
+    bdk::launch_sp_gather(st, S.s, S.start, kt, S.rowperm, S.scale ? S.rs : nullptr, P.off_ptr, P.off_col, P.off_val,
+                          P.bperm, P.xperm, ld, S.y, ldy);
     auto run = [&](auto* Lx, auto* Ux, auto* Ud, auto* Sm) {
       using T = std::remove_pointer_t<decltype(Lx)>;
       for (std::size_t l = 0; l + 1 < S.fl_ptr.size(); ++l)
-        bdk::launch_sp_rows<T>(st, S.fl_ptr[l + 1] - S.fl_ptr[l], S.fl_rows + S.fl_ptr[l], S.lr_ptr, S.lr_col,
-                               S.lr_vidx, Lx, nullptr, S.y);
-      bdk::launch_sp_rows<T>(st, S.m, S.tl_rows, S.lr_ptr, S.lr_col, S.lr_vidx, Lx, nullptr, S.y);
+        bdk::launch_sp_rows<T>(st, kt, S.fl_ptr[l + 1] - S.fl_ptr[l], S.fl_rows + S.fl_ptr[l], S.lr_ptr, S.lr_col,
+                               S.lr_vidx, Lx, nullptr, S.y, ldy);
+      bdk::launch_sp_rows<T>(st, kt, S.m, S.tl_rows, S.lr_ptr, S.lr_col, S.lr_vidx, Lx, nullptr, S.y, ldy);
       // Tail: forward into ty, backward back into y (each output starts unpublished).
-      bdk::launch_fill_sentinel(st, S.ty, S.m);
-      bdk::launch_tail_trsv<T>(st, S.m, Sm, S.m, S.y + S.k0, S.ty, false, P.tickets + tix, P.err);
-      bdk::launch_fill_sentinel(st, S.y + S.k0, S.m);
-      bdk::launch_tail_trsv<T>(st, S.m, Sm, S.m, S.ty, S.y + S.k0, true, P.tickets + tix + 1, P.err);
+      for (int32_t c = 0; c < kt; ++c) bdk::launch_fill_sentinel(st, S.ty + c * ldt, S.m);
+      bdk::launch_tail_trsv<T>(st, kt, S.m, Sm, S.m, S.y + S.k0, ldy, S.ty, ldt, false, P.tickets + tix, P.err);
+      for (int32_t c = 0; c < kt; ++c) bdk::launch_fill_sentinel(st, S.y + c * ldy + S.k0, S.m);
+      bdk::launch_tail_trsv<T>(st, kt, S.m, Sm, S.m, S.ty, ldt, S.y + S.k0, ldy, true, P.tickets + tix + 1, P.err);
       for (std::size_t l = 0; l + 1 < S.bl_ptr.size(); ++l)
-        bdk::launch_sp_rows<T>(st, S.bl_ptr[l + 1] - S.bl_ptr[l], S.bl_rows + S.bl_ptr[l], S.ur_ptr, S.ur_col,
-                               S.ur_vidx, Ux, Ud, S.y);
+        bdk::launch_sp_rows<T>(st, kt, S.bl_ptr[l + 1] - S.bl_ptr[l], S.bl_rows + S.bl_ptr[l], S.ur_ptr, S.ur_col,
+                               S.ur_vidx, Ux, Ud, S.y, ldy);
     };
     if (S.prec == 0) run(S.Lx64, S.Ux64, S.Ud64, S.S64);
     else run(S.Lx32, S.Ux32, S.Ud32, S.S32);
     tix += 2;
-    bdk::launch_sp_scatter_x(st, S.s, S.start, S.q, S.y, P.xperm);
+    bdk::launch_sp_scatter_x(st, S.s, S.start, kt, S.q, S.y, ldy, P.xperm, ld);
   }
-  bdk::launch_unpermute(st, P.n, P.Q, P.xperm, x);
+  bdk::launch_unpermute(st, P.n, kt, P.Q, P.xperm, ld, x, ld);
 }
 
 template <typename T>
@@ -643,7 +653,7 @@ int bdlu_gpu_refactor(bdlu_ctx* c, uint64_t id, const double* values, int where,
     HIPCHECK(hipMemcpyAsync(P.vals, values, sizeof(double) * P.nnz,
                             where == BDLU_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
     bdk::launch_scatter(st, P.nnz, P.vals, P.vmap, P.arena);
-    bdk::launch_anorm(st, P.n, P.csr_ptr, P.vals, P.partials, P.norms + 8);
+    bdk::launch_anorm(st, P.n, P.csr_ptr, P.vals, P.partials, P.norms + kAnormSlot);
     const bool fp32 = precision == BDLU_FP32;
     ProfEvents pev;
     ProfEvents* pe = c->prof ? &pev : nullptr;
@@ -747,52 +757,68 @@ int bdlu_gpu_solve(bdlu_ctx* c, uint64_t id, double* B, int32_t nrhs, int64_t ld
   const int32_t max_iters = ir ? ir->max_iters : 0;
   const bool want_stats = max_iters > 0 || (ir && ir->stop_rule < 0);
   bdlu_solve_stats ss{0, -1.0, -1.0};
+  const int64_t n = P.n;
   try {
     HIPCHECK(hipSetDevice(c->device));
     const hipMemcpyKind in = where == BDLU_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     const hipMemcpyKind out = where == BDLU_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
     bdk::launch_fill_i32(st, P.err, 1, 0);
-    for (int32_t k = 0; k < nrhs; ++k) {
-      double* col = B + static_cast<int64_t>(k) * ldb;
-      HIPCHECK(hipMemcpyAsync(P.b, col, sizeof(double) * P.n, in, st));
+    double anorm = 0.0;
+    if (want_stats) HIPCHECK(hipMemcpyAsync(&anorm, P.norms + kAnormSlot, sizeof(double), hipMemcpyDeviceToHost, st));
+    // Columns in chunks of up to kMaxRhs, each solved (and refined) together.
+    for (int32_t c0 = 0; c0 < nrhs; c0 += kMaxRhs) {
+      const int32_t kc = std::min(kMaxRhs, nrhs - c0);
+      const int32_t kt = kc <= 1 ? 1 : kc <= 2 ? 2 : kc <= 4 ? 4 : 8;
+      for (int32_t k = 0; k < kc; ++k)
+        HIPCHECK(hipMemcpyAsync(P.b + k * n, B + (c0 + k) * ldb, sizeof(double) * n, in, st));
+      if (kt > kc) bdk::launch_fill_f64(st, P.b + kc * n, (kt - kc) * n, 0.0);
       ProfEvents pev;
       if (c->prof) pev.mark(BDLU_PROF_SOLVE, st);
-      block_solve(P, st, P.b, P.x);
+      block_solve(P, st, P.b, P.x, kt);
       if (c->prof) {
         pev.mark(BDLU_PROF_SOLVE, st);
         HIPCHECK(hipStreamSynchronize(st));
         pev.collect(c);
       }
-      int32_t iters = 0;
-      double h[16];
-      auto eval = [&]() {
-        bdk::launch_residual(st, P.n, P.csr_ptr, P.csr_col, P.vals, P.x, P.b, P.r);
-        bdk::launch_norms(st, P.n, P.r, P.x, P.b, P.partials, P.norms);
-        HIPCHECK(hipMemcpyAsync(h, P.norms, sizeof(double) * 9, hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipStreamSynchronize(st));
-      };
       if (want_stats) {
+        double h[9 * kMaxRhs];
+        auto eval = [&]() {
+          bdk::launch_residual(st, P.n, kt, P.csr_ptr, P.csr_col, P.vals, P.x, P.b, P.r, n);
+          bdk::launch_norms(st, P.n, kt, P.r, P.x, P.b, n, P.partials, P.norms);
+          HIPCHECK(hipMemcpyAsync(h, P.norms, sizeof(double) * 9 * kt, hipMemcpyDeviceToHost, st));
+          HIPCHECK(hipStreamSynchronize(st));
+        };
+        auto rel = [&](int k) {
+          const double bn2 = std::sqrt(h[k * 9 + 3]);
+          return bn2 > 0 ? std::sqrt(h[k * 9 + 0]) / bn2 : 0.0;
+        };
+        auto berr = [&](int k) {
+          const double den = anorm * h[k * 9 + 2] + h[k * 9 + 4];
+          return den > 0 ? h[k * 9 + 1] / den : 0.0;
+        };
         eval();
+        int32_t iters = 0;
         for (int it2 = 0; it2 < max_iters; ++it2) {
-          const double bn2 = std::sqrt(h[3]);
-          const double rel = bn2 > 0 ? std::sqrt(h[0]) / bn2 : 0.0;
-          const double berr_den = h[8] * h[2] + h[4];
-          const double berr = berr_den > 0 ? h[1] / berr_den : 0.0;
-          const bool stop = (ir->stop_rule == 1) ? (berr <= ir->tol) : (rel < ir->tol);
-          if (stop || bn2 == 0.0) break;
-          block_solve(P, st, P.r, P.d);
-          bdk::launch_axpy(st, P.n, P.d, P.x);
+          bool done = true;
+          for (int k = 0; k < kc; ++k) {
+            const bool stop = (ir->stop_rule == 1) ? (berr(k) <= ir->tol) : (rel(k) < ir->tol);
+            if (!stop && h[k * 9 + 3] > 0.0) done = false;
+          }
+          if (done) break;
+          block_solve(P, st, P.r, P.d, kt);
+          bdk::launch_axpy(st, P.n, kt, P.d, P.x, n);
           ++iters;
           eval();
         }
         // Worst right-hand side of the call.
-        const double bn2 = std::sqrt(h[3]);
-        ss.rel_residual = std::max(ss.rel_residual, bn2 > 0 ? std::sqrt(h[0]) / bn2 : 0.0);
-        const double den = h[8] * h[2] + h[4];
-        ss.backward_error = std::max(ss.backward_error, den > 0 ? h[1] / den : 0.0);
+        for (int k = 0; k < kc; ++k) {
+          ss.rel_residual = std::max(ss.rel_residual, rel(k));
+          ss.backward_error = std::max(ss.backward_error, berr(k));
+        }
         ss.iterations = std::max(ss.iterations, iters);
       }
-      HIPCHECK(hipMemcpyAsync(col, P.x, sizeof(double) * P.n, out, st));
+      for (int32_t k = 0; k < kc; ++k)
+        HIPCHECK(hipMemcpyAsync(B + (c0 + k) * ldb, P.x + k * n, sizeof(double) * n, out, st));
     }
     int32_t herr = 0;
     HIPCHECK(hipMemcpyAsync(&herr, P.err, sizeof(int32_t), hipMemcpyDeviceToHost, st));

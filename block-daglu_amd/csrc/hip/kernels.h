@@ -65,38 +65,45 @@ void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, 
                     TailTimer* timer = nullptr);
 
 // ---- solve ----
-void launch_permute(hipStream_t st, int32_t n, const int32_t* P, const double* src, double* dst);    // dst[k]=src[P[k]]
-void launch_unpermute(hipStream_t st, int32_t n, const int32_t* Q, const double* src, double* dst);  // dst[Q[k]]=src[k]
+// Vector sets hold kc columns (right-hand sides), column c at base + c * ld.
+// Kernels with per-column registers are instantiated for 1, 2, 4 and 8 columns
+// (kc is rounded up by the caller, padding with zero columns).
+void launch_permute(hipStream_t st, int32_t n, int32_t k, const int32_t* P, const double* src, int64_t lds,
+                    double* dst, int64_t ldd);  // dst[c][i] = src[c][P[i]]
+void launch_unpermute(hipStream_t st, int32_t n, int32_t k, const int32_t* Q, const double* src, int64_t lds,
+                      double* dst, int64_t ldd);  // dst[c][Q[i]] = src[c][i]
 // Dense-block solve of one segment (persistent, sync-free). ticket starts at 0;
 // xperm entries of this segment's blocks start as the unpublished pattern.
-void launch_dense_seg_solve(hipStream_t st, int32_t ntasks, const int32_t* tasks, const int32_t* R,
+void launch_dense_seg_solve(hipStream_t st, int32_t kc, int32_t ntasks, const int32_t* tasks, const int32_t* R,
                             const int64_t* lu0_of_block, const double* lu64, const float* lu32,
                             const int32_t* prec_of_block, const int32_t* gsrc, const int32_t* off_ptr,
                             const int32_t* off_col, const double* off_val, int32_t* ticket, const double* bperm,
-                            double* xperm, int32_t* err);
-// Sparse block solve pieces; y is a length-s work vector in position order.
-void launch_sp_gather(hipStream_t st, int32_t s, int32_t start, const int32_t* rowperm, const double* rs,
+                            double* xperm, int64_t ldv, int32_t* err);
+// Sparse block solve pieces; y is an s x kc work set in position order.
+void launch_sp_gather(hipStream_t st, int32_t s, int32_t start, int32_t kc, const int32_t* rowperm, const double* rs,
                       const int32_t* off_ptr, const int32_t* off_col, const double* off_val, const double* bperm,
-                      const double* xperm, double* y);
+                      const double* xperm, int64_t ldv, double* y, int64_t ldy);
 template <typename T>
-void launch_sp_rows(hipStream_t st, int32_t nrows, const int32_t* rows, const int32_t* ptr, const int32_t* col,
-                    const int32_t* vidx, const T* vals, const T* diag, double* y);
+void launch_sp_rows(hipStream_t st, int32_t kc, int32_t nrows, const int32_t* rows, const int32_t* ptr,
+                    const int32_t* col, const int32_t* vidx, const T* vals, const T* diag, double* y, int64_t ldy);
 // Dense tail triangular solve over 64-row bands. ticket starts at 0; yout
 // starts as the unpublished pattern and does not alias yin.
 template <typename T>
-void launch_tail_trsv(hipStream_t st, int32_t m, const T* S, int64_t lds, const double* yin, double* yout, bool upper,
-                      int32_t* ticket, int32_t* err);
-void launch_sp_scatter_x(hipStream_t st, int32_t s, int32_t start, const int32_t* q, const double* x, double* xperm);
+void launch_tail_trsv(hipStream_t st, int32_t kc, int32_t m, const T* S, int64_t lds, const double* yin, int64_t ldi,
+                      double* yout, int64_t ldo, bool upper, int32_t* ticket, int32_t* err);
+void launch_sp_scatter_x(hipStream_t st, int32_t s, int32_t start, int32_t kc, const int32_t* q, const double* x,
+                         int64_t ldx, double* xperm, int64_t ldv);
 
 // ---- residual / norms / refinement ----
-void launch_residual(hipStream_t st, int32_t n, const int64_t* csr_ptr, const int32_t* csr_col, const double* vals,
-                     const double* x, const double* b, double* r);
-// out[0]=sum r^2, out[1]=max|r|, out[2]=max|x|, out[3]=sum b^2, out[4]=max|b|
-void launch_norms(hipStream_t st, int32_t n, const double* r, const double* x, const double* b, double* partials,
-                  double* out);
+void launch_residual(hipStream_t st, int32_t n, int32_t kc, const int64_t* csr_ptr, const int32_t* csr_col,
+                     const double* vals, const double* x, const double* b, double* r, int64_t ld);
+// Per column c: out[c*9+0]=sum r^2, [1]=max|r|, [2]=max|x|, [3]=sum b^2, [4]=max|b|.
+// partials: 512 * 5 * kc doubles.
+void launch_norms(hipStream_t st, int32_t n, int32_t kc, const double* r, const double* x, const double* b, int64_t ld,
+                  double* partials, double* out);
 // out[0] = max_i sum_j |a_ij|
 void launch_anorm(hipStream_t st, int32_t n, const int64_t* csr_ptr, const double* vals, double* partials,
                   double* out);
-void launch_axpy(hipStream_t st, int32_t n, const double* d, double* x);  // x += d
+void launch_axpy(hipStream_t st, int32_t n, int32_t kc, const double* d, double* x, int64_t ld);  // x += d
 
 }  // namespace bdk

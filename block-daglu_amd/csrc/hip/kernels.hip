@@ -760,19 +760,28 @@ template void launch_tail_lu<float>(hipStream_t, int32_t, float*, int64_t, doubl
 
 // ---------------------------------------------------------------------------
 // Solve.
-__global__ void k_permute(int32_t n, const int32_t* __restrict__ P, const double* __restrict__ src,
-                          double* __restrict__ dst) {
-  for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) dst[k] = src[P[k]];
+// Multi-column vectors: column c of a vector set lives at base + c * ld.
+__global__ void k_permute(int32_t n, int32_t k, const int32_t* __restrict__ P, const double* __restrict__ src,
+                          int64_t lds, double* __restrict__ dst, int64_t ldd) {
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int32_t pi = P[i];
+    for (int32_t c = 0; c < k; ++c) dst[c * ldd + i] = src[c * lds + pi];
+  }
 }
-__global__ void k_unpermute(int32_t n, const int32_t* __restrict__ Q, const double* __restrict__ src,
-                            double* __restrict__ dst) {
-  for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) dst[Q[k]] = src[k];
+__global__ void k_unpermute(int32_t n, int32_t k, const int32_t* __restrict__ Q, const double* __restrict__ src,
+                            int64_t lds, double* __restrict__ dst, int64_t ldd) {
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int32_t qi = Q[i];
+    for (int32_t c = 0; c < k; ++c) dst[c * ldd + qi] = src[c * lds + i];
+  }
 }
-void launch_permute(hipStream_t st, int32_t n, const int32_t* P, const double* src, double* dst) {
-  hipLaunchKernelGGL(k_permute, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, n, P, src, dst);
+void launch_permute(hipStream_t st, int32_t n, int32_t k, const int32_t* P, const double* src, int64_t lds,
+                    double* dst, int64_t ldd) {
+  hipLaunchKernelGGL(k_permute, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, n, k, P, src, lds, dst, ldd);
 }
-void launch_unpermute(hipStream_t st, int32_t n, const int32_t* Q, const double* src, double* dst) {
-  hipLaunchKernelGGL(k_unpermute, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, n, Q, src, dst);
+void launch_unpermute(hipStream_t st, int32_t n, int32_t k, const int32_t* Q, const double* src, int64_t lds,
+                      double* dst, int64_t ldd) {
+  hipLaunchKernelGGL(k_unpermute, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, n, k, Q, src, lds, dst, ldd);
 }
 
 __device__ __forceinline__ double readlane_f64(double v, int src) {
@@ -815,12 +824,13 @@ __device__ __forceinline__ void stage_factor(double* __restrict__ fs, const T* _
 // Deadlock-free: the lowest unfinished ticket is always the current block of
 // some workgroup (a claimed, unstarted ticket is higher than its holder's
 // current one), and every x entry it reads belongs to a lower ticket.
+template <int K>
 __global__ __launch_bounds__(64) void k_dense_seg_solve(
     int32_t ntasks, const int32_t* __restrict__ tasks, const int32_t* __restrict__ R,
     const int64_t* __restrict__ lu0_of_block, const double* __restrict__ lu64, const float* __restrict__ lu32,
     const int32_t* __restrict__ prec_of_block, const int32_t* __restrict__ gsrc, const int32_t* __restrict__ off_ptr,
     const int32_t* __restrict__ off_col, const double* __restrict__ off_val, int32_t* __restrict__ ticket,
-    const double* __restrict__ bperm, double* __restrict__ xperm, int32_t* __restrict__ err) {
+    const double* __restrict__ bperm, double* __restrict__ xperm, int64_t ldv, int32_t* __restrict__ err) {
   __shared__ double fs[64 * 64];
   constexpr int kPre = 4;
   const int lane = threadIdx.x;
@@ -838,7 +848,9 @@ __global__ __launch_bounds__(64) void k_dense_seg_solve(
     const int32_t r = r0 + lane;
     // --- independent of other blocks ---
     const int32_t src = act ? gsrc[r] : 0;
-    const double bv = act ? bperm[r] : 0.0;
+    double acc[K];
+#pragma unroll
+    for (int c = 0; c < K; ++c) acc[c] = act ? bperm[c * ldv + r] : 0.0;
     const int32_t p0 = act ? off_ptr[r] : 0, p1 = act ? off_ptr[r + 1] : 0;
     int32_t oc[kPre];
     double ov[kPre];
@@ -855,111 +867,162 @@ __global__ __launch_bounds__(64) void k_dense_seg_solve(
     __syncthreads();
     const double rd = act ? 1.0 / fs[lane * s + lane] : 1.0;
     // Each lane's row of L\U into registers (clamped, unconditional LDS reads):
-    // the substitution chain below then runs on registers and readlane only.
+    // the substitution chains below then run on registers and readlane only.
     double row[64];
     const int li = lane < s ? lane : s - 1;
 #pragma unroll
     for (int j = 0; j < 64; ++j) row[j] = fs[(j < s ? j : s - 1) * s + li];
     // --- published x of other blocks: all first polls in flight together ---
-    uint64_t xu[kPre];
-#pragma unroll
-    for (int u = 0; u < kPre; ++u) xu[u] = oc[u] >= 0 ? ld_x_relaxed(xperm + oc[u]) : 0ull;
-    double acc = bv;
+    uint64_t xu[kPre][K];
 #pragma unroll
     for (int u = 0; u < kPre; ++u)
-      if (oc[u] >= 0) acc = acc - ov[u] * finish_wait_x(xperm + oc[u], xu[u], late);
-    for (int32_t p = p0 + kPre; p < p1; ++p) acc = acc - off_val[p] * wait_x(xperm + off_col[p], late);
-    // Row gather (the block's pivoting), then unit-lower and upper substitution.
-    acc = __shfl(acc, src);
-    if (!act) acc = 0.0;
+#pragma unroll
+      for (int c = 0; c < K; ++c) xu[u][c] = oc[u] >= 0 ? ld_x_relaxed(xperm + c * ldv + oc[u]) : 0ull;
+#pragma unroll
+    for (int u = 0; u < kPre; ++u)
+      if (oc[u] >= 0) {
+#pragma unroll
+        for (int c = 0; c < K; ++c) acc[c] = acc[c] - ov[u] * finish_wait_x(xperm + c * ldv + oc[u], xu[u][c], late);
+      }
+    for (int32_t p = p0 + kPre; p < p1; ++p) {
+      const double v = off_val[p];
+      const int32_t j = off_col[p];
+#pragma unroll
+      for (int c = 0; c < K; ++c) acc[c] = acc[c] - v * wait_x(xperm + c * ldv + j, late);
+    }
+    // Row gather (the block's pivoting), then unit-lower and upper substitution
+    // (K independent chains interleaved).
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      acc[c] = __shfl(acc[c], src);
+      if (!act) acc[c] = 0.0;
+    }
 #pragma unroll
     for (int j = 0; j < 64; ++j) {
       if (j < s) {
-        const double xj = readlane_f64(acc, j);
-        const double t = __builtin_fma(-row[j], xj, acc);
-        acc = (lane > j && act) ? t : acc;
+#pragma unroll
+        for (int c = 0; c < K; ++c) {
+          const double xj = readlane_f64(acc[c], j);
+          const double t2 = __builtin_fma(-row[j], xj, acc[c]);
+          acc[c] = (lane > j && act) ? t2 : acc[c];
+        }
       }
     }
 #pragma unroll
     for (int j = 63; j >= 0; --j) {
       if (j < s) {
-        acc = (lane == j) ? acc * rd : acc;
-        const double xj = readlane_f64(acc, j);
-        const double t = __builtin_fma(-row[j], xj, acc);
-        acc = (lane < j) ? t : acc;
+#pragma unroll
+        for (int c = 0; c < K; ++c) {
+          acc[c] = (lane == j) ? acc[c] * rd : acc[c];
+          const double xj = readlane_f64(acc[c], j);
+          const double t2 = __builtin_fma(-row[j], xj, acc[c]);
+          acc[c] = (lane < j) ? t2 : acc[c];
+        }
       }
     }
-    if (act) st_x_relaxed(xperm + r, acc);
+    if (act) {
+#pragma unroll
+      for (int c = 0; c < K; ++c) st_x_relaxed(xperm + c * ldv + r, acc[c]);
+    }
     __syncthreads();  // fs is restaged by the next block
     t = uniform_from_lane0(tn);
   }
   if (late) atomicOr(err, 1);
 }
 
-void launch_dense_seg_solve(hipStream_t st, int32_t ntasks, const int32_t* tasks, const int32_t* R,
+void launch_dense_seg_solve(hipStream_t st, int32_t kc, int32_t ntasks, const int32_t* tasks, const int32_t* R,
                             const int64_t* lu0_of_block, const double* lu64, const float* lu32,
                             const int32_t* prec_of_block, const int32_t* gsrc, const int32_t* off_ptr,
                             const int32_t* off_col, const double* off_val, int32_t* ticket, const double* bperm,
-                            double* xperm, int32_t* err) {
+                            double* xperm, int64_t ldv, int32_t* err) {
   if (ntasks <= 0) return;
   const int g = ntasks < 1024 ? ntasks : 1024;  // up to four single-wave workgroups per CU
-  hipLaunchKernelGGL(k_dense_seg_solve, dim3(g), dim3(64), 0, st, ntasks, tasks, R, lu0_of_block, lu64, lu32,
-                     prec_of_block, gsrc, off_ptr, off_col, off_val, ticket, bperm, xperm, err);
+#define BDLU_SEG(KK)                                                                                              \
+  hipLaunchKernelGGL(k_dense_seg_solve<KK>, dim3(g), dim3(64), 0, st, ntasks, tasks, R, lu0_of_block, lu64, lu32, \
+                     prec_of_block, gsrc, off_ptr, off_col, off_val, ticket, bperm, xperm, ldv, err)
+  switch (kc) {
+    case 1: BDLU_SEG(1); break;
+    case 2: BDLU_SEG(2); break;
+    case 4: BDLU_SEG(4); break;
+    default: BDLU_SEG(8); break;
+  }
+#undef BDLU_SEG
 }
 
-// Sparse block: y[k] = (bperm[r] - A_off(r,:) xperm) / Rs(row), r = start + rowperm[k].
-__global__ void k_sp_gather(int32_t s, int32_t start, const int32_t* __restrict__ rowperm,
+// Sparse block: y[k] = (bperm[r] - A_off(r,:) xperm) / Rs(row), r = start + rowperm[k],
+// for each of the kc columns (bperm/xperm column stride ldv, y column stride ldy).
+__global__ void k_sp_gather(int32_t s, int32_t start, int32_t kc, const int32_t* __restrict__ rowperm,
                             const double* __restrict__ rs, const int32_t* __restrict__ off_ptr,
                             const int32_t* __restrict__ off_col, const double* __restrict__ off_val,
-                            const double* __restrict__ bperm, const double* __restrict__ xperm,
-                            double* __restrict__ y) {
+                            const double* __restrict__ bperm, const double* __restrict__ xperm, int64_t ldv,
+                            double* __restrict__ y, int64_t ldy) {
   for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < s; k += gridDim.x * blockDim.x) {
     const int32_t lr = rowperm[k];
     const int32_t r = start + lr;
-    double acc = bperm[r];
-    for (int32_t p = off_ptr[r]; p < off_ptr[r + 1]; ++p) acc = acc - off_val[p] * xperm[off_col[p]];
-    y[k] = rs ? acc / rs[lr] : acc;
+    const double scale = rs ? rs[lr] : 1.0;
+    for (int32_t c = 0; c < kc; ++c) {
+      double acc = bperm[c * ldv + r];
+      for (int32_t p = off_ptr[r]; p < off_ptr[r + 1]; ++p) acc = acc - off_val[p] * xperm[c * ldv + off_col[p]];
+      y[c * ldy + k] = rs ? acc / scale : acc;
+    }
   }
 }
-void launch_sp_gather(hipStream_t st, int32_t s, int32_t start, const int32_t* rowperm, const double* rs,
+void launch_sp_gather(hipStream_t st, int32_t s, int32_t start, int32_t kc, const int32_t* rowperm, const double* rs,
                       const int32_t* off_ptr, const int32_t* off_col, const double* off_val, const double* bperm,
-                      const double* xperm, double* y) {
-  hipLaunchKernelGGL(k_sp_gather, dim3(grid_for(s, 256, 4096)), dim3(256), 0, st, s, start, rowperm, rs, off_ptr,
-                     off_col, off_val, bperm, xperm, y);
+                      const double* xperm, int64_t ldv, double* y, int64_t ldy) {
+  hipLaunchKernelGGL(k_sp_gather, dim3(grid_for(s, 256, 4096)), dim3(256), 0, st, s, start, kc, rowperm, rs, off_ptr,
+                     off_col, off_val, bperm, xperm, ldv, y, ldy);
 }
 
-// Row-oriented sparse triangular step over one level: for each listed row k,
-// y[k] = (y[k] - sum vals[vidx[p]] * y[col[p]]) [/ diag[k]]. A wavefront per row.
-template <typename T>
+// Row-oriented sparse triangular step over one level: for each listed row k and
+// each of the K columns, y[k] = (y[k] - sum vals[vidx[p]] * y[col[p]]) [/ diag[k]].
+// A wavefront per row; the row's entries are read once for all K columns.
+template <typename T, int K>
 __global__ __launch_bounds__(64) void k_sp_rows(int32_t nrows, const int32_t* __restrict__ rows,
                                                 const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                                                 const int32_t* __restrict__ vidx, const T* __restrict__ vals,
-                                                const T* __restrict__ diag, double* __restrict__ y) {
+                                                const T* __restrict__ diag, double* __restrict__ y, int64_t ldy) {
   const int lane = threadIdx.x;
   for (int32_t t = blockIdx.x; t < nrows; t += gridDim.x) {
     const int32_t k = rows[t];
-    double acc = 0.0;
-    for (int32_t p = ptr[k] + lane; p < ptr[k + 1]; p += kWave)
-      acc += static_cast<double>(vals[vidx[p]]) * y[col[p]];
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-    // Every lane holds the same sum and stores the same value (no lane-0-only
+    double acc[K];
+#pragma unroll
+    for (int c = 0; c < K; ++c) acc[c] = 0.0;
+    for (int32_t p = ptr[k] + lane; p < ptr[k + 1]; p += kWave) {
+      const double v = static_cast<double>(vals[vidx[p]]);
+      const int32_t j = col[p];
+#pragma unroll
+      for (int c = 0; c < K; ++c) acc[c] += v * y[c * ldy + j];
+    }
+#pragma unroll
+    for (int c = 0; c < K; ++c)
+      for (int off = 32; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off);
+    // Every lane holds the same sums and stores the same values (no lane-0-only
     // region inside the loop).
-    double v = y[k] - acc;
-    if (diag) v = v / static_cast<double>(diag[k]);
-    y[k] = v;
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      double v = y[c * ldy + k] - acc[c];
+      if (diag) v = v / static_cast<double>(diag[k]);
+      y[c * ldy + k] = v;
+    }
   }
 }
 template <typename T>
-void launch_sp_rows(hipStream_t st, int32_t nrows, const int32_t* rows, const int32_t* ptr, const int32_t* col,
-                    const int32_t* vidx, const T* vals, const T* diag, double* y) {
+void launch_sp_rows(hipStream_t st, int32_t kc, int32_t nrows, const int32_t* rows, const int32_t* ptr,
+                    const int32_t* col, const int32_t* vidx, const T* vals, const T* diag, double* y, int64_t ldy) {
   if (nrows <= 0) return;
-  hipLaunchKernelGGL(k_sp_rows<T>, dim3(grid_for(nrows, 1, 8192)), dim3(64), 0, st, nrows, rows, ptr, col, vidx,
-                     vals, diag, y);
+  const dim3 g(grid_for(nrows, 1, 8192)), b(64);
+  switch (kc) {
+    case 1: hipLaunchKernelGGL((k_sp_rows<T, 1>), g, b, 0, st, nrows, rows, ptr, col, vidx, vals, diag, y, ldy); break;
+    case 2: hipLaunchKernelGGL((k_sp_rows<T, 2>), g, b, 0, st, nrows, rows, ptr, col, vidx, vals, diag, y, ldy); break;
+    case 4: hipLaunchKernelGGL((k_sp_rows<T, 4>), g, b, 0, st, nrows, rows, ptr, col, vidx, vals, diag, y, ldy); break;
+    default: hipLaunchKernelGGL((k_sp_rows<T, 8>), g, b, 0, st, nrows, rows, ptr, col, vidx, vals, diag, y, ldy); break;
+  }
 }
-template void launch_sp_rows<double>(hipStream_t, int32_t, const int32_t*, const int32_t*, const int32_t*,
-                                     const int32_t*, const double*, const double*, double*);
-template void launch_sp_rows<float>(hipStream_t, int32_t, const int32_t*, const int32_t*, const int32_t*,
-                                    const int32_t*, const float*, const float*, double*);
+template void launch_sp_rows<double>(hipStream_t, int32_t, int32_t, const int32_t*, const int32_t*, const int32_t*,
+                                     const int32_t*, const double*, const double*, double*, int64_t);
+template void launch_sp_rows<float>(hipStream_t, int32_t, int32_t, const int32_t*, const int32_t*, const int32_t*,
+                                    const int32_t*, const float*, const float*, double*, int64_t);
 
 // Dense tail triangular solve (unit-lower forward or upper backward) on the
 // column-major m x m factor, one wavefront per 64-row band, bands claimed in
@@ -968,9 +1031,10 @@ template void launch_sp_rows<float>(hipStream_t, int32_t, const int32_t*, const 
 // band's factor tile loaded before the poll, then solves its diagonal tile by
 // substitution and publishes its 64 values. yin and yout must not alias; yout
 // starts filled with kXSentinel.
-template <typename T>
+template <typename T, int K>
 __global__ __launch_bounds__(64) void k_tail_trsv(int32_t m, const T* __restrict__ S, int64_t lds,
-                                                  const double* __restrict__ yin, double* __restrict__ yout, int upper,
+                                                  const double* __restrict__ yin, int64_t ldi,
+                                                  double* __restrict__ yout, int64_t ldo, int upper,
                                                   int32_t* __restrict__ ticket, int32_t* __restrict__ err) {
   const int lane = threadIdx.x;
   const int32_t np = (m + kNb - 1) / kNb;
@@ -983,13 +1047,17 @@ __global__ __launch_bounds__(64) void k_tail_trsv(int32_t m, const T* __restrict
   const int32_t rows = (m - r0) < kNb ? (m - r0) : kNb;
   const int32_t row = r0 + lane;
   const bool live = lane < rows;
-  double acc = live ? yin[row] : 0.0;
+  double acc[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) acc[c] = live ? yin[c * ldi + row] : 0.0;
   bool late = false;
   // Tile loads are unconditional with clamped indices, so all 64 issue back to
   // back (a per-element condition becomes a branch that waits on each load).
-  // Lanes past the band never store; columns past the matrix meet xv = 0.
+  // Lanes past the band never store; columns past the matrix meet xv = 0. The
+  // tile is shared by the K right-hand sides.
   const int32_t rowc = live ? row : r0;
-  T v[kNb];
+  constexpr int kH = kNb / 2;  // the tile is consumed in two halves of 32 columns
+  T v[kH];
   for (int32_t q = 0; q < np - 1; ++q) {
     int32_t j;
     if (upper) {
@@ -1002,117 +1070,182 @@ __global__ __launch_bounds__(64) void k_tail_trsv(int32_t m, const T* __restrict
     const int32_t c0 = j * kNb;
     const int32_t cols = (m - c0) < kNb ? (m - c0) : kNb;
 #pragma unroll
-    for (int c = 0; c < kNb; ++c) v[c] = S[int64_t(c0 + (c < cols ? c : cols - 1)) * lds + rowc];
-    const double xv = lane < cols ? wait_x(yout + c0 + lane, late) : 0.0;
+    for (int c = 0; c < kH; ++c) v[c] = S[int64_t(c0 + (c < cols ? c : cols - 1)) * lds + rowc];
+    double xv[K];
 #pragma unroll
-    for (int c = 0; c < kNb; ++c) acc = acc - static_cast<double>(v[c]) * readlane_f64(xv, c);
+    for (int k = 0; k < K; ++k) xv[k] = lane < cols ? wait_x(yout + k * ldo + c0 + lane, late) : 0.0;
+#pragma unroll
+    for (int c = 0; c < kH; ++c)
+#pragma unroll
+      for (int k = 0; k < K; ++k) acc[k] = acc[k] - static_cast<double>(v[c]) * readlane_f64(xv[k], c);
+#pragma unroll
+    for (int c = 0; c < kH; ++c) v[c] = S[int64_t(c0 + (kH + c < cols ? kH + c : cols - 1)) * lds + rowc];
+#pragma unroll
+    for (int c = 0; c < kH; ++c)
+#pragma unroll
+      for (int k = 0; k < K; ++k) acc[k] = acc[k] - static_cast<double>(v[c]) * readlane_f64(xv[k], kH + c);
   }
-#pragma unroll
-  for (int c = 0; c < kNb; ++c) v[c] = S[int64_t(r0 + (c < rows ? c : rows - 1)) * lds + rowc];
   if (!upper) {
 #pragma unroll
-    for (int c = 0; c < kNb; ++c) {
-      const double xc = readlane_f64(acc, c);
-      if (lane > c) acc = acc - static_cast<double>(v[c]) * xc;
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+      for (int c = 0; c < kH; ++c) v[c] = S[int64_t(r0 + (h * kH + c < rows ? h * kH + c : rows - 1)) * lds + rowc];
+#pragma unroll
+      for (int c = 0; c < kH; ++c)
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const double xc = readlane_f64(acc[k], h * kH + c);
+          if (lane > h * kH + c) acc[k] = acc[k] - static_cast<double>(v[c]) * xc;
+        }
     }
   } else {
 #pragma unroll
-    for (int c = kNb - 1; c >= 0; --c) {
-      if (c < rows) {
-        if (lane == c) acc = acc / static_cast<double>(v[c]);
-        const double xc = readlane_f64(acc, c);
-        if (lane < c) acc = acc - static_cast<double>(v[c]) * xc;
+    for (int h = 1; h >= 0; --h) {
+#pragma unroll
+      for (int c = 0; c < kH; ++c) v[c] = S[int64_t(r0 + (h * kH + c < rows ? h * kH + c : rows - 1)) * lds + rowc];
+#pragma unroll
+      for (int c = kH - 1; c >= 0; --c) {
+        if (h * kH + c < rows) {
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            if (lane == h * kH + c) acc[k] = acc[k] / static_cast<double>(v[c]);
+            const double xc = readlane_f64(acc[k], h * kH + c);
+            if (lane < h * kH + c) acc[k] = acc[k] - static_cast<double>(v[c]) * xc;
+          }
+        }
       }
     }
   }
-  if (live) st_x_relaxed(yout + row, acc);
+  if (live) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) st_x_relaxed(yout + k * ldo + row, acc[k]);
+  }
   if (late) atomicOr(err, 2);
 }
 template <typename T>
-void launch_tail_trsv(hipStream_t st, int32_t m, const T* S, int64_t lds, const double* yin, double* yout, bool upper,
-                      int32_t* ticket, int32_t* err) {
+void launch_tail_trsv(hipStream_t st, int32_t kc, int32_t m, const T* S, int64_t lds, const double* yin, int64_t ldi,
+                      double* yout, int64_t ldo, bool upper, int32_t* ticket, int32_t* err) {
   if (m <= 0) return;
-  const int np = (m + kNb - 1) / kNb;
-  hipLaunchKernelGGL(k_tail_trsv<T>, dim3(np), dim3(64), 0, st, m, S, lds, yin, yout, upper ? 1 : 0, ticket, err);
+  const dim3 g((m + kNb - 1) / kNb), b(64);
+  const int up = upper ? 1 : 0;
+  switch (kc) {
+    case 1: hipLaunchKernelGGL((k_tail_trsv<T, 1>), g, b, 0, st, m, S, lds, yin, ldi, yout, ldo, up, ticket, err); break;
+    case 2: hipLaunchKernelGGL((k_tail_trsv<T, 2>), g, b, 0, st, m, S, lds, yin, ldi, yout, ldo, up, ticket, err); break;
+    case 4: hipLaunchKernelGGL((k_tail_trsv<T, 4>), g, b, 0, st, m, S, lds, yin, ldi, yout, ldo, up, ticket, err); break;
+    default: hipLaunchKernelGGL((k_tail_trsv<T, 8>), g, b, 0, st, m, S, lds, yin, ldi, yout, ldo, up, ticket, err); break;
+  }
 }
-template void launch_tail_trsv<double>(hipStream_t, int32_t, const double*, int64_t, const double*, double*, bool,
-                                       int32_t*, int32_t*);
-template void launch_tail_trsv<float>(hipStream_t, int32_t, const float*, int64_t, const double*, double*, bool,
-                                      int32_t*, int32_t*);
+template void launch_tail_trsv<double>(hipStream_t, int32_t, int32_t, const double*, int64_t, const double*, int64_t,
+                                       double*, int64_t, bool, int32_t*, int32_t*);
+template void launch_tail_trsv<float>(hipStream_t, int32_t, int32_t, const float*, int64_t, const double*, int64_t,
+                                      double*, int64_t, bool, int32_t*, int32_t*);
 
-__global__ void k_sp_scatter_x(int32_t s, int32_t start, const int32_t* __restrict__ q, const double* __restrict__ x,
-                               double* __restrict__ xperm) {
+__global__ void k_sp_scatter_x(int32_t s, int32_t start, int32_t kc, const int32_t* __restrict__ q,
+                               const double* __restrict__ x, int64_t ldx, double* __restrict__ xperm, int64_t ldv) {
   for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < s; k += gridDim.x * blockDim.x)
-    xperm[start + q[k]] = x[k];
+    for (int32_t c = 0; c < kc; ++c) xperm[c * ldv + start + q[k]] = x[c * ldx + k];
 }
-void launch_sp_scatter_x(hipStream_t st, int32_t s, int32_t start, const int32_t* q, const double* x, double* xperm) {
-  hipLaunchKernelGGL(k_sp_scatter_x, dim3(grid_for(s, 256, 4096)), dim3(256), 0, st, s, start, q, x, xperm);
+void launch_sp_scatter_x(hipStream_t st, int32_t s, int32_t start, int32_t kc, const int32_t* q, const double* x,
+                         int64_t ldx, double* xperm, int64_t ldv) {
+  hipLaunchKernelGGL(k_sp_scatter_x, dim3(grid_for(s, 256, 4096)), dim3(256), 0, st, s, start, kc, q, x, ldx, xperm,
+                     ldv);
 }
 
 // ---------------------------------------------------------------------------
 // Residual r = b - A x over the canonical matrix, row by row in canonical
 // order (the reference's refinement loop, src/solver.cpp:1497-1506).
+template <int K>
 __global__ void k_residual(int32_t n, const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
                            const double* __restrict__ vals, const double* __restrict__ x,
-                           const double* __restrict__ b, double* __restrict__ r) {
+                           const double* __restrict__ b, double* __restrict__ r, int64_t ld) {
   for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double acc = b[i];
-    for (int64_t p = ptr[i]; p < ptr[i + 1]; ++p) acc = acc - vals[p] * x[col[p]];
-    r[i] = acc;
+    double acc[K];
+#pragma unroll
+    for (int c = 0; c < K; ++c) acc[c] = b[c * ld + i];
+    for (int64_t p = ptr[i]; p < ptr[i + 1]; ++p) {
+      const double v = vals[p];
+      const int32_t j = col[p];
+#pragma unroll
+      for (int c = 0; c < K; ++c) acc[c] = acc[c] - v * x[c * ld + j];
+    }
+#pragma unroll
+    for (int c = 0; c < K; ++c) r[c * ld + i] = acc[c];
   }
 }
-void launch_residual(hipStream_t st, int32_t n, const int64_t* csr_ptr, const int32_t* csr_col, const double* vals,
-                     const double* x, const double* b, double* r) {
-  hipLaunchKernelGGL(k_residual, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, n, csr_ptr, csr_col, vals, x, b, r);
+void launch_residual(hipStream_t st, int32_t n, int32_t kc, const int64_t* csr_ptr, const int32_t* csr_col,
+                     const double* vals, const double* x, const double* b, double* r, int64_t ld) {
+  const dim3 g(grid_for(n, 256, 8192)), bl(256);
+  switch (kc) {
+    case 1: hipLaunchKernelGGL(k_residual<1>, g, bl, 0, st, n, csr_ptr, csr_col, vals, x, b, r, ld); break;
+    case 2: hipLaunchKernelGGL(k_residual<2>, g, bl, 0, st, n, csr_ptr, csr_col, vals, x, b, r, ld); break;
+    case 4: hipLaunchKernelGGL(k_residual<4>, g, bl, 0, st, n, csr_ptr, csr_col, vals, x, b, r, ld); break;
+    default: hipLaunchKernelGGL(k_residual<8>, g, bl, 0, st, n, csr_ptr, csr_col, vals, x, b, r, ld); break;
+  }
 }
 
-// Deterministic reductions: fixed grid, fixed per-block order, then one block.
+// Deterministic reductions: fixed grid, fixed per-block order, then a fixed
+// lane tree. Per column c: out[c*9 + 0] = sum r^2, [1] max|r|, [2] max|x|,
+// [3] sum b^2, [4] max|b|.
 constexpr int kRedBlocks = 512;
-__global__ __launch_bounds__(256) void k_norms_partial(int32_t n, const double* __restrict__ r,
+__global__ __launch_bounds__(256) void k_norms_partial(int32_t n, int32_t kc, const double* __restrict__ r,
                                                        const double* __restrict__ x, const double* __restrict__ b,
-                                                       double* __restrict__ part) {
+                                                       int64_t ld, double* __restrict__ part) {
   __shared__ double sh[5][256];
-  double v[5] = {0, 0, 0, 0, 0};
-  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const double ri = r[i], xi = x[i], bi = b[i];
-    v[0] += ri * ri;
-    v[1] = fmax(v[1], fabs(ri));
-    v[2] = fmax(v[2], fabs(xi));
-    v[3] += bi * bi;
-    v[4] = fmax(v[4], fabs(bi));
-  }
-  for (int q = 0; q < 5; ++q) sh[q][threadIdx.x] = v[q];
-  __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
-    if (threadIdx.x < h) {
-      sh[0][threadIdx.x] += sh[0][threadIdx.x + h];
-      sh[1][threadIdx.x] = fmax(sh[1][threadIdx.x], sh[1][threadIdx.x + h]);
-      sh[2][threadIdx.x] = fmax(sh[2][threadIdx.x], sh[2][threadIdx.x + h]);
-      sh[3][threadIdx.x] += sh[3][threadIdx.x + h];
-      sh[4][threadIdx.x] = fmax(sh[4][threadIdx.x], sh[4][threadIdx.x + h]);
+  for (int32_t c = 0; c < kc; ++c) {
+    double v[5] = {0, 0, 0, 0, 0};
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+      const double ri = r[c * ld + i], xi = x[c * ld + i], bi = b[c * ld + i];
+      v[0] += ri * ri;
+      v[1] = fmax(v[1], fabs(ri));
+      v[2] = fmax(v[2], fabs(xi));
+      v[3] += bi * bi;
+      v[4] = fmax(v[4], fabs(bi));
     }
+    for (int q = 0; q < 5; ++q) sh[q][threadIdx.x] = v[q];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if (threadIdx.x < h) {
+        sh[0][threadIdx.x] += sh[0][threadIdx.x + h];
+        sh[1][threadIdx.x] = fmax(sh[1][threadIdx.x], sh[1][threadIdx.x + h]);
+        sh[2][threadIdx.x] = fmax(sh[2][threadIdx.x], sh[2][threadIdx.x + h]);
+        sh[3][threadIdx.x] += sh[3][threadIdx.x + h];
+        sh[4][threadIdx.x] = fmax(sh[4][threadIdx.x], sh[4][threadIdx.x + h]);
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0)
+      for (int q = 0; q < 5; ++q) part[(c * kRedBlocks + blockIdx.x) * 5 + q] = sh[q][0];
     __syncthreads();
   }
-  if (threadIdx.x == 0)
-    for (int q = 0; q < 5; ++q) part[blockIdx.x * 5 + q] = sh[q][0];
 }
-__global__ __launch_bounds__(256) void k_norms_final(int32_t nb, const double* __restrict__ part,
-                                                     double* __restrict__ out) {
-  if (threadIdx.x != 0) return;
+// One wavefront per column: lane l folds partials l, l+64, ... in order, then a
+// fixed butterfly.
+__global__ __launch_bounds__(64) void k_norms_final(int32_t nb, const double* __restrict__ part,
+                                                    double* __restrict__ out) {
+  const int lane = threadIdx.x, c = blockIdx.x;
   double v[5] = {0, 0, 0, 0, 0};
-  for (int32_t i = 0; i < nb; ++i) {
-    v[0] += part[i * 5 + 0];
-    v[1] = fmax(v[1], part[i * 5 + 1]);
-    v[2] = fmax(v[2], part[i * 5 + 2]);
-    v[3] += part[i * 5 + 3];
-    v[4] = fmax(v[4], part[i * 5 + 4]);
+  for (int32_t i = lane; i < nb; i += kWave) {
+    const double* p = part + (c * nb + i) * 5;
+    v[0] += p[0];
+    v[1] = fmax(v[1], p[1]);
+    v[2] = fmax(v[2], p[2]);
+    v[3] += p[3];
+    v[4] = fmax(v[4], p[4]);
   }
-  for (int q = 0; q < 5; ++q) out[q] = v[q];
+  for (int off = 32; off > 0; off >>= 1) {
+    v[0] += __shfl_xor(v[0], off);
+    v[1] = fmax(v[1], __shfl_xor(v[1], off));
+    v[2] = fmax(v[2], __shfl_xor(v[2], off));
+    v[3] += __shfl_xor(v[3], off);
+    v[4] = fmax(v[4], __shfl_xor(v[4], off));
+  }
+  if (lane == 0)
+    for (int q = 0; q < 5; ++q) out[c * 9 + q] = v[q];
 }
-void launch_norms(hipStream_t st, int32_t n, const double* r, const double* x, const double* b, double* partials,
-                  double* out) {
-  hipLaunchKernelGGL(k_norms_partial, dim3(kRedBlocks), dim3(256), 0, st, n, r, x, b, partials);
-  hipLaunchKernelGGL(k_norms_final, dim3(1), dim3(64), 0, st, kRedBlocks, partials, out);
+void launch_norms(hipStream_t st, int32_t n, int32_t kc, const double* r, const double* x, const double* b, int64_t ld,
+                  double* partials, double* out) {
+  hipLaunchKernelGGL(k_norms_partial, dim3(kRedBlocks), dim3(256), 0, st, n, kc, r, x, b, ld, partials);
+  hipLaunchKernelGGL(k_norms_final, dim3(kc), dim3(64), 0, st, kRedBlocks, partials, out);
 }
 
 __global__ __launch_bounds__(256) void k_anorm_partial(int32_t n, const int64_t* __restrict__ ptr,
@@ -1144,11 +1277,12 @@ void launch_anorm(hipStream_t st, int32_t n, const int64_t* csr_ptr, const doubl
   hipLaunchKernelGGL(k_max_final, dim3(1), dim3(64), 0, st, kRedBlocks, partials, out);
 }
 
-__global__ void k_axpy(int32_t n, const double* __restrict__ d, double* __restrict__ x) {
-  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = x[i] + d[i];
+__global__ void k_axpy(int32_t n, int32_t kc, const double* __restrict__ d, double* __restrict__ x, int64_t ld) {
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    for (int32_t c = 0; c < kc; ++c) x[c * ld + i] = x[c * ld + i] + d[c * ld + i];
 }
-void launch_axpy(hipStream_t st, int32_t n, const double* d, double* x) {
-  hipLaunchKernelGGL(k_axpy, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, n, d, x);
+void launch_axpy(hipStream_t st, int32_t n, int32_t kc, const double* d, double* x, int64_t ld) {
+  hipLaunchKernelGGL(k_axpy, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, n, kc, d, x, ld);
 }
 
 }  // namespace bdk

@@ -187,3 +187,28 @@ def test_fp32_ir_reaches_1e12(gpu):
              f"fallbacks={rep['fp64_fallback_blocks']}")
         assert rr <= 1e-12
         assert be <= 1e-12
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_multi_rhs_matches_single(gpu, precision):
+    """k right-hand sides solved together (one pass of the K-column kernels) give
+    what k single solves give: bit for bit in plain FP64; with FP32+IR the
+    columns refine together, so each meets the residual bound."""
+    bd = gpu
+    _say(f"case: multi-RHS precision={precision}")
+    n, r, c, v, _ = bd.generate(small_cfg1(bd))
+    rng = np.random.default_rng(7)
+    B = np.asfortranarray(rng.uniform(-1, 1, (n, 5)))
+    s = bd.Solver()
+    s.set_precision(bd.Precision(precision))
+    assert s.set_matrix(r, c, v, n) and s.analyze() and s.factor(), s.last_error_message()
+    X = B.copy(order="F")
+    assert s.solve_multi_in_place(X, 5), s.last_error_message()
+    for j in range(5):
+        x = B[:, j].copy()
+        assert s.solve_in_place(x)
+        rr = rel_residual(n, r, c, v, X[:, j], B[:, j])
+        if precision == 0:
+            assert np.array_equal(x, X[:, j]), f"column {j} differs"
+        else:
+            assert rr <= 1e-12, rr
