@@ -102,9 +102,30 @@ def log(msg):
 
 
 def rel_residual(n, r, c, v, x, b):
-    y = np.zeros(n)
-    np.add.at(y, r, v * x[c])
-    return float(np.linalg.norm(b - y) / np.linalg.norm(b))
+    """||b - A x||_2 / ||b||_2, the worst column when x and b hold several."""
+    X = x.reshape(n, -1)
+    Bm = b.reshape(n, -1)
+    worst = 0.0
+    for j in range(X.shape[1]):
+        y = np.zeros(n)
+        np.add.at(y, r, v * X[c, j])
+        worst = max(worst, float(np.linalg.norm(Bm[:, j] - y) / np.linalg.norm(Bm[:, j])))
+    return worst
+
+
+def backward_error(n, r, c, v, x, b):
+    """||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf), the worst column."""
+    X = x.reshape(n, -1)
+    Bm = b.reshape(n, -1)
+    rowsum = np.zeros(n)
+    np.add.at(rowsum, r, np.abs(v))
+    worst = 0.0
+    for j in range(X.shape[1]):
+        y = np.zeros(n)
+        np.add.at(y, r, v * X[c, j])
+        den = rowsum.max() * np.abs(X[:, j]).max() + np.abs(Bm[:, j]).max()
+        worst = max(worst, float(np.abs(Bm[:, j] - y).max() / den))
+    return worst
 
 
 def run_gpu(bd, torch, n, r, c, v_sets, b, precision, steps, warmup, profile):
@@ -120,7 +141,10 @@ def run_gpu(bd, torch, n, r, c, v_sets, b, precision, steps, warmup, profile):
     order = bd.canonical_order(r, c)
     dev = torch.device("cuda:0")
     vals_dev = torch.tensor(np.stack([vs[order] for vs in v_sets]), dtype=torch.float64, device=dev)
-    b_dev = torch.tensor(b, dtype=torch.float64, device=dev)
+    # right-hand sides as a (k, n) row-major tensor = n x k column-major
+    bk = b.reshape(n, -1)
+    nrhs = bk.shape[1]
+    b_dev = torch.tensor(np.ascontiguousarray(bk.T), dtype=torch.float64, device=dev)
     x_dev = torch.empty_like(b_dev)
     torch.cuda.synchronize()
     ext = torch.cuda.ExternalStream(s.device_stream())
@@ -131,7 +155,7 @@ def run_gpu(bd, torch, n, r, c, v_sets, b, precision, steps, warmup, profile):
             x_dev.copy_(b_dev)
         ok = s.factor_device(vals_dev[i % len(v_sets)].data_ptr())
         assert ok, s.last_error_message()
-        ok = s.solve_device(x_dev.data_ptr(), 1, n)
+        ok = s.solve_device(x_dev.data_ptr(), nrhs, n)
         assert ok, s.last_error_message()
 
     for i in range(warmup):
@@ -157,7 +181,7 @@ def run_gpu(bd, torch, n, r, c, v_sets, b, precision, steps, warmup, profile):
         f"wall={wall:.3f}s gpu={gpu_ms:.1f}ms p50={1e3 * float(np.median(times)):.2f}ms "
         f"max={1e3 * max(times):.2f}ms ir_iters={rep['ir_iterations']} repivots={rep['repivot_blocks']} "
         f"fp64_fallbacks={rep['fp64_fallback_blocks']} rr_engine={rep['rel_residual']:.2e}")
-    x = x_dev.cpu().numpy()
+    x = x_dev.cpu().numpy().T  # n x k
     last_vals = v_sets[(steps - 1) % len(v_sets)]
     prof = None
     if profile:
@@ -340,9 +364,17 @@ def main():
         return
     t = time.time()
     n, r, c, v, B = bd.generate(args.config)
-    b = B[:, 0].copy()
+    b = np.asfortranarray(B)  # n x k right-hand sides
     v_sets = [v] + [bd.perturb(v, seed=1000 + i) for i in range(1, 10)]
-    log(f"workload n={n} nnz={len(r)} generated in {time.time() - t:.1f}s")
+    log(f"workload n={n} nnz={len(r)} rhs={b.shape[1]} generated in {time.time() - t:.1f}s")
+    workload = {
+        1: f"circuit-like n={n} nnz={len(r)}: one 20000 power-law block + 9000 blocks of size 1-40, "
+           f"10 refactors at +-5%, 1 RHS",
+        2: f"batched blocks n={n} nnz={len(r)}: 50000 blocks log-uniform 1-64 + 8 dense-ish 512-2048 (30% fill), "
+           f"10 refactors at +-5%, {b.shape[1]} RHS",
+        3: f"mixed-precision stress n={n} nnz={len(r)}: circuit-like pattern, rows/cols scaled by 10^U(-4,4), "
+           f"10 refactors at +-5%, 1 RHS, FP32+IR to backward error <= 1e-12",
+    }.get(args.config, f"config {args.config} n={n} nnz={len(r)}")
 
     # Plain FP64 (the reference default, no refinement), FP64 + FP64 refinement to
     # ||r||/||b|| < 1e-12 (REFINE_ITERS semantics), and FP32 factors + FP64 refinement.
@@ -351,7 +383,10 @@ def main():
     os.environ["BLOCKDAGLU_REFINE_ITERS"] = "10"
     res64 = run_gpu(bd, torch, n, r, c, v_sets, b, 0, args.steps, args.warmup, False)
     os.environ.pop("BLOCKDAGLU_REFINE_ITERS", None)
+    if args.config == 3:
+        os.environ["BLOCKDAGLU_GPU_IR_STOP"] = "backward"
     res32 = run_gpu(bd, torch, n, r, c, v_sets, b, 1, args.steps, args.warmup, not args.no_profile)
+    os.environ.pop("BLOCKDAGLU_GPU_IR_STOP", None)
 
     def summarize(res):
         times, wall, gpu_ms, x, last_vals, rep, prof = res
@@ -360,6 +395,7 @@ def main():
                 "gpu_ms_per_step": gpu_ms / args.steps,
                 "p50_ms": float(np.percentile(ts, 50) * 1e3), "p99_ms": float(np.percentile(ts, 99) * 1e3),
                 "rel_residual": rel_residual(n, r, c, last_vals, x, b),
+                "backward_error": backward_error(n, r, c, last_vals, x, b),
                 "ir_iterations": rep["ir_iterations"], "fp64_fallback_blocks": rep["fp64_fallback_blocks"]}
 
     s64p, s64, s32 = summarize(res64p), summarize(res64), summarize(res32)
@@ -402,16 +438,19 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 factors + f64 refinement",
         "data": "synthetic (seeded generator)",
-        "config": {"workload": f"circuit-like n={n} nnz={len(r)}: one 20000 power-law block + 9000 blocks "
-                               f"of size 1-40, 10 refactors at +-5%, 1 RHS", "parallelism": "single GPU"},
+        "config": {"workload": workload, "parallelism": "single GPU"},
         "p50_ms": s32["p50_ms"], "p99_ms": s32["p99_ms"], "rel_residual": s32["rel_residual"],
+        "backward_error": s32["backward_error"],
+        "forward_error_vs_fp64": float(np.linalg.norm(res32[3] - res64[3]) / np.linalg.norm(res64[3])),
         "ir_iterations": s32["ir_iterations"], "fp64_fallback_blocks": s32["fp64_fallback_blocks"],
         "fp64": {"value": s64["value"], "ms_per_step": s64["ms_per_step"], "p50_ms": s64["p50_ms"],
                  "p99_ms": s64["p99_ms"], "rel_residual": s64["rel_residual"], "ir_iterations": s64["ir_iterations"],
-                 "dtype": "f64 + f64 refinement"},
+                 "backward_error": s64["backward_error"], "dtype": "f64 + f64 refinement"},
         "fp64_plain": {"value": s64p["value"], "ms_per_step": s64p["ms_per_step"], "p50_ms": s64p["p50_ms"],
-                       "p99_ms": s64p["p99_ms"], "rel_residual": s64p["rel_residual"], "dtype": "f64"},
+                       "p99_ms": s64p["p99_ms"], "rel_residual": s64p["rel_residual"],
+                       "backward_error": s64p["backward_error"], "dtype": "f64"},
         "fp32ir_over_fp64": s32["value"] / s64["value"],
+        "fp32ir_over_fp64_plain": s32["value"] / s64p["value"],
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernel_profile": prof,

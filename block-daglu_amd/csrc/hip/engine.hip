@@ -134,6 +134,20 @@ struct Plan {
   double* partials = nullptr;
   double* norms = nullptr;  // per column c: [c*9 + 0..4] residual norms; [kAnormSlot] ||A||_inf
   bool factored = false;
+  // Captured launch sequences, replayed per call: the refactor per precision,
+  // and the solve per (column count, vector pair). Lazily instantiated.
+  hipGraphExec_t refactor_graph[2] = {nullptr, nullptr};
+  std::map<std::vector<int32_t>, hipGraphExec_t> solve_graph;
+  void drop_graphs() {
+    for (auto& e : refactor_graph)
+      if (e) {
+        (void)hipGraphExecDestroy(e);
+        e = nullptr;
+      }
+    for (auto& kv : solve_graph) (void)hipGraphExecDestroy(kv.second);
+    solve_graph.clear();
+  }
+  ~Plan() { drop_graphs(); }
 };
 
 }  // namespace
@@ -144,6 +158,7 @@ struct bdlu_ctx {
   double fp32_growth = 1e4;
   double fp64_growth = 1e8;
   bool prof = false;
+  bool graphs = true;  // capture launch sequences into HIP graphs (BLOCKDAGLU_GPU_GRAPH=0 disables)
   double prof_ms[BDLU_PROF_KINDS] = {0};
   int64_t prof_n[BDLU_PROF_KINDS] = {0};
   double prof_flops[BDLU_PROF_KINDS] = {0};
@@ -477,8 +492,48 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
 }
 
 // Solve A x = b for the resident factors; b and x are device vectors of length n.
+// Runs `body` (a fixed sequence of kernel launches on `st`): the first time it is
+// captured into a graph and instantiated into `exec`, then the graph is
+// replayed. With use == false the launches are issued directly.
+template <class F>
+void run_captured(hipStream_t st, hipGraphExec_t& exec, bool use, F&& body) {
+  if (!use) {
+    body();
+    return;
+  }
+  if (!exec) {
+    HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipGraph_t graph = nullptr;
+    try {
+      body();
+    } catch (...) {
+      (void)hipStreamEndCapture(st, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+      throw;
+    }
+    HIPCHECK(hipStreamEndCapture(st, &graph));
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      exec = nullptr;
+      throw HipError{std::string("hipGraphInstantiate: ") + hipGetErrorString(e)};
+    }
+  }
+  HIPCHECK(hipGraphLaunch(exec, st));
+}
+
 // Solves kt columns at once: b and x are n x kt sets (column stride n).
-void block_solve(Plan& P, hipStream_t st, const double* b, double* x, int32_t kt) {
+void block_solve_launches(Plan& P, hipStream_t st, const double* b, double* x, int32_t kt);
+void block_solve(Plan& P, hipStream_t st, const double* b, double* x, int32_t kt, bool graphs) {
+  // The launch sequence depends on the column count, the vector pair and the
+  // precision each sparse block was last factored in (its factor arrays).
+  std::vector<int32_t> key{kt, b == P.b ? 0 : 1};
+  for (const SparseDev& S : P.sp) key.push_back(S.prec);
+  hipGraphExec_t& exec = P.solve_graph[key];
+  run_captured(st, exec, graphs, [&] { block_solve_launches(P, st, b, x, kt); });
+}
+
+void block_solve_launches(Plan& P, hipStream_t st, const double* b, double* x, int32_t kt) {
   const int64_t ld = P.n;
   bdk::launch_fill_i32(st, P.tickets, P.nticket, 0);
   bdk::launch_fill_sentinel(st, P.xperm, int64_t(P.n) * kt);
@@ -571,6 +626,7 @@ int bdlu_gpu_create(int device, uint64_t hbm_budget_bytes, bdlu_ctx** out) {
   c->budget = hbm_budget_bytes;
   if (const char* e = std::getenv("BLOCKDAGLU_GPU_FP32_GROWTH")) c->fp32_growth = std::max(1.0, std::atof(e));
   if (const char* e = std::getenv("BLOCKDAGLU_GPU_REPIVOT_GROWTH")) c->fp64_growth = std::max(1.0, std::atof(e));
+  if (const char* e = std::getenv("BLOCKDAGLU_GPU_GRAPH")) c->graphs = std::atoi(e) != 0;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return BDLU_E_BACKEND;
@@ -652,25 +708,28 @@ int bdlu_gpu_refactor(bdlu_ctx* c, uint64_t id, const double* values, int where,
     HIPCHECK(hipSetDevice(c->device));
     HIPCHECK(hipMemcpyAsync(P.vals, values, sizeof(double) * P.nnz,
                             where == BDLU_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
-    bdk::launch_scatter(st, P.nnz, P.vals, P.vmap, P.arena);
-    bdk::launch_anorm(st, P.n, P.csr_ptr, P.vals, P.partials, P.norms + kAnormSlot);
     const bool fp32 = precision == BDLU_FP32;
     ProfEvents pev;
     ProfEvents* pe = c->prof ? &pev : nullptr;
-    // Dense blocks (dynamic partial pivoting, reference semantics).
-    if (pe) pe->mark(BDLU_PROF_DENSE_LU, st);
-    if (fp32)
-      bdk::launch_dense_lu<float>(st, P.ndense, P.d_size, P.d_start, P.d_val0, P.d_lu0, P.arena, P.lu32, P.piv,
-                                  P.block_status, P.d_block);
-    else
-      bdk::launch_dense_lu<double>(st, P.ndense, P.d_size, P.d_start, P.d_val0, P.d_lu0, P.arena, P.lu64, P.piv,
-                                   P.block_status, P.d_block);
-    if (pe) pe->mark(BDLU_PROF_DENSE_LU, st);
-    for (SparseDev& S : P.sp) {
-      S.prec = fp32 ? 1 : 0;
-      if (fp32) factor_sparse<float>(P, S, st, S.Lx32, S.Ux32, S.Ud32, S.S32, pe, c->fp32_growth);
-      else factor_sparse<double>(P, S, st, S.Lx64, S.Ux64, S.Ud64, S.S64, pe, c->fp64_growth);
-    }
+    for (SparseDev& S : P.sp) S.prec = fp32 ? 1 : 0;
+    auto launches = [&] {
+      bdk::launch_scatter(st, P.nnz, P.vals, P.vmap, P.arena);
+      bdk::launch_anorm(st, P.n, P.csr_ptr, P.vals, P.partials, P.norms + kAnormSlot);
+      // Dense blocks (dynamic partial pivoting, reference semantics).
+      if (pe) pe->mark(BDLU_PROF_DENSE_LU, st);
+      if (fp32)
+        bdk::launch_dense_lu<float>(st, P.ndense, P.d_size, P.d_start, P.d_val0, P.d_lu0, P.arena, P.lu32, P.piv,
+                                    P.block_status, P.d_block);
+      else
+        bdk::launch_dense_lu<double>(st, P.ndense, P.d_size, P.d_start, P.d_val0, P.d_lu0, P.arena, P.lu64, P.piv,
+                                     P.block_status, P.d_block);
+      if (pe) pe->mark(BDLU_PROF_DENSE_LU, st);
+      for (SparseDev& S : P.sp) {
+        if (fp32) factor_sparse<float>(P, S, st, S.Lx32, S.Ux32, S.Ud32, S.S32, pe, c->fp32_growth);
+        else factor_sparse<double>(P, S, st, S.Lx64, S.Ux64, S.Ud64, S.S64, pe, c->fp64_growth);
+      }
+    };
+    run_captured(st, P.refactor_graph[fp32 ? 1 : 0], c->graphs && !pe, launches);
     // Collect statuses.
     std::vector<int32_t> bst(P.nblocks, 0);
     if (P.ndense) HIPCHECK(hipMemcpyAsync(bst.data(), P.block_status, sizeof(int32_t) * P.nblocks,
@@ -774,7 +833,7 @@ int bdlu_gpu_solve(bdlu_ctx* c, uint64_t id, double* B, int32_t nrhs, int64_t ld
       if (kt > kc) bdk::launch_fill_f64(st, P.b + kc * n, (kt - kc) * n, 0.0);
       ProfEvents pev;
       if (c->prof) pev.mark(BDLU_PROF_SOLVE, st);
-      block_solve(P, st, P.b, P.x, kt);
+      block_solve(P, st, P.b, P.x, kt, c->graphs && !c->prof);
       if (c->prof) {
         pev.mark(BDLU_PROF_SOLVE, st);
         HIPCHECK(hipStreamSynchronize(st));
@@ -805,7 +864,7 @@ int bdlu_gpu_solve(bdlu_ctx* c, uint64_t id, double* B, int32_t nrhs, int64_t ld
             if (!stop && h[k * 9 + 3] > 0.0) done = false;
           }
           if (done) break;
-          block_solve(P, st, P.r, P.d, kt);
+          block_solve(P, st, P.r, P.d, kt, c->graphs && !c->prof);
           bdk::launch_axpy(st, P.n, kt, P.d, P.x, n);
           ++iters;
           eval();

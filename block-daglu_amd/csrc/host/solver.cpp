@@ -348,7 +348,11 @@ bool Solver::factor_values(const double* values, bool on_device) {
 namespace {
 bdlu_ir_opts ir_options(Precision p) {
   bdlu_ir_opts o{};
+  // Default: the reference's stop rule ||r||_2/||b||_2 < 1e-12 (src/solver.cpp:1508-1521);
+  // BLOCKDAGLU_GPU_IR_STOP=backward stops on normwise backward error <= 1e-12 instead.
   o.stop_rule = 0;
+  if (const char* e = std::getenv("BLOCKDAGLU_GPU_IR_STOP"))
+    if (std::string(e) == "backward") o.stop_rule = 1;
   o.tol = 1e-12;
   int iters = 0;
   if (const char* e = blockdaglu_getenv_compat("USERALGO_REFINE_ITERS")) {

@@ -100,3 +100,53 @@ def test_serve_budget_eviction(gpu):
         assert res["error"] == bd.SolverError.None_, res
         assert np.array_equal(X, x)
     tight.close()
+
+
+def test_graph_replays_match_eager(gpu, monkeypatch):
+    """Refactor and solve launch sequences captured in HIP graphs and replayed
+    (several replays, with other requests and host synchronisation between
+    them) give exactly what the same launches issued directly give."""
+    bd = gpu
+    _say("case: graph replays vs eager launches")
+    pats = _patterns(bd, count=2)
+    reqs = _requests(bd, pats, 8, seed=21)
+    outs = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("BLOCKDAGLU_GPU_GRAPH", mode)
+        for prec in (bd.Precision.FP64, bd.Precision.FP32_IR):
+            srv = bd.Server(slots=1, hbm_budget_gb=8.0, precision=prec)
+            pid = [srv.add_pattern(r, c, v, n) for (n, r, c, v) in pats]
+            outs[(mode, prec)] = [srv.wait(srv.submit(pid[p], vals, B)) for (p, vals, B) in reqs]
+            srv.close()
+    for prec in (bd.Precision.FP64, bd.Precision.FP32_IR):
+        for (xg, rg), (xe, re) in zip(outs[("1", prec)], outs[("0", prec)]):
+            assert rg["error"] == bd.SolverError.None_ and re["error"] == bd.SolverError.None_
+            assert np.array_equal(xg, xe), "graph replay differs from eager launches"
+            assert rg["ir_iterations"] == re["ir_iterations"]
+
+
+def test_precision_switch_on_a_cached_plan(gpu):
+    """A plan factored in FP64 and then reused by an FP32+IR solver must solve with
+    its FP32 factors: same result and iteration count as a fresh context."""
+    bd = gpu
+    _say("case: precision switch on a cached plan")
+    (n, r, c, v), = _patterns(bd, count=1)
+    b = np.random.default_rng(5).uniform(-1, 1, n)
+    s64 = bd.Solver()
+    assert s64.set_matrix(r, c, v, n) and s64.analyze() and s64.factor()
+    x64 = b.copy()
+    assert s64.solve_in_place(x64)
+    s64.close()
+    s32 = bd.Solver()
+    s32.set_precision(bd.Precision.FP32_IR)
+    assert s32.set_matrix(r, c, v, n) and s32.analyze() and s32.factor()
+    x32 = b.copy()
+    assert s32.solve_in_place(x32)
+    rep = s32.report()
+    s32.close()
+    srv = bd.Server(slots=1, hbm_budget_gb=8.0, precision=bd.Precision.FP32_IR)
+    X, res = srv.wait(srv.submit(srv.add_pattern(r, c, v, n), v, b))
+    srv.close()
+    _say(f"  plan_reused={rep['plan_reused']} iters solver={rep['ir_iterations']} fresh={res['ir_iterations']}")
+    assert np.array_equal(X, x32)
+    assert rep["ir_iterations"] == res["ir_iterations"]
