@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <numeric>
+#include <functional>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -134,6 +136,16 @@ struct Plan {
   double* partials = nullptr;
   double* norms = nullptr;  // per column c: [c*9 + 0..4] residual norms; [kAnormSlot] ||A||_inf
   bool factored = false;
+  // Launch schedule, picked per plan by the autotuner (every choice gives the
+  // same results bit for bit; only the time differs).
+  struct Schedule {
+    int32_t small_tile_below = 0;  // trailing updates of size < this use 64x64 tiles
+    bool dense_lpt = false;        // dense blocks launched largest first
+    int32_t seg_grid = 1024;       // persistent workgroups of the dense-block solve
+    bool factor_tuned[2] = {false, false};
+    bool solve_tuned = false;
+  } sched;
+  int32_t* d_lpt = nullptr;  // dense blocks by size, largest first
   // Captured launch sequences, replayed per call: the refactor per precision,
   // and the solve per (column count, vector pair). Lazily instantiated.
   hipGraphExec_t refactor_graph[2] = {nullptr, nullptr};
@@ -159,6 +171,7 @@ struct bdlu_ctx {
   double fp64_growth = 1e8;
   bool prof = false;
   bool graphs = true;  // capture launch sequences into HIP graphs (BLOCKDAGLU_GPU_GRAPH=0 disables)
+  bool autotune = true;  // time schedule candidates on a plan's first calls (BLOCKDAGLU_GPU_AUTOTUNE=0 disables)
   double prof_ms[BDLU_PROF_KINDS] = {0};
   int64_t prof_n[BDLU_PROF_KINDS] = {0};
   double prof_flops[BDLU_PROF_KINDS] = {0};
@@ -338,6 +351,13 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
   P.d_size = A.copy(P.h_dense_size, st);
   P.d_start = A.copy(P.h_dense_start, st);
   P.d_block = A.copy(P.h_dense_block, st);
+  {
+    std::vector<int32_t> lpt(P.h_dense_size.size());
+    std::iota(lpt.begin(), lpt.end(), 0);
+    std::stable_sort(lpt.begin(), lpt.end(),
+                     [&](int32_t a, int32_t b) { return P.h_dense_size[a] > P.h_dense_size[b]; });
+    P.d_lpt = A.copy(lpt, st);
+  }
   P.d_val0 = A.copy(P.h_dense_val0, st);
   P.d_lu0 = A.copy(P.h_dense_lu0, st);
   P.lu0_of_block = A.copy(lu0_of_block, st);
@@ -544,7 +564,7 @@ void block_solve_launches(Plan& P, hipStream_t st, const double* b, double* x, i
       // Each dense block's factors are read in the precision it was factored in.
       bdk::launch_dense_seg_solve(st, kt, ph.ntasks, ph.tasks, P.R, P.lu0_of_block, P.lu64, P.lu32, P.prec_of_block,
                                   P.piv, P.off_ptr, P.off_col, P.off_val, P.tickets + tix, P.bperm, P.xperm, ld,
-                                  P.err);
+                                  P.err, P.sched.seg_grid);
       ++tix;
       continue;
     }
@@ -576,6 +596,65 @@ void block_solve_launches(Plan& P, hipStream_t st, const double* b, double* x, i
   bdk::launch_unpermute(st, P.n, kt, P.Q, P.xperm, ld, x, ld);
 }
 
+// Autotuner: times each candidate schedule once (after one warm-up run) with
+// events on the engine stream and keeps the fastest. Candidates only change
+// tile sizes, workgroup order and grid size, never the arithmetic.
+float time_launches(hipStream_t st, const std::function<void()>& f) {
+  hipEvent_t a, b;
+  HIPCHECK(hipEventCreate(&a));
+  HIPCHECK(hipEventCreate(&b));
+  HIPCHECK(hipEventRecord(a, st));
+  f();
+  HIPCHECK(hipEventRecord(b, st));
+  HIPCHECK(hipEventSynchronize(b));
+  float ms = 0.f;
+  HIPCHECK(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms;
+}
+
+void autotune_factor(Plan& P, hipStream_t st, const std::function<void()>& launches) {
+  int32_t max_m = 0;
+  for (const SparseDev& S : P.sp) max_m = std::max(max_m, S.m);
+  std::vector<int32_t> tiles{0};
+  if (max_m > 256) tiles = {0, 768, 2048, 1 << 30};
+  std::vector<bool> lpt{false};
+  if (P.ndense > 1) lpt = {false, true};
+  launches();  // warm-up
+  float best = 1e30f;
+  Plan::Schedule pick = P.sched;
+  for (int32_t t : tiles)
+    for (bool l : lpt) {
+      P.sched.small_tile_below = t;
+      P.sched.dense_lpt = l;
+      const float ms = time_launches(st, launches);
+      if (ms < best) {
+        best = ms;
+        pick = P.sched;
+      }
+    }
+  P.sched.small_tile_below = pick.small_tile_below;
+  P.sched.dense_lpt = pick.dense_lpt;
+}
+
+void autotune_solve(Plan& P, hipStream_t st, int32_t kt) {
+  std::vector<int32_t> grids{256, 512, 1024};
+  auto run = [&] { block_solve_launches(P, st, P.b, P.x, kt); };
+  run();  // warm-up
+  float best = 1e30f;
+  int32_t pick = P.sched.seg_grid;
+  for (int32_t g : grids) {
+    P.sched.seg_grid = g;
+    const float ms = time_launches(st, run);
+    if (ms < best) {
+      best = ms;
+      pick = g;
+    }
+  }
+  P.sched.seg_grid = pick;
+}
+
 template <typename T>
 void factor_sparse(Plan& P, SparseDev& S, hipStream_t st, T* Lx, T* Ux, T* Ud, T* Sm, ProfEvents* pe,
                    double growth_limit) {
@@ -600,7 +679,8 @@ void factor_sparse(Plan& P, SparseDev& S, hipStream_t st, T* Lx, T* Ux, T* Ud, T
     tt.panel_events = &pe->ev[BDLU_PROF_TAIL_PANEL];
   }
   T* inv = sizeof(T) == 8 ? reinterpret_cast<T*>(S.inv64) : reinterpret_cast<T*>(S.inv32);
-  bdk::launch_tail_lu<T>(st, S.m, Sm, S.m, growth_limit, S.status, inv, pe ? &tt : nullptr);
+  bdk::launch_tail_lu<T>(st, S.m, Sm, S.m, growth_limit, S.status, inv, pe ? &tt : nullptr,
+                         P.sched.small_tile_below);
 }
 
 }  // namespace
@@ -627,6 +707,7 @@ int bdlu_gpu_create(int device, uint64_t hbm_budget_bytes, bdlu_ctx** out) {
   if (const char* e = std::getenv("BLOCKDAGLU_GPU_FP32_GROWTH")) c->fp32_growth = std::max(1.0, std::atof(e));
   if (const char* e = std::getenv("BLOCKDAGLU_GPU_REPIVOT_GROWTH")) c->fp64_growth = std::max(1.0, std::atof(e));
   if (const char* e = std::getenv("BLOCKDAGLU_GPU_GRAPH")) c->graphs = std::atoi(e) != 0;
+  if (const char* e = std::getenv("BLOCKDAGLU_GPU_AUTOTUNE")) c->autotune = std::atoi(e) != 0;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return BDLU_E_BACKEND;
@@ -686,6 +767,19 @@ int bdlu_gpu_plan_release(bdlu_ctx* c, uint64_t id) {
   return BDLU_OK;
 }
 
+int bdlu_gpu_plan_schedule(bdlu_ctx* c, uint64_t id, int32_t* out4) {
+  if (!c || !out4) return BDLU_E_INVALID;
+  std::lock_guard<std::mutex> g(c->mu);
+  auto it = c->plans.find(id);
+  if (it == c->plans.end()) return set_err(c, BDLU_E_INVALID, "unknown plan");
+  const Plan::Schedule& sc = it->second->sched;
+  out4[0] = sc.small_tile_below;
+  out4[1] = sc.dense_lpt ? 1 : 0;
+  out4[2] = sc.seg_grid;
+  out4[3] = (sc.factor_tuned[0] ? 1 : 0) | (sc.factor_tuned[1] ? 2 : 0) | (sc.solve_tuned ? 4 : 0);
+  return BDLU_OK;
+}
+
 int bdlu_gpu_plan_bytes(bdlu_ctx* c, uint64_t id, uint64_t* bytes) {
   if (!c || !bytes) return BDLU_E_INVALID;
   std::lock_guard<std::mutex> g(c->mu);
@@ -717,19 +811,26 @@ int bdlu_gpu_refactor(bdlu_ctx* c, uint64_t id, const double* values, int where,
       bdk::launch_anorm(st, P.n, P.csr_ptr, P.vals, P.partials, P.norms + kAnormSlot);
       // Dense blocks (dynamic partial pivoting, reference semantics).
       if (pe) pe->mark(BDLU_PROF_DENSE_LU, st);
+      const int32_t* order = P.sched.dense_lpt ? P.d_lpt : nullptr;
       if (fp32)
         bdk::launch_dense_lu<float>(st, P.ndense, P.d_size, P.d_start, P.d_val0, P.d_lu0, P.arena, P.lu32, P.piv,
-                                    P.block_status, P.d_block);
+                                    P.block_status, P.d_block, order);
       else
         bdk::launch_dense_lu<double>(st, P.ndense, P.d_size, P.d_start, P.d_val0, P.d_lu0, P.arena, P.lu64, P.piv,
-                                     P.block_status, P.d_block);
+                                     P.block_status, P.d_block, order);
       if (pe) pe->mark(BDLU_PROF_DENSE_LU, st);
       for (SparseDev& S : P.sp) {
         if (fp32) factor_sparse<float>(P, S, st, S.Lx32, S.Ux32, S.Ud32, S.S32, pe, c->fp32_growth);
         else factor_sparse<double>(P, S, st, S.Lx64, S.Ux64, S.Ud64, S.S64, pe, c->fp64_growth);
       }
     };
-    run_captured(st, P.refactor_graph[fp32 ? 1 : 0], c->graphs && !pe, launches);
+    const int pi = fp32 ? 1 : 0;
+    if (c->autotune && !pe && !P.sched.factor_tuned[pi]) {
+      autotune_factor(P, st, launches);
+      P.sched.factor_tuned[pi] = true;
+      P.drop_graphs();  // recapture with the chosen schedule
+    }
+    run_captured(st, P.refactor_graph[pi], c->graphs && !pe, launches);
     // Collect statuses.
     std::vector<int32_t> bst(P.nblocks, 0);
     if (P.ndense) HIPCHECK(hipMemcpyAsync(bst.data(), P.block_status, sizeof(int32_t) * P.nblocks,
@@ -832,6 +933,11 @@ int bdlu_gpu_solve(bdlu_ctx* c, uint64_t id, double* B, int32_t nrhs, int64_t ld
         HIPCHECK(hipMemcpyAsync(P.b + k * n, B + (c0 + k) * ldb, sizeof(double) * n, in, st));
       if (kt > kc) bdk::launch_fill_f64(st, P.b + kc * n, (kt - kc) * n, 0.0);
       ProfEvents pev;
+      if (c->autotune && !c->prof && !P.sched.solve_tuned) {
+        autotune_solve(P, st, kt);
+        P.sched.solve_tuned = true;
+        P.drop_graphs();
+      }
       if (c->prof) pev.mark(BDLU_PROF_SOLVE, st);
       block_solve(P, st, P.b, P.x, kt, c->graphs && !c->prof);
       if (c->prof) {

@@ -40,7 +40,7 @@ void launch_fill_sentinel(hipStream_t st, double* p, int64_t n);
 template <typename T>
 void launch_dense_lu(hipStream_t st, int32_t nd, const int32_t* sizes, const int32_t* starts, const int64_t* val0,
                      const int64_t* lu0, const double* arena, T* lu, int32_t* piv, int32_t* status,
-                     const int32_t* block_of);
+                     const int32_t* block_of, const int32_t* order = nullptr);
 
 void launch_rowscale(hipStream_t st, const SpView& v, const double* arena, double* rs);
 template <typename T>
@@ -62,7 +62,7 @@ struct TailTimer {
 // inv: 2*64*64 scratch for the diagonal blocks' inverses.
 template <typename T>
 void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, int32_t* status, T* inv,
-                    TailTimer* timer = nullptr);
+                    TailTimer* timer = nullptr, int32_t small_tile_below = 0);
 
 // ---- solve ----
 // Vector sets hold kc columns (right-hand sides), column c at base + c * ld.
@@ -78,7 +78,7 @@ void launch_dense_seg_solve(hipStream_t st, int32_t kc, int32_t ntasks, const in
                             const int64_t* lu0_of_block, const double* lu64, const float* lu32,
                             const int32_t* prec_of_block, const int32_t* gsrc, const int32_t* off_ptr,
                             const int32_t* off_col, const double* off_val, int32_t* ticket, const double* bperm,
-                            double* xperm, int64_t ldv, int32_t* err);
+                            double* xperm, int64_t ldv, int32_t* err, int32_t grid = 1024);
 // Sparse block solve pieces; y is an s x kc work set in position order.
 void launch_sp_gather(hipStream_t st, int32_t s, int32_t start, int32_t kc, const int32_t* rowperm, const double* rs,
                       const int32_t* off_ptr, const int32_t* off_col, const double* off_val, const double* bperm,

@@ -152,10 +152,12 @@ __global__ __launch_bounds__(64) void k_dense_lu(int32_t nd, const int32_t* __re
                                                  const int32_t* __restrict__ starts, const int64_t* __restrict__ val0,
                                                  const int64_t* __restrict__ lu0, const double* __restrict__ arena,
                                                  T* __restrict__ lu, int32_t* __restrict__ piv,
-                                                 int32_t* __restrict__ status, const int32_t* __restrict__ block_of) {
+                                                 int32_t* __restrict__ status, const int32_t* __restrict__ block_of,
+                                                 const int32_t* __restrict__ order) {
   __shared__ T a[64 * kLdsPad];
   const int lane = threadIdx.x;
-  const int32_t t = blockIdx.x;
+  // order: which dense block each workgroup takes (e.g. largest first), or none.
+  const int32_t t = order ? order[blockIdx.x] : static_cast<int32_t>(blockIdx.x);
   if (t >= nd) return;
   const int s = __builtin_amdgcn_readfirstlane(sizes[t]);
   const double* src = arena + val0[t];
@@ -234,15 +236,17 @@ __global__ __launch_bounds__(64) void k_dense_lu(int32_t nd, const int32_t* __re
 template <typename T>
 void launch_dense_lu(hipStream_t st, int32_t nd, const int32_t* sizes, const int32_t* starts, const int64_t* val0,
                      const int64_t* lu0, const double* arena, T* lu, int32_t* piv, int32_t* status,
-                     const int32_t* block_of) {
+                     const int32_t* block_of, const int32_t* order) {
   if (nd <= 0) return;
   hipLaunchKernelGGL(k_dense_lu<T>, dim3(nd), dim3(64), 0, st, nd, sizes, starts, val0, lu0, arena, lu, piv,
-                     status, block_of);
+                     status, block_of, order);
 }
 template void launch_dense_lu<double>(hipStream_t, int32_t, const int32_t*, const int32_t*, const int64_t*,
-                                      const int64_t*, const double*, double*, int32_t*, int32_t*, const int32_t*);
+                                      const int64_t*, const double*, double*, int32_t*, int32_t*, const int32_t*,
+                                      const int32_t*);
 template void launch_dense_lu<float>(hipStream_t, int32_t, const int32_t*, const int32_t*, const int64_t*,
-                                     const int64_t*, const double*, float*, int32_t*, int32_t*, const int32_t*);
+                                     const int64_t*, const double*, float*, int32_t*, int32_t*, const int32_t*,
+                                     const int32_t*);
 
 // ---------------------------------------------------------------------------
 // Sparse blocks: KLU scale=1 row sums, in a fixed order per row.
@@ -591,12 +595,17 @@ struct TileJob {
 // All global loads are unconditional with clamped indices (a conditional load
 // turns into a branch that waits for it). Workgroups below
 // j0.tiles_m*j0.tiles_n run job 0, the rest job 1.
-constexpr int kGT = 128;       // output tile edge
-constexpr int kGLd = kGT + 1;  // LDS row length (padded against bank conflicts)
-template <typename T, bool kUpdate>
+// kTile: output tile edge (128, or 64 for small trailing matrices, where 128
+// leaves most CUs idle); each wavefront computes a kTile/2 square.
+template <typename T, bool kUpdate, int kTile>
 __device__ __forceinline__ void tile_gemm(const TileJob<T>& j0, const TileJob<T>& j1, double limit,
                                           int32_t* __restrict__ status) {
   using V = typename MfmaOps<T>::v4;
+  constexpr int kGT = kTile;
+  constexpr int kGLd = kGT + 1;    // LDS row length (padded against bank conflicts)
+  constexpr int kWT = kGT / 2;     // wavefront sub-tile edge
+  constexpr int kMB = kGT / 32;    // 16x16 MFMA blocks per wavefront and dimension
+  constexpr int kRounds = kGT / 64;  // rounds of 16 staged elements per thread and operand
   __shared__ T As[kNb * kGLd];  // As[k*kGLd + i] = A(i0 + i, k)
   __shared__ T Bs[kNb * kGLd];  // Bs[k*kGLd + j] = +-B(k, c0 + j)
   const int32_t n0 = j0.tiles_m * j0.tiles_n;
@@ -606,15 +615,15 @@ __device__ __forceinline__ void tile_gemm(const TileJob<T>& j0, const TileJob<T>
   const int32_t i0 = (tb % J.tiles_m) * kGT, c0 = (tb / J.tiles_m) * kGT;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+  const int wr = (wave >> 1) * kWT, wc = (wave & 1) * kWT;
   const int li = lane & 15, lk = lane >> 4;
   const int32_t M = J.M, N = J.N, K = J.K;
-  V acc[4][4];  // acc[c][a]: C^T block (j-block c, i-block a)
+  V acc[kMB][kMB];  // acc[c][a]: C^T block (j-block c, i-block a)
   if (kUpdate) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
+    for (int c = 0; c < kMB; ++c)
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < kMB; ++a)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int32_t i = min(i0 + wr + a * 16 + li, M - 1);
@@ -623,15 +632,15 @@ __device__ __forceinline__ void tile_gemm(const TileJob<T>& j0, const TileJob<T>
         }
   } else {
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
+    for (int c = 0; c < kMB; ++c)
 #pragma unroll
-      for (int a = 0; a < 4; ++a) acc[c][a] = V{T(0), T(0), T(0), T(0)};
+      for (int a = 0; a < kMB; ++a) acc[c][a] = V{T(0), T(0), T(0), T(0)};
   }
   // A tile: 128 x 64, B tile: 64 x 128, 32 elements each per thread, loaded in
   // two rounds of 16 with the unit-stride index running across threads.
   const T sgn = kUpdate ? T(-1) : T(1);
 #pragma unroll
-  for (int half = 0; half < 2; ++half) {
+  for (int half = 0; half < kRounds; ++half) {
     T v[16];
     int ii[16], kk[16];
 #pragma unroll
@@ -639,7 +648,7 @@ __device__ __forceinline__ void tile_gemm(const TileJob<T>& j0, const TileJob<T>
       const int e = (half * 16 + u) * 256 + tid;
       if (J.a_r == 1) {
         ii[u] = e & (kGT - 1);
-        kk[u] = e >> 7;
+        kk[u] = e / kGT;
       } else {
         kk[u] = e & (kNb - 1);
         ii[u] = e >> 6;
@@ -651,7 +660,7 @@ __device__ __forceinline__ void tile_gemm(const TileJob<T>& j0, const TileJob<T>
     for (int u = 0; u < 16; ++u) As[kk[u] * kGLd + ii[u]] = (i0 + ii[u] < M && kk[u] < K) ? v[u] : T(0);
   }
 #pragma unroll
-  for (int half = 0; half < 2; ++half) {
+  for (int half = 0; half < kRounds; ++half) {
     T v[16];
     int jj[16], kk[16];
 #pragma unroll
@@ -662,7 +671,7 @@ __device__ __forceinline__ void tile_gemm(const TileJob<T>& j0, const TileJob<T>
         jj[u] = e >> 6;
       } else {
         jj[u] = e & (kGT - 1);
-        kk[u] = e >> 7;
+        kk[u] = e / kGT;
       }
       const int32_t gj = min(c0 + jj[u], N - 1), gk = min(kk[u], K - 1);
       v[u] = J.B[int64_t(gk) * J.b_r + int64_t(gj) * J.b_c];
@@ -673,21 +682,21 @@ __device__ __forceinline__ void tile_gemm(const TileJob<T>& j0, const TileJob<T>
   }
   __syncthreads();
   for (int k = 0; k < kNb; k += 4) {
-    T af[4], bf[4];
+    T af[kMB], bf[kMB];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) af[a] = As[(k + lk) * kGLd + wr + a * 16 + li];
+    for (int a = 0; a < kMB; ++a) af[a] = As[(k + lk) * kGLd + wr + a * 16 + li];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) bf[c] = Bs[(k + lk) * kGLd + wc + c * 16 + li];
+    for (int c = 0; c < kMB; ++c) bf[c] = Bs[(k + lk) * kGLd + wc + c * 16 + li];
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
+    for (int c = 0; c < kMB; ++c)
 #pragma unroll
-      for (int a = 0; a < 4; ++a) acc[c][a] = MfmaOps<T>::mma(bf[c], af[a], acc[c][a]);
+      for (int a = 0; a < kMB; ++a) acc[c][a] = MfmaOps<T>::mma(bf[c], af[a], acc[c][a]);
   }
   T lmax = T(0);
 #pragma unroll
-  for (int c = 0; c < 4; ++c)
+  for (int c = 0; c < kMB; ++c)
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < kMB; ++a)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int32_t i = i0 + wr + a * 16 + li;
@@ -705,14 +714,14 @@ __device__ __forceinline__ void tile_gemm(const TileJob<T>& j0, const TileJob<T>
 }
 // Trailing update (op 0 only) and the panel pair (ops 1 and 2): two kernels so
 // profiles and counters separate them.
-template <typename T>
+template <typename T, int kTile>
 __global__ __launch_bounds__(256) void k_tail_update(TileJob<T> j, double limit, int32_t* __restrict__ status) {
-  tile_gemm<T, true>(j, j, limit, status);
+  tile_gemm<T, true, kTile>(j, j, limit, status);
 }
-template <typename T>
+template <typename T, int kTile>
 __global__ __launch_bounds__(256) void k_tail_panel(TileJob<T> j0, TileJob<T> j1, double limit,
                                                     int32_t* __restrict__ status) {
-  tile_gemm<T, false>(j0, j1, limit, status);
+  tile_gemm<T, false, kTile>(j0, j1, limit, status);
 }
 
 namespace {
@@ -725,38 +734,51 @@ hipEvent_t timer_event(std::vector<hipEvent_t>* pool, hipStream_t st) {
 }
 }  // namespace
 
+// Panel jobs and trailing update of one panel step with kTile x kTile tiles.
+template <typename T, int kTile>
+void tail_step(hipStream_t st, T* S, int64_t lds, int32_t p, int32_t b, int32_t rest, double limit,
+               int32_t* status, T* inv, TailTimer* timer) {
+  const int32_t tr = (rest + kTile - 1) / kTile;
+  TileJob<T> l21{rest, b, b, 2, tr, 1, S + int64_t(p) * lds + p + b, 1, lds, inv + kNb * kNb, kNb, 1,
+                 S + int64_t(p) * lds + p + b, 1, lds};
+  TileJob<T> u12{b, rest, b, 1, 1, tr, inv, kNb, 1, S + int64_t(p + b) * lds + p, 1, lds,
+                 S + int64_t(p + b) * lds + p, 1, lds};
+  hipLaunchKernelGGL((k_tail_panel<T, kTile>), dim3(2 * tr), dim3(256), 0, st, l21, u12, limit, status);
+  if (timer && timer->panel_events) timer_event(timer->panel_events, st);
+  TileJob<T> upd{rest, rest, b, 0, tr, tr, S + int64_t(p) * lds + p + b, 1, lds,
+                 S + int64_t(p + b) * lds + p, 1, lds, S + int64_t(p + b) * lds + p + b, 1, lds};
+  if (timer && timer->gemm_events) timer_event(timer->gemm_events, st);
+  hipLaunchKernelGGL((k_tail_update<T, kTile>), dim3(tr * tr), dim3(256), 0, st, upd, limit, status);
+  if (timer && timer->gemm_events) {
+    timer_event(timer->gemm_events, st);
+    const double r = rest;
+    timer->gemm_flops->push_back(2.0 * r * r * b);
+    timer->gemm_bytes->push_back(sizeof(T) * (2.0 * r * r + 2.0 * r * b));
+  }
+}
+
 template <typename T>
 void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, int32_t* status, T* inv,
-                    TailTimer* timer) {
+                    TailTimer* timer, int32_t small_tile_below) {
   for (int32_t p = 0; p < m; p += kNb) {
     const int32_t b = (m - p) < kNb ? (m - p) : kNb;
     const int32_t rest = m - p - b;
     if (timer && timer->panel_events) timer_event(timer->panel_events, st);
     hipLaunchKernelGGL(k_diag_lu_inv<T>, dim3(1), dim3(kDiagWaves * kNb), 0, st, S, lds, p, b, limit, status, inv);
-    const int32_t tr = (rest + kGT - 1) / kGT;
-    if (rest > 0) {
-      TileJob<T> l21{rest, b, b, 2, tr, 1, S + int64_t(p) * lds + p + b, 1, lds, inv + kNb * kNb, kNb, 1,
-                     S + int64_t(p) * lds + p + b, 1, lds};
-      TileJob<T> u12{b, rest, b, 1, 1, tr, inv, kNb, 1, S + int64_t(p + b) * lds + p, 1, lds,
-                     S + int64_t(p + b) * lds + p, 1, lds};
-      hipLaunchKernelGGL(k_tail_panel<T>, dim3(2 * tr), dim3(256), 0, st, l21, u12, limit, status);
+    if (rest <= 0) {
+      if (timer && timer->panel_events) timer_event(timer->panel_events, st);
+      break;
     }
-    if (timer && timer->panel_events) timer_event(timer->panel_events, st);
-    if (rest <= 0) break;
-    TileJob<T> upd{rest, rest, b, 0, tr, tr, S + int64_t(p) * lds + p + b, 1, lds,
-                   S + int64_t(p + b) * lds + p, 1, lds, S + int64_t(p + b) * lds + p + b, 1, lds};
-    if (timer && timer->gemm_events) timer_event(timer->gemm_events, st);
-    hipLaunchKernelGGL(k_tail_update<T>, dim3(tr * tr), dim3(256), 0, st, upd, limit, status);
-    if (timer && timer->gemm_events) {
-      timer_event(timer->gemm_events, st);
-      const double r = rest;
-      timer->gemm_flops->push_back(2.0 * r * r * b);
-      timer->gemm_bytes->push_back(sizeof(T) * (2.0 * r * r + 2.0 * r * b));
-    }
+    if (rest < small_tile_below)
+      tail_step<T, 64>(st, S, lds, p, b, rest, limit, status, inv, timer);
+    else
+      tail_step<T, 128>(st, S, lds, p, b, rest, limit, status, inv, timer);
   }
 }
-template void launch_tail_lu<double>(hipStream_t, int32_t, double*, int64_t, double, int32_t*, double*, TailTimer*);
-template void launch_tail_lu<float>(hipStream_t, int32_t, float*, int64_t, double, int32_t*, float*, TailTimer*);
+template void launch_tail_lu<double>(hipStream_t, int32_t, double*, int64_t, double, int32_t*, double*, TailTimer*,
+                                     int32_t);
+template void launch_tail_lu<float>(hipStream_t, int32_t, float*, int64_t, double, int32_t*, float*, TailTimer*,
+                                    int32_t);
 
 // ---------------------------------------------------------------------------
 // Solve.
@@ -934,9 +956,9 @@ void launch_dense_seg_solve(hipStream_t st, int32_t kc, int32_t ntasks, const in
                             const int64_t* lu0_of_block, const double* lu64, const float* lu32,
                             const int32_t* prec_of_block, const int32_t* gsrc, const int32_t* off_ptr,
                             const int32_t* off_col, const double* off_val, int32_t* ticket, const double* bperm,
-                            double* xperm, int64_t ldv, int32_t* err) {
+                            double* xperm, int64_t ldv, int32_t* err, int32_t grid) {
   if (ntasks <= 0) return;
-  const int g = ntasks < 1024 ? ntasks : 1024;  // up to four single-wave workgroups per CU
+  const int g = ntasks < grid ? ntasks : grid;  // persistent single-wave workgroups
 #define BDLU_SEG(KK)                                                                                              \
   hipLaunchKernelGGL(k_dense_seg_solve<KK>, dim3(g), dim3(64), 0, st, ntasks, tasks, R, lu0_of_block, lu64, lu32, \
                      prec_of_block, gsrc, off_ptr, off_col, off_val, ticket, bperm, xperm, ldv, err)

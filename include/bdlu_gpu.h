@@ -120,6 +120,14 @@ const char* bdlu_gpu_last_error(bdlu_ctx* ctx);
 int bdlu_gpu_plan_upload(bdlu_ctx* ctx, const bdlu_plan_desc* plan, uint64_t* plan_id);
 int bdlu_gpu_plan_release(bdlu_ctx* ctx, uint64_t plan_id);
 int bdlu_gpu_plan_bytes(bdlu_ctx* ctx, uint64_t plan_id, uint64_t* bytes);
+// Launch schedule the autotuner picked for a plan (no counterpart in the
+// reference): out[0] trailing updates below this size use 64x64 tiles,
+// out[1] dense blocks launched largest first (0/1), out[2] persistent
+// workgroups of the dense-block solve, out[3] bit 0/1: refactor tuned for
+// FP64/FP32, bit 2: solve tuned. Tuning happens on a plan's first refactor
+// (per precision) and first solve unless BLOCKDAGLU_GPU_AUTOTUNE=0; every
+// candidate gives bit-identical results.
+int bdlu_gpu_plan_schedule(bdlu_ctx* ctx, uint64_t plan_id, int32_t* out4);
 
 // New values for the plan's pattern, canonical order. precision: BDLU_FP64 or
 // BDLU_FP32. Sparse blocks refactor on the analysis' pivot order and check the
