@@ -128,6 +128,22 @@ def backward_error(n, r, c, v, x, b):
     return worst
 
 
+def componentwise_error(n, r, c, v, x, b):
+    """max_i |b - A x|_i / (|A| |x| + |b|)_i (Oettli-Prager), the worst column."""
+    X = x.reshape(n, -1)
+    Bm = b.reshape(n, -1)
+    worst = 0.0
+    for j in range(X.shape[1]):
+        y = np.zeros(n)
+        np.add.at(y, r, v * X[c, j])
+        den = np.abs(Bm[:, j]).copy()
+        np.add.at(den, r, np.abs(v * X[c, j]))
+        res = np.abs(Bm[:, j] - y)
+        ratio = np.divide(res, den, out=np.where(res > 0, np.inf, 0.0), where=den > 0)
+        worst = max(worst, float(ratio.max()))
+    return worst
+
+
 def run_gpu(bd, torch, n, r, c, v_sets, b, precision, steps, warmup, profile):
     """Returns (seconds per step list, final x of the last step, report, prof)."""
     L = bd.lib()
@@ -207,7 +223,7 @@ import importlib.util
 spec = importlib.util.spec_from_file_location("blockdaglu_amd", sys.argv[1] + "/block-daglu_amd/__init__.py")
 bd = importlib.util.module_from_spec(spec); sys.modules["blockdaglu_amd"] = bd; spec.loader.exec_module(bd)
 from oracle import oracle as orc
-n, r, c, v, B = bd.generate(int(sys.argv[2]))
+n, r, c, v, B = bd.generate(int(sys.argv[2]), keep_order=int(sys.argv[3]))
 o = orc.OracleSolver(r, c, v)
 t = time.perf_counter(); assert o.analyze() == 0; print("analyze", time.perf_counter() - t, flush=True)
 t = time.perf_counter(); assert o.factor() == 0; print("factor", time.perf_counter() - t, o.info()["threads"], flush=True)
@@ -215,7 +231,7 @@ t = time.perf_counter(); rc, x = o.solve(B[:, 0]); print("solve", time.perf_coun
 """
 
 
-def cpu_baseline(config, budget_s):
+def cpu_baseline(config, budget_s, keep_order=0):
     """The oracle (CPU restatement of the reference BLOCK route) on one request of
     the same workload: set_matrix -> analyze -> factor -> solve (the reference API
     has no pattern-only refactor), reference thread policy (src/solver.cpp:366-390),
@@ -223,7 +239,7 @@ def cpu_baseline(config, budget_s):
     its throughput is measured; otherwise 1/budget_s is an upper bound on it and
     the sample says which phases completed."""
     import subprocess
-    cmd = [sys.executable, "-c", CPU_SCRIPT, str(ROOT), str(config)]
+    cmd = [sys.executable, "-c", CPU_SCRIPT, str(ROOT), str(config), str(keep_order)]
     t0 = time.perf_counter()
     proc = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True)
     try:
@@ -358,6 +374,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=25.0, help="seconds of CPU baseline sample")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--keep-order", action="store_true",
+                    help="skip the generator's structure-hiding permutation (same blocks and values); the "
+                         "batched-blocks workload needs it for a host analysis of seconds")
+    ap.add_argument("--ir-stop", choices=["residual", "backward", "componentwise"],
+                    help="refinement stop rule at 1e-12 (default: componentwise for the scaled stress case, "
+                         "the reference's ||r||/||b|| otherwise)")
     ap.add_argument("--profile-summary", metavar="DB", help="print the kernel table of a rocprofv3 database and exit")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
                     help="compute HBM bytes per launch of the roofline kernel from two --pmc runs, write "
@@ -396,7 +418,7 @@ def main():
         bench_serving(bd, args)
         return
     t = time.time()
-    n, r, c, v, B = bd.generate(args.config)
+    n, r, c, v, B = bd.generate(args.config, keep_order=int(args.keep_order))
     b = np.asfortranarray(B)  # n x k right-hand sides
     v_sets = [v] + [bd.perturb(v, seed=1000 + i) for i in range(1, 10)]
     log(f"workload n={n} nnz={len(r)} rhs={b.shape[1]} generated in {time.time() - t:.1f}s")
@@ -408,16 +430,24 @@ def main():
         3: f"mixed-precision stress n={n} nnz={len(r)}: circuit-like pattern, rows/cols scaled by 10^U(-4,4), "
            f"10 refactors at +-5%, 1 RHS, FP32+IR to backward error <= 1e-12",
     }.get(args.config, f"config {args.config} n={n} nnz={len(r)}")
+    if args.keep_order:
+        workload += (" (generated order kept: the random permutation that hides the block structure is skipped, "
+                     "so the one-time host analysis is seconds instead of a long maximum-transversal search; "
+                     "blocks, fills, coupling and values are the same)")
 
     # Plain FP64 (the reference default, no refinement), FP64 + FP64 refinement to
     # ||r||/||b|| < 1e-12 (REFINE_ITERS semantics), and FP32 factors + FP64 refinement.
+    # The badly scaled stress case stops both refinements on componentwise backward
+    # error (scale-invariant); ||r||/||b|| is dominated by the largest rows there.
+    ir_stop = args.ir_stop or ("componentwise" if args.config == 3 else "residual")
     os.environ.pop("BLOCKDAGLU_REFINE_ITERS", None)
+    os.environ.pop("BLOCKDAGLU_GPU_IR_STOP", None)
     res64p = run_gpu(bd, torch, n, r, c, v_sets, b, 0, args.steps, args.warmup, False)
+    if ir_stop != "residual":
+        os.environ["BLOCKDAGLU_GPU_IR_STOP"] = ir_stop
     os.environ["BLOCKDAGLU_REFINE_ITERS"] = "10"
     res64 = run_gpu(bd, torch, n, r, c, v_sets, b, 0, args.steps, args.warmup, False)
     os.environ.pop("BLOCKDAGLU_REFINE_ITERS", None)
-    if args.config == 3:
-        os.environ["BLOCKDAGLU_GPU_IR_STOP"] = "backward"
     res32 = run_gpu(bd, torch, n, r, c, v_sets, b, 1, args.steps, args.warmup, not args.no_profile)
     os.environ.pop("BLOCKDAGLU_GPU_IR_STOP", None)
 
@@ -429,6 +459,7 @@ def main():
                 "p50_ms": float(np.percentile(ts, 50) * 1e3), "p99_ms": float(np.percentile(ts, 99) * 1e3),
                 "rel_residual": rel_residual(n, r, c, last_vals, x, b),
                 "backward_error": backward_error(n, r, c, last_vals, x, b),
+                "componentwise_error": componentwise_error(n, r, c, last_vals, x, b),
                 "ir_iterations": rep["ir_iterations"], "fp64_fallback_blocks": rep["fp64_fallback_blocks"]}
 
     s64p, s64, s32 = summarize(res64p), summarize(res64), summarize(res32)
@@ -457,7 +488,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
-        cpu = cpu_baseline(args.config, args.cpu_budget)
+        cpu = cpu_baseline(args.config, args.cpu_budget, int(args.keep_order))
         log(f"cpu baseline: {cpu['sample']}")
     out = {
         "metric": METRIC,
@@ -472,17 +503,20 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 factors + f64 refinement",
         "data": "synthetic (seeded generator)",
-        "config": {"workload": workload, "parallelism": "single GPU"},
+        "config": {"workload": workload, "parallelism": "single GPU", "keep_order": bool(args.keep_order)},
         "p50_ms": s32["p50_ms"], "p99_ms": s32["p99_ms"], "rel_residual": s32["rel_residual"],
-        "backward_error": s32["backward_error"],
+        "backward_error": s32["backward_error"], "componentwise_error": s32["componentwise_error"],
+        "ir_stop_rule": ir_stop,
         "forward_error_vs_fp64": float(np.linalg.norm(res32[3] - res64[3]) / np.linalg.norm(res64[3])),
         "ir_iterations": s32["ir_iterations"], "fp64_fallback_blocks": s32["fp64_fallback_blocks"],
         "fp64": {"value": s64["value"], "ms_per_step": s64["ms_per_step"], "p50_ms": s64["p50_ms"],
                  "p99_ms": s64["p99_ms"], "rel_residual": s64["rel_residual"], "ir_iterations": s64["ir_iterations"],
-                 "backward_error": s64["backward_error"], "dtype": "f64 + f64 refinement"},
+                 "backward_error": s64["backward_error"], "componentwise_error": s64["componentwise_error"],
+                 "dtype": "f64 + f64 refinement"},
         "fp64_plain": {"value": s64p["value"], "ms_per_step": s64p["ms_per_step"], "p50_ms": s64p["p50_ms"],
                        "p99_ms": s64p["p99_ms"], "rel_residual": s64p["rel_residual"],
-                       "backward_error": s64p["backward_error"], "dtype": "f64"},
+                       "backward_error": s64p["backward_error"],
+                       "componentwise_error": s64p["componentwise_error"], "dtype": "f64"},
         "fp32ir_over_fp64": s32["value"] / s64["value"],
         "fp32ir_over_fp64_plain": s32["value"] / s64p["value"],
         "roofline": roof,

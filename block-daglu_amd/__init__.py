@@ -55,7 +55,7 @@ class _Report(C.Structure):
     _fields_ = [("nblocks", C.c_int32), ("dense_blocks", C.c_int32), ("sparse_blocks", C.c_int32),
                 ("max_block", C.c_int32), ("fp64_fallback_blocks", C.c_int32), ("repivot_blocks", C.c_int32),
                 ("ir_iterations", C.c_int32), ("plan_reused", C.c_int32), ("rel_residual", C.c_double),
-                ("backward_error", C.c_double)]
+                ("backward_error", C.c_double), ("componentwise_error", C.c_double)]
 
 
 class GenSpec(C.Structure):
@@ -65,13 +65,13 @@ class GenSpec(C.Structure):
                 ("large_size_hi", C.c_int32), ("large_fill", C.c_int32), ("large_fill_param", C.c_double),
                 ("extra_per_row", C.c_double), ("coupling_kind", C.c_int32), ("coupling_rate", C.c_double),
                 ("diag_factor", C.c_double), ("nrhs", C.c_int32), ("scale_exp", C.c_double),
-                ("scale_seed", C.c_uint64)]
+                ("scale_seed", C.c_uint64), ("keep_order", C.c_int32)]
 
 
 class _ServeResult(C.Structure):
     _fields_ = [("error", C.c_int32), ("slot", C.c_int32), ("ir_iterations", C.c_int32),
                 ("fp64_fallback_blocks", C.c_int32), ("repivots", C.c_int32), ("rel_residual", C.c_double),
-                ("backward_error", C.c_double), ("t_submit_ms", C.c_double), ("t_start_ms", C.c_double),
+                ("backward_error", C.c_double), ("componentwise_error", C.c_double), ("t_submit_ms", C.c_double), ("t_start_ms", C.c_double),
                 ("t_done_ms", C.c_double)]
 
 

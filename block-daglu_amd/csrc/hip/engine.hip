@@ -124,6 +124,7 @@ struct Plan {
   std::vector<int32_t> h_prec;
   // sparse
   std::vector<SparseDev> sp;
+  std::vector<int32_t> sp_order;  // sparse blocks by decreasing tail size (refactor order)
   std::vector<int32_t> sp_of_block;
   void* xscratch = nullptr;
   int32_t nslots = 0;
@@ -133,8 +134,9 @@ struct Plan {
   int32_t nticket = 0;
   int32_t* err = nullptr;
   double *b = nullptr, *x = nullptr, *r = nullptr, *d = nullptr, *bperm = nullptr, *xperm = nullptr;
+  double* cw = nullptr;  // per-row componentwise backward-error ratios of the last residual
   double* partials = nullptr;
-  double* norms = nullptr;  // per column c: [c*9 + 0..4] residual norms; [kAnormSlot] ||A||_inf
+  double* norms = nullptr;  // per column c: [c*9 + 0..5] residual norms; [kAnormSlot] ||A||_inf
   bool factored = false;
   // Launch schedule, picked per plan by the autotuner (every choice gives the
   // same results bit for bit; only the time differs).
@@ -180,6 +182,21 @@ struct bdlu_ctx {
   uint64_t budget = 0;
   uint64_t used = 0;
   hipStream_t stream = nullptr;
+  // Side streams (default priority) for independent parts of a refactor: the
+  // dense-block LU and the dense tails of different sparse blocks. Joined back
+  // into `stream` with events, so callers see one ordered stream.
+  static constexpr int kSide = 4;
+  hipStream_t side[kSide] = {};
+  bool fork = true;  // BLOCKDAGLU_GPU_STREAMS=1 keeps every launch on one stream
+  std::vector<hipEvent_t> events;
+  hipEvent_t event(std::size_t i) {
+    while (events.size() <= i) {
+      hipEvent_t e = nullptr;
+      HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      events.push_back(e);
+    }
+    return events[i];
+  }
   std::mutex mu;
   std::string last_error;
   uint64_t next_id = 1;
@@ -456,6 +473,10 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
     S.status = A.get<int32_t>(1);
     P.sp.push_back(std::move(S));
   }
+  P.sp_order.resize(P.sp.size());
+  for (std::size_t i = 0; i < P.sp.size(); ++i) P.sp_order[i] = static_cast<int32_t>(i);
+  std::stable_sort(P.sp_order.begin(), P.sp_order.end(),
+                   [&](int32_t x, int32_t y) { return P.sp[x].m > P.sp[y].m; });
   if (!P.sp.empty()) {
     P.nslots = static_cast<int32_t>(std::min<int64_t>(1024, max_width));
     const std::size_t xbytes = sizeof(double) * static_cast<std::size_t>(P.nslots) * max_s;
@@ -505,7 +526,8 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
   P.d = A.get<double>(nk);
   P.bperm = A.get<double>(nk);
   P.xperm = A.get<double>(nk);
-  P.partials = A.get<double>(512 * 5 * kMaxRhs);
+  P.cw = A.get<double>(nk);
+  P.partials = A.get<double>(512 * 6 * kMaxRhs);
   P.norms = A.get<double>(kAnormSlot + 8);
   HIPCHECK(hipStreamSynchronize(st));
   return pl;
@@ -655,9 +677,11 @@ void autotune_solve(Plan& P, hipStream_t st, int32_t kt) {
   P.sched.seg_grid = pick;
 }
 
+// Sparse leading columns and the assembly of the dense tail (these share the
+// plan's column scratch, so sparse blocks take this phase one at a time).
 template <typename T>
-void factor_sparse(Plan& P, SparseDev& S, hipStream_t st, T* Lx, T* Ux, T* Ud, T* Sm, ProfEvents* pe,
-                   double growth_limit) {
+void factor_sparse_cols(Plan& P, SparseDev& S, hipStream_t st, T* Lx, T* Ux, T* Ud, T* Sm, ProfEvents* pe,
+                        double growth_limit) {
   bdk::SpView view = S.view;
   view.inv_tol = growth_limit;
   bdk::launch_fill_i32(st, S.status, 1, 0);
@@ -671,6 +695,12 @@ void factor_sparse(Plan& P, SparseDev& S, hipStream_t st, T* Lx, T* Ux, T* Ud, T
   else bdk::launch_fill_f32(st, reinterpret_cast<float*>(Sm), int64_t(S.m) * S.m, 0.0f);
   bdk::launch_sp_tail<T>(st, view, P.arena, S.rs, xs, P.nslots, Lx, Ux, Sm, S.m);
   if (pe) pe->mark(BDLU_PROF_SPARSE, st);
+}
+
+// LU of the dense tail (touches only the block's own tail, inverse scratch and
+// status, so tails of different sparse blocks may run concurrently).
+template <typename T>
+void factor_sparse_tail(Plan& P, SparseDev& S, hipStream_t st, T* Sm, ProfEvents* pe, double growth_limit) {
   bdk::TailTimer tt;
   if (pe) {
     tt.gemm_events = &pe->ev[BDLU_PROF_TAIL_GEMM];
@@ -708,7 +738,15 @@ int bdlu_gpu_create(int device, uint64_t hbm_budget_bytes, bdlu_ctx** out) {
   if (const char* e = std::getenv("BLOCKDAGLU_GPU_REPIVOT_GROWTH")) c->fp64_growth = std::max(1.0, std::atof(e));
   if (const char* e = std::getenv("BLOCKDAGLU_GPU_GRAPH")) c->graphs = std::atoi(e) != 0;
   if (const char* e = std::getenv("BLOCKDAGLU_GPU_AUTOTUNE")) c->autotune = std::atoi(e) != 0;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (const char* e = std::getenv("BLOCKDAGLU_GPU_STREAMS")) c->fork = std::atoi(e) > 1;
+  bool ok = hipSetDevice(device) == hipSuccess &&
+            hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
+  for (int i = 0; ok && i < bdlu_ctx::kSide; ++i)
+    ok = hipStreamCreateWithFlags(&c->side[i], hipStreamNonBlocking) == hipSuccess;
+  if (!ok) {
+    for (hipStream_t q : c->side)
+      if (q) (void)hipStreamDestroy(q);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return BDLU_E_BACKEND;
   }
@@ -722,8 +760,11 @@ void bdlu_gpu_destroy(bdlu_ctx* c) {
     std::lock_guard<std::mutex> g(c->mu);
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    for (hipStream_t q : c->side) (void)hipStreamSynchronize(q);
     for (auto& kv : c->plans) kv.second->mem.release();
     c->plans.clear();
+    for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+    for (hipStream_t q : c->side) (void)hipStreamDestroy(q);
     (void)hipStreamDestroy(c->stream);
   }
   delete c;
@@ -809,19 +850,58 @@ int bdlu_gpu_refactor(bdlu_ctx* c, uint64_t id, const double* values, int where,
     auto launches = [&] {
       bdk::launch_scatter(st, P.nnz, P.vals, P.vmap, P.arena);
       bdk::launch_anorm(st, P.n, P.csr_ptr, P.vals, P.partials, P.norms + kAnormSlot);
-      // Dense blocks (dynamic partial pivoting, reference semantics).
+      // Dense blocks (dynamic partial pivoting, reference semantics), on a side
+      // stream unless profiling (the profile events time one stream).
+      const bool fork = c->fork && !pe;
+      hipStream_t ds = st;
+      if (fork) {
+        HIPCHECK(hipEventRecord(c->event(0), st));
+        HIPCHECK(hipStreamWaitEvent(c->side[0], c->event(0), 0));
+        ds = c->side[0];
+      }
       if (pe) pe->mark(BDLU_PROF_DENSE_LU, st);
       const int32_t* order = P.sched.dense_lpt ? P.d_lpt : nullptr;
       if (fp32)
-        bdk::launch_dense_lu<float>(st, P.ndense, P.d_size, P.d_start, P.d_val0, P.d_lu0, P.arena, P.lu32, P.piv,
+        bdk::launch_dense_lu<float>(ds, P.ndense, P.d_size, P.d_start, P.d_val0, P.d_lu0, P.arena, P.lu32, P.piv,
                                     P.block_status, P.d_block, order);
       else
-        bdk::launch_dense_lu<double>(st, P.ndense, P.d_size, P.d_start, P.d_val0, P.d_lu0, P.arena, P.lu64, P.piv,
+        bdk::launch_dense_lu<double>(ds, P.ndense, P.d_size, P.d_start, P.d_val0, P.d_lu0, P.arena, P.lu64, P.piv,
                                      P.block_status, P.d_block, order);
       if (pe) pe->mark(BDLU_PROF_DENSE_LU, st);
-      for (SparseDev& S : P.sp) {
-        if (fp32) factor_sparse<float>(P, S, st, S.Lx32, S.Ux32, S.Ud32, S.S32, pe, c->fp32_growth);
-        else factor_sparse<double>(P, S, st, S.Lx64, S.Ux64, S.Ud64, S.S64, pe, c->fp64_growth);
+      if (fork) HIPCHECK(hipEventRecord(c->event(1), ds));
+      // Sparse blocks, largest tail first. With several blocks, each tail goes
+      // to the least loaded side stream (load: tail size cubed) once its
+      // columns are done; the column phases stay on the main stream.
+      const bool spread = fork && P.sp.size() > 1;
+      double load[bdlu_ctx::kSide] = {0};
+      bool used[bdlu_ctx::kSide] = {false};
+      for (std::size_t q = 0; q < P.sp_order.size(); ++q) {
+        SparseDev& S = P.sp[P.sp_order[q]];
+        if (fp32) factor_sparse_cols<float>(P, S, st, S.Lx32, S.Ux32, S.Ud32, S.S32, pe, c->fp32_growth);
+        else factor_sparse_cols<double>(P, S, st, S.Lx64, S.Ux64, S.Ud64, S.S64, pe, c->fp64_growth);
+        hipStream_t ts = st;
+        if (spread && S.m > 0) {
+          int pick = 1;
+          for (int k = 2; k < bdlu_ctx::kSide; ++k)
+            if (load[k] < load[pick]) pick = k;
+          load[pick] += double(S.m) * S.m * S.m;
+          used[pick] = true;
+          HIPCHECK(hipEventRecord(c->event(2 + q), st));
+          HIPCHECK(hipStreamWaitEvent(c->side[pick], c->event(2 + q), 0));
+          ts = c->side[pick];
+        }
+        if (fp32) factor_sparse_tail<float>(P, S, ts, S.S32, pe, c->fp32_growth);
+        else factor_sparse_tail<double>(P, S, ts, S.S64, pe, c->fp64_growth);
+      }
+      if (fork) {
+        // Join: the main stream waits for the dense-block LU and every side stream used.
+        HIPCHECK(hipStreamWaitEvent(st, c->event(1), 0));
+        for (int k = 1; k < bdlu_ctx::kSide; ++k) {
+          if (!used[k]) continue;
+          const std::size_t e = 2 + P.sp_order.size() + k;
+          HIPCHECK(hipEventRecord(c->event(e), c->side[k]));
+          HIPCHECK(hipStreamWaitEvent(st, c->event(e), 0));
+        }
       }
     };
     const int pi = fp32 ? 1 : 0;
@@ -880,7 +960,8 @@ int bdlu_gpu_refactor(bdlu_ctx* c, uint64_t id, const double* values, int where,
       int32_t s0 = sst[i];
       if (fp32 && s0 != 0) {
         // FP32 refactor failed its check: redo this block in FP64.
-        factor_sparse<double>(P, S, st, S.Lx64, S.Ux64, S.Ud64, S.S64, nullptr, c->fp64_growth);
+        factor_sparse_cols<double>(P, S, st, S.Lx64, S.Ux64, S.Ud64, S.S64, nullptr, c->fp64_growth);
+        factor_sparse_tail<double>(P, S, st, S.S64, nullptr, c->fp64_growth);
         S.prec = 0;
         HIPCHECK(hipMemcpyAsync(&s0, S.status, sizeof(int32_t), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
@@ -916,7 +997,7 @@ int bdlu_gpu_solve(bdlu_ctx* c, uint64_t id, double* B, int32_t nrhs, int64_t ld
   hipStream_t st = c->stream;
   const int32_t max_iters = ir ? ir->max_iters : 0;
   const bool want_stats = max_iters > 0 || (ir && ir->stop_rule < 0);
-  bdlu_solve_stats ss{0, -1.0, -1.0};
+  bdlu_solve_stats ss{0, -1.0, -1.0, -1.0};
   const int64_t n = P.n;
   try {
     HIPCHECK(hipSetDevice(c->device));
@@ -948,8 +1029,8 @@ int bdlu_gpu_solve(bdlu_ctx* c, uint64_t id, double* B, int32_t nrhs, int64_t ld
       if (want_stats) {
         double h[9 * kMaxRhs];
         auto eval = [&]() {
-          bdk::launch_residual(st, P.n, kt, P.csr_ptr, P.csr_col, P.vals, P.x, P.b, P.r, n);
-          bdk::launch_norms(st, P.n, kt, P.r, P.x, P.b, n, P.partials, P.norms);
+          bdk::launch_residual(st, P.n, kt, P.csr_ptr, P.csr_col, P.vals, P.x, P.b, P.r, P.cw, n);
+          bdk::launch_norms(st, P.n, kt, P.r, P.x, P.b, P.cw, n, P.partials, P.norms);
           HIPCHECK(hipMemcpyAsync(h, P.norms, sizeof(double) * 9 * kt, hipMemcpyDeviceToHost, st));
           HIPCHECK(hipStreamSynchronize(st));
         };
@@ -966,7 +1047,9 @@ int bdlu_gpu_solve(bdlu_ctx* c, uint64_t id, double* B, int32_t nrhs, int64_t ld
         for (int it2 = 0; it2 < max_iters; ++it2) {
           bool done = true;
           for (int k = 0; k < kc; ++k) {
-            const bool stop = (ir->stop_rule == 1) ? (berr(k) <= ir->tol) : (rel(k) < ir->tol);
+            const bool stop = ir->stop_rule == 1   ? berr(k) <= ir->tol
+                              : ir->stop_rule == 2 ? h[k * 9 + 5] <= ir->tol
+                                                   : rel(k) < ir->tol;
             if (!stop && h[k * 9 + 3] > 0.0) done = false;
           }
           if (done) break;
@@ -979,6 +1062,7 @@ int bdlu_gpu_solve(bdlu_ctx* c, uint64_t id, double* B, int32_t nrhs, int64_t ld
         for (int k = 0; k < kc; ++k) {
           ss.rel_residual = std::max(ss.rel_residual, rel(k));
           ss.backward_error = std::max(ss.backward_error, berr(k));
+          ss.componentwise_error = std::max(ss.componentwise_error, h[k * 9 + 5]);
         }
         ss.iterations = std::max(ss.iterations, iters);
       }

@@ -95,12 +95,13 @@ void launch_sp_scatter_x(hipStream_t st, int32_t s, int32_t start, int32_t kc, c
                          int64_t ldx, double* xperm, int64_t ldv);
 
 // ---- residual / norms / refinement ----
+// r = b - A x; cw_i = |r_i| / (|A||x| + |b|)_i (componentwise backward-error ratio).
 void launch_residual(hipStream_t st, int32_t n, int32_t kc, const int64_t* csr_ptr, const int32_t* csr_col,
-                     const double* vals, const double* x, const double* b, double* r, int64_t ld);
-// Per column c: out[c*9+0]=sum r^2, [1]=max|r|, [2]=max|x|, [3]=sum b^2, [4]=max|b|.
-// partials: 512 * 5 * kc doubles.
-void launch_norms(hipStream_t st, int32_t n, int32_t kc, const double* r, const double* x, const double* b, int64_t ld,
-                  double* partials, double* out);
+                     const double* vals, const double* x, const double* b, double* r, double* cw, int64_t ld);
+// Per column c: out[c*9+0]=sum r^2, [1]=max|r|, [2]=max|x|, [3]=sum b^2, [4]=max|b|,
+// [5]=max cw. partials: 512 * 6 * kc doubles.
+void launch_norms(hipStream_t st, int32_t n, int32_t kc, const double* r, const double* x, const double* b,
+                  const double* cw, int64_t ld, double* partials, double* out);
 // out[0] = max_i sum_j |a_ij|
 void launch_anorm(hipStream_t st, int32_t n, const int64_t* csr_ptr, const double* vals, double* partials,
                   double* out);

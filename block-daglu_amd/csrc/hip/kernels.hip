@@ -168,15 +168,6 @@ __global__ __launch_bounds__(64) void k_dense_lu(int32_t nd, const int32_t* __re
   __syncthreads();
   int32_t mypiv = lane;
   int32_t st = 0;
-  T amax_in = T(0);
-  if (sizeof(T) == 4) {
-    for (int idx = lane; idx < s * s; idx += kWave) {
-      const int j = idx / s, i = idx - j * s;
-      amax_in = tmax(amax_in, tabs(a[j * kLdsPad + i]));
-    }
-    for (int off = 32; off > 0; off >>= 1) amax_in = tmax(amax_in, __shfl_xor(amax_in, off));
-  }
-  T umin = T(0);
   for (int k = 0; k < s; ++k) {
     T v = (lane >= k && lane < s) ? tabs(a[k * kLdsPad + lane]) : T(-1);
     int idx = lane;
@@ -193,7 +184,6 @@ __global__ __launch_bounds__(64) void k_dense_lu(int32_t nd, const int32_t* __re
       st |= kStZeroPivot;
       break;
     }
-    if (k == 0 || v < umin) umin = v;
     const int p = uniform_from_lane0(idx);
     mypiv = (lane == k) ? p : mypiv;
     if (p != k) {
@@ -213,9 +203,19 @@ __global__ __launch_bounds__(64) void k_dense_lu(int32_t nd, const int32_t* __re
     __syncthreads();
   }
   if (sizeof(T) == 4 && st == 0) {
-    // FP32 small-pivot check: a pivot far below the block's scale means FP32
-    // factors cannot drive FP64 refinement; the block is redone in FP64.
-    if (umin < amax_in * T(7.62939453125e-06)) st |= kStCheck;
+    // FP32 small-pivot check, invariant under row and column scaling: pivot k
+    // was computed as a_kk - sum_i l_ki u_ik, so its rounding error is about
+    // u * (|L||U|)_kk (Higham, Accuracy and Stability of Numerical Algorithms,
+    // Thm 9.3). A pivot below 2^-12 of that sum has lost more than half of
+    // FP32's 24 bits to cancellation; the block is then redone in FP64.
+    int bad = 0;
+    if (lane < s) {
+      const T ukk = tabs(a[lane * kLdsPad + lane]);
+      T mass = ukk;
+      for (int i = 0; i < lane; ++i) mass += tabs(a[i * kLdsPad + lane]) * tabs(a[lane * kLdsPad + i]);
+      bad = (ukk * T(4096) < mass) ? 1 : 0;
+    }
+    if (__ballot(bad) != 0ull) st |= kStCheck;
   }
   T* dst = lu + lu0[t];
   for (int idx = lane; idx < s * s; idx += kWave) {
@@ -390,9 +390,8 @@ template void launch_sp_tail<float>(hipStream_t, const SpView&, const double*, c
 
 // ---------------------------------------------------------------------------
 // Dense tail: no-pivot blocked right-looking LU, panel width 64. Per panel:
-//   1. k_diag_lu_inv: LU of the 64x64 diagonal block (one 1024-thread
-//      workgroup, register tiles, one barrier per step), then L11^{-1} and
-//      U11^{-1};
+//   1. k_diag_lu_inv: LU of the 64x64 diagonal block (one wavefront, rows
+//      in registers), then L11^{-1} and U11^{-1} (one wavefront each);
 //   2. k_tail_panel: L21 = A21 U11^{-1} and U12 = L11^{-1} A12 as MFMA tile
 //      products (one launch, both jobs), with the |L| growth check;
 //   3. k_tail_update: trailing update A22 -= L21 U12 on MFMA.
@@ -401,152 +400,118 @@ constexpr int kNb = 64;
 // inv[0..4095]: L11^{-1} row-major (unit lower); inv[4096..8191]: U11^{-1}
 // row-major (upper). Rows/columns at or past b are an identity pad.
 //
-// 1024 threads: thread (w = wave, j = lane) owns column j of rows w + 16r,
-// r = 0..3, in registers. A wave issues at most one instruction per four
-// cycles, so many waves with short per-step streams set the step time.
-// Right-looking LU with one barrier per step: at step k every thread reads
-// pivot row k and pivot column k from LDS (final since step k-1, all reads
-// issued before any use) and the owners of row k+1 / column k+1 publish them
-// for step k+1. Row tests depend on the wave index and the step only, so they
-// are scalar. The two triangular inverses then run interleaved (L11^{-1}
-// forward, U11^{-1} backward), each step publishing one finished row through a
-// two-slot ring.
-constexpr int kDiagWaves = 16;
-constexpr int kDiagRows = kNb / kDiagWaves;
+// Two wavefronts. The LU is one dependent chain of 64 steps, so it runs on a
+// single wave with no barriers: lane i keeps row i in registers, and at step k
+// every lane takes pivot row k from lane k with readlane (scalar registers,
+// compile-time lane) and updates its own row. Both triangular inverses are
+// then independent per column: wave 0 forms L11^{-1} and wave 1 (after writing
+// the factor back) U11^{-1}, lane j one column each, right-looking so each
+// step has 63-k independent FMAs, with the factor entries read from a
+// column-major LDS copy at wave-uniform addresses (broadcast reads). Per
+// element the operation sequence is that of the textbook right-looking LU and
+// forward/backward substitution, so the results do not depend on how the work
+// is split.
+constexpr int kDiagThreads = 2 * kNb;
+
+__device__ __forceinline__ float readlane_t(float v, int src) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), src));
+}
+__device__ __forceinline__ double readlane_t(double v, int src) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(u)), src);
+  const uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(u >> 32)), src);
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
 
 template <typename T>
-__global__ __launch_bounds__(kDiagWaves * kNb) void k_diag_lu_inv(T* __restrict__ S, int64_t lds, int32_t p,
-                                                                  int32_t b, double limit,
-                                                                  int32_t* __restrict__ status, T* __restrict__ inv) {
-  __shared__ T f[kNb * (kNb + 1)];  // f[i*65 + j] = LU(i, j)
-  __shared__ T xr[2][kNb], yr[2][kNb];
+__global__ __launch_bounds__(kDiagThreads) void k_diag_lu_inv(T* __restrict__ S, int64_t lds, int32_t p, int32_t b,
+                                                              double limit, int32_t* __restrict__ status,
+                                                              T* __restrict__ inv) {
+  constexpr int LC = kNb + 2;  // column-major LDS copy: fc[j*66 + i] = LU(i, j); 16-byte aligned columns
+  __shared__ T fc[kNb * LC];
   __shared__ int32_t st_sh;
-  constexpr int LD = kNb + 1;
-  const int j = threadIdx.x & (kNb - 1);
+  const int lane = threadIdx.x & (kNb - 1);
   const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-  constexpr int kThreads = kDiagWaves * kNb;
   if (threadIdx.x == 0) st_sh = 0;
-  {
-    T v[kNb * kNb / kThreads];
-#pragma unroll
-    for (int u = 0; u < kNb * kNb / kThreads; ++u) {
-      const int c = u * kDiagWaves + w, r = j;  // element (r, c): coalesced down column c
-      v[u] = (r < b && c < b) ? S[int64_t(p + c) * lds + p + r] : (r == c ? T(1) : T(0));
-    }
-#pragma unroll
-    for (int u = 0; u < kNb * kNb / kThreads; ++u) f[j * LD + u * kDiagWaves + w] = v[u];
-  }
-  __syncthreads();
-  T a[kDiagRows];
-#pragma unroll
-  for (int r = 0; r < kDiagRows; ++r) a[r] = f[(w + kDiagWaves * r) * LD + j];
   int32_t st = 0;
-  for (int k = 0; k < kNb; ++k) {
-    const T akk = f[k * LD + k];
-    const T ukj_raw = f[k * LD + j];
-    T cv[kDiagRows];
+  if (w == 0) {
+    const int i = lane;
+    T a[kNb];
+    const T* src = S + int64_t(p) * lds + p + (i < b ? i : 0);
 #pragma unroll
-    for (int r = 0; r < kDiagRows; ++r) cv[r] = f[(w + kDiagWaves * r) * LD + k];
-    if (akk == T(0)) st |= kStZeroPivot;
-    const T rp = T(1) / akk;
-    // Columns left of k keep their values (a - lik * 0), column k takes l_ik.
-    const T ukj = (j > k) ? ukj_raw : T(0);
-    if (w + kDiagWaves * (kDiagRows - 1) > k) {  // this wave still has active rows
-#pragma unroll
-      for (int r = 0; r < kDiagRows; ++r) {
-        if (w + kDiagWaves * r > k) {
-          const T lik = cv[r] * rp;
-          a[r] = (j == k) ? lik : tfma(-lik, ukj, a[r]);
-        }
-      }
+    for (int j = 0; j < kNb; ++j) {
+      // Clamped address, unconditional load (a conditional load branches and waits).
+      const T v = src[int64_t(j < b ? j : 0) * lds];
+      a[j] = (i < b && j < b) ? v : (i == j ? T(1) : T(0));
     }
-    // Publish row k+1 and column k+1. The column owner writes all its rows:
-    // entries above row k+1 are not read again before the final publish.
-    const int kn = k + 1;
-    if (kn < kNb) {
-      // Register a[kn / kDiagWaves] of the owner wave, picked by scalar branches (a
-      // value select over the register array makes the compiler index it
-      // dynamically, i.e. move it to scratch memory).
-      if ((kn & (kDiagWaves - 1)) == w) {
+    // The next pivot's reciprocal is started as soon as that pivot is final
+    // (after its own column's update), so the division overlaps the rest of
+    // the step.
+    T akk = readlane_t(a[0], 0);
+    T rp = T(1) / akk;
 #pragma unroll
-        for (int r = 0; r < kDiagRows; ++r)
-          if (r == kn / kDiagWaves && j >= kn) f[kn * LD + j] = a[r];
+    for (int k = 0; k < kNb; ++k) {
+      if (akk == T(0)) st |= kStZeroPivot;
+      const bool below = i > k;
+      const T lik = a[k] * rp;
+      // Rows at or above k take the update with a zero multiplier, which leaves
+      // them unchanged (up to the sign of an exact zero) without a select per element.
+      const T m = below ? -lik : T(0);
+      if (k + 1 < kNb) {
+        a[k + 1] = tfma(m, readlane_t(a[k + 1], k), a[k + 1]);
+        akk = readlane_t(a[k + 1], k + 1);
+        rp = T(1) / akk;
       }
-      if (j == kn) {
 #pragma unroll
-        for (int r = 0; r < kDiagRows; ++r) f[(w + kDiagWaves * r) * LD + kn] = a[r];
-      }
+      for (int j = k + 2; j < kNb; ++j) a[j] = tfma(m, readlane_t(a[j], k), a[j]);
+      a[k] = below ? lik : a[k];
     }
-    __syncthreads();
-  }
-  // |L| check on the finished factor (a NaN fails it).
-  T lmax = T(0);
+    // |L| check on the finished factor (a NaN fails it).
+    T lmax = T(0);
 #pragma unroll
-  for (int r = 0; r < kDiagRows; ++r) {
-    const T al = (j < w + kDiagWaves * r) ? tabs(a[r]) : T(0);
-    lmax = (al <= lmax || lmax != lmax) ? lmax : al;
-  }
-  // Publish the full LU (L below the diagonal) for the inverse sweeps and the
-  // coalesced write-back.
+    for (int j = 0; j < kNb; ++j) {
+      const T al = (j < i) ? tabs(a[j]) : T(0);
+      lmax = (al <= lmax || lmax != lmax) ? lmax : al;
+    }
+    if (!(lmax <= static_cast<T>(limit))) st |= kStCheck;
 #pragma unroll
-  for (int r = 0; r < kDiagRows; ++r) f[(w + kDiagWaves * r) * LD + j] = a[r];
-  __syncthreads();
-#pragma unroll
-  for (int u = 0; u < kNb * kNb / kThreads; ++u) {
-    const int c = u * kDiagWaves + w, r = j;
-    if (r < b && c < b) S[int64_t(p + c) * lds + p + r] = f[r * LD + c];
-  }
-  // X = L11^{-1} (forward: X[i] -= L[i][k] X[k] for i > k) and
-  // Y = U11^{-1} (backward: Y[k] /= U[k][k]; Y[i] -= U[i][k] Y[k] for i < k).
-  // Registers keep Y undivided; a row is divided when it is published and the
-  // registers once at the end (the same operation on the same values).
-  T x[kDiagRows], y[kDiagRows];
-#pragma unroll
-  for (int r = 0; r < kDiagRows; ++r) {
-    const int i = w + kDiagWaves * r;
-    x[r] = (i == j) ? T(1) : T(0);
-    y[r] = (i == j) ? T(1) : T(0);
-    if (i == 0) xr[0][j] = x[r];
-    if (i == kNb - 1) yr[(kNb - 1) & 1][j] = y[r] / f[i * LD + i];
+    for (int j = 0; j < kNb; ++j) fc[j * LC + i] = a[j];
   }
   __syncthreads();
-  for (int s = 0; s < kNb - 1; ++s) {
-    const int kx = s, ky = kNb - 1 - s;
-    const T xk = xr[kx & 1][j];
-    const T yk = yr[ky & 1][j];
-    T lv[kDiagRows], uv[kDiagRows];
+  const int j = lane;
+  if (w == 0) {
+    // Column j of L11^{-1}, right-looking: x_i -= L(i,k) x_k for i > k,
+    // k = 0..62 (per element the same order as forward substitution).
+    T x[kNb];
 #pragma unroll
-    for (int r = 0; r < kDiagRows; ++r) {
-      lv[r] = f[(w + kDiagWaves * r) * LD + kx];
-      uv[r] = f[(w + kDiagWaves * r) * LD + ky];
+    for (int i = 0; i < kNb; ++i) x[i] = (i == j) ? T(1) : T(0);
+#pragma unroll
+    for (int k = 0; k < kNb - 1; ++k) {
+#pragma unroll
+      for (int i = k + 1; i < kNb; ++i) x[i] = tfma(-fc[k * LC + i], x[k], x[i]);
     }
 #pragma unroll
-    for (int r = 0; r < kDiagRows; ++r) {
-      const int i = w + kDiagWaves * r;
-      if (i > kx) x[r] = tfma(-lv[r], xk, x[r]);
-      if (i < ky) y[r] = tfma(-uv[r], yk, y[r]);
-    }
-    const int xn = kx + 1, yn = ky - 1;
-    if ((xn & (kDiagWaves - 1)) == w) {
+    for (int i = 0; i < kNb; ++i) inv[i * kNb + j] = x[i];
+  } else {
+    // The factor back to S (coalesced down each column), then column j of
+    // U11^{-1}, backward: y_k /= U(k,k), then y_i -= U(i,k) y_k for i < k.
+    const int i = lane;
+    T* dst = S + int64_t(p) * lds + p + i;
 #pragma unroll
-      for (int r = 0; r < kDiagRows; ++r)
-        if (r == xn / kDiagWaves) xr[xn & 1][j] = x[r];
-    }
-    if ((yn & (kDiagWaves - 1)) == w) {
-      const T d = f[yn * LD + yn];
+    for (int c = 0; c < kNb; ++c)
+      if (i < b && c < b) dst[int64_t(c) * lds] = fc[c * LC + i];
+    T y[kNb];
 #pragma unroll
-      for (int r = 0; r < kDiagRows; ++r)
-        if (r == yn / kDiagWaves) yr[yn & 1][j] = y[r] / d;
+    for (int q = 0; q < kNb; ++q) y[q] = (q == j) ? T(1) : T(0);
+#pragma unroll
+    for (int k = kNb - 1; k >= 0; --k) {
+      y[k] = y[k] / fc[k * LC + k];
+#pragma unroll
+      for (int q = 0; q < k; ++q) y[q] = tfma(-fc[k * LC + q], y[k], y[q]);
     }
-    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kNb; ++q) inv[kNb * kNb + q * kNb + j] = y[q];
   }
-#pragma unroll
-  for (int r = 0; r < kDiagRows; ++r) {
-    const int i = w + kDiagWaves * r;
-    inv[i * kNb + j] = x[r];
-    inv[kNb * kNb + i * kNb + j] = y[r] / f[i * LD + i];
-  }
-  if (!(lmax <= static_cast<T>(limit))) st |= kStCheck;
   if (st) atomicOr(&st_sh, st);
   __syncthreads();
   if (threadIdx.x == 0 && st_sh) atomicOr(status, st_sh);
@@ -764,7 +729,7 @@ void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, 
     const int32_t b = (m - p) < kNb ? (m - p) : kNb;
     const int32_t rest = m - p - b;
     if (timer && timer->panel_events) timer_event(timer->panel_events, st);
-    hipLaunchKernelGGL(k_diag_lu_inv<T>, dim3(1), dim3(kDiagWaves * kNb), 0, st, S, lds, p, b, limit, status, inv);
+    hipLaunchKernelGGL(k_diag_lu_inv<T>, dim3(1), dim3(kDiagThreads), 0, st, S, lds, p, b, limit, status, inv);
     if (rest <= 0) {
       if (timer && timer->panel_events) timer_event(timer->panel_events, st);
       break;
@@ -1176,45 +1141,61 @@ void launch_sp_scatter_x(hipStream_t st, int32_t s, int32_t start, int32_t kc, c
 // ---------------------------------------------------------------------------
 // Residual r = b - A x over the canonical matrix, row by row in canonical
 // order (the reference's refinement loop, src/solver.cpp:1497-1506).
+// r = b - A x and, per row, the componentwise backward-error ratio
+// |r_i| / (|A| |x| + |b|)_i (Oettli-Prager) into cw.
 template <int K>
 __global__ void k_residual(int32_t n, const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
                            const double* __restrict__ vals, const double* __restrict__ x,
-                           const double* __restrict__ b, double* __restrict__ r, int64_t ld) {
+                           const double* __restrict__ b, double* __restrict__ r, double* __restrict__ cw,
+                           int64_t ld) {
   for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double acc[K];
+    double acc[K], den[K];
 #pragma unroll
-    for (int c = 0; c < K; ++c) acc[c] = b[c * ld + i];
+    for (int c = 0; c < K; ++c) {
+      acc[c] = b[c * ld + i];
+      den[c] = fabs(acc[c]);
+    }
     for (int64_t p = ptr[i]; p < ptr[i + 1]; ++p) {
       const double v = vals[p];
       const int32_t j = col[p];
 #pragma unroll
-      for (int c = 0; c < K; ++c) acc[c] = acc[c] - v * x[c * ld + j];
+      for (int c = 0; c < K; ++c) {
+        const double xj = x[c * ld + j];
+        acc[c] = acc[c] - v * xj;
+        den[c] += fabs(v * xj);
+      }
     }
 #pragma unroll
-    for (int c = 0; c < K; ++c) r[c * ld + i] = acc[c];
+    for (int c = 0; c < K; ++c) {
+      r[c * ld + i] = acc[c];
+      const double a = fabs(acc[c]);
+      cw[c * ld + i] = den[c] > 0.0 ? a / den[c] : (a == 0.0 ? 0.0 : INFINITY);
+    }
   }
 }
 void launch_residual(hipStream_t st, int32_t n, int32_t kc, const int64_t* csr_ptr, const int32_t* csr_col,
-                     const double* vals, const double* x, const double* b, double* r, int64_t ld) {
+                     const double* vals, const double* x, const double* b, double* r, double* cw, int64_t ld) {
   const dim3 g(grid_for(n, 256, 8192)), bl(256);
   switch (kc) {
-    case 1: hipLaunchKernelGGL(k_residual<1>, g, bl, 0, st, n, csr_ptr, csr_col, vals, x, b, r, ld); break;
-    case 2: hipLaunchKernelGGL(k_residual<2>, g, bl, 0, st, n, csr_ptr, csr_col, vals, x, b, r, ld); break;
-    case 4: hipLaunchKernelGGL(k_residual<4>, g, bl, 0, st, n, csr_ptr, csr_col, vals, x, b, r, ld); break;
-    default: hipLaunchKernelGGL(k_residual<8>, g, bl, 0, st, n, csr_ptr, csr_col, vals, x, b, r, ld); break;
+    case 1: hipLaunchKernelGGL(k_residual<1>, g, bl, 0, st, n, csr_ptr, csr_col, vals, x, b, r, cw, ld); break;
+    case 2: hipLaunchKernelGGL(k_residual<2>, g, bl, 0, st, n, csr_ptr, csr_col, vals, x, b, r, cw, ld); break;
+    case 4: hipLaunchKernelGGL(k_residual<4>, g, bl, 0, st, n, csr_ptr, csr_col, vals, x, b, r, cw, ld); break;
+    default: hipLaunchKernelGGL(k_residual<8>, g, bl, 0, st, n, csr_ptr, csr_col, vals, x, b, r, cw, ld); break;
   }
 }
 
 // Deterministic reductions: fixed grid, fixed per-block order, then a fixed
 // lane tree. Per column c: out[c*9 + 0] = sum r^2, [1] max|r|, [2] max|x|,
-// [3] sum b^2, [4] max|b|.
+// [3] sum b^2, [4] max|b|, [5] max componentwise backward-error ratio.
 constexpr int kRedBlocks = 512;
+constexpr int kNormQ = 6;
 __global__ __launch_bounds__(256) void k_norms_partial(int32_t n, int32_t kc, const double* __restrict__ r,
                                                        const double* __restrict__ x, const double* __restrict__ b,
-                                                       int64_t ld, double* __restrict__ part) {
-  __shared__ double sh[5][256];
+                                                       const double* __restrict__ cw, int64_t ld,
+                                                       double* __restrict__ part) {
+  __shared__ double sh[kNormQ][256];
   for (int32_t c = 0; c < kc; ++c) {
-    double v[5] = {0, 0, 0, 0, 0};
+    double v[kNormQ] = {0, 0, 0, 0, 0, 0};
     for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
       const double ri = r[c * ld + i], xi = x[c * ld + i], bi = b[c * ld + i];
       v[0] += ri * ri;
@@ -1222,21 +1203,20 @@ __global__ __launch_bounds__(256) void k_norms_partial(int32_t n, int32_t kc, co
       v[2] = fmax(v[2], fabs(xi));
       v[3] += bi * bi;
       v[4] = fmax(v[4], fabs(bi));
+      v[5] = fmax(v[5], cw[c * ld + i]);
     }
-    for (int q = 0; q < 5; ++q) sh[q][threadIdx.x] = v[q];
+    for (int q = 0; q < kNormQ; ++q) sh[q][threadIdx.x] = v[q];
     __syncthreads();
     for (int h = 128; h > 0; h >>= 1) {
       if (threadIdx.x < h) {
         sh[0][threadIdx.x] += sh[0][threadIdx.x + h];
-        sh[1][threadIdx.x] = fmax(sh[1][threadIdx.x], sh[1][threadIdx.x + h]);
-        sh[2][threadIdx.x] = fmax(sh[2][threadIdx.x], sh[2][threadIdx.x + h]);
         sh[3][threadIdx.x] += sh[3][threadIdx.x + h];
-        sh[4][threadIdx.x] = fmax(sh[4][threadIdx.x], sh[4][threadIdx.x + h]);
+        for (int q : {1, 2, 4, 5}) sh[q][threadIdx.x] = fmax(sh[q][threadIdx.x], sh[q][threadIdx.x + h]);
       }
       __syncthreads();
     }
     if (threadIdx.x == 0)
-      for (int q = 0; q < 5; ++q) part[(c * kRedBlocks + blockIdx.x) * 5 + q] = sh[q][0];
+      for (int q = 0; q < kNormQ; ++q) part[(c * kRedBlocks + blockIdx.x) * kNormQ + q] = sh[q][0];
     __syncthreads();
   }
 }
@@ -1245,28 +1225,24 @@ __global__ __launch_bounds__(256) void k_norms_partial(int32_t n, int32_t kc, co
 __global__ __launch_bounds__(64) void k_norms_final(int32_t nb, const double* __restrict__ part,
                                                     double* __restrict__ out) {
   const int lane = threadIdx.x, c = blockIdx.x;
-  double v[5] = {0, 0, 0, 0, 0};
+  double v[kNormQ] = {0, 0, 0, 0, 0, 0};
   for (int32_t i = lane; i < nb; i += kWave) {
-    const double* p = part + (c * nb + i) * 5;
+    const double* p = part + (c * nb + i) * kNormQ;
     v[0] += p[0];
-    v[1] = fmax(v[1], p[1]);
-    v[2] = fmax(v[2], p[2]);
     v[3] += p[3];
-    v[4] = fmax(v[4], p[4]);
+    for (int q : {1, 2, 4, 5}) v[q] = fmax(v[q], p[q]);
   }
   for (int off = 32; off > 0; off >>= 1) {
     v[0] += __shfl_xor(v[0], off);
-    v[1] = fmax(v[1], __shfl_xor(v[1], off));
-    v[2] = fmax(v[2], __shfl_xor(v[2], off));
     v[3] += __shfl_xor(v[3], off);
-    v[4] = fmax(v[4], __shfl_xor(v[4], off));
+    for (int q : {1, 2, 4, 5}) v[q] = fmax(v[q], __shfl_xor(v[q], off));
   }
   if (lane == 0)
-    for (int q = 0; q < 5; ++q) out[c * 9 + q] = v[q];
+    for (int q = 0; q < kNormQ; ++q) out[c * 9 + q] = v[q];
 }
-void launch_norms(hipStream_t st, int32_t n, int32_t kc, const double* r, const double* x, const double* b, int64_t ld,
-                  double* partials, double* out) {
-  hipLaunchKernelGGL(k_norms_partial, dim3(kRedBlocks), dim3(256), 0, st, n, kc, r, x, b, ld, partials);
+void launch_norms(hipStream_t st, int32_t n, int32_t kc, const double* r, const double* x, const double* b,
+                  const double* cw, int64_t ld, double* partials, double* out) {
+  hipLaunchKernelGGL(k_norms_partial, dim3(kRedBlocks), dim3(256), 0, st, n, kc, r, x, b, cw, ld, partials);
   hipLaunchKernelGGL(k_norms_final, dim3(kc), dim3(64), 0, st, kRedBlocks, partials, out);
 }
 

@@ -102,6 +102,7 @@ void bdlu_solver_report(void* s, bdlu_report* out) {
   out->plan_reused = r.plan_reused ? 1 : 0;
   out->rel_residual = r.rel_residual;
   out->backward_error = r.backward_error;
+  out->componentwise_error = r.componentwise_error;
 }
 
 int32_t bdlu_solver_nblocks(void* s) {
@@ -306,6 +307,7 @@ int bdlu_serve_wait(void* s, int64_t request, double* X, bdlu_serve_result* out)
     out->repivots = r.repivots;
     out->rel_residual = r.rel_residual;
     out->backward_error = r.backward_error;
+    out->componentwise_error = r.componentwise_error;
     out->t_submit_ms = r.t_submit_ms;
     out->t_start_ms = r.t_start_ms;
     out->t_done_ms = r.t_done_ms;

@@ -200,11 +200,14 @@ Workload build(const bdlu_gen_spec& sp) {
       }
     }
   }
-  // Hide the structure.
+  // Hide the structure (unless asked to keep the generated order; the values and
+  // the pattern come from their own streams, so they do not change).
   std::vector<int32_t> pr(n), pc(n);
   for (int32_t i = 0; i < n; ++i) pr[i] = pc[i] = i;
-  for (int32_t i = n - 1; i > 0; --i) std::swap(pr[i], pr[rperm.below(i + 1)]);
-  for (int32_t i = n - 1; i > 0; --i) std::swap(pc[i], pc[rperm.below(i + 1)]);
+  if (!sp.keep_order) {
+    for (int32_t i = n - 1; i > 0; --i) std::swap(pr[i], pr[rperm.below(i + 1)]);
+    for (int32_t i = n - 1; i > 0; --i) std::swap(pc[i], pc[rperm.below(i + 1)]);
+  }
   for (std::size_t k = 0; k < w.row.size(); ++k) {
     w.row[k] = pr[w.row[k]];
     w.col[k] = pc[w.col[k]];

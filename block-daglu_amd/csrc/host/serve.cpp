@@ -256,6 +256,7 @@ class ServeEngine {
       r.res.ir_iterations = ss.iterations;
       r.res.rel_residual = ss.rel_residual;
       r.res.backward_error = ss.backward_error;
+      r.res.componentwise_error = ss.componentwise_error;
       if (rc != BDLU_OK) r.res.error = SolverError::SolveFailed;
     } else {
       r.res.error = rc == BDLU_E_BACKEND ? SolverError::BackendUnavailable : SolverError::FactorizationFailed;

@@ -349,10 +349,13 @@ namespace {
 bdlu_ir_opts ir_options(Precision p) {
   bdlu_ir_opts o{};
   // Default: the reference's stop rule ||r||_2/||b||_2 < 1e-12 (src/solver.cpp:1508-1521);
-  // BLOCKDAGLU_GPU_IR_STOP=backward stops on normwise backward error <= 1e-12 instead.
+  // BLOCKDAGLU_GPU_IR_STOP=backward stops on normwise backward error <= 1e-12 instead,
+  // =componentwise on componentwise backward error <= 1e-12.
   o.stop_rule = 0;
-  if (const char* e = std::getenv("BLOCKDAGLU_GPU_IR_STOP"))
+  if (const char* e = std::getenv("BLOCKDAGLU_GPU_IR_STOP")) {
     if (std::string(e) == "backward") o.stop_rule = 1;
+    if (std::string(e) == "componentwise") o.stop_rule = 2;
+  }
   o.tol = 1e-12;
   int iters = 0;
   if (const char* e = blockdaglu_getenv_compat("USERALGO_REFINE_ITERS")) {
@@ -395,6 +398,7 @@ bool Solver::solve_multi_in_place(std::vector<double>& B, int32_t nrhs) {
   report_.ir_iterations = ss.iterations;
   report_.rel_residual = ss.rel_residual;
   report_.backward_error = ss.backward_error;
+  report_.componentwise_error = ss.componentwise_error;
   for (double v : B)
     if (!std::isfinite(v)) return fail(SolverError::SolveFailed, "block solve produced a non-finite value");
   return true;
@@ -415,6 +419,7 @@ bool Solver::solve_device(double* d_B, int32_t nrhs, int64_t ldb) {
   report_.ir_iterations = ss.iterations;
   report_.rel_residual = ss.rel_residual;
   report_.backward_error = ss.backward_error;
+  report_.componentwise_error = ss.componentwise_error;
   return true;
 }
 

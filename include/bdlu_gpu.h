@@ -103,6 +103,7 @@ typedef struct bdlu_ir_opts {
   int32_t max_iters;        // 0: plain solve. >0: refinement steps at most
   int32_t stop_rule;        // 0: reference rule ||r||_2/||b||_2 < tol (src/solver.cpp:1508-1521)
                             // 1: normwise backward error <= tol
+                            // 2: componentwise backward error <= tol
   double tol;
 } bdlu_ir_opts;
 
@@ -110,6 +111,7 @@ typedef struct bdlu_solve_stats {
   int32_t iterations;
   double rel_residual;      // ||b - A x||_2 / ||b||_2 (FP64)
   double backward_error;    // ||r||_inf / (||A||_inf ||x||_inf + ||b||_inf)
+  double componentwise_error;  // max_i |r_i| / (|A| |x| + |b|)_i (Oettli-Prager)
 } bdlu_solve_stats;
 
 int bdlu_gpu_device_count(int* count);

@@ -41,7 +41,7 @@ typedef struct bdlu_report {
   int32_t nblocks, dense_blocks, sparse_blocks, max_block;
   int32_t fp64_fallback_blocks, repivot_blocks, ir_iterations;
   int32_t plan_reused;
-  double rel_residual, backward_error;
+  double rel_residual, backward_error, componentwise_error;
 } bdlu_report;
 
 void* bdlu_solver_create(void);
@@ -87,6 +87,7 @@ typedef struct bdlu_serve_result {
   int32_t repivots;
   double rel_residual;           // worst right-hand side of the request
   double backward_error;         // worst right-hand side of the request
+  double componentwise_error;    // worst right-hand side of the request
   double t_submit_ms, t_start_ms, t_done_ms;  // server clock
 } bdlu_serve_result;
 void* bdlu_serve_create(int device, int slots, uint64_t hbm_budget_bytes, int precision /* 0 FP64, 1 FP32+IR */);
@@ -143,6 +144,7 @@ typedef struct bdlu_gen_spec {
   int32_t nrhs;
   double scale_exp;       // > 0: A <- D_r A D_c with d ~ 10^U(-scale_exp, scale_exp)
   uint64_t scale_seed;
+  int32_t keep_order;     // 1: skip the random row/column permutation that hides the block structure
 } bdlu_gen_spec;
 
 void bdlu_gen_config_spec(int cfg, int pattern, bdlu_gen_spec* spec);

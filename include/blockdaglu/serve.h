@@ -26,6 +26,7 @@ struct ServeResult {
   int32_t repivots = 0;
   double rel_residual = -1.0;    // max over the request's right-hand sides
   double backward_error = -1.0;  // max over the request's right-hand sides
+  double componentwise_error = -1.0;  // max over the request's right-hand sides
   double t_submit_ms = 0.0;      // server clock (ms since the server started)
   double t_start_ms = 0.0;
   double t_done_ms = 0.0;

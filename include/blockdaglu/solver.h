@@ -54,6 +54,7 @@ struct EngineReport {
   int32_t ir_iterations = 0;
   double rel_residual = -1.0;        // ||b - A x||_2 / ||b||_2 (FP64, on device)
   double backward_error = -1.0;      // ||r||_inf / (||A||_inf ||x||_inf + ||b||_inf)
+  double componentwise_error = -1.0; // max_i |r_i| / (|A| |x| + |b|)_i
   bool plan_reused = false;          // analyze() found a resident plan for this pattern
 };
 

@@ -150,3 +150,34 @@ def test_precision_switch_on_a_cached_plan(gpu):
     _say(f"  plan_reused={rep['plan_reused']} iters solver={rep['ir_iterations']} fresh={res['ir_iterations']}")
     assert np.array_equal(X, x32)
     assert rep["ir_iterations"] == res["ir_iterations"]
+
+
+def test_side_streams_match_one_stream(gpu, monkeypatch):
+    """A refactor that spreads the dense-block LU and the dense tails of several
+    sparse blocks over side streams (joined by events, captured in one graph)
+    gives exactly what the same launches on one stream give."""
+    bd = gpu
+    _say("case: side streams vs one stream, several sparse blocks")
+    sp = bd.config_spec(1)
+    sp.seed = 700
+    sp.n_small_blocks = 80
+    sp.n_small_total = 1600
+    sp.n_large_blocks = 4
+    sp.large_size_lo, sp.large_size_hi = 200, 700
+    n, r, c, v, _ = bd.generate(sp)
+    rng = np.random.default_rng(9)
+    reqs = [(bd.perturb(v, seed=90 + i), np.asfortranarray(rng.uniform(-1, 1, (n, 1 + i % 3)))) for i in range(4)]
+    outs = {}
+    for mode in ("4", "1"):
+        monkeypatch.setenv("BLOCKDAGLU_GPU_STREAMS", mode)
+        for prec in (bd.Precision.FP64, bd.Precision.FP32_IR):
+            srv = bd.Server(slots=1, hbm_budget_gb=8.0, precision=prec)
+            pid = srv.add_pattern(r, c, v, n)
+            outs[(mode, prec)] = [srv.wait(srv.submit(pid, vals, B)) for vals, B in reqs]
+            srv.close()
+    for prec in (bd.Precision.FP64, bd.Precision.FP32_IR):
+        for (xs, rs), (x1, r1) in zip(outs[("4", prec)], outs[("1", prec)]):
+            assert rs["error"] == bd.SolverError.None_ and r1["error"] == bd.SolverError.None_
+            assert np.array_equal(xs, x1), "side-stream refactor differs from one stream"
+            assert rs["ir_iterations"] == r1["ir_iterations"]
+    _say(f"  n={n} requests={len(reqs)} equal")
