@@ -30,7 +30,7 @@ HIP_SRCS = ["csrc/hip/kernels.hip", "csrc/hip/engine.hip"]
 HOST_SRCS = [
     "csrc/host/canon.cpp", "csrc/host/btf.cpp", "csrc/host/decomp.cpp", "csrc/host/colamd.cpp",
     "csrc/host/symbolic.cpp", "csrc/host/denselu.cpp", "csrc/host/plan.cpp", "csrc/host/solver.cpp",
-    "csrc/host/capi.cpp", "csrc/host/gen.cpp", "csrc/host/serve.cpp",
+    "csrc/host/capi.cpp", "csrc/host/gen.cpp", "csrc/host/export.cpp", "csrc/host/serve.cpp",
 ]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

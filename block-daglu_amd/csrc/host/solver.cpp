@@ -24,6 +24,7 @@
 
 #include "../../../include/bdlu_gpu.h"
 #include "blockdaglu/env.h"
+#include "export.h"
 #include "host.h"
 #include "plan.h"
 
@@ -152,6 +153,7 @@ struct SolverImpl {
   PlanKey key;
   std::unique_ptr<Resident> plan;
   bool singular_at_analysis = false;
+  std::unique_ptr<bdlu::ExportFactors> exported;  // factor export, built on first request
   ~SolverImpl() { give_back(std::move(plan)); }
 };
 
@@ -182,16 +184,17 @@ bool Solver::set_matrix(CooMatrix m, int32_t dimension) {
   s.prepared = true;
   s.validated = s.decomposed = s.factored = false;
   s.singular_at_analysis = false;
+  s.exported.reset();
   s.canon = bdlu::Canonical{};
   n_ = dimension >= 0 ? dimension : -1;
   report_ = EngineReport{};
   return true;
 }
 
-bool Solver::analyze() {
-  clear_error_ok();
+// Validation and canonicalization of the stored matrix (reference
+// src/solver.cpp:642-722), once per set_matrix.
+bool Solver::validate() {
   SolverImpl& s = *impl_;
-  if (!s.prepared) return fail(SolverError::NotPrepared, "set_matrix must be called before analyze");
   if (!s.validated) {
     std::string msg;
     const bdlu::CanonStatus st = bdlu::canonicalize(s.original, s.dim_hint, s.canon, msg);
@@ -209,6 +212,14 @@ bool Solver::analyze() {
     s.validated = true;
   }
   n_ = s.canon.n;
+  return true;
+}
+
+bool Solver::analyze() {
+  clear_error_ok();
+  SolverImpl& s = *impl_;
+  if (!s.prepared) return fail(SolverError::NotPrepared, "set_matrix must be called before analyze");
+  if (!validate()) return false;
   PlanKey k;
   k.n = s.canon.n;
   k.nnz = static_cast<int64_t>(s.canon.nnz());
@@ -434,16 +445,79 @@ void* Solver::device_stream() const {
   return c ? bdlu_gpu_stream(c) : nullptr;
 }
 
-// Exports: the reference builds them from a separate global KLU factorization
-// with btf=0 (src/solver.cpp:1541-1725); that route is not part of this engine.
-CooMatrix Solver::get_rearranged_matrix() {
-  fail(SolverError::ExportFailed, "factor export is not available in the BLOCK engine");
-  return {};
+// Exports (reference src/solver.cpp:1598-1725): a global sparse LU of the
+// matrix given to set_matrix, built on the first request and kept until the
+// next set_matrix; they need neither analyze() nor a device.
+bool Solver::ensure_export() {
+  SolverImpl& s = *impl_;
+  if (s.exported) return true;
+  if (!s.prepared) return fail(SolverError::NotPrepared, "set_matrix must be called before exporting factors");
+  if (!validate()) return false;
+  auto e = std::make_unique<bdlu::ExportFactors>();
+  std::string msg;
+  try {
+    if (!bdlu::export_factors(s.canon, *e, msg)) return fail(SolverError::ExportFailed, msg);
+  } catch (const std::exception& ex) {
+    return fail(SolverError::ExportFailed, ex.what());
+  }
+  s.exported = std::move(e);
+  return true;
 }
-CooMatrix Solver::getQ() { return get_rearranged_matrix(); }
-CooMatrix Solver::getP() { return get_rearranged_matrix(); }
-CooMatrix Solver::getL() { return get_rearranged_matrix(); }
-CooMatrix Solver::getU() { return get_rearranged_matrix(); }
+
+// P (R \ A) Q entry by entry, in the canonical entry order; R holds the row
+// scale factors of the original rows (all 1 unless the export scales).
+CooMatrix Solver::get_rearranged_matrix() {
+  clear_error_ok();
+  if (!ensure_export()) return {};
+  const bdlu::ExportFactors& e = *impl_->exported;
+  const bdlu::Canonical& a = impl_->canon;
+  std::vector<int32_t> pinv(e.n), qinv(e.n);
+  for (int32_t k = 0; k < e.n; ++k) {
+    pinv[e.P[k]] = k;
+    qinv[e.Q[k]] = k;
+  }
+  CooMatrix out;
+  out.reserve(a.nnz());
+  for (std::size_t t = 0; t < a.nnz(); ++t) {
+    const double r = e.Rs.empty() ? 1.0 : e.Rs[a.row[t]];
+    out.emplace_back(pinv[a.row[t]], qinv[a.col[t]], r != 0.0 ? a.val[t] / r : a.val[t]);
+  }
+  return out;
+}
+
+// Q as a matrix: entry (Q[k], k) = 1, so (A Q)(:, k) = A(:, Q[k]).
+CooMatrix Solver::getQ() {
+  clear_error_ok();
+  if (!ensure_export()) return {};
+  const bdlu::ExportFactors& e = *impl_->exported;
+  CooMatrix out;
+  out.reserve(e.n);
+  for (int32_t k = 0; k < e.n; ++k) out.emplace_back(e.Q[k], k, 1.0);
+  return out;
+}
+
+// P as a matrix: entry (k, P[k]) = 1, so (P A)(k, :) = A(P[k], :).
+CooMatrix Solver::getP() {
+  clear_error_ok();
+  if (!ensure_export()) return {};
+  const bdlu::ExportFactors& e = *impl_->exported;
+  CooMatrix out;
+  out.reserve(e.n);
+  for (int32_t k = 0; k < e.n; ++k) out.emplace_back(k, e.P[k], 1.0);
+  return out;
+}
+
+CooMatrix Solver::getL() {
+  clear_error_ok();
+  if (!ensure_export()) return {};
+  return impl_->exported->L;
+}
+
+CooMatrix Solver::getU() {
+  clear_error_ok();
+  if (!ensure_export()) return {};
+  return impl_->exported->U;
+}
 
 const std::vector<int32_t>& Solver::block_row_perm() const {
   static const std::vector<int32_t> empty;

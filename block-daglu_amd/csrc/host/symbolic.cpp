@@ -108,7 +108,8 @@ struct Reach {
 }  // namespace
 
 bool analyze_sparse_block(int32_t s, const std::vector<int32_t>& rows, const std::vector<int32_t>& cols,
-                          const std::vector<double>& vals, const SymbolicOptions& opt, SparseBlockPlan& out) {
+                          const std::vector<double>& vals, const SymbolicOptions& opt, SparseBlockPlan& out,
+                          SparseFactorValues* values) {
   out = SparseBlockPlan{};
   out.s = s;
   const int32_t ne = static_cast<int32_t>(rows.size());
@@ -130,7 +131,10 @@ bool analyze_sparse_block(int32_t s, const std::vector<int32_t>& rows, const std
   std::vector<double> rs(s, 1.0);
   if (out.scale) {
     std::fill(rs.begin(), rs.end(), 0.0);
-    for (int32_t t = 0; t < ne; ++t) rs[rows[t]] += std::abs(vals[t]);
+    if (opt.scale_kind == 2)
+      for (int32_t t = 0; t < ne; ++t) rs[rows[t]] = std::max(rs[rows[t]], std::abs(vals[t]));
+    else
+      for (int32_t t = 0; t < ne; ++t) rs[rows[t]] += std::abs(vals[t]);
     for (int32_t i = 0; i < s; ++i)
       if (rs[i] == 0.0) rs[i] = 1.0;
   }
@@ -215,12 +219,18 @@ bool analyze_sparse_block(int32_t s, const std::vector<int32_t>& rows, const std
     for (int32_t k : reach) {
       out.u_idx.push_back(k);
       out.updates += out.l_ptr[k + 1] - out.l_ptr[k];
+      if (values) values->ux.push_back(x[out.prow[k]]);
     }
     out.u_ptr.push_back(static_cast<int32_t>(out.u_idx.size()));
+    if (values) values->udiag.push_back(pivot);
     for (int32_t k : reach) x[out.prow[k]] = 0.0;
     for (int32_t i : cand) x[i] = 0.0;
   }
   out.k0 = k0;
+  if (values) {
+    values->lx = lx;
+    values->rs = rs;
+  }
   out.m = s - k0;
 
   // Tail: remaining rows, starting diagonal bookkeeping, and U(0:k0, j) patterns.

@@ -48,14 +48,25 @@ struct SymbolicOptions {
   double tail_tol = 1.0;
   double tail_density = 0.2;     // switch to the dense tail when |L(:,j)| > density*(s-j)
   int32_t tail_min = 32;         // ... and at least this many columns remain
-  int32_t scale_min_n = 4096;    // KLU scale=1 for blocks at least this large
+  int32_t scale_min_n = 4096;    // KLU scaling for blocks at least this large
+  int32_t scale_kind = 1;        // 1: row sums (KLU scale=1), 2: row maxima (KLU scale=2)
+};
+
+// Numeric values of the first factorization, for callers that export factors
+// (analyze_sparse_block otherwise keeps patterns and pivot rows only).
+struct SparseFactorValues {
+  std::vector<double> rs;     // s: row scale factors by local row (1 when unscaled)
+  std::vector<double> lx;     // L(:,k) below the pivot, parallel to l_row
+  std::vector<double> ux;     // U(k,j) above the diagonal, parallel to u_idx
+  std::vector<double> udiag;  // k0: pivots
 };
 SymbolicOptions symbolic_options_from_env();
 
 // rows/cols/vals: the block's local entries (canonical order). Returns false if
 // a column has no admissible pivot (numerically or structurally singular).
 bool analyze_sparse_block(int32_t s, const std::vector<int32_t>& rows, const std::vector<int32_t>& cols,
-                          const std::vector<double>& vals, const SymbolicOptions& opt, SparseBlockPlan& out);
+                          const std::vector<double>& vals, const SymbolicOptions& opt, SparseBlockPlan& out,
+                          SparseFactorValues* values = nullptr);
 
 // Threshold partial pivoting with diagonal preference and KLU's diagonal
 // tracking on a dense m x m column-major matrix (factored in place, rows

@@ -107,6 +107,8 @@ class Solver final {
 
  private:
   bool clear_error_ok();
+  bool validate();
+  bool ensure_export();
   bool factor_values(const double* values, bool on_device);
   bool fail(SolverError error, std::string message);
 
