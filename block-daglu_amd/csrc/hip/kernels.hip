@@ -680,8 +680,9 @@ __device__ __forceinline__ void tile_gemm(const TileJob<T>& j0, const TileJob<T>
 // Trailing update (op 0 only) and the panel pair (ops 1 and 2): two kernels so
 // profiles and counters separate them.
 template <typename T, int kTile>
-__global__ __launch_bounds__(256) void k_tail_update(TileJob<T> j, double limit, int32_t* __restrict__ status) {
-  tile_gemm<T, true, kTile>(j, j, limit, status);
+__global__ __launch_bounds__(256) void k_tail_update(TileJob<T> j0, TileJob<T> j1, double limit,
+                                                     int32_t* __restrict__ status) {
+  tile_gemm<T, true, kTile>(j0, j1, limit, status);
 }
 template <typename T, int kTile>
 __global__ __launch_bounds__(256) void k_tail_panel(TileJob<T> j0, TileJob<T> j1, double limit,
@@ -713,7 +714,7 @@ void tail_step(hipStream_t st, T* S, int64_t lds, int32_t p, int32_t b, int32_t 
   TileJob<T> upd{rest, rest, b, 0, tr, tr, S + int64_t(p) * lds + p + b, 1, lds,
                  S + int64_t(p + b) * lds + p, 1, lds, S + int64_t(p + b) * lds + p + b, 1, lds};
   if (timer && timer->gemm_events) timer_event(timer->gemm_events, st);
-  hipLaunchKernelGGL((k_tail_update<T, kTile>), dim3(tr * tr), dim3(256), 0, st, upd, limit, status);
+  hipLaunchKernelGGL((k_tail_update<T, kTile>), dim3(tr * tr), dim3(256), 0, st, upd, upd, limit, status);
   if (timer && timer->gemm_events) {
     timer_event(timer->gemm_events, st);
     const double r = rest;
@@ -722,9 +723,67 @@ void tail_step(hipStream_t st, T* S, int64_t lds, int32_t p, int32_t b, int32_t 
   }
 }
 
+// One step with lookahead: the panel pair of step p, then the trailing update
+// in two launches: (A) the next diagonal block alone, after which the next
+// step's diagonal LU starts on `helper`, and (B) the rest of the trailing
+// matrix (two tile jobs: everything right of the next block column, and the
+// block column below the next diagonal block) on `st`, which then waits for
+// the diagonal LU. Every element gets the same MFMA sequence as in the single
+// update, so results are identical.
+template <typename T, int kTile>
+void tail_step_lookahead(hipStream_t st, hipStream_t helper, hipEvent_t e_upd, hipEvent_t e_diag, T* S,
+                         int64_t lds, int32_t p, int32_t b, int32_t rest, double limit, int32_t* status, T* inv) {
+  const int32_t tr = (rest + kTile - 1) / kTile;
+  TileJob<T> l21{rest, b, b, 2, tr, 1, S + int64_t(p) * lds + p + b, 1, lds, inv + kNb * kNb, kNb, 1,
+                 S + int64_t(p) * lds + p + b, 1, lds};
+  TileJob<T> u12{b, rest, b, 1, 1, tr, inv, kNb, 1, S + int64_t(p + b) * lds + p, 1, lds,
+                 S + int64_t(p + b) * lds + p, 1, lds};
+  hipLaunchKernelGGL((k_tail_panel<T, kTile>), dim3(2 * tr), dim3(256), 0, st, l21, u12, limit, status);
+  const int32_t q = p + b;                      // next diagonal block
+  const int32_t nb = rest < kNb ? rest : kNb;   // its size
+  const T* L21 = S + int64_t(p) * lds + q;      // rest x b, column stride lds
+  const T* U12 = S + int64_t(q) * lds + p;      // b x rest, column stride lds
+  TileJob<T> a{nb, nb, b, 0, 1, 1, L21, 1, lds, U12, 1, lds, S + int64_t(q) * lds + q, 1, lds};
+  hipLaunchKernelGGL((k_tail_update<T, 64>), dim3(1), dim3(256), 0, st, a, a, limit, status);
+  (void)hipEventRecord(e_upd, st);
+  (void)hipStreamWaitEvent(helper, e_upd, 0);
+  hipLaunchKernelGGL(k_diag_lu_inv<T>, dim3(1), dim3(kDiagThreads), 0, helper, S, lds, q, nb, limit, status, inv);
+  (void)hipEventRecord(e_diag, helper);
+  if (rest > nb) {
+    // Job 0: all trailing rows, columns right of the next block column.
+    const int32_t n0 = rest - nb;
+    TileJob<T> right{rest, n0, b, 0, tr, (n0 + kTile - 1) / kTile, L21, 1, lds, U12 + int64_t(nb) * lds, 1, lds,
+                     S + int64_t(q + nb) * lds + q, 1, lds};
+    // Job 1: the next block column below its diagonal block.
+    TileJob<T> below{n0, nb, b, 0, (n0 + kTile - 1) / kTile, 1, L21 + nb, 1, lds, U12, 1, lds,
+                     S + int64_t(q) * lds + q + nb, 1, lds};
+    const int32_t g = right.tiles_m * right.tiles_n + below.tiles_m * below.tiles_n;
+    hipLaunchKernelGGL((k_tail_update<T, kTile>), dim3(g), dim3(256), 0, st, right, below, limit, status);
+  }
+  (void)hipStreamWaitEvent(st, e_diag, 0);
+}
+
 template <typename T>
 void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, int32_t* status, T* inv,
-                    TailTimer* timer, int32_t small_tile_below) {
+                    TailTimer* timer, int32_t small_tile_below, hipStream_t helper, hipEvent_t* events) {
+  if (helper && events && !timer) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_diag_lu_inv<T>, dim3(1), dim3(kDiagThreads), 0, st, S, lds, 0, m < kNb ? m : kNb, limit,
+                       status, inv);
+    int32_t step = 0;
+    for (int32_t p = 0; p < m; p += kNb, ++step) {
+      const int32_t b = (m - p) < kNb ? (m - p) : kNb;
+      const int32_t rest = m - p - b;
+      if (rest <= 0) break;
+      if (rest < small_tile_below)
+        tail_step_lookahead<T, 64>(st, helper, events[2 * step], events[2 * step + 1], S, lds, p, b, rest, limit,
+                                   status, inv);
+      else
+        tail_step_lookahead<T, 128>(st, helper, events[2 * step], events[2 * step + 1], S, lds, p, b, rest, limit,
+                                    status, inv);
+    }
+    return;
+  }
   for (int32_t p = 0; p < m; p += kNb) {
     const int32_t b = (m - p) < kNb ? (m - p) : kNb;
     const int32_t rest = m - p - b;
@@ -741,9 +800,9 @@ void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, 
   }
 }
 template void launch_tail_lu<double>(hipStream_t, int32_t, double*, int64_t, double, int32_t*, double*, TailTimer*,
-                                     int32_t);
+                                     int32_t, hipStream_t, hipEvent_t*);
 template void launch_tail_lu<float>(hipStream_t, int32_t, float*, int64_t, double, int32_t*, float*, TailTimer*,
-                                    int32_t);
+                                    int32_t, hipStream_t, hipEvent_t*);
 
 // ---------------------------------------------------------------------------
 // Solve.

@@ -152,18 +152,21 @@ def test_precision_switch_on_a_cached_plan(gpu):
     assert rep["ir_iterations"] == res["ir_iterations"]
 
 
-def test_side_streams_match_one_stream(gpu, monkeypatch):
+@pytest.mark.parametrize("large", [4, 1])
+def test_side_streams_match_one_stream(gpu, monkeypatch, large):
     """A refactor that spreads the dense-block LU and the dense tails of several
-    sparse blocks over side streams (joined by events, captured in one graph)
-    gives exactly what the same launches on one stream give."""
+    sparse blocks over side streams, or (one sparse block) runs each next
+    diagonal LU of the dense tail on a side stream beside the trailing update
+    (lookahead), joined by events and captured in one graph, gives exactly what
+    the same work on one stream gives."""
     bd = gpu
-    _say("case: side streams vs one stream, several sparse blocks")
+    _say(f"case: side streams vs one stream, {large} large block(s)")
     sp = bd.config_spec(1)
-    sp.seed = 700
+    sp.seed = 700 + large
     sp.n_small_blocks = 80
     sp.n_small_total = 1600
-    sp.n_large_blocks = 4
-    sp.large_size_lo, sp.large_size_hi = 200, 700
+    sp.n_large_blocks = large
+    sp.large_size_lo, sp.large_size_hi = (200, 700) if large > 1 else (2500, 2500)
     n, r, c, v, _ = bd.generate(sp)
     rng = np.random.default_rng(9)
     reqs = [(bd.perturb(v, seed=90 + i), np.asfortranarray(rng.uniform(-1, 1, (n, 1 + i % 3)))) for i in range(4)]
