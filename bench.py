@@ -37,7 +37,13 @@ MFMA_PEAK = {"f64": 78.6, "f32": 157.3}  # TFLOP/s dense
 
 
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_tail_update_circuit200k.json"
-ROOFLINE_KERNEL = "k_tail_update<float>"
+ROOFLINE_KERNEL = "k_tail_update<float, *>"
+
+
+def is_roofline_kernel(name: str) -> bool:
+    """Both tile variants of the FP32 trailing update, by demangled or mangled
+    name (not the next-block launch k_tail_update_next)."""
+    return "k_tail_update<float," in name or "k_tail_updateIfLi" in name
 
 
 def _db_view(db, name):
@@ -55,10 +61,17 @@ def profile_summary(db_path: str) -> str:
         "group by s.kernel_name order by sum(d.end - d.start) desc"))
     total = sum(r[2] for r in rows) or 1
     out = ["| kernel | calls | total ms | avg us | min us | max us | share |", "|---|---:|---:|---:|---:|---:|---:|"]
+    fam = [0, 0]
     for name, cnt, tot, mn, mx in rows:
         short = name.split("(")[0].replace("void ", "")
+        if is_roofline_kernel(name):
+            fam[0] += cnt
+            fam[1] += tot
         out.append(f"| `{short}` | {cnt} | {tot / 1e6:.2f} | {tot / cnt / 1e3:.2f} | {mn / 1e3:.2f} | {mx / 1e3:.2f} | "
                    f"{100.0 * tot / total:.1f}% |")
+    if fam[0]:
+        out.append(f"\nRoofline kernel `{ROOFLINE_KERNEL}` (both tile variants): {fam[0]} launches, "
+                   f"average {fam[1] / fam[0] / 1e3:.2f} us.")
     return "\n".join(out)
 
 
@@ -79,10 +92,21 @@ def traffic_from_counters(fetch_db: str, write_db: str) -> dict:
         factor = COPY_BYTES / (float(np.median(cal)) * 1024.0)
         vals = [v for (name, v) in db.execute(
             "select kernel_name, value from counters_collection where counter_name = ?", (which,))
-            if "k_tail_update<float>" in name]
+            if is_roofline_kernel(name)]
         out[which] = {"copy_counter_kib": float(np.median(cal)), "factor": factor, "dispatches": len(vals),
                       "mean_kib": float(np.mean(vals)), "mean_bytes_scaled": float(np.mean(vals)) * 1024.0 * factor}
     out["bytes_per_launch"] = out["FETCH_SIZE"]["mean_bytes_scaled"] + out["WRITE_SIZE"]["mean_bytes_scaled"]
+    # The same counters without the copy factor: FETCH_SIZE is derived from read
+    # requests counted in 64-byte units, which reads a 128-byte request (as the
+    # copy issues) as half its size but a 64-byte request in full.
+    out["bytes_per_launch_unscaled"] = 1024.0 * (out["FETCH_SIZE"]["mean_kib"] + out["WRITE_SIZE"]["mean_kib"])
+    out["kernel"] = ROOFLINE_KERNEL
+    out["commands"] = [
+        "rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmc_fetch -o run -- python3 bench.py --steps 2 --warmup 1 "
+        "--no-cpu-baseline --no-profile --copy-calibration",
+        "rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pmc_write -o run -- python3 bench.py --steps 2 --warmup 1 "
+        "--no-cpu-baseline --no-profile --copy-calibration",
+        "python bench.py --traffic gpurun_out/pmc_fetch/run_results.db gpurun_out/pmc_write/run_results.db"]
     return out
 
 
@@ -295,19 +319,26 @@ def bench_serving(bd, args):
     for prec in (bd.Precision.FP64, bd.Precision.FP32_IR):
         srv = bd.Server(slots=args.slots, hbm_budget_gb=64.0, precision=prec)
         t = time.time()
-        pid = [srv.add_pattern(p[1], p[2], p[3], p[0]) for p in pats]
+        pid = []
+        for k, p in enumerate(pats):
+            pid.append(srv.add_pattern(p[1], p[2], p[3], p[0]))
+            log(f"  {prec.name}: pattern {k} (n={p[0]}) registered, {time.time() - t:.1f}s")
         t_an = time.time() - t
         # warm every slot on every pattern (plan upload, first factor)
         ids = [srv.submit(pid[p], pats[p][4][0], pats[p][5][:, :1]) for p in range(16) for _ in range(args.slots)]
-        for i in ids:
+        for k, i in enumerate(ids):
             srv.wait(i)
+            if (k + 1) % args.slots == 0:
+                log(f"  {prec.name}: warm-up, pattern {k // args.slots} on every slot, {time.time() - t:.1f}s")
         # capacity: closed loop, all requests queued at once
         ncap = min(nreq, 400)
         t = time.perf_counter()
         ids = [srv.submit(pid[req_pat[i]], pats[req_pat[i]][4][req_val[i]], pats[req_pat[i]][5][:, :req_k[i]])
                for i in range(ncap)]
-        for i in ids:
+        for k, i in enumerate(ids):
             srv.wait(i)
+            if (k + 1) % 100 == 0:
+                log(f"  {prec.name}: capacity run {k + 1}/{ncap}")
         cap = ncap / (time.perf_counter() - t)
         if lam is None:
             lam = 0.8 * cap

@@ -188,21 +188,15 @@ struct bdlu_ctx {
   static constexpr int kSide = 4;
   hipStream_t side[kSide] = {};
   bool fork = true;  // BLOCKDAGLU_GPU_STREAMS=1 keeps every launch on one stream
-  bool lookahead = true;  // BLOCKDAGLU_GPU_LOOKAHEAD=0 turns off the dense-tail lookahead
-  std::vector<hipEvent_t> events;       // fork/join points of a refactor
-  std::vector<hipEvent_t> step_events;  // per-step lookahead points of a dense tail
-  static hipEvent_t grow(std::vector<hipEvent_t>& pool, std::size_t i) {
-    while (pool.size() <= i) {
+  bool lookahead = true;  // BLOCKDAGLU_GPU_LOOKAHEAD=0: next diagonal LU in its own launch
+  std::vector<hipEvent_t> events;  // fork/join points of a refactor
+  hipEvent_t event(std::size_t i) {
+    while (events.size() <= i) {
       hipEvent_t e = nullptr;
       HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-      pool.push_back(e);
+      events.push_back(e);
     }
-    return pool[i];
-  }
-  hipEvent_t event(std::size_t i) { return grow(events, i); }
-  hipEvent_t* step_event_array(std::size_t count) {
-    if (count) grow(step_events, count - 1);
-    return step_events.data();
+    return events[i];
   }
   std::mutex mu;
   std::string last_error;
@@ -708,7 +702,7 @@ void factor_sparse_cols(Plan& P, SparseDev& S, hipStream_t st, T* Lx, T* Ux, T* 
 // status, so tails of different sparse blocks may run concurrently).
 template <typename T>
 void factor_sparse_tail(Plan& P, SparseDev& S, hipStream_t st, T* Sm, ProfEvents* pe, double growth_limit,
-                        hipStream_t helper = nullptr, hipEvent_t* events = nullptr) {
+                        bool lookahead = false) {
   bdk::TailTimer tt;
   if (pe) {
     tt.gemm_events = &pe->ev[BDLU_PROF_TAIL_GEMM];
@@ -718,7 +712,7 @@ void factor_sparse_tail(Plan& P, SparseDev& S, hipStream_t st, T* Sm, ProfEvents
   }
   T* inv = sizeof(T) == 8 ? reinterpret_cast<T*>(S.inv64) : reinterpret_cast<T*>(S.inv32);
   bdk::launch_tail_lu<T>(st, S.m, Sm, S.m, growth_limit, S.status, inv, pe ? &tt : nullptr,
-                         P.sched.small_tile_below, helper, events);
+                         P.sched.small_tile_below, lookahead);
 }
 
 }  // namespace
@@ -773,7 +767,6 @@ void bdlu_gpu_destroy(bdlu_ctx* c) {
     for (auto& kv : c->plans) kv.second->mem.release();
     c->plans.clear();
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
-    for (hipEvent_t e : c->step_events) (void)hipEventDestroy(e);
     for (hipStream_t q : c->side) (void)hipStreamDestroy(q);
     (void)hipStreamDestroy(c->stream);
   }
@@ -890,14 +883,6 @@ int bdlu_gpu_refactor(bdlu_ctx* c, uint64_t id, const double* values, int where,
         if (fp32) factor_sparse_cols<float>(P, S, st, S.Lx32, S.Ux32, S.Ud32, S.S32, pe, c->fp32_growth);
         else factor_sparse_cols<double>(P, S, st, S.Lx64, S.Ux64, S.Ud64, S.S64, pe, c->fp64_growth);
         hipStream_t ts = st;
-        // A tail on the main stream uses a side stream for its lookahead.
-        hipStream_t helper = nullptr;
-        hipEvent_t* steps = nullptr;
-        if (fork && !spread && S.m > 0 && c->lookahead) {
-          helper = c->side[1];
-          steps = c->step_event_array(2 * static_cast<std::size_t>((S.m + 63) / 64));
-          used[1] = true;
-        }
         if (spread && S.m > 0) {
           int pick = 1;
           for (int k = 2; k < bdlu_ctx::kSide; ++k)
@@ -908,8 +893,8 @@ int bdlu_gpu_refactor(bdlu_ctx* c, uint64_t id, const double* values, int where,
           HIPCHECK(hipStreamWaitEvent(c->side[pick], c->event(2 + q), 0));
           ts = c->side[pick];
         }
-        if (fp32) factor_sparse_tail<float>(P, S, ts, S.S32, pe, c->fp32_growth, helper, steps);
-        else factor_sparse_tail<double>(P, S, ts, S.S64, pe, c->fp64_growth, helper, steps);
+        if (fp32) factor_sparse_tail<float>(P, S, ts, S.S32, pe, c->fp32_growth, c->lookahead);
+        else factor_sparse_tail<double>(P, S, ts, S.S64, pe, c->fp64_growth, c->lookahead);
       }
       if (fork) {
         // Join: the main stream waits for the dense-block LU and every side stream used.

@@ -60,13 +60,11 @@ struct TailTimer {
 // No-pivot blocked LU of the m x m column-major S (the tail after its rows were
 // placed in the fixed pivot order). Check: |L| <= limit and nonzero pivots.
 // inv: 2*64*64 scratch for the diagonal blocks' inverses.
-// helper/events: with a second stream and 2*ceil(m/64) events, each step's
-// diagonal LU of the next block runs on `helper` beside the bulk of the
-// trailing update (lookahead); results are identical either way.
+// lookahead: each step's next diagonal LU runs as one workgroup of the bulk
+// trailing-update launch (ignored while timing); results are identical either way.
 template <typename T>
 void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, int32_t* status, T* inv,
-                    TailTimer* timer = nullptr, int32_t small_tile_below = 0, hipStream_t helper = nullptr,
-                    hipEvent_t* events = nullptr);
+                    TailTimer* timer = nullptr, int32_t small_tile_below = 0, bool lookahead = false);
 
 // ---- solve ----
 // Vector sets hold kc columns (right-hand sides), column c at base + c * ld.

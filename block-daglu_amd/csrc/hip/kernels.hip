@@ -423,13 +423,17 @@ __device__ __forceinline__ double readlane_t(double v, int src) {
   return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
 }
 
+constexpr int kDiagLC = kNb + 2;  // column-major LDS copy: fc[j*66 + i] = LU(i, j); 16-byte aligned columns
+
+// Body of the diagonal LU + inverses for a workgroup of 128 or 256 threads
+// (waves 2 and 3, when present, only take part in the barriers). fc: LDS of
+// 64*66 T; st_sh: one LDS int.
 template <typename T>
-__global__ __launch_bounds__(kDiagThreads) void k_diag_lu_inv(T* __restrict__ S, int64_t lds, int32_t p, int32_t b,
-                                                              double limit, int32_t* __restrict__ status,
-                                                              T* __restrict__ inv) {
-  constexpr int LC = kNb + 2;  // column-major LDS copy: fc[j*66 + i] = LU(i, j); 16-byte aligned columns
-  __shared__ T fc[kNb * LC];
-  __shared__ int32_t st_sh;
+__device__ __forceinline__ void diag_lu_inv_body(T* __restrict__ S, int64_t lds, int32_t p, int32_t b, double limit,
+                                                 int32_t* __restrict__ status, T* __restrict__ inv, T* fc,
+                                                 int32_t* st_sh_ptr) {
+  constexpr int LC = kDiagLC;
+  int32_t& st_sh = *st_sh_ptr;
   const int lane = threadIdx.x & (kNb - 1);
   const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
   if (threadIdx.x == 0) st_sh = 0;
@@ -492,7 +496,7 @@ __global__ __launch_bounds__(kDiagThreads) void k_diag_lu_inv(T* __restrict__ S,
     }
 #pragma unroll
     for (int i = 0; i < kNb; ++i) inv[i * kNb + j] = x[i];
-  } else {
+  } else if (w == 1) {
     // The factor back to S (coalesced down each column), then column j of
     // U11^{-1}, backward: y_k /= U(k,k), then y_i -= U(i,k) y_k for i < k.
     const int i = lane;
@@ -515,6 +519,15 @@ __global__ __launch_bounds__(kDiagThreads) void k_diag_lu_inv(T* __restrict__ S,
   if (st) atomicOr(&st_sh, st);
   __syncthreads();
   if (threadIdx.x == 0 && st_sh) atomicOr(status, st_sh);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kDiagThreads) void k_diag_lu_inv(T* __restrict__ S, int64_t lds, int32_t p, int32_t b,
+                                                              double limit, int32_t* __restrict__ status,
+                                                              T* __restrict__ inv) {
+  __shared__ T fc[kNb * kDiagLC];
+  __shared__ int32_t st_sh;
+  diag_lu_inv_body<T>(S, lds, p, b, limit, status, inv, fc, &st_sh);
 }
 
 // Inputs of both forms: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16].
@@ -562,21 +575,24 @@ struct TileJob {
 // j0.tiles_m*j0.tiles_n run job 0, the rest job 1.
 // kTile: output tile edge (128, or 64 for small trailing matrices, where 128
 // leaves most CUs idle); each wavefront computes a kTile/2 square.
+template <int kTile, typename T>
+constexpr std::size_t tile_lds_elems() { return 2 * kNb * (kTile + 1); }
+
 template <typename T, bool kUpdate, int kTile>
 __device__ __forceinline__ void tile_gemm(const TileJob<T>& j0, const TileJob<T>& j1, double limit,
-                                          int32_t* __restrict__ status) {
+                                          int32_t* __restrict__ status, T* lds, int32_t bid) {
   using V = typename MfmaOps<T>::v4;
   constexpr int kGT = kTile;
   constexpr int kGLd = kGT + 1;    // LDS row length (padded against bank conflicts)
   constexpr int kWT = kGT / 2;     // wavefront sub-tile edge
   constexpr int kMB = kGT / 32;    // 16x16 MFMA blocks per wavefront and dimension
   constexpr int kRounds = kGT / 64;  // rounds of 16 staged elements per thread and operand
-  __shared__ T As[kNb * kGLd];  // As[k*kGLd + i] = A(i0 + i, k)
-  __shared__ T Bs[kNb * kGLd];  // Bs[k*kGLd + j] = +-B(k, c0 + j)
+  T* As = lds;                // As[k*kGLd + i] = A(i0 + i, k)
+  T* Bs = lds + kNb * kGLd;   // Bs[k*kGLd + j] = +-B(k, c0 + j)
   const int32_t n0 = j0.tiles_m * j0.tiles_n;
-  const bool second = static_cast<int32_t>(blockIdx.x) >= n0;
+  const bool second = bid >= n0;
   const TileJob<T>& J = second ? j1 : j0;
-  const int32_t tb = second ? static_cast<int32_t>(blockIdx.x) - n0 : static_cast<int32_t>(blockIdx.x);
+  const int32_t tb = second ? bid - n0 : bid;
   const int32_t i0 = (tb % J.tiles_m) * kGT, c0 = (tb / J.tiles_m) * kGT;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -682,12 +698,39 @@ __device__ __forceinline__ void tile_gemm(const TileJob<T>& j0, const TileJob<T>
 template <typename T, int kTile>
 __global__ __launch_bounds__(256) void k_tail_update(TileJob<T> j0, TileJob<T> j1, double limit,
                                                      int32_t* __restrict__ status) {
-  tile_gemm<T, true, kTile>(j0, j1, limit, status);
+  __shared__ T lds[tile_lds_elems<kTile, T>()];
+  tile_gemm<T, true, kTile>(j0, j1, limit, status, lds, static_cast<int32_t>(blockIdx.x));
+}
+// The rest of a trailing update (jobs j0, j1 from workgroup 1 on) with the next
+// step's diagonal LU + inverses in workgroup 0, which starts with the launch
+// (the next diagonal block was updated by the launch before). The tile and
+// diagonal code share one LDS buffer.
+template <typename T, int kTile>
+__global__ __launch_bounds__(256) void k_tail_update_diag(TileJob<T> j0, TileJob<T> j1, double limit,
+                                                          int32_t* __restrict__ status, T* __restrict__ S,
+                                                          int64_t lds_s, int32_t q, int32_t nb, T* __restrict__ inv) {
+  constexpr std::size_t kTileElems = tile_lds_elems<kTile, T>();
+  constexpr std::size_t kDiagElems = kNb * kDiagLC + 16 / sizeof(T);
+  __shared__ __attribute__((aligned(16))) T lds[kTileElems > kDiagElems ? kTileElems : kDiagElems];
+  if (blockIdx.x == 0) {
+    diag_lu_inv_body<T>(S, lds_s, q, nb, limit, status, inv, lds,
+                        reinterpret_cast<int32_t*>(lds + kNb * kDiagLC));
+    return;
+  }
+  tile_gemm<T, true, kTile>(j0, j1, limit, status, lds, static_cast<int32_t>(blockIdx.x) - 1);
+}
+// The next diagonal block's share of a trailing update, launched alone ahead of
+// the rest when the next diagonal LU runs beside the update (same tile code).
+template <typename T>
+__global__ __launch_bounds__(256) void k_tail_update_next(TileJob<T> j, double limit, int32_t* __restrict__ status) {
+  __shared__ T lds[tile_lds_elems<64, T>()];
+  tile_gemm<T, true, 64>(j, j, limit, status, lds, static_cast<int32_t>(blockIdx.x));
 }
 template <typename T, int kTile>
 __global__ __launch_bounds__(256) void k_tail_panel(TileJob<T> j0, TileJob<T> j1, double limit,
                                                     int32_t* __restrict__ status) {
-  tile_gemm<T, false, kTile>(j0, j1, limit, status);
+  __shared__ T lds[tile_lds_elems<kTile, T>()];
+  tile_gemm<T, false, kTile>(j0, j1, limit, status, lds, static_cast<int32_t>(blockIdx.x));
 }
 
 namespace {
@@ -724,15 +767,15 @@ void tail_step(hipStream_t st, T* S, int64_t lds, int32_t p, int32_t b, int32_t 
 }
 
 // One step with lookahead: the panel pair of step p, then the trailing update
-// in two launches: (A) the next diagonal block alone, after which the next
-// step's diagonal LU starts on `helper`, and (B) the rest of the trailing
-// matrix (two tile jobs: everything right of the next block column, and the
-// block column below the next diagonal block) on `st`, which then waits for
-// the diagonal LU. Every element gets the same MFMA sequence as in the single
-// update, so results are identical.
+// in two launches: (A) the next diagonal block alone, then (B) the rest of the
+// trailing matrix (two tile jobs: everything right of the next block column,
+// and the block column below the next diagonal block) with the next step's
+// diagonal LU as workgroup 0 of the same launch, so it runs beside the bulk of
+// the update from the start. Every element gets the same MFMA sequence as in
+// the single update, so results are identical.
 template <typename T, int kTile>
-void tail_step_lookahead(hipStream_t st, hipStream_t helper, hipEvent_t e_upd, hipEvent_t e_diag, T* S,
-                         int64_t lds, int32_t p, int32_t b, int32_t rest, double limit, int32_t* status, T* inv) {
+void tail_step_lookahead(hipStream_t st, T* S, int64_t lds, int32_t p, int32_t b, int32_t rest, double limit,
+                         int32_t* status, T* inv) {
   const int32_t tr = (rest + kTile - 1) / kTile;
   TileJob<T> l21{rest, b, b, 2, tr, 1, S + int64_t(p) * lds + p + b, 1, lds, inv + kNb * kNb, kNb, 1,
                  S + int64_t(p) * lds + p + b, 1, lds};
@@ -744,43 +787,35 @@ void tail_step_lookahead(hipStream_t st, hipStream_t helper, hipEvent_t e_upd, h
   const T* L21 = S + int64_t(p) * lds + q;      // rest x b, column stride lds
   const T* U12 = S + int64_t(q) * lds + p;      // b x rest, column stride lds
   TileJob<T> a{nb, nb, b, 0, 1, 1, L21, 1, lds, U12, 1, lds, S + int64_t(q) * lds + q, 1, lds};
-  hipLaunchKernelGGL((k_tail_update<T, 64>), dim3(1), dim3(256), 0, st, a, a, limit, status);
-  (void)hipEventRecord(e_upd, st);
-  (void)hipStreamWaitEvent(helper, e_upd, 0);
-  hipLaunchKernelGGL(k_diag_lu_inv<T>, dim3(1), dim3(kDiagThreads), 0, helper, S, lds, q, nb, limit, status, inv);
-  (void)hipEventRecord(e_diag, helper);
-  if (rest > nb) {
-    // Job 0: all trailing rows, columns right of the next block column.
-    const int32_t n0 = rest - nb;
-    TileJob<T> right{rest, n0, b, 0, tr, (n0 + kTile - 1) / kTile, L21, 1, lds, U12 + int64_t(nb) * lds, 1, lds,
-                     S + int64_t(q + nb) * lds + q, 1, lds};
-    // Job 1: the next block column below its diagonal block.
-    TileJob<T> below{n0, nb, b, 0, (n0 + kTile - 1) / kTile, 1, L21 + nb, 1, lds, U12, 1, lds,
-                     S + int64_t(q) * lds + q + nb, 1, lds};
-    const int32_t g = right.tiles_m * right.tiles_n + below.tiles_m * below.tiles_n;
-    hipLaunchKernelGGL((k_tail_update<T, kTile>), dim3(g), dim3(256), 0, st, right, below, limit, status);
-  }
-  (void)hipStreamWaitEvent(st, e_diag, 0);
+  hipLaunchKernelGGL(k_tail_update_next<T>, dim3(1), dim3(256), 0, st, a, limit, status);
+  const int32_t n0 = rest - nb;
+  // Job 0: all trailing rows, columns right of the next block column.
+  TileJob<T> right{rest, n0, b, 0, tr, (n0 + kTile - 1) / kTile, L21, 1, lds, U12 + int64_t(nb) * lds, 1, lds,
+                   S + int64_t(q + nb) * lds + q, 1, lds};
+  // Job 1: the next block column below its diagonal block.
+  TileJob<T> below{n0, nb, b, 0, (n0 + kTile - 1) / kTile, 1, L21 + nb, 1, lds, U12, 1, lds,
+                   S + int64_t(q) * lds + q + nb, 1, lds};
+  if (n0 <= 0) right.tiles_m = right.tiles_n = below.tiles_m = below.tiles_n = 0;
+  const int32_t g = 1 + right.tiles_m * right.tiles_n + below.tiles_m * below.tiles_n;
+  hipLaunchKernelGGL((k_tail_update_diag<T, kTile>), dim3(g), dim3(256), 0, st, right, below, limit, status, S, lds,
+                     q, nb, inv);
 }
 
 template <typename T>
 void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, int32_t* status, T* inv,
-                    TailTimer* timer, int32_t small_tile_below, hipStream_t helper, hipEvent_t* events) {
-  if (helper && events && !timer) {
+                    TailTimer* timer, int32_t small_tile_below, bool lookahead) {
+  if (lookahead && !timer) {
     if (m <= 0) return;
     hipLaunchKernelGGL(k_diag_lu_inv<T>, dim3(1), dim3(kDiagThreads), 0, st, S, lds, 0, m < kNb ? m : kNb, limit,
                        status, inv);
-    int32_t step = 0;
-    for (int32_t p = 0; p < m; p += kNb, ++step) {
+    for (int32_t p = 0; p < m; p += kNb) {
       const int32_t b = (m - p) < kNb ? (m - p) : kNb;
       const int32_t rest = m - p - b;
       if (rest <= 0) break;
       if (rest < small_tile_below)
-        tail_step_lookahead<T, 64>(st, helper, events[2 * step], events[2 * step + 1], S, lds, p, b, rest, limit,
-                                   status, inv);
+        tail_step_lookahead<T, 64>(st, S, lds, p, b, rest, limit, status, inv);
       else
-        tail_step_lookahead<T, 128>(st, helper, events[2 * step], events[2 * step + 1], S, lds, p, b, rest, limit,
-                                    status, inv);
+        tail_step_lookahead<T, 128>(st, S, lds, p, b, rest, limit, status, inv);
     }
     return;
   }
@@ -800,9 +835,9 @@ void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, 
   }
 }
 template void launch_tail_lu<double>(hipStream_t, int32_t, double*, int64_t, double, int32_t*, double*, TailTimer*,
-                                     int32_t, hipStream_t, hipEvent_t*);
+                                     int32_t, bool);
 template void launch_tail_lu<float>(hipStream_t, int32_t, float*, int64_t, double, int32_t*, float*, TailTimer*,
-                                    int32_t, hipStream_t, hipEvent_t*);
+                                    int32_t, bool);
 
 // ---------------------------------------------------------------------------
 // Solve.
