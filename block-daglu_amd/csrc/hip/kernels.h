@@ -66,6 +66,10 @@ template <typename T>
 void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, int32_t* status, T* inv,
                     TailTimer* timer = nullptr, int32_t small_tile_below = 0, bool lookahead = false);
 
+// Explicit inverses of the dense blocks' factors (L^{-1} strictly below the
+// diagonal, U^{-1} on and above it), at the blocks' factor offsets.
+template <typename T>
+void launch_dense_inv(hipStream_t st, int32_t nd, const int32_t* sizes, const int64_t* lu0, const T* lu, T* inv);
 // ---- solve ----
 // Vector sets hold kc columns (right-hand sides), column c at base + c * ld.
 // Kernels with per-column registers are instantiated for 1, 2, 4 and 8 columns

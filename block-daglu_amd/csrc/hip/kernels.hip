@@ -248,6 +248,78 @@ template void launch_dense_lu<float>(hipStream_t, int32_t, const int32_t*, const
                                      const int64_t*, const double*, float*, int32_t*, int32_t*, const int32_t*,
                                      const int32_t*);
 
+// Explicit inverses of each dense block's factors, for the solve: M(i,j) =
+// L^{-1}(i,j) below the diagonal (unit diagonal implied) and U^{-1}(i,j) on and
+// above it, s x s column-major at the block's factor offset. One workgroup of
+// two wavefronts per block: wave 0 forms the columns of L^{-1}, wave 1 those of
+// U^{-1}, lane j column j, right-looking (63-k independent FMAs per step) over
+// a column-major LDS copy of the factor read at wave-uniform addresses.
+// Applying them in the solve replaces two chains of s dependent substitution
+// steps per block by two matrix-vector products.
+template <typename T>
+__global__ __launch_bounds__(128) void k_dense_inv(int32_t nd, const int32_t* __restrict__ sizes,
+                                                   const int64_t* __restrict__ lu0, const T* __restrict__ lu,
+                                                   T* __restrict__ inv) {
+  constexpr int LC = 66;
+  __shared__ T fc[64 * LC];
+  const int32_t t = static_cast<int32_t>(blockIdx.x);
+  if (t >= nd) return;
+  const int s = __builtin_amdgcn_readfirstlane(sizes[t]);
+  const T* src = lu + lu0[t];
+  T* dst = inv + lu0[t];
+  for (int idx = threadIdx.x; idx < s * s; idx += 128) {
+    const int j = idx / s, i = idx - j * s;
+    fc[j * LC + i] = src[idx];
+  }
+  // Rows and columns past s: identity, so every value the loops read is finite.
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += 128) {
+    const int j = idx >> 6, i = idx & 63;
+    if (i >= s || j >= s) fc[j * LC + i] = (i == j) ? T(1) : T(0);
+  }
+  __syncthreads();
+  const int j = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  T x[64];
+#pragma unroll
+  for (int i = 0; i < 64; ++i) x[i] = (i == j) ? T(1) : T(0);
+  if (w == 0) {
+#pragma unroll
+    for (int k = 0; k < 63; ++k) {
+      if (k + 1 < s) {
+#pragma unroll
+        for (int i = k + 1; i < 64; ++i) x[i] = tfma(-fc[k * LC + i], x[k], x[i]);
+      }
+    }
+    if (j < s) {
+#pragma unroll
+      for (int i = 1; i < 64; ++i)
+        if (i < s && i > j) dst[int64_t(j) * s + i] = x[i];
+    }
+  } else {
+#pragma unroll
+    for (int k = 63; k >= 0; --k) {
+      if (k < s) {
+        x[k] = x[k] / fc[k * LC + k];
+#pragma unroll
+        for (int i = 0; i < k; ++i) x[i] = tfma(-fc[k * LC + i], x[k], x[i]);
+      }
+    }
+    if (j < s) {
+#pragma unroll
+      for (int i = 0; i < 64; ++i)
+        if (i < s && i <= j) dst[int64_t(j) * s + i] = x[i];
+    }
+  }
+}
+
+template <typename T>
+void launch_dense_inv(hipStream_t st, int32_t nd, const int32_t* sizes, const int64_t* lu0, const T* lu, T* inv) {
+  if (nd <= 0) return;
+  hipLaunchKernelGGL(k_dense_inv<T>, dim3(nd), dim3(128), 0, st, nd, sizes, lu0, lu, inv);
+}
+template void launch_dense_inv<double>(hipStream_t, int32_t, const int32_t*, const int64_t*, const double*, double*);
+template void launch_dense_inv<float>(hipStream_t, int32_t, const int32_t*, const int64_t*, const float*, float*);
+
 // ---------------------------------------------------------------------------
 // Sparse blocks: KLU scale=1 row sums, in a fixed order per row.
 __global__ void k_rowscale(SpView v, const double* __restrict__ arena, double* __restrict__ rs) {
@@ -897,8 +969,9 @@ __device__ __forceinline__ void stage_factor(double* __restrict__ fs, const T* _
 // maximal run of dense blocks between sparse blocks in the solve order). Each
 // workgroup (one wavefront) claims block tickets in order (a topological order
 // of the block DAG: solve level, then descending block id) and solves block b:
-// rhs = bperm - A_off * xperm (ubtf:2269-2286), then the pivot gather, unit-L
-// forward and U backward substitution (ubtf:2071-2097). x values are published
+// rhs = bperm - A_off * xperm (ubtf:2269-2286), then the pivot gather and the
+// unit-L and U solves (ubtf:2071-2097), applied as products with the block's
+// explicit inverse factors (lu64/lu32 here hold them). x values are published
 // values (kXSentinel protocol): everything that does not need other blocks' x
 // (factor to LDS, row gather, the first off-diagonal entries, the next ticket)
 // is issued first, then the block polls exactly the x entries its rows use.
@@ -946,9 +1019,8 @@ __global__ __launch_bounds__(64) void k_dense_seg_solve(
     else
       stage_factor(fs, lu64 + lu0_of_block[b], s * s, lane);
     __syncthreads();
-    const double rd = act ? 1.0 / fs[lane * s + lane] : 1.0;
-    // Each lane's row of L\U into registers (clamped, unconditional LDS reads):
-    // the substitution chains below then run on registers and readlane only.
+    // Each lane's row of the block's inverse factors into registers (clamped,
+    // unconditional LDS reads).
     double row[64];
     const int li = lane < s ? lane : s - 1;
 #pragma unroll
@@ -971,34 +1043,33 @@ __global__ __launch_bounds__(64) void k_dense_seg_solve(
 #pragma unroll
       for (int c = 0; c < K; ++c) acc[c] = acc[c] - v * wait_x(xperm + c * ldv + j, late);
     }
-    // Row gather (the block's pivoting), then unit-lower and upper substitution
-    // (K independent chains interleaved).
+    // Row gather (the block's pivoting), then y = L^{-1} (P rhs) and
+    // x = U^{-1} y as matrix-vector products: every step reads the finished
+    // input vector, so there is no dependency chain across the block.
 #pragma unroll
     for (int c = 0; c < K; ++c) {
       acc[c] = __shfl(acc[c], src);
       if (!act) acc[c] = 0.0;
     }
+    double y[K];
+#pragma unroll
+    for (int c = 0; c < K; ++c) y[c] = acc[c];
 #pragma unroll
     for (int j = 0; j < 64; ++j) {
       if (j < s) {
+        const double m = lane > j ? row[j] : 0.0;
 #pragma unroll
-        for (int c = 0; c < K; ++c) {
-          const double xj = readlane_f64(acc[c], j);
-          const double t2 = __builtin_fma(-row[j], xj, acc[c]);
-          acc[c] = (lane > j && act) ? t2 : acc[c];
-        }
+        for (int c = 0; c < K; ++c) y[c] = __builtin_fma(m, readlane_f64(acc[c], j), y[c]);
       }
     }
 #pragma unroll
-    for (int j = 63; j >= 0; --j) {
-      if (j < s) {
+    for (int c = 0; c < K; ++c) acc[c] = 0.0;
 #pragma unroll
-        for (int c = 0; c < K; ++c) {
-          acc[c] = (lane == j) ? acc[c] * rd : acc[c];
-          const double xj = readlane_f64(acc[c], j);
-          const double t2 = __builtin_fma(-row[j], xj, acc[c]);
-          acc[c] = (lane < j) ? t2 : acc[c];
-        }
+    for (int j = 0; j < 64; ++j) {
+      if (j < s) {
+        const double m = lane <= j ? row[j] : 0.0;
+#pragma unroll
+        for (int c = 0; c < K; ++c) acc[c] = __builtin_fma(m, readlane_f64(y[c], j), acc[c]);
       }
     }
     if (act) {
@@ -1137,6 +1208,8 @@ __global__ __launch_bounds__(64) void k_tail_trsv(int32_t m, const T* __restrict
   // Lanes past the band never store; columns past the matrix meet xv = 0. The
   // tile is shared by the K right-hand sides.
   const int32_t rowc = live ? row : r0;
+  // Backward: each lane's reciprocal of its own diagonal entry, off the chain.
+  const double rinv = upper ? 1.0 / static_cast<double>(S[int64_t(rowc) * lds + rowc]) : 1.0;
   constexpr int kH = kNb / 2;  // the tile is consumed in two halves of 32 columns
   T v[kH];
   for (int32_t q = 0; q < np - 1; ++q) {
@@ -1189,7 +1262,7 @@ __global__ __launch_bounds__(64) void k_tail_trsv(int32_t m, const T* __restrict
         if (h * kH + c < rows) {
 #pragma unroll
           for (int k = 0; k < K; ++k) {
-            if (lane == h * kH + c) acc[k] = acc[k] / static_cast<double>(v[c]);
+            if (lane == h * kH + c) acc[k] = acc[k] * rinv;
             const double xc = readlane_f64(acc[k], h * kH + c);
             if (lane < h * kH + c) acc[k] = acc[k] - static_cast<double>(v[c]) * xc;
           }

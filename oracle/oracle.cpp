@@ -16,7 +16,7 @@
 //               diagonal preference and diagonal tracking, tol 0.001, scale=1
 //               row sums)                         -> klu_like_factor
 //   klu_solve                                      -> klu_like_solve
-// Pinning (what checks each function) is listed in the test files and DESIGN.md.
+// Pinning (what checks each function) is listed in the docstrings of the test files.
 //
 // Build: oracle/Makefile (g++ -O3 -ffp-contract=off, no fast-math).
 #include <algorithm>
