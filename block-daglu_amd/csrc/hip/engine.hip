@@ -85,6 +85,7 @@ struct SparseDev {
   double* inv64 = nullptr;  // 2*64*64 diagonal-block inverses (tail LU scratch)
   float* inv32 = nullptr;
   double* ty = nullptr;  // m: tail forward-solve output (published values)
+  bdk::SpTailLong tail_long;  // tail columns with a long sparse reach
   int32_t* status = nullptr;  // device int
   int prec = 0;
 };
@@ -455,10 +456,46 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
     std::vector<int32_t> tl_rows(sd.m);
     for (int32_t t = 0; t < sd.m; ++t) tl_rows[t] = sd.k0 + t;
     S.tl_rows = A.copy(tl_rows, st);
+    {
+      // Tail columns whose reach in the sparse part exceeds kLongReach: their
+      // reach positions grouped by forward level, for k_sp_tail_long.
+      constexpr int32_t kLongReach = 64;
+      const int32_t nlev = static_cast<int32_t>(fl_ptr.size()) - 1;
+      std::vector<int32_t> lev_of(std::max(sd.k0, 1), 0);
+      for (int32_t l = 0; l < nlev; ++l)
+        for (int32_t e = fl_ptr[l]; e < fl_ptr[l + 1]; ++e) lev_of[fl_rows[e]] = l;
+      std::vector<int32_t> is_long(std::max(sd.m, 1), 0), long_t, reach, levp;
+      for (int32_t t = 0; t < sd.m && nlev > 0; ++t) {
+        const int32_t j = sd.k0 + t;
+        const int32_t p0 = sd.u_ptr[j], p1 = sd.u_ptr[j + 1];
+        if (p1 - p0 <= kLongReach) continue;
+        is_long[t] = 1;
+        long_t.push_back(t);
+        std::vector<int32_t> cnt(nlev + 1, 0);
+        for (int32_t p = p0; p < p1; ++p) cnt[lev_of[sd.u_idx[p]] + 1]++;
+        for (int32_t l = 0; l < nlev; ++l) cnt[l + 1] += cnt[l];
+        const int32_t base = static_cast<int32_t>(reach.size());
+        for (int32_t l = 0; l <= nlev; ++l) levp.push_back(base + cnt[l]);
+        reach.resize(base + (p1 - p0));
+        for (int32_t p = p0; p < p1; ++p) reach[base + cnt[lev_of[sd.u_idx[p]]]++] = sd.u_idx[p];
+      }
+      if (!long_t.empty()) {
+        bdk::SpTailLong& lg = S.tail_long;
+        lg.nlong = static_cast<int32_t>(long_t.size());
+        lg.nlev = nlev;
+        lg.is_long = A.copy(is_long, st);
+        lg.long_t = A.copy(long_t, st);
+        lg.reach = A.copy(reach, st);
+        lg.lev = A.copy(levp, st);
+      }
+    }
     S.bl_rows = A.copy(bl_rows, st);
     S.lr_ptr = A.copy(lr_ptr, st);
     S.lr_col = A.copy(lr_col, st);
     S.lr_vidx = A.copy(lr_vidx, st);
+    S.tail_long.lr_ptr = S.lr_ptr;
+    S.tail_long.lr_col = S.lr_col;
+    S.tail_long.lr_vidx = S.lr_vidx;
     S.ur_ptr = A.copy(ur_ptr, st);
     S.ur_col = A.copy(ur_col, st);
     S.ur_vidx = A.copy(ur_vidx, st);
@@ -483,7 +520,13 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
   std::stable_sort(P.sp_order.begin(), P.sp_order.end(),
                    [&](int32_t x, int32_t y) { return P.sp[x].m > P.sp[y].m; });
   if (!P.sp.empty()) {
-    P.nslots = static_cast<int32_t>(std::min<int64_t>(1024, max_width));
+    // Column slots of the sparse phases: each sparse or tail column is a serial
+    // chain of dependent memory steps, so the phase time falls with the number
+    // of columns in flight. One slot holds one s-sized work vector; slots are
+    // sized from a 768 MB scratch budget, at least 1024 and at most 8192.
+    constexpr int64_t kScratchBudget = int64_t(768) << 20;
+    const int64_t by_budget = std::max<int64_t>(1024, kScratchBudget / (int64_t(sizeof(double)) * max_s));
+    P.nslots = static_cast<int32_t>(std::min<int64_t>(std::min<int64_t>(8192, by_budget), max_width));
     const std::size_t xbytes = sizeof(double) * static_cast<std::size_t>(P.nslots) * max_s;
     P.xscratch = A.get<char>(xbytes);
     HIPCHECK(hipMemsetAsync(P.xscratch, 0, xbytes, st));
@@ -698,7 +741,8 @@ void factor_sparse_cols(Plan& P, SparseDev& S, hipStream_t st, T* Lx, T* Ux, T* 
                            P.nslots, Lx, Ux, Ud, S.status);
   if (sizeof(T) == 8) bdk::launch_fill_f64(st, reinterpret_cast<double*>(Sm), int64_t(S.m) * S.m, 0.0);
   else bdk::launch_fill_f32(st, reinterpret_cast<float*>(Sm), int64_t(S.m) * S.m, 0.0f);
-  bdk::launch_sp_tail<T>(st, view, P.arena, S.rs, xs, P.nslots, Lx, Ux, Sm, S.m);
+  bdk::launch_sp_tail<T>(st, view, P.arena, S.rs, xs, P.nslots, Lx, Ux, Sm, S.m,
+                         S.tail_long.nlong > 0 ? &S.tail_long : nullptr);
   if (pe) pe->mark(BDLU_PROF_SPARSE, st);
 }
 

@@ -46,9 +46,23 @@ void launch_rowscale(hipStream_t st, const SpView& v, const double* arena, doubl
 template <typename T>
 void launch_sp_cols(hipStream_t st, const SpView& v, const double* arena, const double* rs, int32_t ncols,
                     const int32_t* cols, T* xs, int32_t nslots, T* Lx, T* Ux, T* Ud, int32_t* status);
+// Tail columns whose sparse reach is long, done row-oriented by level (one
+// workgroup each) after the others: is_long (m) marks them, long_t lists them,
+// reach/lev hold their reach positions grouped by forward level (nlev levels),
+// lr_*: L by position rows.
+struct SpTailLong {
+  int32_t nlong = 0, nlev = 0;
+  const int32_t* is_long = nullptr;
+  const int32_t* long_t = nullptr;
+  const int32_t* reach = nullptr;
+  const int32_t* lev = nullptr;
+  const int32_t* lr_ptr = nullptr;
+  const int32_t* lr_col = nullptr;
+  const int32_t* lr_vidx = nullptr;
+};
 template <typename T>
 void launch_sp_tail(hipStream_t st, const SpView& v, const double* arena, const double* rs, T* xs, int32_t nslots,
-                    const T* Lx, T* Ux, T* S, int64_t lds);
+                    const T* Lx, T* Ux, T* S, int64_t lds, const SpTailLong* lg = nullptr);
 // Optional per-launch event timing of the tail kernels (profiling runs only).
 struct TailTimer {
   std::vector<hipEvent_t>* gemm_events = nullptr;   // pairs (start, stop)

@@ -413,10 +413,11 @@ template <typename T>
 __global__ __launch_bounds__(64) void k_sp_tail(SpView v, const double* __restrict__ arena,
                                                 const double* __restrict__ rs, T* __restrict__ xs,
                                                 const T* __restrict__ Lx, T* __restrict__ Ux, T* __restrict__ S,
-                                                int64_t lds) {
+                                                int64_t lds, const int32_t* __restrict__ is_long) {
   const int lane = threadIdx.x;
   T* x = xs + int64_t(blockIdx.x) * v.s;
   for (int32_t t = blockIdx.x; t < v.m; t += gridDim.x) {
+    if (is_long && is_long[t]) continue;  // done by k_sp_tail_long
     const int32_t j = v.k0 + t;
     T* col = S + int64_t(t) * lds;
     for (int32_t p = v.a_ptr[j] + lane; p < v.a_ptr[j + 1]; p += kWave) {
@@ -448,17 +449,85 @@ __global__ __launch_bounds__(64) void k_sp_tail(SpView v, const double* __restri
     wave_sync_mem();
   }
 }
+// The same tail columns for the few whose sparse reach is long (thousands of
+// entries make the column-oriented chain above the longest path of the phase).
+// One 256-thread workgroup per column, row-oriented: the reach positions come
+// grouped by forward level, and within a level every position k computes
+// U(k,j) = a(k,j) - sum_k' L(k,k') U(k',j) over its row of L (rows of one level
+// depend only on lower levels, so there are no write conflicts and the sums
+// have a fixed order). Then every tail row pulls its L21 products into S(:,t).
+// reach/lev: the column's reach positions by level, lev[c*(nlev+1) + l] the
+// start of level l in `reach` for long column c; lr_*: L by position rows.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sp_tail_long(SpView v, const double* __restrict__ arena,
+                                                      const double* __restrict__ rs, T* __restrict__ xs,
+                                                      const T* __restrict__ Lx, T* __restrict__ Ux,
+                                                      T* __restrict__ S, int64_t lds, int32_t nlong,
+                                                      const int32_t* __restrict__ long_t,
+                                                      const int32_t* __restrict__ reach,
+                                                      const int32_t* __restrict__ lev, int32_t nlev,
+                                                      const int32_t* __restrict__ lr_ptr,
+                                                      const int32_t* __restrict__ lr_col,
+                                                      const int32_t* __restrict__ lr_vidx) {
+  const int tid = threadIdx.x;
+  T* x = xs + int64_t(blockIdx.x) * v.s;  // this workgroup's work slot
+  for (int32_t c = blockIdx.x; c < nlong; c += gridDim.x) {
+  const int32_t t = long_t[c];
+  const int32_t j = v.k0 + t;
+  T* col = S + int64_t(t) * lds;
+  for (int32_t p = v.a_ptr[j] + tid; p < v.a_ptr[j + 1]; p += 256) {
+    const int32_t r = v.a_row[p];
+    const double a = arena[v.a_val0 + p];
+    const T val = static_cast<T>(v.scale ? a / rs[r] : a);
+    const int32_t q = v.rpos[r];
+    if (q < v.k0) x[r] = val;
+    else col[q - v.k0] = val;
+  }
+  __syncthreads();
+  const int32_t* lv = lev + int64_t(c) * (nlev + 1);
+  for (int32_t l = 0; l < nlev; ++l) {
+    for (int32_t e = lv[l] + tid; e < lv[l + 1]; e += 256) {
+      const int32_t k = reach[e];
+      const int32_t r = v.prow[k];
+      T acc = x[r];
+      for (int32_t p = lr_ptr[k]; p < lr_ptr[k + 1]; ++p) acc = acc - Lx[lr_vidx[p]] * x[v.prow[lr_col[p]]];
+      x[r] = acc;
+    }
+    __syncthreads();
+  }
+  for (int32_t tt = tid; tt < v.m; tt += 256) {
+    const int32_t k = v.k0 + tt;
+    const int32_t p0 = lr_ptr[k], p1 = lr_ptr[k + 1];
+    if (p0 == p1) continue;
+    T acc = col[tt];
+    for (int32_t p = p0; p < p1; ++p) acc = acc - Lx[lr_vidx[p]] * x[v.prow[lr_col[p]]];
+    col[tt] = acc;
+  }
+  __syncthreads();
+  for (int32_t up = v.u_ptr[j] + tid; up < v.u_ptr[j + 1]; up += 256) Ux[up] = x[v.prow[v.u_idx[up]]];
+  __syncthreads();
+  for (int32_t up = v.u_ptr[j] + tid; up < v.u_ptr[j + 1]; up += 256) x[v.prow[v.u_idx[up]]] = T(0);
+  __syncthreads();
+  }
+}
+
 template <typename T>
 void launch_sp_tail(hipStream_t st, const SpView& v, const double* arena, const double* rs, T* xs, int32_t nslots,
-                    const T* Lx, T* Ux, T* S, int64_t lds) {
+                    const T* Lx, T* Ux, T* S, int64_t lds, const SpTailLong* lg) {
   if (v.m <= 0) return;
   const int g = v.m < nslots ? v.m : nslots;
-  hipLaunchKernelGGL(k_sp_tail<T>, dim3(g), dim3(64), 0, st, v, arena, rs, xs, Lx, Ux, S, lds);
+  hipLaunchKernelGGL(k_sp_tail<T>, dim3(g), dim3(64), 0, st, v, arena, rs, xs, Lx, Ux, S, lds,
+                     lg ? lg->is_long : nullptr);
+  if (lg && lg->nlong > 0) {
+    const int gl = lg->nlong < nslots ? lg->nlong : nslots;
+    hipLaunchKernelGGL(k_sp_tail_long<T>, dim3(gl), dim3(256), 0, st, v, arena, rs, xs, Lx, Ux, S, lds, lg->nlong,
+                       lg->long_t, lg->reach, lg->lev, lg->nlev, lg->lr_ptr, lg->lr_col, lg->lr_vidx);
+  }
 }
 template void launch_sp_tail<double>(hipStream_t, const SpView&, const double*, const double*, double*, int32_t,
-                                     const double*, double*, double*, int64_t);
+                                     const double*, double*, double*, int64_t, const SpTailLong*);
 template void launch_sp_tail<float>(hipStream_t, const SpView&, const double*, const double*, float*, int32_t,
-                                    const float*, float*, float*, int64_t);
+                                    const float*, float*, float*, int64_t, const SpTailLong*);
 
 // ---------------------------------------------------------------------------
 // Dense tail: no-pivot blocked right-looking LU, panel width 64. Per panel:
