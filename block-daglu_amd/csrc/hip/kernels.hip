@@ -846,8 +846,11 @@ __global__ __launch_bounds__(256) void k_tail_update(TileJob<T> j0, TileJob<T> j
 // step's diagonal LU + inverses in workgroup 0, which starts with the launch
 // (the next diagonal block was updated by the launch before). The tile and
 // diagonal code share one LDS buffer.
+// FP32: two waves per SIMD, as for the plain update (the diagonal path would
+// otherwise raise the register count of the whole kernel).
 template <typename T, int kTile>
-__global__ __launch_bounds__(256) void k_tail_update_diag(TileJob<T> j0, TileJob<T> j1, double limit,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1)))
+void k_tail_update_diag(TileJob<T> j0, TileJob<T> j1, double limit,
                                                           int32_t* __restrict__ status, T* __restrict__ S,
                                                           int64_t lds_s, int32_t q, int32_t nb, T* __restrict__ inv) {
   constexpr std::size_t kTileElems = tile_lds_elems<kTile, T>();
