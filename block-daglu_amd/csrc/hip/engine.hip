@@ -86,6 +86,7 @@ struct SparseDev {
   float* inv32 = nullptr;
   double* ty = nullptr;  // m: tail forward-solve output (published values)
   bdk::SpTailLong tail_long;  // tail columns with a long sparse reach
+  std::vector<bdk::SpColsLong> cols_long;  // per refactor level: columns with a long reach
   int32_t* status = nullptr;  // device int
   int prec = 0;
 };
@@ -488,6 +489,39 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
         lg.reach = A.copy(reach, st);
         lg.lev = A.copy(levp, st);
       }
+      // The same for the sparse columns, per refactor level.
+      std::vector<int32_t> col_long(std::max(sd.k0, 1), 0);
+      S.cols_long.assign(sd.nlev, bdk::SpColsLong{});
+      bool any = false;
+      for (int32_t l = 0; l < sd.nlev && nlev > 0; ++l) {
+        std::vector<int32_t> lj, rch, lp;
+        for (int32_t e = sd.lev_ptr[l]; e < sd.lev_ptr[l + 1]; ++e) {
+          const int32_t j = sd.lev_col[e];
+          const int32_t p0 = sd.u_ptr[j], p1 = sd.u_ptr[j + 1];
+          if (p1 - p0 <= kLongReach) continue;
+          col_long[j] = 1;
+          lj.push_back(j);
+          std::vector<int32_t> cnt(nlev + 1, 0);
+          for (int32_t p = p0; p < p1; ++p) cnt[lev_of[sd.u_idx[p]] + 1]++;
+          for (int32_t q = 0; q < nlev; ++q) cnt[q + 1] += cnt[q];
+          const int32_t base = static_cast<int32_t>(rch.size());
+          for (int32_t q = 0; q <= nlev; ++q) lp.push_back(base + cnt[q]);
+          rch.resize(base + (p1 - p0));
+          for (int32_t p = p0; p < p1; ++p) rch[base + cnt[lev_of[sd.u_idx[p]]]++] = sd.u_idx[p];
+        }
+        if (lj.empty()) continue;
+        any = true;
+        bdk::SpColsLong& g = S.cols_long[l];
+        g.nlong = static_cast<int32_t>(lj.size());
+        g.nlev = nlev;
+        g.long_j = A.copy(lj, st);
+        g.reach = A.copy(rch, st);
+        g.lev = A.copy(lp, st);
+      }
+      if (any) {
+        const int32_t* d_col_long = A.copy(col_long, st);
+        for (bdk::SpColsLong& g : S.cols_long) g.is_long = d_col_long;
+      }
     }
     S.bl_rows = A.copy(bl_rows, st);
     S.lr_ptr = A.copy(lr_ptr, st);
@@ -496,6 +530,11 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
     S.tail_long.lr_ptr = S.lr_ptr;
     S.tail_long.lr_col = S.lr_col;
     S.tail_long.lr_vidx = S.lr_vidx;
+    for (bdk::SpColsLong& g : S.cols_long) {
+      g.lr_ptr = S.lr_ptr;
+      g.lr_col = S.lr_col;
+      g.lr_vidx = S.lr_vidx;
+    }
     S.ur_ptr = A.copy(ur_ptr, st);
     S.ur_col = A.copy(ur_col, st);
     S.ur_vidx = A.copy(ur_vidx, st);
@@ -738,7 +777,8 @@ void factor_sparse_cols(Plan& P, SparseDev& S, hipStream_t st, T* Lx, T* Ux, T* 
   T* xs = static_cast<T*>(P.xscratch);
   for (std::size_t l = 0; l + 1 < S.lev_ptr.size(); ++l)
     bdk::launch_sp_cols<T>(st, view, P.arena, S.rs, S.lev_ptr[l + 1] - S.lev_ptr[l], S.lev_col + S.lev_ptr[l], xs,
-                           P.nslots, Lx, Ux, Ud, S.status);
+                           P.nslots, Lx, Ux, Ud, S.status,
+                           l < S.cols_long.size() && S.cols_long[l].nlong > 0 ? &S.cols_long[l] : nullptr);
   if (sizeof(T) == 8) bdk::launch_fill_f64(st, reinterpret_cast<double*>(Sm), int64_t(S.m) * S.m, 0.0);
   else bdk::launch_fill_f32(st, reinterpret_cast<float*>(Sm), int64_t(S.m) * S.m, 0.0f);
   bdk::launch_sp_tail<T>(st, view, P.arena, S.rs, xs, P.nslots, Lx, Ux, Sm, S.m,

@@ -43,9 +43,24 @@ void launch_dense_lu(hipStream_t st, int32_t nd, const int32_t* sizes, const int
                      const int32_t* block_of, const int32_t* order = nullptr);
 
 void launch_rowscale(hipStream_t st, const SpView& v, const double* arena, double* rs);
+// Columns of one refactor level whose reach is long, done row-oriented by
+// level (one workgroup each) after the others: is_long (k0) marks them,
+// long_j lists this level's, reach/lev hold their reach positions grouped by
+// forward level (nlev levels), lr_*: L by position rows.
+struct SpColsLong {
+  int32_t nlong = 0, nlev = 0;
+  const int32_t* is_long = nullptr;
+  const int32_t* long_j = nullptr;
+  const int32_t* reach = nullptr;
+  const int32_t* lev = nullptr;
+  const int32_t* lr_ptr = nullptr;
+  const int32_t* lr_col = nullptr;
+  const int32_t* lr_vidx = nullptr;
+};
 template <typename T>
 void launch_sp_cols(hipStream_t st, const SpView& v, const double* arena, const double* rs, int32_t ncols,
-                    const int32_t* cols, T* xs, int32_t nslots, T* Lx, T* Ux, T* Ud, int32_t* status);
+                    const int32_t* cols, T* xs, int32_t nslots, T* Lx, T* Ux, T* Ud, int32_t* status,
+                    const SpColsLong* lg = nullptr);
 // Tail columns whose sparse reach is long, done row-oriented by level (one
 // workgroup each) after the others: is_long (m) marks them, long_t lists them,
 // reach/lev hold their reach positions grouped by forward level (nlev levels),

@@ -342,13 +342,14 @@ __global__ __launch_bounds__(64) void k_sp_cols(SpView v, const double* __restri
                                                 const double* __restrict__ rs, int32_t ncols,
                                                 const int32_t* __restrict__ cols, T* __restrict__ xs,
                                                 T* __restrict__ Lx, T* __restrict__ Ux, T* __restrict__ Ud,
-                                                int32_t* __restrict__ status) {
+                                                int32_t* __restrict__ status, const int32_t* __restrict__ is_long) {
   const int lane = threadIdx.x;
   T* x = xs + int64_t(blockIdx.x) * v.s;
   int32_t st = 0;
   T lmax_all = T(0);
   for (int32_t c = blockIdx.x; c < ncols; c += gridDim.x) {
     const int32_t j = cols[c];
+    if (is_long && is_long[j]) continue;  // done by k_sp_cols_long
     for (int32_t p = v.a_ptr[j] + lane; p < v.a_ptr[j + 1]; p += kWave) {
       const int32_t r = v.a_row[p];
       const double a = arena[v.a_val0 + p];
@@ -392,17 +393,115 @@ __global__ __launch_bounds__(64) void k_sp_cols(SpView v, const double* __restri
   if (lane == 0 && st) atomicOr(status, st);
 }
 
+// The same for the few columns of a level whose reach is long (the column
+// chain above would set the level's time): one 256-thread workgroup per
+// column, row-oriented over the forward levels of the reach, as
+// k_sp_tail_long; then the pivot row and the rows of L(:,j) each pull their
+// row of L over the columns before j (later columns are not yet refactored).
+template <typename T>
+__global__ __launch_bounds__(256) void k_sp_cols_long(SpView v, const double* __restrict__ arena,
+                                                      const double* __restrict__ rs, T* __restrict__ xs,
+                                                      T* __restrict__ Lx, T* __restrict__ Ux, T* __restrict__ Ud,
+                                                      int32_t* __restrict__ status, int32_t nlong,
+                                                      const int32_t* __restrict__ long_j,
+                                                      const int32_t* __restrict__ reach,
+                                                      const int32_t* __restrict__ lev, int32_t nlev,
+                                                      const int32_t* __restrict__ lr_ptr,
+                                                      const int32_t* __restrict__ lr_col,
+                                                      const int32_t* __restrict__ lr_vidx) {
+  __shared__ int32_t st_sh;
+  __shared__ T piv_sh;
+  const int tid = threadIdx.x;
+  T* x = xs + int64_t(blockIdx.x) * v.s;
+  int32_t st = 0;
+  T lmax_all = T(0);
+  for (int32_t c = blockIdx.x; c < nlong; c += gridDim.x) {
+    const int32_t j = long_j[c];
+    for (int32_t p = v.a_ptr[j] + tid; p < v.a_ptr[j + 1]; p += 256) {
+      const int32_t r = v.a_row[p];
+      const double a = arena[v.a_val0 + p];
+      x[r] = static_cast<T>(v.scale ? a / rs[r] : a);
+    }
+    __syncthreads();
+    const int32_t* lv = lev + int64_t(c) * (nlev + 1);
+    for (int32_t l = 0; l < nlev; ++l) {
+      for (int32_t e = lv[l] + tid; e < lv[l + 1]; e += 256) {
+        const int32_t k = reach[e];
+        const int32_t r = v.prow[k];
+        T acc = x[r];
+        for (int32_t p = lr_ptr[k]; p < lr_ptr[k + 1]; ++p) {
+          const T xv = x[v.prow[lr_col[p]]];
+          acc = xv != T(0) ? acc - Lx[lr_vidx[p]] * xv : acc;
+        }
+        x[r] = acc;
+      }
+      __syncthreads();
+    }
+    // Rows at positions >= j: the pivot row (e = -1) and the rows of L(:,j).
+    // Each reads only x of reach positions (< j), never another row of this
+    // set, so every row is written as soon as it is summed.
+    const int32_t nl = v.l_ptr[j + 1] - v.l_ptr[j];
+    for (int32_t e = tid - 1; e < nl; e += 256) {
+      const int32_t r = e < 0 ? v.prow[j] : v.l_row[v.l_ptr[j] + e];
+      const int32_t kr = v.rpos[r];
+      T acc = x[r];
+      // Columns before j outside its reach have x = 0 here but need not be
+      // refactored yet (same or later level): their L values are skipped.
+      for (int32_t p = lr_ptr[kr]; p < lr_ptr[kr + 1] && lr_col[p] < j; ++p) {
+        const T xv = x[v.prow[lr_col[p]]];
+        acc = xv != T(0) ? acc - Lx[lr_vidx[p]] * xv : acc;
+      }
+      x[r] = acc;
+      if (e < 0) piv_sh = acc;
+    }
+    __syncthreads();
+    const T piv = piv_sh;
+    if (tid == 0) {
+      Ud[j] = piv;
+      if (piv == T(0)) st |= kStZeroPivot;
+    }
+    for (int32_t p = v.l_ptr[j] + tid; p < v.l_ptr[j + 1]; p += 256) {
+      const int32_t r = v.l_row[p];
+      const T l = x[r] / piv;
+      Lx[p] = l;
+      const T al = tabs(l);
+      if (!(al <= lmax_all)) lmax_all = al;
+    }
+    for (int32_t up = v.u_ptr[j] + tid; up < v.u_ptr[j + 1]; up += 256) Ux[up] = x[v.prow[v.u_idx[up]]];
+    __syncthreads();
+    for (int32_t p = v.l_ptr[j] + tid; p < v.l_ptr[j + 1]; p += 256) x[v.l_row[p]] = T(0);
+    for (int32_t up = v.u_ptr[j] + tid; up < v.u_ptr[j + 1]; up += 256) x[v.prow[v.u_idx[up]]] = T(0);
+    if (tid == 0) x[v.prow[j]] = T(0);
+    __syncthreads();
+  }
+  if (!(lmax_all <= static_cast<T>(v.inv_tol))) st |= kStCheck;
+  if (tid == 0) st_sh = 0;
+  __syncthreads();
+  if (st) atomicOr(&st_sh, st);
+  __syncthreads();
+  if (tid == 0 && st_sh) atomicOr(status, st_sh);
+}
+
 template <typename T>
 void launch_sp_cols(hipStream_t st, const SpView& v, const double* arena, const double* rs, int32_t ncols,
-                    const int32_t* cols, T* xs, int32_t nslots, T* Lx, T* Ux, T* Ud, int32_t* status) {
+                    const int32_t* cols, T* xs, int32_t nslots, T* Lx, T* Ux, T* Ud, int32_t* status,
+                    const SpColsLong* lg) {
   if (ncols <= 0) return;
   const int g = ncols < nslots ? ncols : nslots;
-  hipLaunchKernelGGL(k_sp_cols<T>, dim3(g), dim3(64), 0, st, v, arena, rs, ncols, cols, xs, Lx, Ux, Ud, status);
+  hipLaunchKernelGGL(k_sp_cols<T>, dim3(g), dim3(64), 0, st, v, arena, rs, ncols, cols, xs, Lx, Ux, Ud, status,
+                     lg ? lg->is_long : nullptr);
+  if (lg && lg->nlong > 0) {
+    const int gl = lg->nlong < nslots ? lg->nlong : nslots;
+    hipLaunchKernelGGL(k_sp_cols_long<T>, dim3(gl), dim3(256), 0, st, v, arena, rs, xs, Lx, Ux, Ud, status,
+                       lg->nlong, lg->long_j, lg->reach, lg->lev, lg->nlev, lg->lr_ptr, lg->lr_col, lg->lr_vidx);
+  }
 }
 template void launch_sp_cols<double>(hipStream_t, const SpView&, const double*, const double*, int32_t,
-                                     const int32_t*, double*, int32_t, double*, double*, double*, int32_t*);
+                                     const int32_t*, double*, int32_t, double*, double*, double*, int32_t*,
+                                     const SpColsLong*);
 template void launch_sp_cols<float>(hipStream_t, const SpView&, const double*, const double*, int32_t,
-                                    const int32_t*, float*, int32_t, float*, float*, float*, int32_t*);
+                                    const int32_t*, float*, int32_t, float*, float*, float*, int32_t*,
+                                    const SpColsLong*);
 
 // Tail columns: U(0:k0, j) and the Schur column S(:, t) = A(tail, j) -
 // L(tail, 0:k0) U(0:k0, j). S arrives zero-filled, in the tail's fixed
