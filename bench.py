@@ -103,9 +103,9 @@ def traffic_from_counters(fetch_db: str, write_db: str) -> dict:
     out["kernel"] = ROOFLINE_KERNEL
     out["commands"] = [
         "rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmc_fetch -o run -- python3 bench.py --steps 2 --warmup 1 "
-        "--no-cpu-baseline --no-profile --copy-calibration",
+        "--no-cpu-baseline --copy-calibration",
         "rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pmc_write -o run -- python3 bench.py --steps 2 --warmup 1 "
-        "--no-cpu-baseline --no-profile --copy-calibration",
+        "--no-cpu-baseline --copy-calibration",
         "python bench.py --traffic gpurun_out/pmc_fetch/run_results.db gpurun_out/pmc_write/run_results.db"]
     return out
 
