@@ -248,9 +248,10 @@ template void launch_dense_lu<float>(hipStream_t, int32_t, const int32_t*, const
                                      const int64_t*, const double*, float*, int32_t*, int32_t*, const int32_t*,
                                      const int32_t*);
 
-// Explicit inverses of each dense block's factors, for the solve: M(i,j) =
-// L^{-1}(i,j) below the diagonal (unit diagonal implied) and U^{-1}(i,j) on and
-// above it, s x s column-major at the block's factor offset. One workgroup of
+// Explicit inverses of each dense block's factors, for the solve, as two full
+// s x s column-major matrices at the block's factor offset: L^{-1} (unit
+// diagonal stored, zeros above) in invl and U^{-1} (zeros below) in invu, so
+// the solve applies them without masks. One workgroup of
 // two wavefronts per block: wave 0 forms the columns of L^{-1}, wave 1 those of
 // U^{-1}, lane j column j, right-looking (63-k independent FMAs per step) over
 // a column-major LDS copy of the factor read at wave-uniform addresses.
@@ -259,14 +260,13 @@ template void launch_dense_lu<float>(hipStream_t, int32_t, const int32_t*, const
 template <typename T>
 __global__ __launch_bounds__(128) void k_dense_inv(int32_t nd, const int32_t* __restrict__ sizes,
                                                    const int64_t* __restrict__ lu0, const T* __restrict__ lu,
-                                                   T* __restrict__ inv) {
+                                                   T* __restrict__ invl, T* __restrict__ invu) {
   constexpr int LC = 66;
   __shared__ T fc[64 * LC];
   const int32_t t = static_cast<int32_t>(blockIdx.x);
   if (t >= nd) return;
   const int s = __builtin_amdgcn_readfirstlane(sizes[t]);
   const T* src = lu + lu0[t];
-  T* dst = inv + lu0[t];
   for (int idx = threadIdx.x; idx < s * s; idx += 128) {
     const int j = idx / s, i = idx - j * s;
     fc[j * LC + i] = src[idx];
@@ -291,9 +291,10 @@ __global__ __launch_bounds__(128) void k_dense_inv(int32_t nd, const int32_t* __
       }
     }
     if (j < s) {
+      T* dst = invl + lu0[t];
 #pragma unroll
-      for (int i = 1; i < 64; ++i)
-        if (i < s && i > j) dst[int64_t(j) * s + i] = x[i];
+      for (int i = 0; i < 64; ++i)
+        if (i < s) dst[int64_t(j) * s + i] = i > j ? x[i] : (i == j ? T(1) : T(0));
     }
   } else {
 #pragma unroll
@@ -305,20 +306,24 @@ __global__ __launch_bounds__(128) void k_dense_inv(int32_t nd, const int32_t* __
       }
     }
     if (j < s) {
+      T* dst = invu + lu0[t];
 #pragma unroll
       for (int i = 0; i < 64; ++i)
-        if (i < s && i <= j) dst[int64_t(j) * s + i] = x[i];
+        if (i < s) dst[int64_t(j) * s + i] = i <= j ? x[i] : T(0);
     }
   }
 }
 
 template <typename T>
-void launch_dense_inv(hipStream_t st, int32_t nd, const int32_t* sizes, const int64_t* lu0, const T* lu, T* inv) {
+void launch_dense_inv(hipStream_t st, int32_t nd, const int32_t* sizes, const int64_t* lu0, const T* lu, T* invl,
+                      T* invu) {
   if (nd <= 0) return;
-  hipLaunchKernelGGL(k_dense_inv<T>, dim3(nd), dim3(128), 0, st, nd, sizes, lu0, lu, inv);
+  hipLaunchKernelGGL(k_dense_inv<T>, dim3(nd), dim3(128), 0, st, nd, sizes, lu0, lu, invl, invu);
 }
-template void launch_dense_inv<double>(hipStream_t, int32_t, const int32_t*, const int64_t*, const double*, double*);
-template void launch_dense_inv<float>(hipStream_t, int32_t, const int32_t*, const int64_t*, const float*, float*);
+template void launch_dense_inv<double>(hipStream_t, int32_t, const int32_t*, const int64_t*, const double*, double*,
+                                       double*);
+template void launch_dense_inv<float>(hipStream_t, int32_t, const int32_t*, const int64_t*, const float*, float*,
+                                      float*);
 
 // ---------------------------------------------------------------------------
 // Sparse blocks: KLU scale=1 row sums, in a fixed order per row.
@@ -1142,21 +1147,56 @@ __device__ __forceinline__ void stage_factor(double* __restrict__ fs, const T* _
 // of the block DAG: solve level, then descending block id) and solves block b:
 // rhs = bperm - A_off * xperm (ubtf:2269-2286), then the pivot gather and the
 // unit-L and U solves (ubtf:2071-2097), applied as products with the block's
-// explicit inverse factors (lu64/lu32 here hold them). x values are published
+// explicit inverse factors (lu64/lu32 hold U^{-1}, linv64/linv32 L^{-1}, both
+// column-major s x s at the block's factor offset). x values are published
 // values (kXSentinel protocol): everything that does not need other blocks' x
 // (factor to LDS, row gather, the first off-diagonal entries, the next ticket)
 // is issued first, then the block polls exactly the x entries its rows use.
 // Deadlock-free: the lowest unfinished ticket is always the current block of
 // some workgroup (a claimed, unstarted ticket is higher than its holder's
 // current one), and every x entry it reads belongs to a lower ticket.
+// The two products of a block solve over the first N columns (N >= s; entries
+// past s are zero on one side of each product): y = L^{-1} v from the rows in
+// registers, then x = U^{-1} y from the LDS tile, the input vector broadcast
+// from LDS at wave-uniform addresses. No masks and no per-step branches.
+template <int N, int K>
+__device__ __forceinline__ void block_inverse_products(const double (&rowl)[64], const double* __restrict__ fs,
+                                                       int s, int li, double* __restrict__ vb, double (&v)[K],
+                                                       int lane) {
+#pragma unroll
+  for (int c = 0; c < K; ++c) vb[c * 64 + lane] = v[c];
+  __syncthreads();
+  double y[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) y[c] = 0.0;
+#pragma unroll
+  for (int j = 0; j < N; ++j)
+#pragma unroll
+    for (int c = 0; c < K; ++c) y[c] = __builtin_fma(rowl[j], vb[c * 64 + j], y[c]);
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < K; ++c) vb[c * 64 + lane] = y[c];
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < K; ++c) v[c] = 0.0;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    const double u = fs[(j < s ? j : s - 1) * s + li];  // past s: a finite entry times a zero
+#pragma unroll
+    for (int c = 0; c < K; ++c) v[c] = __builtin_fma(u, vb[c * 64 + j], v[c]);
+  }
+}
+
 template <int K>
 __global__ __launch_bounds__(64) void k_dense_seg_solve(
     int32_t ntasks, const int32_t* __restrict__ tasks, const int32_t* __restrict__ R,
     const int64_t* __restrict__ lu0_of_block, const double* __restrict__ lu64, const float* __restrict__ lu32,
+    const double* __restrict__ linv64, const float* __restrict__ linv32,
     const int32_t* __restrict__ prec_of_block, const int32_t* __restrict__ gsrc, const int32_t* __restrict__ off_ptr,
     const int32_t* __restrict__ off_col, const double* __restrict__ off_val, int32_t* __restrict__ ticket,
     const double* __restrict__ bperm, double* __restrict__ xperm, int64_t ldv, int32_t* __restrict__ err) {
   __shared__ double fs[64 * 64];
+  __shared__ double vb[K * 64];
   constexpr int kPre = 4;
   const int lane = threadIdx.x;
   int32_t tk = 0;
@@ -1185,17 +1225,27 @@ __global__ __launch_bounds__(64) void k_dense_seg_solve(
       oc[u] = ok ? off_col[p0 + u] : -1;
       ov[u] = ok ? off_val[p0 + u] : 0.0;
     }
-    if (__builtin_amdgcn_readfirstlane(prec_of_block[b]) != 0)
-      stage_factor(fs, lu32 + lu0_of_block[b], s * s, lane);
-    else
-      stage_factor(fs, lu64 + lu0_of_block[b], s * s, lane);
-    __syncthreads();
-    // Each lane's row of the block's inverse factors into registers (clamped,
-    // unconditional LDS reads).
-    double row[64];
+    // U^{-1} into LDS; each lane's row of L^{-1} straight into registers
+    // (clamped, unconditional loads; zero past s and on lanes past s).
     const int li = lane < s ? lane : s - 1;
+    const int64_t o = lu0_of_block[b];
+    double rowl[64];
+    if (__builtin_amdgcn_readfirstlane(prec_of_block[b]) != 0) {
+      stage_factor(fs, lu32 + o, s * s, lane);
 #pragma unroll
-    for (int j = 0; j < 64; ++j) row[j] = fs[(j < s ? j : s - 1) * s + li];
+      for (int j = 0; j < 64; ++j) {
+        const double l = static_cast<double>(linv32[o + (j < s ? j : s - 1) * s + li]);
+        rowl[j] = (j < s && act) ? l : 0.0;
+      }
+    } else {
+      stage_factor(fs, lu64 + o, s * s, lane);
+#pragma unroll
+      for (int j = 0; j < 64; ++j) {
+        const double l = linv64[o + (j < s ? j : s - 1) * s + li];
+        rowl[j] = (j < s && act) ? l : 0.0;
+      }
+    }
+    __syncthreads();
     // --- published x of other blocks: all first polls in flight together ---
     uint64_t xu[kPre][K];
 #pragma unroll
@@ -1222,27 +1272,9 @@ __global__ __launch_bounds__(64) void k_dense_seg_solve(
       acc[c] = __shfl(acc[c], src);
       if (!act) acc[c] = 0.0;
     }
-    double y[K];
-#pragma unroll
-    for (int c = 0; c < K; ++c) y[c] = acc[c];
-#pragma unroll
-    for (int j = 0; j < 64; ++j) {
-      if (j < s) {
-        const double m = lane > j ? row[j] : 0.0;
-#pragma unroll
-        for (int c = 0; c < K; ++c) y[c] = __builtin_fma(m, readlane_f64(acc[c], j), y[c]);
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < K; ++c) acc[c] = 0.0;
-#pragma unroll
-    for (int j = 0; j < 64; ++j) {
-      if (j < s) {
-        const double m = lane <= j ? row[j] : 0.0;
-#pragma unroll
-        for (int c = 0; c < K; ++c) acc[c] = __builtin_fma(m, readlane_f64(y[c], j), acc[c]);
-      }
-    }
+    if (s <= 16) block_inverse_products<16, K>(rowl, fs, s, li, vb, acc, lane);
+    else if (s <= 32) block_inverse_products<32, K>(rowl, fs, s, li, vb, acc, lane);
+    else block_inverse_products<64, K>(rowl, fs, s, li, vb, acc, lane);
     if (act) {
 #pragma unroll
       for (int c = 0; c < K; ++c) st_x_relaxed(xperm + c * ldv + r, acc[c]);
@@ -1255,14 +1287,15 @@ __global__ __launch_bounds__(64) void k_dense_seg_solve(
 
 void launch_dense_seg_solve(hipStream_t st, int32_t kc, int32_t ntasks, const int32_t* tasks, const int32_t* R,
                             const int64_t* lu0_of_block, const double* lu64, const float* lu32,
-                            const int32_t* prec_of_block, const int32_t* gsrc, const int32_t* off_ptr,
-                            const int32_t* off_col, const double* off_val, int32_t* ticket, const double* bperm,
+                            const double* linv64, const float* linv32, const int32_t* prec_of_block,
+                            const int32_t* gsrc, const int32_t* off_ptr, const int32_t* off_col,
+                            const double* off_val, int32_t* ticket, const double* bperm,
                             double* xperm, int64_t ldv, int32_t* err, int32_t grid) {
   if (ntasks <= 0) return;
   const int g = ntasks < grid ? ntasks : grid;  // persistent single-wave workgroups
 #define BDLU_SEG(KK)                                                                                              \
   hipLaunchKernelGGL(k_dense_seg_solve<KK>, dim3(g), dim3(64), 0, st, ntasks, tasks, R, lu0_of_block, lu64, lu32, \
-                     prec_of_block, gsrc, off_ptr, off_col, off_val, ticket, bperm, xperm, ldv, err)
+                     linv64, linv32, prec_of_block, gsrc, off_ptr, off_col, off_val, ticket, bperm, xperm, ldv, err)
   switch (kc) {
     case 1: BDLU_SEG(1); break;
     case 2: BDLU_SEG(2); break;
