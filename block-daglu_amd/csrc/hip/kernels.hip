@@ -1381,73 +1381,119 @@ template void launch_sp_rows<float>(hipStream_t, int32_t, int32_t, const int32_t
                                     const int32_t*, const float*, const float*, double*, int64_t);
 
 // Dense tail triangular solve (unit-lower forward or upper backward) on the
-// column-major m x m factor, one wavefront per 64-row band, bands claimed in
-// dependency order by ticket. A band accumulates the contributions of earlier
-// bands as their values are published (kXSentinel protocol on yout), with the
-// band's factor tile loaded before the poll, then solves its diagonal tile by
-// substitution and publishes its 64 values. yin and yout must not alias; yout
-// starts filled with kXSentinel.
+// column-major m x m factor, one workgroup of four wavefronts per 64-row band,
+// bands claimed in dependency order by ticket. The tiles of the band's earlier
+// bands (in publication order) are dealt round-robin to the four waves; each
+// wave keeps the next half tile's loads in flight while it takes the current
+// one, polls the 64 values of the tile's band as they are published
+// (kXSentinel protocol on yout) and broadcasts them through LDS. The four
+// partial sums are combined in a fixed order, then the first wave solves the
+// band's diagonal tile (staged in LDS before any poll) by substitution and
+// publishes its 64 values. yin and yout must not alias; yout starts filled
+// with kXSentinel.
 template <typename T, int K>
-__global__ __launch_bounds__(64) void k_tail_trsv(int32_t m, const T* __restrict__ S, int64_t lds,
-                                                  const double* __restrict__ yin, int64_t ldi,
-                                                  double* __restrict__ yout, int64_t ldo, int upper,
-                                                  int32_t* __restrict__ ticket, int32_t* __restrict__ err) {
-  const int lane = threadIdx.x;
+__global__ __launch_bounds__(256) void k_tail_trsv(int32_t m, const T* __restrict__ S, int64_t lds,
+                                                   const double* __restrict__ yin, int64_t ldi,
+                                                   double* __restrict__ yout, int64_t ldo, int upper,
+                                                   int32_t* __restrict__ ticket, int32_t* __restrict__ err) {
+  constexpr int kW = 4;        // wavefronts per band
+  constexpr int kH = kNb / 2;  // the diagonal tile is taken in halves of 32 columns
+  __shared__ double xs[kW][K][kNb];
+  __shared__ double part[kW][K][kNb];
+  __shared__ T dt[kNb * kNb];  // diagonal tile, column-major
+  __shared__ int32_t tk_sh;
+  const int tid = threadIdx.x;
+  const int lane = tid & (kNb - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  if (tid == 0) tk_sh = atomicAdd(ticket, 1);
+  __syncthreads();
+  const int32_t t = __builtin_amdgcn_readfirstlane(tk_sh);
   const int32_t np = (m + kNb - 1) / kNb;
-  int32_t t0 = 0;
-  if (lane == 0) t0 = atomicAdd(ticket, 1);
-  const int32_t t = uniform_from_lane0(t0);
-  if (t >= np) return;
+  if (t >= np) return;  // uniform over the workgroup
   const int32_t w = upper ? (np - 1 - t) : t;
   const int32_t r0 = w * kNb;
   const int32_t rows = (m - r0) < kNb ? (m - r0) : kNb;
   const int32_t row = r0 + lane;
   const bool live = lane < rows;
+  // Loads are unconditional with clamped indices, so they issue back to back
+  // (a per-element condition becomes a branch that waits on each load). Lanes
+  // past the band never store; columns past the matrix meet x = 0.
+  const int32_t rowc = live ? row : r0;
+  for (int idx = tid; idx < kNb * kNb; idx += kW * kNb) {
+    const int c = idx >> 6, i = idx & (kNb - 1);
+    const T v = S[int64_t(r0 + (c < rows ? c : rows - 1)) * lds + r0 + (i < rows ? i : rows - 1)];
+    dt[idx] = (c < rows && i < rows) ? v : T(0);
+  }
+  const int32_t nd = upper ? (np - 1 - w) : w;  // earlier bands
+  bool late = false;
   double acc[K];
 #pragma unroll
-  for (int c = 0; c < K; ++c) acc[c] = live ? yin[c * ldi + row] : 0.0;
-  bool late = false;
-  // Tile loads are unconditional with clamped indices, so all 64 issue back to
-  // back (a per-element condition becomes a branch that waits on each load).
-  // Lanes past the band never store; columns past the matrix meet xv = 0. The
-  // tile is shared by the K right-hand sides.
-  const int32_t rowc = live ? row : r0;
-  // Backward: each lane's reciprocal of its own diagonal entry, off the chain.
-  const double rinv = upper ? 1.0 / static_cast<double>(S[int64_t(rowc) * lds + rowc]) : 1.0;
-  constexpr int kH = kNb / 2;  // the tile is consumed in two halves of 32 columns
-  T v[kH];
-  for (int32_t q = 0; q < np - 1; ++q) {
-    int32_t j;
-    if (upper) {
-      j = np - 1 - q;
-      if (j <= w) break;
-    } else {
-      j = q;
-      if (j >= w) break;
+  for (int c = 0; c < K; ++c) acc[c] = 0.0;
+  // Tiles are loaded in pieces of kP columns (fewer with more right-hand
+  // sides, for registers), two pieces in flight.
+  constexpr int kP = K >= 4 ? 16 : 32;
+  constexpr int kPieces = kNb / kP;
+  T buf[2][kP];
+  auto load_piece = [&](T(&v)[kP], int32_t q, int h) {
+    const int32_t j = upper ? (np - 1 - q) : q;
+    const int32_t cols = (m - j * kNb) < kNb ? (m - j * kNb) : kNb;
+#pragma unroll
+    for (int c = 0; c < kP; ++c) {
+      const int cc = h * kP + c;
+      v[c] = S[int64_t(j * kNb + (cc < cols ? cc : cols - 1)) * lds + rowc];
     }
-    const int32_t c0 = j * kNb;
-    const int32_t cols = (m - c0) < kNb ? (m - c0) : kNb;
+  };
+  int32_t q = wv;
+  if (q < nd) load_piece(buf[0], q, 0);
+  while (q < nd) {
+    const int32_t j = upper ? (np - 1 - q) : q;
+    const int32_t cols = (m - j * kNb) < kNb ? (m - j * kNb) : kNb;
+    const int32_t qn = q + kW;
+    load_piece(buf[1], q, 1);
 #pragma unroll
-    for (int c = 0; c < kH; ++c) v[c] = S[int64_t(c0 + (c < cols ? c : cols - 1)) * lds + rowc];
-    double xv[K];
+    for (int k = 0; k < K; ++k) xs[wv][k][lane] = lane < cols ? wait_x(yout + k * ldo + j * kNb + lane, late) : 0.0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-    for (int k = 0; k < K; ++k) xv[k] = lane < cols ? wait_x(yout + k * ldo + c0 + lane, late) : 0.0;
+    for (int h = 0; h < kPieces; ++h) {
 #pragma unroll
-    for (int c = 0; c < kH; ++c)
+      for (int c = 0; c < kP; ++c)
 #pragma unroll
-      for (int k = 0; k < K; ++k) acc[k] = acc[k] - static_cast<double>(v[c]) * readlane_f64(xv[k], c);
-#pragma unroll
-    for (int c = 0; c < kH; ++c) v[c] = S[int64_t(c0 + (kH + c < cols ? kH + c : cols - 1)) * lds + rowc];
-#pragma unroll
-    for (int c = 0; c < kH; ++c)
-#pragma unroll
-      for (int k = 0; k < K; ++k) acc[k] = acc[k] - static_cast<double>(v[c]) * readlane_f64(xv[k], kH + c);
+        for (int k = 0; k < K; ++k)
+          acc[k] = __builtin_fma(static_cast<double>(buf[h & 1][c]), xs[wv][k][h * kP + c], acc[k]);
+      // The buffer just consumed takes the piece after the one in flight
+      // (unconditionally: after the last tile it reloads this one, which keeps
+      // the loop free of branches around loads and their register copies).
+      if (h + 2 < kPieces)
+        load_piece(buf[h & 1], q, h + 2);
+      else
+        load_piece(buf[h & 1], qn < nd ? qn : q, h + 2 - kPieces);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    q = qn;
   }
+#pragma unroll
+  for (int k = 0; k < K; ++k) part[wv][k][lane] = acc[k];
+  __syncthreads();
+  if (wv != 0) {
+    if (late) atomicOr(err, 2);
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double sum = (part[0][k][lane] + part[1][k][lane]) + (part[2][k][lane] + part[3][k][lane]);
+    acc[k] = (live ? yin[k * ldi + row] : 0.0) - sum;
+  }
+  // Diagonal tile by substitution; row `lane` of the tile is dt[c * 64 + lane].
+  T v[kH];
   if (!upper) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
 #pragma unroll
-      for (int c = 0; c < kH; ++c) v[c] = S[int64_t(r0 + (h * kH + c < rows ? h * kH + c : rows - 1)) * lds + rowc];
+      for (int c = 0; c < kH; ++c) v[c] = dt[(h * kH + c) * kNb + lane];
 #pragma unroll
       for (int c = 0; c < kH; ++c)
 #pragma unroll
@@ -1457,10 +1503,12 @@ __global__ __launch_bounds__(64) void k_tail_trsv(int32_t m, const T* __restrict
         }
     }
   } else {
+    // Each lane's reciprocal of its own diagonal entry (1 on the pad).
+    const double rinv = 1.0 / static_cast<double>(live ? dt[lane * kNb + lane] : T(1));
 #pragma unroll
     for (int h = 1; h >= 0; --h) {
 #pragma unroll
-      for (int c = 0; c < kH; ++c) v[c] = S[int64_t(r0 + (h * kH + c < rows ? h * kH + c : rows - 1)) * lds + rowc];
+      for (int c = 0; c < kH; ++c) v[c] = dt[(h * kH + c) * kNb + lane];
 #pragma unroll
       for (int c = kH - 1; c >= 0; --c) {
         if (h * kH + c < rows) {
@@ -1484,7 +1532,7 @@ template <typename T>
 void launch_tail_trsv(hipStream_t st, int32_t kc, int32_t m, const T* S, int64_t lds, const double* yin, int64_t ldi,
                       double* yout, int64_t ldo, bool upper, int32_t* ticket, int32_t* err) {
   if (m <= 0) return;
-  const dim3 g((m + kNb - 1) / kNb), b(64);
+  const dim3 g((m + kNb - 1) / kNb), b(256);
   const int up = upper ? 1 : 0;
   switch (kc) {
     case 1: hipLaunchKernelGGL((k_tail_trsv<T, 1>), g, b, 0, st, m, S, lds, yin, ldi, yout, ldo, up, ticket, err); break;
