@@ -82,7 +82,7 @@ struct SparseDev {
   float *Lx32 = nullptr, *Ux32 = nullptr, *Ud32 = nullptr, *S32 = nullptr;
   double* rs = nullptr;
   double* y = nullptr;
-  double* inv64 = nullptr;  // 2*64*64 diagonal-block inverses (tail LU scratch)
+  double* inv64 = nullptr;  // 2*64*64 per panel: the tail's diagonal-block inverses
   float* inv32 = nullptr;
   double* ty = nullptr;  // m: tail forward-solve output (published values)
   bdk::SpTailLong tail_long;  // tail columns with a long sparse reach
@@ -551,8 +551,9 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
     S.S32 = A.get<float>(static_cast<std::size_t>(sd.m) * sd.m);
     S.rs = A.get<double>(sd.s);
     S.y = A.get<double>(static_cast<std::size_t>(sd.s) * kMaxRhs);
-    S.inv64 = A.get<double>(2 * 64 * 64);
-    S.inv32 = A.get<float>(2 * 64 * 64);
+    const std::size_t npan = (static_cast<std::size_t>(sd.m) + 63) / 64;
+    S.inv64 = A.get<double>(2 * 64 * 64 * std::max<std::size_t>(npan, 1));
+    S.inv32 = A.get<float>(2 * 64 * 64 * std::max<std::size_t>(npan, 1));
     S.ty = A.get<double>(static_cast<std::size_t>(std::max(sd.m, 1)) * kMaxRhs);
     S.status = A.get<int32_t>(1);
     P.sp.push_back(std::move(S));
@@ -685,7 +686,7 @@ void block_solve_launches(Plan& P, hipStream_t st, const double* b, double* x, i
     const int64_t ldy = S.s, ldt = std::max(S.m, 1);
     bdk::launch_sp_gather(st, S.s, S.start, kt, S.rowperm, S.scale ? S.rs : nullptr, P.off_ptr, P.off_col, P.off_val,
                           P.bperm, P.xperm, ld, S.y, ldy);
-    auto run = [&](auto* Lx, auto* Ux, auto* Ud, auto* Sm) {
+    auto run = [&](auto* Lx, auto* Ux, auto* Ud, auto* Sm, auto* inv) {
       using T = std::remove_pointer_t<decltype(Lx)>;
       for (std::size_t l = 0; l + 1 < S.fl_ptr.size(); ++l)
         bdk::launch_sp_rows<T>(st, kt, S.fl_ptr[l + 1] - S.fl_ptr[l], S.fl_rows + S.fl_ptr[l], S.lr_ptr, S.lr_col,
@@ -693,15 +694,17 @@ void block_solve_launches(Plan& P, hipStream_t st, const double* b, double* x, i
       bdk::launch_sp_rows<T>(st, kt, S.m, S.tl_rows, S.lr_ptr, S.lr_col, S.lr_vidx, Lx, nullptr, S.y, ldy);
       // Tail: forward into ty, backward back into y (each output starts unpublished).
       for (int32_t c = 0; c < kt; ++c) bdk::launch_fill_sentinel(st, S.ty + c * ldt, S.m);
-      bdk::launch_tail_trsv<T>(st, kt, S.m, Sm, S.m, S.y + S.k0, ldy, S.ty, ldt, false, P.tickets + tix, P.err);
+      bdk::launch_tail_trsv<T>(st, kt, S.m, Sm, S.m, inv, S.y + S.k0, ldy, S.ty, ldt, false, P.tickets + tix,
+                               P.err);
       for (int32_t c = 0; c < kt; ++c) bdk::launch_fill_sentinel(st, S.y + c * ldy + S.k0, S.m);
-      bdk::launch_tail_trsv<T>(st, kt, S.m, Sm, S.m, S.ty, ldt, S.y + S.k0, ldy, true, P.tickets + tix + 1, P.err);
+      bdk::launch_tail_trsv<T>(st, kt, S.m, Sm, S.m, inv, S.ty, ldt, S.y + S.k0, ldy, true, P.tickets + tix + 1,
+                               P.err);
       for (std::size_t l = 0; l + 1 < S.bl_ptr.size(); ++l)
         bdk::launch_sp_rows<T>(st, kt, S.bl_ptr[l + 1] - S.bl_ptr[l], S.bl_rows + S.bl_ptr[l], S.ur_ptr, S.ur_col,
                                S.ur_vidx, Ux, Ud, S.y, ldy);
     };
-    if (S.prec == 0) run(S.Lx64, S.Ux64, S.Ud64, S.S64);
-    else run(S.Lx32, S.Ux32, S.Ud32, S.S32);
+    if (S.prec == 0) run(S.Lx64, S.Ux64, S.Ud64, S.S64, S.inv64);
+    else run(S.Lx32, S.Ux32, S.Ud32, S.S32, S.inv32);
     tix += 2;
     bdk::launch_sp_scatter_x(st, S.s, S.start, kt, S.q, S.y, ldy, P.xperm, ld);
   }
@@ -1229,14 +1232,14 @@ int bdlu_gpu_debug_dense_lu(bdlu_ctx* c, int32_t m, double* A, int precision) {
       std::vector<float> h(mm);
       for (std::size_t i = 0; i < mm; ++i) h[i] = static_cast<float>(A[i]);
       float* d = tmp.copy(h, st);
-      float* inv = tmp.get<float>(2 * 64 * 64);
+      float* inv = tmp.get<float>(2 * 64 * 64 * ((static_cast<std::size_t>(m) + 63) / 64));
       bdk::launch_tail_lu<float>(st, m, d, m, 1e300, status, inv);
       HIPCHECK(hipMemcpyAsync(h.data(), d, sizeof(float) * mm, hipMemcpyDeviceToHost, st));
       HIPCHECK(hipStreamSynchronize(st));
       for (std::size_t i = 0; i < mm; ++i) A[i] = h[i];
     } else {
       double* d = tmp.copy(A, mm, st);
-      double* inv = tmp.get<double>(2 * 64 * 64);
+      double* inv = tmp.get<double>(2 * 64 * 64 * ((static_cast<std::size_t>(m) + 63) / 64));
       bdk::launch_tail_lu<double>(st, m, d, m, 1e300, status, inv);
       HIPCHECK(hipMemcpyAsync(A, d, sizeof(double) * mm, hipMemcpyDeviceToHost, st));
       HIPCHECK(hipStreamSynchronize(st));

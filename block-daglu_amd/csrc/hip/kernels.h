@@ -88,7 +88,8 @@ struct TailTimer {
 
 // No-pivot blocked LU of the m x m column-major S (the tail after its rows were
 // placed in the fixed pivot order). Check: |L| <= limit and nonzero pivots.
-// inv: 2*64*64 scratch for the diagonal blocks' inverses.
+// inv: 2*64*64 per 64-column panel (ceil(m/64) panels): the inverses of the
+// diagonal blocks, L^{-1} then U^{-1}, row-major, kept for the tail solve.
 // lookahead: each step's next diagonal LU runs as one workgroup of the bulk
 // trailing-update launch (ignored while timing); results are identical either way.
 template <typename T>
@@ -123,11 +124,12 @@ void launch_sp_gather(hipStream_t st, int32_t s, int32_t start, int32_t kc, cons
 template <typename T>
 void launch_sp_rows(hipStream_t st, int32_t kc, int32_t nrows, const int32_t* rows, const int32_t* ptr,
                     const int32_t* col, const int32_t* vidx, const T* vals, const T* diag, double* y, int64_t ldy);
-// Dense tail triangular solve over 64-row bands. ticket starts at 0; yout
-// starts as the unpublished pattern and does not alias yin.
+// Dense tail triangular solve over 64-row bands; inv holds the diagonal tiles'
+// inverses as launch_tail_lu leaves them. ticket starts at 0; yout starts as the
+// unpublished pattern and does not alias yin.
 template <typename T>
-void launch_tail_trsv(hipStream_t st, int32_t kc, int32_t m, const T* S, int64_t lds, const double* yin, int64_t ldi,
-                      double* yout, int64_t ldo, bool upper, int32_t* ticket, int32_t* err);
+void launch_tail_trsv(hipStream_t st, int32_t kc, int32_t m, const T* S, int64_t lds, const T* inv, const double* yin,
+                      int64_t ldi, double* yout, int64_t ldo, bool upper, int32_t* ticket, int32_t* err);
 void launch_sp_scatter_x(hipStream_t st, int32_t s, int32_t start, int32_t kc, const int32_t* q, const double* x,
                          int64_t ldx, double* xperm, int64_t ldv);
 

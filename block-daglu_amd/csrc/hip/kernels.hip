@@ -643,7 +643,8 @@ template void launch_sp_tail<float>(hipStream_t, const SpView&, const double*, c
 constexpr int kNb = 64;
 
 // inv[0..4095]: L11^{-1} row-major (unit lower); inv[4096..8191]: U11^{-1}
-// row-major (upper). Rows/columns at or past b are an identity pad.
+// row-major (upper). Rows/columns at or past b are an identity pad. Each panel
+// keeps its pair (the tail solve applies them to its diagonal tiles).
 //
 // Two wavefronts. The LU is one dependent chain of 64 steps, so it runs on a
 // single wave with no barriers: lane i keeps row i in registers, and at step k
@@ -995,6 +996,7 @@ hipEvent_t timer_event(std::vector<hipEvent_t>* pool, hipStream_t st) {
 template <typename T, int kTile>
 void tail_step(hipStream_t st, T* S, int64_t lds, int32_t p, int32_t b, int32_t rest, double limit,
                int32_t* status, T* inv, TailTimer* timer) {
+  // inv: this panel's pair of diagonal-block inverses.
   const int32_t tr = (rest + kTile - 1) / kTile;
   TileJob<T> l21{rest, b, b, 2, tr, 1, S + int64_t(p) * lds + p + b, 1, lds, inv + kNb * kNb, kNb, 1,
                  S + int64_t(p) * lds + p + b, 1, lds};
@@ -1024,6 +1026,7 @@ void tail_step(hipStream_t st, T* S, int64_t lds, int32_t p, int32_t b, int32_t 
 template <typename T, int kTile>
 void tail_step_lookahead(hipStream_t st, T* S, int64_t lds, int32_t p, int32_t b, int32_t rest, double limit,
                          int32_t* status, T* inv) {
+  // inv: this panel's pair of diagonal-block inverses; the next panel's follows.
   const int32_t tr = (rest + kTile - 1) / kTile;
   TileJob<T> l21{rest, b, b, 2, tr, 1, S + int64_t(p) * lds + p + b, 1, lds, inv + kNb * kNb, kNb, 1,
                  S + int64_t(p) * lds + p + b, 1, lds};
@@ -1046,7 +1049,7 @@ void tail_step_lookahead(hipStream_t st, T* S, int64_t lds, int32_t p, int32_t b
   if (n0 <= 0) right.tiles_m = right.tiles_n = below.tiles_m = below.tiles_n = 0;
   const int32_t g = 1 + right.tiles_m * right.tiles_n + below.tiles_m * below.tiles_n;
   hipLaunchKernelGGL((k_tail_update_diag<T, kTile>), dim3(g), dim3(256), 0, st, right, below, limit, status, S, lds,
-                     q, nb, inv);
+                     q, nb, inv + 2 * kNb * kNb);
 }
 
 template <typename T>
@@ -1056,14 +1059,15 @@ void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, 
     if (m <= 0) return;
     hipLaunchKernelGGL(k_diag_lu_inv<T>, dim3(1), dim3(kDiagThreads), 0, st, S, lds, 0, m < kNb ? m : kNb, limit,
                        status, inv);
+    const int64_t kInvPair = 2 * kNb * kNb;
     for (int32_t p = 0; p < m; p += kNb) {
       const int32_t b = (m - p) < kNb ? (m - p) : kNb;
       const int32_t rest = m - p - b;
       if (rest <= 0) break;
       if (rest < small_tile_below)
-        tail_step_lookahead<T, 64>(st, S, lds, p, b, rest, limit, status, inv);
+        tail_step_lookahead<T, 64>(st, S, lds, p, b, rest, limit, status, inv + p / kNb * kInvPair);
       else
-        tail_step_lookahead<T, 128>(st, S, lds, p, b, rest, limit, status, inv);
+        tail_step_lookahead<T, 128>(st, S, lds, p, b, rest, limit, status, inv + p / kNb * kInvPair);
     }
     return;
   }
@@ -1071,15 +1075,16 @@ void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, 
     const int32_t b = (m - p) < kNb ? (m - p) : kNb;
     const int32_t rest = m - p - b;
     if (timer && timer->panel_events) timer_event(timer->panel_events, st);
-    hipLaunchKernelGGL(k_diag_lu_inv<T>, dim3(1), dim3(kDiagThreads), 0, st, S, lds, p, b, limit, status, inv);
+    T* pinv = inv + int64_t(p / kNb) * 2 * kNb * kNb;
+    hipLaunchKernelGGL(k_diag_lu_inv<T>, dim3(1), dim3(kDiagThreads), 0, st, S, lds, p, b, limit, status, pinv);
     if (rest <= 0) {
       if (timer && timer->panel_events) timer_event(timer->panel_events, st);
       break;
     }
     if (rest < small_tile_below)
-      tail_step<T, 64>(st, S, lds, p, b, rest, limit, status, inv, timer);
+      tail_step<T, 64>(st, S, lds, p, b, rest, limit, status, pinv, timer);
     else
-      tail_step<T, 128>(st, S, lds, p, b, rest, limit, status, inv, timer);
+      tail_step<T, 128>(st, S, lds, p, b, rest, limit, status, pinv, timer);
   }
 }
 template void launch_tail_lu<double>(hipStream_t, int32_t, double*, int64_t, double, int32_t*, double*, TailTimer*,
@@ -1381,26 +1386,29 @@ template void launch_sp_rows<float>(hipStream_t, int32_t, int32_t, const int32_t
                                     const int32_t*, const float*, const float*, double*, int64_t);
 
 // Dense tail triangular solve (unit-lower forward or upper backward) on the
-// column-major m x m factor, one workgroup of four wavefronts per 64-row band,
+// column-major m x m factor and the inverses of its 64 x 64 diagonal tiles (inv:
+// per band, L^{-1} then U^{-1}, each row-major with an identity pad), one
+// workgroup of four wavefronts per 64-row band,
 // bands claimed in dependency order by ticket. The tiles of the band's earlier
 // bands (in publication order) are dealt round-robin to the four waves; each
 // wave keeps the next half tile's loads in flight while it takes the current
 // one, polls the 64 values of the tile's band as they are published
 // (kXSentinel protocol on yout) and broadcasts them through LDS. The four
-// partial sums are combined in a fixed order, then the first wave solves the
-// band's diagonal tile (staged in LDS before any poll) by substitution and
-// publishes its 64 values. yin and yout must not alias; yout starts filled
+// partial sums are combined in a fixed order, then the first wave applies the
+// inverse of the band's diagonal tile (staged in LDS before any poll) as a
+// matrix-vector product and publishes its 64 values. yin and yout must not alias; yout starts filled
 // with kXSentinel.
 template <typename T, int K>
 __global__ __launch_bounds__(256) void k_tail_trsv(int32_t m, const T* __restrict__ S, int64_t lds,
-                                                   const double* __restrict__ yin, int64_t ldi,
+                                                   const T* __restrict__ inv, const double* __restrict__ yin,
+                                                   int64_t ldi,
                                                    double* __restrict__ yout, int64_t ldo, int upper,
                                                    int32_t* __restrict__ ticket, int32_t* __restrict__ err) {
-  constexpr int kW = 4;        // wavefronts per band
-  constexpr int kH = kNb / 2;  // the diagonal tile is taken in halves of 32 columns
+  constexpr int kW = 4;         // wavefronts per band
+  constexpr int kDL = kNb + 1;  // LDS row length of the diagonal inverse (padded against bank conflicts)
   __shared__ double xs[kW][K][kNb];
   __shared__ double part[kW][K][kNb];
-  __shared__ T dt[kNb * kNb];  // diagonal tile, column-major
+  __shared__ T dt[kNb * kDL];   // inverse of the diagonal tile, row-major
   __shared__ int32_t tk_sh;
   const int tid = threadIdx.x;
   const int lane = tid & (kNb - 1);
@@ -1419,11 +1427,8 @@ __global__ __launch_bounds__(256) void k_tail_trsv(int32_t m, const T* __restric
   // (a per-element condition becomes a branch that waits on each load). Lanes
   // past the band never store; columns past the matrix meet x = 0.
   const int32_t rowc = live ? row : r0;
-  for (int idx = tid; idx < kNb * kNb; idx += kW * kNb) {
-    const int c = idx >> 6, i = idx & (kNb - 1);
-    const T v = S[int64_t(r0 + (c < rows ? c : rows - 1)) * lds + r0 + (i < rows ? i : rows - 1)];
-    dt[idx] = (c < rows && i < rows) ? v : T(0);
-  }
+  const T* tinv = inv + (int64_t(w) * 2 + (upper ? 1 : 0)) * kNb * kNb;
+  for (int idx = tid; idx < kNb * kNb; idx += kW * kNb) dt[(idx >> 6) * kDL + (idx & (kNb - 1))] = tinv[idx];
   const int32_t nd = upper ? (np - 1 - w) : w;  // earlier bands
   bool late = false;
   double acc[K];
@@ -1487,40 +1492,21 @@ __global__ __launch_bounds__(256) void k_tail_trsv(int32_t m, const T* __restric
     const double sum = (part[0][k][lane] + part[1][k][lane]) + (part[2][k][lane] + part[3][k][lane]);
     acc[k] = (live ? yin[k * ldi + row] : 0.0) - sum;
   }
-  // Diagonal tile by substitution; row `lane` of the tile is dt[c * 64 + lane].
-  T v[kH];
-  if (!upper) {
+  // x = inverse(diagonal tile) * (yin - sum): the right-hand side broadcast
+  // through LDS, row `lane` of the inverse at dt[lane * kDL + c]. Rows past the
+  // band take a zero right-hand side (the inverse's pad is the identity).
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+  for (int k = 0; k < K; ++k) xs[0][k][lane] = live ? acc[k] : 0.0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-      for (int c = 0; c < kH; ++c) v[c] = dt[(h * kH + c) * kNb + lane];
+  for (int k = 0; k < K; ++k) acc[k] = 0.0;
 #pragma unroll
-      for (int c = 0; c < kH; ++c)
+  for (int c = 0; c < kNb; ++c) {
+    const double a = static_cast<double>(dt[lane * kDL + c]);
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-          const double xc = readlane_f64(acc[k], h * kH + c);
-          if (lane > h * kH + c) acc[k] = acc[k] - static_cast<double>(v[c]) * xc;
-        }
-    }
-  } else {
-    // Each lane's reciprocal of its own diagonal entry (1 on the pad).
-    const double rinv = 1.0 / static_cast<double>(live ? dt[lane * kNb + lane] : T(1));
-#pragma unroll
-    for (int h = 1; h >= 0; --h) {
-#pragma unroll
-      for (int c = 0; c < kH; ++c) v[c] = dt[(h * kH + c) * kNb + lane];
-#pragma unroll
-      for (int c = kH - 1; c >= 0; --c) {
-        if (h * kH + c < rows) {
-#pragma unroll
-          for (int k = 0; k < K; ++k) {
-            if (lane == h * kH + c) acc[k] = acc[k] * rinv;
-            const double xc = readlane_f64(acc[k], h * kH + c);
-            if (lane < h * kH + c) acc[k] = acc[k] - static_cast<double>(v[c]) * xc;
-          }
-        }
-      }
-    }
+    for (int k = 0; k < K; ++k) acc[k] = __builtin_fma(a, xs[0][k][c], acc[k]);
   }
   if (live) {
 #pragma unroll
@@ -1529,22 +1515,25 @@ __global__ __launch_bounds__(256) void k_tail_trsv(int32_t m, const T* __restric
   if (late) atomicOr(err, 2);
 }
 template <typename T>
-void launch_tail_trsv(hipStream_t st, int32_t kc, int32_t m, const T* S, int64_t lds, const double* yin, int64_t ldi,
-                      double* yout, int64_t ldo, bool upper, int32_t* ticket, int32_t* err) {
+void launch_tail_trsv(hipStream_t st, int32_t kc, int32_t m, const T* S, int64_t lds, const T* inv, const double* yin,
+                      int64_t ldi, double* yout, int64_t ldo, bool upper, int32_t* ticket, int32_t* err) {
   if (m <= 0) return;
   const dim3 g((m + kNb - 1) / kNb), b(256);
   const int up = upper ? 1 : 0;
+#define BDLU_TRSV(KK)                                                                                        \
+  hipLaunchKernelGGL((k_tail_trsv<T, KK>), g, b, 0, st, m, S, lds, inv, yin, ldi, yout, ldo, up, ticket, err)
   switch (kc) {
-    case 1: hipLaunchKernelGGL((k_tail_trsv<T, 1>), g, b, 0, st, m, S, lds, yin, ldi, yout, ldo, up, ticket, err); break;
-    case 2: hipLaunchKernelGGL((k_tail_trsv<T, 2>), g, b, 0, st, m, S, lds, yin, ldi, yout, ldo, up, ticket, err); break;
-    case 4: hipLaunchKernelGGL((k_tail_trsv<T, 4>), g, b, 0, st, m, S, lds, yin, ldi, yout, ldo, up, ticket, err); break;
-    default: hipLaunchKernelGGL((k_tail_trsv<T, 8>), g, b, 0, st, m, S, lds, yin, ldi, yout, ldo, up, ticket, err); break;
+    case 1: BDLU_TRSV(1); break;
+    case 2: BDLU_TRSV(2); break;
+    case 4: BDLU_TRSV(4); break;
+    default: BDLU_TRSV(8); break;
   }
+#undef BDLU_TRSV
 }
-template void launch_tail_trsv<double>(hipStream_t, int32_t, int32_t, const double*, int64_t, const double*, int64_t,
-                                       double*, int64_t, bool, int32_t*, int32_t*);
-template void launch_tail_trsv<float>(hipStream_t, int32_t, int32_t, const float*, int64_t, const double*, int64_t,
-                                      double*, int64_t, bool, int32_t*, int32_t*);
+template void launch_tail_trsv<double>(hipStream_t, int32_t, int32_t, const double*, int64_t, const double*,
+                                       const double*, int64_t, double*, int64_t, bool, int32_t*, int32_t*);
+template void launch_tail_trsv<float>(hipStream_t, int32_t, int32_t, const float*, int64_t, const float*,
+                                      const double*, int64_t, double*, int64_t, bool, int32_t*, int32_t*);
 
 __global__ void k_sp_scatter_x(int32_t s, int32_t start, int32_t kc, const int32_t* __restrict__ q,
                                const double* __restrict__ x, int64_t ldx, double* __restrict__ xperm, int64_t ldv) {
