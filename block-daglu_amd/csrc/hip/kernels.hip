@@ -809,22 +809,21 @@ struct TileJob {
   int64_t c_r, c_c;
 };
 
-// 128x128 output tile per 256-thread workgroup (each wavefront a 64x64
-// quarter, 4x4 MFMA 16x16 blocks), K <= 64 staged whole in LDS (zero padded),
-// so C may alias A (a tile covers all K columns of its rows) or B (all K rows
-// of its columns). Each MFMA forms a block of C^T (B^T as the row operand,
-// A^T as the column operand): the output's fast lane index then runs down a
-// column of C, so every C access is a 16-element contiguous run. For op 0 the
-// accumulators start as C and B is negated, so the MFMA chain yields C - AB.
-// All global loads are unconditional with clamped indices (a conditional load
-// turns into a branch that waits for it). Workgroups below
+// Panel products (ops 1 and 2). 128x128 output tile per 256-thread workgroup
+// (each wavefront a 64x64 quarter, 4x4 MFMA 16x16 blocks), K <= 64 staged
+// whole in LDS (zero padded), so C may alias A (a tile covers all K columns of
+// its rows) or B (all K rows of its columns). Each MFMA forms a block of C^T
+// (B^T as the row operand, A^T as the column operand): the output's fast lane
+// index then runs down a column of C, so every C access is a 16-element
+// contiguous run. All global loads are unconditional with clamped indices (a
+// conditional load turns into a branch that waits for it). Workgroups below
 // j0.tiles_m*j0.tiles_n run job 0, the rest job 1.
 // kTile: output tile edge (128, or 64 for small trailing matrices, where 128
 // leaves most CUs idle); each wavefront computes a kTile/2 square.
 template <int kTile, typename T>
 constexpr std::size_t tile_lds_elems() { return 2 * kNb * (kTile + 1); }
 
-template <typename T, bool kUpdate, int kTile>
+template <typename T, int kTile>
 __device__ __forceinline__ void tile_gemm(const TileJob<T>& j0, const TileJob<T>& j1, double limit,
                                           int32_t* __restrict__ status, T* lds, int32_t bid) {
   using V = typename MfmaOps<T>::v4;
@@ -846,26 +845,12 @@ __device__ __forceinline__ void tile_gemm(const TileJob<T>& j0, const TileJob<T>
   const int li = lane & 15, lk = lane >> 4;
   const int32_t M = J.M, N = J.N, K = J.K;
   V acc[kMB][kMB];  // acc[c][a]: C^T block (j-block c, i-block a)
-  if (kUpdate) {
 #pragma unroll
-    for (int c = 0; c < kMB; ++c)
+  for (int c = 0; c < kMB; ++c)
 #pragma unroll
-      for (int a = 0; a < kMB; ++a)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int32_t i = min(i0 + wr + a * 16 + li, M - 1);
-          const int32_t j = min(c0 + wc + c * 16 + MfmaOps<T>::row(lk, r), N - 1);
-          acc[c][a][r] = J.C[i + int64_t(j) * J.c_c];
-        }
-  } else {
-#pragma unroll
-    for (int c = 0; c < kMB; ++c)
-#pragma unroll
-      for (int a = 0; a < kMB; ++a) acc[c][a] = V{T(0), T(0), T(0), T(0)};
-  }
+    for (int a = 0; a < kMB; ++a) acc[c][a] = V{T(0), T(0), T(0), T(0)};
   // A tile: 128 x 64, B tile: 64 x 128, 32 elements each per thread, loaded in
   // two rounds of 16 with the unit-stride index running across threads.
-  const T sgn = kUpdate ? T(-1) : T(1);
 #pragma unroll
   for (int half = 0; half < kRounds; ++half) {
     T v[16];
@@ -905,7 +890,7 @@ __device__ __forceinline__ void tile_gemm(const TileJob<T>& j0, const TileJob<T>
     }
 #pragma unroll
     for (int u = 0; u < 16; ++u)
-      Bs[kk[u] * kGLd + jj[u]] = (c0 + jj[u] < N && kk[u] < K) ? sgn * v[u] : T(0);
+      Bs[kk[u] * kGLd + jj[u]] = (c0 + jj[u] < N && kk[u] < K) ? v[u] : T(0);
   }
   __syncthreads();
   for (int k = 0; k < kNb; k += 4) {
@@ -933,19 +918,123 @@ __device__ __forceinline__ void tile_gemm(const TileJob<T>& j0, const TileJob<T>
         const T av = (i < M && j < N) ? tabs(v) : T(0);
         lmax = (av <= lmax || lmax != lmax) ? lmax : av;
       }
-  if (!kUpdate && J.op == 2) {
+  if (J.op == 2) {
     int32_t st = !(lmax <= static_cast<T>(limit)) ? kStCheck : 0;
     for (int off = 32; off > 0; off >>= 1) st |= __shfl_xor(st, off);
     if (lane == 0 && st) atomicOr(status, st);
   }
 }
-// Trailing update (op 0 only) and the panel pair (ops 1 and 2): two kernels so
-// profiles and counters separate them.
+// Trailing update C -= A B (op 0), A (M x K) and B (K x N) column-major
+// (a_r = b_r = 1), same tiles, waves and MFMA blocks as the panel product. The
+// accumulators start as C and B is negated, so the MFMA chain yields C - AB.
+// K runs in chunks of 16 through a double-buffered LDS ring: the next chunk's
+// global loads are in flight while the MFMAs take the current one, and the C
+// loads overlap the first chunk's staging. Per element the MFMA sequence is
+// that of a single staged pass over K = 0..63. C does not alias A or B.
+constexpr int kUpdKc = 16;
+template <int kTile, typename T>
+constexpr std::size_t update_lds_elems() { return 2 * 2 * kUpdKc * (kTile + 1); }
+
+template <typename T, int kTile>
+__device__ __forceinline__ void tile_update(const TileJob<T>& j0, const TileJob<T>& j1, T* lds, int32_t bid) {
+  using V = typename MfmaOps<T>::v4;
+  constexpr int kLd = kTile + 1;                // LDS row length (padded against bank conflicts)
+  constexpr int kWT = kTile / 2;                // wavefront sub-tile edge
+  constexpr int kMB = kTile / 32;               // 16x16 MFMA blocks per wavefront and dimension
+  constexpr int kPer = kTile * kUpdKc / 256;    // staged elements per thread, operand and chunk
+  constexpr int kChunks = kNb / kUpdKc;
+  // Ring slot s: As = lds + s * 2 * kUpdKc * kLd (As[k*kLd + i] = A(i0 + i, k0 + k)),
+  // Bs = As + kUpdKc * kLd (Bs[k*kLd + j] = -B(k0 + k, c0 + j)).
+  const int32_t n0 = j0.tiles_m * j0.tiles_n;
+  const bool second = bid >= n0;
+  const TileJob<T>& J = second ? j1 : j0;
+  const int32_t tb = second ? bid - n0 : bid;
+  const int32_t i0 = (tb % J.tiles_m) * kTile, c0 = (tb / J.tiles_m) * kTile;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int wr = (wave >> 1) * kWT, wc = (wave & 1) * kWT;
+  const int li = lane & 15, lk = lane >> 4;
+  const int32_t M = J.M, N = J.N, K = J.K;
+  V acc[kMB][kMB];  // acc[c][a]: C^T block (j-block c, i-block a)
+#pragma unroll
+  for (int c = 0; c < kMB; ++c)
+#pragma unroll
+    for (int a = 0; a < kMB; ++a)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int32_t i = min(i0 + wr + a * 16 + li, M - 1);
+        const int32_t j = min(c0 + wc + c * 16 + MfmaOps<T>::row(lk, r), N - 1);
+        acc[c][a][r] = J.C[i + int64_t(j) * J.c_c];
+      }
+  // A chunk: kTile rows x 16 columns with the row index running across
+  // threads; B chunk: 16 rows x kTile columns with the row (k) index running
+  // across threads (both unit stride in memory).
+  T ra[kPer], rb[kPer];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int e = u * 256 + tid;
+      const int ii = e % kTile, ka = e / kTile;
+      ra[u] = J.A[int64_t(min(i0 + ii, M - 1)) + int64_t(min(k0 + ka, K - 1)) * J.a_c];
+      const int kb = e % kUpdKc, jj = e / kUpdKc;
+      rb[u] = J.B[int64_t(min(k0 + kb, K - 1)) + int64_t(min(c0 + jj, N - 1)) * J.b_c];
+    }
+  };
+  auto lstore = [&](int slot, int k0) {
+    T* As = lds + slot * 2 * kUpdKc * kLd;
+    T* Bs = As + kUpdKc * kLd;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int e = u * 256 + tid;
+      const int ii = e % kTile, ka = e / kTile;
+      As[ka * kLd + ii] = (i0 + ii < M && k0 + ka < K) ? ra[u] : T(0);
+      const int kb = e % kUpdKc, jj = e / kUpdKc;
+      Bs[kb * kLd + jj] = (c0 + jj < N && k0 + kb < K) ? -rb[u] : T(0);
+    }
+  };
+  gload(0);
+  lstore(0, 0);
+  __syncthreads();
+#pragma unroll
+  for (int ch = 0; ch < kChunks; ++ch) {
+    if (ch + 1 < kChunks) gload((ch + 1) * kUpdKc);
+    const T* As = lds + (ch & 1) * 2 * kUpdKc * kLd;
+    const T* Bs = As + kUpdKc * kLd;
+#pragma unroll
+    for (int k = 0; k < kUpdKc; k += 4) {
+      T af[kMB], bf[kMB];
+#pragma unroll
+      for (int a = 0; a < kMB; ++a) af[a] = As[(k + lk) * kLd + wr + a * 16 + li];
+#pragma unroll
+      for (int c = 0; c < kMB; ++c) bf[c] = Bs[(k + lk) * kLd + wc + c * 16 + li];
+#pragma unroll
+      for (int c = 0; c < kMB; ++c)
+#pragma unroll
+        for (int a = 0; a < kMB; ++a) acc[c][a] = MfmaOps<T>::mma(bf[c], af[a], acc[c][a]);
+    }
+    // The other slot was last read before the previous barrier.
+    if (ch + 1 < kChunks) lstore((ch + 1) & 1, (ch + 1) * kUpdKc);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int c = 0; c < kMB; ++c)
+#pragma unroll
+    for (int a = 0; a < kMB; ++a)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int32_t i = i0 + wr + a * 16 + li;
+        const int32_t j = c0 + wc + c * 16 + MfmaOps<T>::row(lk, r);
+        if (i < M && j < N) J.C[i + int64_t(j) * J.c_c] = acc[c][a][r];
+      }
+}
+
+// Trailing update and the panel pair: separate kernels so profiles and
+// counters separate them.
 template <typename T, int kTile>
 __global__ __launch_bounds__(256) void k_tail_update(TileJob<T> j0, TileJob<T> j1, double limit,
                                                      int32_t* __restrict__ status) {
-  __shared__ T lds[tile_lds_elems<kTile, T>()];
-  tile_gemm<T, true, kTile>(j0, j1, limit, status, lds, static_cast<int32_t>(blockIdx.x));
+  __shared__ T lds[update_lds_elems<kTile, T>()];
+  tile_update<T, kTile>(j0, j1, lds, static_cast<int32_t>(blockIdx.x));
 }
 // The rest of a trailing update (jobs j0, j1 from workgroup 1 on) with the next
 // step's diagonal LU + inverses in workgroup 0, which starts with the launch
@@ -958,7 +1047,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
 void k_tail_update_diag(TileJob<T> j0, TileJob<T> j1, double limit,
                                                           int32_t* __restrict__ status, T* __restrict__ S,
                                                           int64_t lds_s, int32_t q, int32_t nb, T* __restrict__ inv) {
-  constexpr std::size_t kTileElems = tile_lds_elems<kTile, T>();
+  constexpr std::size_t kTileElems = update_lds_elems<kTile, T>();
   constexpr std::size_t kDiagElems = kNb * kDiagLC + 16 / sizeof(T);
   __shared__ __attribute__((aligned(16))) T lds[kTileElems > kDiagElems ? kTileElems : kDiagElems];
   if (blockIdx.x == 0) {
@@ -966,20 +1055,20 @@ void k_tail_update_diag(TileJob<T> j0, TileJob<T> j1, double limit,
                         reinterpret_cast<int32_t*>(lds + kNb * kDiagLC));
     return;
   }
-  tile_gemm<T, true, kTile>(j0, j1, limit, status, lds, static_cast<int32_t>(blockIdx.x) - 1);
+  tile_update<T, kTile>(j0, j1, lds, static_cast<int32_t>(blockIdx.x) - 1);
 }
 // The next diagonal block's share of a trailing update, launched alone ahead of
 // the rest when the next diagonal LU runs beside the update (same tile code).
 template <typename T>
 __global__ __launch_bounds__(256) void k_tail_update_next(TileJob<T> j, double limit, int32_t* __restrict__ status) {
-  __shared__ T lds[tile_lds_elems<64, T>()];
-  tile_gemm<T, true, 64>(j, j, limit, status, lds, static_cast<int32_t>(blockIdx.x));
+  __shared__ T lds[update_lds_elems<64, T>()];
+  tile_update<T, 64>(j, j, lds, static_cast<int32_t>(blockIdx.x));
 }
 template <typename T, int kTile>
 __global__ __launch_bounds__(256) void k_tail_panel(TileJob<T> j0, TileJob<T> j1, double limit,
                                                     int32_t* __restrict__ status) {
   __shared__ T lds[tile_lds_elems<kTile, T>()];
-  tile_gemm<T, false, kTile>(j0, j1, limit, status, lds, static_cast<int32_t>(blockIdx.x));
+  tile_gemm<T, kTile>(j0, j1, limit, status, lds, static_cast<int32_t>(blockIdx.x));
 }
 
 namespace {
