@@ -80,6 +80,9 @@ struct SparseDev {
   int64_t nL = 0, nU = 0;
   double *Lx64 = nullptr, *Ux64 = nullptr, *Ud64 = nullptr, *S64 = nullptr;
   float *Lx32 = nullptr, *Ux32 = nullptr, *Ud32 = nullptr, *S32 = nullptr;
+  // Column stride of the dense tail S (m rounded up to 64 elements, so every
+  // column starts on a cache-line boundary in both precisions).
+  int64_t ldt = 1;
   double* rs = nullptr;
   double* y = nullptr;
   double* inv64 = nullptr;  // 2*64*64 per panel: the tail's diagonal-block inverses
@@ -407,6 +410,7 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
     S.s = sd.s;
     S.k0 = sd.k0;
     S.m = sd.m;
+    S.ldt = std::max<int64_t>((int64_t(sd.m) + 63) / 64 * 64, 1);
     S.scale = sd.scale;
     S.tol = sd.tol;
     if (sd.s != d.R[sd.block + 1] - d.R[sd.block] || sd.k0 + sd.m != sd.s) throw HipError{"sparse block sizes"};
@@ -544,11 +548,11 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
     S.Lx64 = A.get<double>(S.nL);
     S.Ux64 = A.get<double>(S.nU);
     S.Ud64 = A.get<double>(sd.k0);
-    S.S64 = A.get<double>(static_cast<std::size_t>(sd.m) * sd.m);
+    S.S64 = A.get<double>(static_cast<std::size_t>(S.ldt) * sd.m);
     S.Lx32 = A.get<float>(S.nL);
     S.Ux32 = A.get<float>(S.nU);
     S.Ud32 = A.get<float>(sd.k0);
-    S.S32 = A.get<float>(static_cast<std::size_t>(sd.m) * sd.m);
+    S.S32 = A.get<float>(static_cast<std::size_t>(S.ldt) * sd.m);
     S.rs = A.get<double>(sd.s);
     S.y = A.get<double>(static_cast<std::size_t>(sd.s) * kMaxRhs);
     const std::size_t npan = (static_cast<std::size_t>(sd.m) + 63) / 64;
@@ -694,10 +698,10 @@ void block_solve_launches(Plan& P, hipStream_t st, const double* b, double* x, i
       bdk::launch_sp_rows<T>(st, kt, S.m, S.tl_rows, S.lr_ptr, S.lr_col, S.lr_vidx, Lx, nullptr, S.y, ldy);
       // Tail: forward into ty, backward back into y (each output starts unpublished).
       for (int32_t c = 0; c < kt; ++c) bdk::launch_fill_sentinel(st, S.ty + c * ldt, S.m);
-      bdk::launch_tail_trsv<T>(st, kt, S.m, Sm, S.m, inv, S.y + S.k0, ldy, S.ty, ldt, false, P.tickets + tix,
+      bdk::launch_tail_trsv<T>(st, kt, S.m, Sm, S.ldt, inv, S.y + S.k0, ldy, S.ty, ldt, false, P.tickets + tix,
                                P.err);
       for (int32_t c = 0; c < kt; ++c) bdk::launch_fill_sentinel(st, S.y + c * ldy + S.k0, S.m);
-      bdk::launch_tail_trsv<T>(st, kt, S.m, Sm, S.m, inv, S.ty, ldt, S.y + S.k0, ldy, true, P.tickets + tix + 1,
+      bdk::launch_tail_trsv<T>(st, kt, S.m, Sm, S.ldt, inv, S.ty, ldt, S.y + S.k0, ldy, true, P.tickets + tix + 1,
                                P.err);
       for (std::size_t l = 0; l + 1 < S.bl_ptr.size(); ++l)
         bdk::launch_sp_rows<T>(st, kt, S.bl_ptr[l + 1] - S.bl_ptr[l], S.bl_rows + S.bl_ptr[l], S.ur_ptr, S.ur_col,
@@ -785,9 +789,9 @@ void factor_sparse_cols(Plan& P, SparseDev& S, hipStream_t st, T* Lx, T* Ux, T* 
     bdk::launch_sp_cols<T>(st, view, P.arena, S.rs, S.lev_ptr[l + 1] - S.lev_ptr[l], S.lev_col + S.lev_ptr[l], xs,
                            P.nslots, Lx, Ux, Ud, S.status,
                            l < S.cols_long.size() && S.cols_long[l].nlong > 0 ? &S.cols_long[l] : nullptr);
-  if (sizeof(T) == 8) bdk::launch_fill_f64(st, reinterpret_cast<double*>(Sm), int64_t(S.m) * S.m, 0.0);
-  else bdk::launch_fill_f32(st, reinterpret_cast<float*>(Sm), int64_t(S.m) * S.m, 0.0f);
-  bdk::launch_sp_tail<T>(st, view, P.arena, S.rs, xs, P.nslots, Lx, Ux, Sm, S.m,
+  if (sizeof(T) == 8) bdk::launch_fill_f64(st, reinterpret_cast<double*>(Sm), S.ldt * S.m, 0.0);
+  else bdk::launch_fill_f32(st, reinterpret_cast<float*>(Sm), S.ldt * S.m, 0.0f);
+  bdk::launch_sp_tail<T>(st, view, P.arena, S.rs, xs, P.nslots, Lx, Ux, Sm, S.ldt,
                          S.tail_long.nlong > 0 ? &S.tail_long : nullptr);
   if (pe) pe->mark(BDLU_PROF_SPARSE, st);
 }
@@ -805,7 +809,7 @@ void factor_sparse_tail(Plan& P, SparseDev& S, hipStream_t st, T* Sm, ProfEvents
     tt.panel_events = &pe->ev[BDLU_PROF_TAIL_PANEL];
   }
   T* inv = sizeof(T) == 8 ? reinterpret_cast<T*>(S.inv64) : reinterpret_cast<T*>(S.inv32);
-  bdk::launch_tail_lu<T>(st, S.m, Sm, S.m, growth_limit, S.status, inv, pe ? &tt : nullptr,
+  bdk::launch_tail_lu<T>(st, S.m, Sm, S.ldt, growth_limit, S.status, inv, pe ? &tt : nullptr,
                          P.sched.small_tile_below, lookahead);
 }
 
