@@ -636,7 +636,8 @@ template void launch_sp_tail<float>(hipStream_t, const SpView&, const double*, c
 // ---------------------------------------------------------------------------
 // Dense tail: no-pivot blocked right-looking LU, panel width 64. Per panel:
 //   1. k_diag_lu_inv: LU of the 64x64 diagonal block (one wavefront, rows
-//      in registers), then L11^{-1} and U11^{-1} (one wavefront each);
+//      in registers), with L11^{-1} and U11^{-1} formed alongside (one
+//      wavefront each);
 //   2. k_tail_panel: L21 = A21 U11^{-1} and U12 = L11^{-1} A12 as MFMA tile
 //      products (one launch, both jobs), with the |L| growth check;
 //   3. k_tail_update: trailing update A22 -= L21 U12 on MFMA.
@@ -646,18 +647,23 @@ constexpr int kNb = 64;
 // row-major (upper). Rows/columns at or past b are an identity pad. Each panel
 // keeps its pair (the tail solve applies them to its diagonal tiles).
 //
-// Two wavefronts. The LU is one dependent chain of 64 steps, so it runs on a
-// single wave with no barriers: lane i keeps row i in registers, and at step k
-// every lane takes pivot row k from lane k with readlane (scalar registers,
-// compile-time lane) and updates its own row. Both triangular inverses are
-// then independent per column: wave 0 forms L11^{-1} and wave 1 (after writing
-// the factor back) U11^{-1}, lane j one column each, right-looking so each
-// step has 63-k independent FMAs, with the factor entries read from a
-// column-major LDS copy at wave-uniform addresses (broadcast reads). Per
-// element the operation sequence is that of the textbook right-looking LU and
-// forward/backward substitution, so the results do not depend on how the work
-// is split.
-constexpr int kDiagThreads = 2 * kNb;
+// Three wavefronts, the inverses formed while the LU runs. The LU is one
+// dependent chain of 64 steps, so it runs on a single wave (wave 0) with no
+// barriers: lane i keeps row i in registers, and at step k every lane takes
+// pivot row k from lane k with readlane (scalar registers, compile-time lane)
+// and updates its own row. After step k, column k of the working matrix is
+// final in both factors (L(i,k) below the diagonal, U(i,k) on and above it);
+// each lane publishes its entry to a column-major LDS copy (one store per lane
+// and step). Wave 1 forms L11^{-1} column by column (lane j: x_i -= L(i,k) x_k
+// for i > k, k ascending, as forward substitution) and wave 2 forms U11^{-1}
+// row by row by bordering (lane i: U^{-1}(i,k) = -(sum_{m<k} U^{-1}(i,m)
+// U(m,k)) / U(k,k), k ascending), reading the published entries at
+// wave-uniform addresses (broadcast reads). The waves meet at a workgroup
+// barrier every 8 LU steps (straight-line code, no polling loops inside the
+// unrolled steps): wave 0 passes barrier t after step 8t-1, waves 1 and 2
+// before their steps 8(t-1), so they run one block of 8 steps behind the LU
+// with less work per step, and every wave executes the same barriers.
+constexpr int kDiagThreads = 3 * kNb;
 
 __device__ __forceinline__ float readlane_t(float v, int src) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), src));
@@ -669,20 +675,28 @@ __device__ __forceinline__ double readlane_t(double v, int src) {
   return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
 }
 
-constexpr int kDiagLC = kNb + 2;  // column-major LDS copy: fc[j*66 + i] = LU(i, j); 16-byte aligned columns
+constexpr int kDiagLC = kNb + 2;  // Lc[k*66 + i] = LU(i, k)
+constexpr int kDiagStep = 8;      // LU steps between barriers
+template <typename T>
+constexpr std::size_t diag_lds_elems() {
+  return kNb * kDiagLC + 16 / sizeof(T);  // + one int32 (status)
+}
 
-// Body of the diagonal LU + inverses for a workgroup of 128 or 256 threads
-// (waves 2 and 3, when present, only take part in the barriers). fc: LDS of
-// 64*66 T; st_sh: one LDS int.
+// Body of the diagonal LU + inverses for a workgroup of 192 or 256 threads
+// (wave 3, when present, only takes part in the barriers). fc: LDS of
+// diag_lds_elems<T>() elements, 16-byte aligned.
 template <typename T>
 __device__ __forceinline__ void diag_lu_inv_body(T* __restrict__ S, int64_t lds, int32_t p, int32_t b, double limit,
-                                                 int32_t* __restrict__ status, T* __restrict__ inv, T* fc,
-                                                 int32_t* st_sh_ptr) {
+                                                 int32_t* __restrict__ status, T* __restrict__ inv, T* fc) {
   constexpr int LC = kDiagLC;
-  int32_t& st_sh = *st_sh_ptr;
+  constexpr int kV = 16 / sizeof(T);  // elements per 16-byte store
+  using VT = T __attribute__((ext_vector_type(16 / sizeof(T))));
+  T* Lc = fc;
+  int32_t* st_sh = reinterpret_cast<int32_t*>(fc + kNb * LC);
   const int lane = threadIdx.x & (kNb - 1);
   const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-  if (threadIdx.x == 0) st_sh = 0;
+  if (threadIdx.x == 0) *st_sh = 0;
+  __syncthreads();
   int32_t st = 0;
   if (w == 0) {
     const int i = lane;
@@ -715,6 +729,8 @@ __device__ __forceinline__ void diag_lu_inv_body(T* __restrict__ S, int64_t lds,
 #pragma unroll
       for (int j = k + 2; j < kNb; ++j) a[j] = tfma(m, readlane_t(a[j], k), a[j]);
       a[k] = below ? lik : a[k];
+      Lc[k * LC + i] = a[k];  // column k, final in both factors
+      if ((k + 1) % kDiagStep == 0) __syncthreads();
     }
     // |L| check on the finished factor (a NaN fails it).
     T lmax = T(0);
@@ -724,56 +740,69 @@ __device__ __forceinline__ void diag_lu_inv_body(T* __restrict__ S, int64_t lds,
       lmax = (al <= lmax || lmax != lmax) ? lmax : al;
     }
     if (!(lmax <= static_cast<T>(limit))) st |= kStCheck;
+    // The factor back to S, coalesced down each column.
+    if (i < b) {
+      T* dst = S + int64_t(p) * lds + p + i;
 #pragma unroll
-    for (int j = 0; j < kNb; ++j) fc[j * LC + i] = a[j];
-  }
-  __syncthreads();
-  const int j = lane;
-  if (w == 0) {
-    // Column j of L11^{-1}, right-looking: x_i -= L(i,k) x_k for i > k,
-    // k = 0..62 (per element the same order as forward substitution).
+      for (int c = 0; c < kNb; ++c)
+        if (c < b) dst[int64_t(c) * lds] = a[c];
+    }
+  } else if (w == 1) {
+    // Column j of L11^{-1}.
+    const int j = lane;
     T x[kNb];
 #pragma unroll
-    for (int i = 0; i < kNb; ++i) x[i] = (i == j) ? T(1) : T(0);
+    for (int q = 0; q < kNb; ++q) x[q] = (q == j) ? T(1) : T(0);
 #pragma unroll
     for (int k = 0; k < kNb - 1; ++k) {
+      if (k % kDiagStep == 0) __syncthreads();
 #pragma unroll
-      for (int i = k + 1; i < kNb; ++i) x[i] = tfma(-fc[k * LC + i], x[k], x[i]);
+      for (int q = k + 1; q < kNb; ++q) x[q] = tfma(-Lc[k * LC + q], x[k], x[q]);
+      __builtin_amdgcn_sched_barrier(0);  // one step's reads in flight at a time (registers)
     }
 #pragma unroll
-    for (int i = 0; i < kNb; ++i) inv[i * kNb + j] = x[i];
-  } else if (w == 1) {
-    // The factor back to S (coalesced down each column), then column j of
-    // U11^{-1}, backward: y_k /= U(k,k), then y_i -= U(i,k) y_k for i < k.
+    for (int q = 0; q < kNb; ++q) inv[q * kNb + j] = x[q];
+  } else if (w == 2) {
+    // Row i of U11^{-1}: entries left of the diagonal stay zero, so each sum
+    // runs over m = i..k-1 in effect (four partial sums, combined in a fixed order).
     const int i = lane;
-    T* dst = S + int64_t(p) * lds + p + i;
+    T z[kNb];
 #pragma unroll
-    for (int c = 0; c < kNb; ++c)
-      if (i < b && c < b) dst[int64_t(c) * lds] = fc[c * LC + i];
-    T y[kNb];
+    for (int q = 0; q < kNb; ++q) z[q] = T(0);
 #pragma unroll
-    for (int q = 0; q < kNb; ++q) y[q] = (q == j) ? T(1) : T(0);
+    for (int k = 0; k < kNb; ++k) {
+      if (k % kDiagStep == 0) __syncthreads();
+      T ps[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-    for (int k = kNb - 1; k >= 0; --k) {
-      y[k] = y[k] / fc[k * LC + k];
-#pragma unroll
-      for (int q = 0; q < k; ++q) y[q] = tfma(-fc[k * LC + q], y[k], y[q]);
+      for (int m = 0; m < k; ++m) ps[m & 3] = tfma(z[m], Lc[k * LC + m], ps[m & 3]);
+      const T sum = (ps[0] + ps[1]) + (ps[2] + ps[3]);
+      z[k] = (i < k ? -sum : (i == k ? T(1) : T(0))) / Lc[k * LC + k];
+      __builtin_amdgcn_sched_barrier(0);
     }
+    const int j = i;
+    T* dst = inv + kNb * kNb + j * kNb;
 #pragma unroll
-    for (int q = 0; q < kNb; ++q) inv[kNb * kNb + q * kNb + j] = y[q];
+    for (int q = 0; q < kNb; q += kV) {
+      VT v;
+#pragma unroll
+      for (int e = 0; e < kV; ++e) v[e] = z[q + e];
+      *reinterpret_cast<VT*>(dst + q) = v;
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < kNb / kDiagStep; ++t) __syncthreads();
   }
-  if (st) atomicOr(&st_sh, st);
+  if (st) atomicOr(st_sh, st);
   __syncthreads();
-  if (threadIdx.x == 0 && st_sh) atomicOr(status, st_sh);
+  if (threadIdx.x == 0 && *st_sh) atomicOr(status, *st_sh);
 }
 
 template <typename T>
 __global__ __launch_bounds__(kDiagThreads) void k_diag_lu_inv(T* __restrict__ S, int64_t lds, int32_t p, int32_t b,
                                                               double limit, int32_t* __restrict__ status,
                                                               T* __restrict__ inv) {
-  __shared__ T fc[kNb * kDiagLC];
-  __shared__ int32_t st_sh;
-  diag_lu_inv_body<T>(S, lds, p, b, limit, status, inv, fc, &st_sh);
+  __shared__ __attribute__((aligned(16))) T fc[diag_lds_elems<T>()];
+  diag_lu_inv_body<T>(S, lds, p, b, limit, status, inv, fc);
 }
 
 // Inputs of both forms: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16].
@@ -1037,9 +1066,9 @@ __global__ __launch_bounds__(256) void k_tail_update(TileJob<T> j0, TileJob<T> j
   tile_update<T, kTile>(j0, j1, lds, static_cast<int32_t>(blockIdx.x));
 }
 // The rest of a trailing update (jobs j0, j1 from workgroup 1 on) with the next
-// step's diagonal LU + inverses in workgroup 0, which starts with the launch
-// (the next diagonal block was updated by the launch before). The tile and
-// diagonal code share one LDS buffer.
+// step's diagonal LU + inverses in workgroup 0 (three of its four waves), which
+// starts with the launch (the next diagonal block was updated by the launch
+// before). The tile and diagonal code share one LDS buffer.
 // FP32: two waves per SIMD, as for the plain update (the diagonal path would
 // otherwise raise the register count of the whole kernel).
 template <typename T, int kTile>
@@ -1048,11 +1077,10 @@ void k_tail_update_diag(TileJob<T> j0, TileJob<T> j1, double limit,
                                                           int32_t* __restrict__ status, T* __restrict__ S,
                                                           int64_t lds_s, int32_t q, int32_t nb, T* __restrict__ inv) {
   constexpr std::size_t kTileElems = update_lds_elems<kTile, T>();
-  constexpr std::size_t kDiagElems = kNb * kDiagLC + 16 / sizeof(T);
+  constexpr std::size_t kDiagElems = diag_lds_elems<T>();
   __shared__ __attribute__((aligned(16))) T lds[kTileElems > kDiagElems ? kTileElems : kDiagElems];
   if (blockIdx.x == 0) {
-    diag_lu_inv_body<T>(S, lds_s, q, nb, limit, status, inv, lds,
-                        reinterpret_cast<int32_t*>(lds + kNb * kDiagLC));
+    diag_lu_inv_body<T>(S, lds_s, q, nb, limit, status, inv, lds);
     return;
   }
   tile_update<T, kTile>(j0, j1, lds, static_cast<int32_t>(blockIdx.x) - 1);
