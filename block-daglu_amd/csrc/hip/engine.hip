@@ -582,15 +582,23 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
   // Solve phases: blocks in descending order; runs of dense blocks form one
   // persistent-kernel segment, each sparse block is its own phase. Inside a
   // segment the tickets follow the solve levels (level(b) = 1 + max level of
-  // the blocks b reads, descending block id within a level): a topological
-  // order in which blocks that are ready come before blocks still waiting.
+  // the blocks b reads): a topological order in which blocks that are ready
+  // come before blocks still waiting. Within a level, blocks with the longest
+  // chain of dependents after them (height) come first, so the critical path
+  // is claimed as soon as it can run; ties keep descending block id.
   std::vector<int32_t> lev(static_cast<std::size_t>(nb), 0);
   for (int32_t b = nb - 1; b >= 0; --b)
     for (int32_t e = d.dep_ptr[b]; e < d.dep_ptr[b + 1]; ++e) lev[b] = std::max(lev[b], lev[d.dep_blk[e]] + 1);
+  std::vector<int32_t> height(static_cast<std::size_t>(nb), 0);  // blocks read only blocks of higher id
+  for (int32_t c = 0; c < nb; ++c)
+    for (int32_t e = d.dep_ptr[c]; e < d.dep_ptr[c + 1]; ++e)
+      height[d.dep_blk[e]] = std::max(height[d.dep_blk[e]], height[c] + 1);
   std::vector<int32_t> cur;
   auto flush = [&]() {
     if (cur.empty()) return;
-    std::stable_sort(cur.begin(), cur.end(), [&](int32_t x, int32_t y) { return lev[x] < lev[y]; });
+    std::stable_sort(cur.begin(), cur.end(), [&](int32_t x, int32_t y) {
+      return lev[x] != lev[y] ? lev[x] < lev[y] : height[x] > height[y];
+    });
     Phase ph;
     ph.sparse = false;
     ph.index = static_cast<int32_t>(P.phases.size());

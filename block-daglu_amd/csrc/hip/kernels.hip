@@ -1380,11 +1380,25 @@ __global__ __launch_bounds__(64) void k_dense_seg_solve(
 #pragma unroll
         for (int c = 0; c < K; ++c) acc[c] = acc[c] - ov[u] * finish_wait_x(xperm + c * ldv + oc[u], xu[u][c], late);
       }
-    for (int32_t p = p0 + kPre; p < p1; ++p) {
-      const double v = off_val[p];
-      const int32_t j = off_col[p];
+    // The rest in groups of kPre, each group's first polls issued together.
+    for (int32_t pg = p0 + kPre; pg < p1; pg += kPre) {
 #pragma unroll
-      for (int c = 0; c < K; ++c) acc[c] = acc[c] - v * wait_x(xperm + c * ldv + j, late);
+      for (int u = 0; u < kPre; ++u) {
+        const bool ok = pg + u < p1;
+        oc[u] = ok ? off_col[pg + u] : -1;
+        ov[u] = ok ? off_val[pg + u] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kPre; ++u)
+#pragma unroll
+        for (int c = 0; c < K; ++c) xu[u][c] = oc[u] >= 0 ? ld_x_relaxed(xperm + c * ldv + oc[u]) : 0ull;
+#pragma unroll
+      for (int u = 0; u < kPre; ++u)
+        if (oc[u] >= 0) {
+#pragma unroll
+          for (int c = 0; c < K; ++c)
+            acc[c] = acc[c] - ov[u] * finish_wait_x(xperm + c * ldv + oc[u], xu[u][c], late);
+        }
     }
     // Row gather (the block's pivoting), then y = L^{-1} (P rhs) and
     // x = U^{-1} y as matrix-vector products: every step reads the finished
