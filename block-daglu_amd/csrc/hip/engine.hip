@@ -110,6 +110,10 @@ struct Plan {
   double* vals = nullptr;
   int64_t* csr_ptr = nullptr;
   int32_t* csr_col = nullptr;
+  // Rows with more than kLongRow entries, for the wavefront-per-row residual.
+  uint8_t* res_is_long = nullptr;
+  int32_t* res_long_rows = nullptr;
+  int32_t res_nlong = 0;
   int32_t *P = nullptr, *Q = nullptr, *R = nullptr;
   int32_t *off_ptr = nullptr, *off_col = nullptr;
   double* off_val = nullptr;
@@ -349,6 +353,22 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
   P.vals = A.get<double>(d.nnz);
   P.csr_ptr = A.copy(d.csr_ptr, n + 1, st);
   P.csr_col = A.copy(d.csr_col, d.nnz, st);
+  // (host vectors live until the upload's final stream synchronization)
+  std::vector<uint8_t> res_flag(static_cast<std::size_t>(n), 0);
+  std::vector<int32_t> res_rows;
+  {
+    constexpr int64_t kLongRow = 64;
+    for (int32_t i = 0; i < n; ++i)
+      if (d.csr_ptr[i + 1] - d.csr_ptr[i] > kLongRow) {
+        res_flag[i] = 1;
+        res_rows.push_back(i);
+      }
+    P.res_nlong = static_cast<int32_t>(res_rows.size());
+    if (P.res_nlong > 0) {
+      P.res_is_long = A.copy(res_flag, st);
+      P.res_long_rows = A.copy(res_rows, st);
+    }
+  }
   P.P = A.copy(d.P, n, st);
   P.Q = A.copy(d.Q, n, st);
   P.R = A.copy(d.R, nb + 1, st);
@@ -1143,7 +1163,8 @@ int bdlu_gpu_solve(bdlu_ctx* c, uint64_t id, double* B, int32_t nrhs, int64_t ld
       if (want_stats) {
         double h[9 * kMaxRhs];
         auto eval = [&]() {
-          bdk::launch_residual(st, P.n, kt, P.csr_ptr, P.csr_col, P.vals, P.x, P.b, P.r, P.cw, n);
+          bdk::launch_residual(st, P.n, kt, P.csr_ptr, P.csr_col, P.vals, P.x, P.b, P.r, P.cw, n, P.res_is_long,
+                               P.res_nlong, P.res_long_rows);
           bdk::launch_norms(st, P.n, kt, P.r, P.x, P.b, P.cw, n, P.partials, P.norms);
           HIPCHECK(hipMemcpyAsync(h, P.norms, sizeof(double) * 9 * kt, hipMemcpyDeviceToHost, st));
           HIPCHECK(hipStreamSynchronize(st));

@@ -135,8 +135,11 @@ void launch_sp_scatter_x(hipStream_t st, int32_t s, int32_t start, int32_t kc, c
 
 // ---- residual / norms / refinement ----
 // r = b - A x; cw_i = |r_i| / (|A||x| + |b|)_i (componentwise backward-error ratio).
+// Rows flagged in is_long (nlong of them, listed in long_rows) are taken one
+// wavefront each; with nlong == 0 the flags are not read.
 void launch_residual(hipStream_t st, int32_t n, int32_t kc, const int64_t* csr_ptr, const int32_t* csr_col,
-                     const double* vals, const double* x, const double* b, double* r, double* cw, int64_t ld);
+                     const double* vals, const double* x, const double* b, double* r, double* cw, int64_t ld,
+                     const uint8_t* is_long, int32_t nlong, const int32_t* long_rows);
 // Per column c: out[c*9+0]=sum r^2, [1]=max|r|, [2]=max|x|, [3]=sum b^2, [4]=max|b|,
 // [5]=max cw. partials: 512 * 6 * kc doubles.
 void launch_norms(hipStream_t st, int32_t n, int32_t kc, const double* r, const double* x, const double* b,

@@ -1686,8 +1686,9 @@ template <int K>
 __global__ void k_residual(int32_t n, const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
                            const double* __restrict__ vals, const double* __restrict__ x,
                            const double* __restrict__ b, double* __restrict__ r, double* __restrict__ cw,
-                           int64_t ld) {
+                           int64_t ld, const uint8_t* __restrict__ is_long) {
   for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (is_long && is_long[i]) continue;  // k_residual_long takes it
     double acc[K], den[K];
 #pragma unroll
     for (int c = 0; c < K; ++c) {
@@ -1712,15 +1713,71 @@ __global__ void k_residual(int32_t n, const int64_t* __restrict__ ptr, const int
     }
   }
 }
-void launch_residual(hipStream_t st, int32_t n, int32_t kc, const int64_t* csr_ptr, const int32_t* csr_col,
-                     const double* vals, const double* x, const double* b, double* r, double* cw, int64_t ld) {
-  const dim3 g(grid_for(n, 256, 8192)), bl(256);
-  switch (kc) {
-    case 1: hipLaunchKernelGGL(k_residual<1>, g, bl, 0, st, n, csr_ptr, csr_col, vals, x, b, r, cw, ld); break;
-    case 2: hipLaunchKernelGGL(k_residual<2>, g, bl, 0, st, n, csr_ptr, csr_col, vals, x, b, r, cw, ld); break;
-    case 4: hipLaunchKernelGGL(k_residual<4>, g, bl, 0, st, n, csr_ptr, csr_col, vals, x, b, r, cw, ld); break;
-    default: hipLaunchKernelGGL(k_residual<8>, g, bl, 0, st, n, csr_ptr, csr_col, vals, x, b, r, cw, ld); break;
+// Rows with many entries (dense-ish blocks), one wavefront each: lane l takes
+// entries l, l+64, ... in order, then a fixed xor-butterfly sums the lanes, so
+// the result does not depend on scheduling. Same outputs as k_residual.
+template <int K>
+__global__ __launch_bounds__(256) void k_residual_long(int32_t nlong, const int32_t* __restrict__ rows,
+                                                       const int64_t* __restrict__ ptr,
+                                                       const int32_t* __restrict__ col,
+                                                       const double* __restrict__ vals,
+                                                       const double* __restrict__ x, const double* __restrict__ b,
+                                                       double* __restrict__ r, double* __restrict__ cw, int64_t ld) {
+  const int lane = threadIdx.x & 63;
+  const int32_t wpb = blockDim.x >> 6;
+  for (int32_t t = blockIdx.x * wpb + (threadIdx.x >> 6); t < nlong; t += gridDim.x * wpb) {
+    const int32_t i = rows[t];
+    double s[K], d[K];
+#pragma unroll
+    for (int c = 0; c < K; ++c) s[c] = d[c] = 0.0;
+    for (int64_t p = ptr[i] + lane; p < ptr[i + 1]; p += 64) {
+      const double v = vals[p];
+      const int32_t j = col[p];
+#pragma unroll
+      for (int c = 0; c < K; ++c) {
+        const double t2 = v * x[c * ld + j];
+        s[c] += t2;
+        d[c] += fabs(t2);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < K; ++c)
+      for (int off = 32; off > 0; off >>= 1) {
+        s[c] += __shfl_xor(s[c], off);
+        d[c] += __shfl_xor(d[c], off);
+      }
+    if (lane == 0) {
+#pragma unroll
+      for (int c = 0; c < K; ++c) {
+        const double bi = b[c * ld + i];
+        const double acc = bi - s[c];
+        const double den = fabs(bi) + d[c];
+        r[c * ld + i] = acc;
+        const double a = fabs(acc);
+        cw[c * ld + i] = den > 0.0 ? a / den : (a == 0.0 ? 0.0 : INFINITY);
+      }
+    }
   }
+}
+void launch_residual(hipStream_t st, int32_t n, int32_t kc, const int64_t* csr_ptr, const int32_t* csr_col,
+                     const double* vals, const double* x, const double* b, double* r, double* cw, int64_t ld,
+                     const uint8_t* is_long, int32_t nlong, const int32_t* long_rows) {
+  const dim3 g(grid_for(n, 256, 8192)), bl(256);
+#define BDLU_RES(KK)                                                                                             \
+  do {                                                                                                           \
+    hipLaunchKernelGGL(k_residual<KK>, g, bl, 0, st, n, csr_ptr, csr_col, vals, x, b, r, cw, ld,                \
+                       nlong > 0 ? is_long : nullptr);                                                           \
+    if (nlong > 0)                                                                                               \
+      hipLaunchKernelGGL(k_residual_long<KK>, dim3(grid_for(nlong, 4, 4096)), dim3(256), 0, st, nlong,          \
+                         long_rows, csr_ptr, csr_col, vals, x, b, r, cw, ld);                                    \
+  } while (0)
+  switch (kc) {
+    case 1: BDLU_RES(1); break;
+    case 2: BDLU_RES(2); break;
+    case 4: BDLU_RES(4); break;
+    default: BDLU_RES(8); break;
+  }
+#undef BDLU_RES
 }
 
 // Deterministic reductions: fixed grid, fixed per-block order, then a fixed
