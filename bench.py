@@ -42,7 +42,7 @@ ROOFLINE_KERNEL = "k_tail_update<float, *>"
 
 def is_roofline_kernel(name: str) -> bool:
     """Both tile variants of the FP32 trailing update, by demangled or mangled
-    name (not the next-block launch k_tail_update_next)."""
+    name (not the fused launch k_tail_update_diag of the timed steps)."""
     return "k_tail_update<float," in name or "k_tail_updateIfLi" in name
 
 

@@ -1065,32 +1065,30 @@ __global__ __launch_bounds__(256) void k_tail_update(TileJob<T> j0, TileJob<T> j
   __shared__ T lds[update_lds_elems<kTile, T>()];
   tile_update<T, kTile>(j0, j1, lds, static_cast<int32_t>(blockIdx.x));
 }
-// The rest of a trailing update (jobs j0, j1 from workgroup 1 on) with the next
-// step's diagonal LU + inverses in workgroup 0 (three of its four waves), which
-// starts with the launch (the next diagonal block was updated by the launch
-// before). The tile and diagonal code share one LDS buffer.
+// A trailing update with the next step's diagonal block folded in: workgroup 0
+// first applies the update to the next diagonal block (job `next`, one 64x64
+// tile, the same tile code), then factors it and forms its inverses (three of
+// its four waves); workgroups from 1 on update the rest (jobs j0, j1), so the
+// diagonal work runs beside the bulk of the update from the start. The tile
+// and diagonal code share one LDS buffer.
 // FP32: two waves per SIMD, as for the plain update (the diagonal path would
 // otherwise raise the register count of the whole kernel).
 template <typename T, int kTile>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1)))
-void k_tail_update_diag(TileJob<T> j0, TileJob<T> j1, double limit,
-                                                          int32_t* __restrict__ status, T* __restrict__ S,
-                                                          int64_t lds_s, int32_t q, int32_t nb, T* __restrict__ inv) {
+void k_tail_update_diag(TileJob<T> j0, TileJob<T> j1, TileJob<T> next, double limit,
+                        int32_t* __restrict__ status, T* __restrict__ S, int64_t lds_s, int32_t q, int32_t nb,
+                        T* __restrict__ inv) {
   constexpr std::size_t kTileElems = update_lds_elems<kTile, T>();
   constexpr std::size_t kDiagElems = diag_lds_elems<T>();
   __shared__ __attribute__((aligned(16))) T lds[kTileElems > kDiagElems ? kTileElems : kDiagElems];
   if (blockIdx.x == 0) {
+    // The tile ends on a barrier after its last LDS read; the diagonal body's
+    // first barrier makes the block's new values visible to its loads.
+    tile_update<T, 64>(next, next, lds, 0);
     diag_lu_inv_body<T>(S, lds_s, q, nb, limit, status, inv, lds);
     return;
   }
   tile_update<T, kTile>(j0, j1, lds, static_cast<int32_t>(blockIdx.x) - 1);
-}
-// The next diagonal block's share of a trailing update, launched alone ahead of
-// the rest when the next diagonal LU runs beside the update (same tile code).
-template <typename T>
-__global__ __launch_bounds__(256) void k_tail_update_next(TileJob<T> j, double limit, int32_t* __restrict__ status) {
-  __shared__ T lds[update_lds_elems<64, T>()];
-  tile_update<T, 64>(j, j, lds, static_cast<int32_t>(blockIdx.x));
 }
 template <typename T, int kTile>
 __global__ __launch_bounds__(256) void k_tail_panel(TileJob<T> j0, TileJob<T> j1, double limit,
@@ -1133,13 +1131,12 @@ void tail_step(hipStream_t st, T* S, int64_t lds, int32_t p, int32_t b, int32_t 
   }
 }
 
-// One step with lookahead: the panel pair of step p, then the trailing update
-// in two launches: (A) the next diagonal block alone, then (B) the rest of the
+// One step with lookahead: the panel pair of step p, then one launch for the
+// whole trailing update: workgroup 0 updates the next diagonal block and
+// factors it (the next step's diagonal LU), the others update the rest of the
 // trailing matrix (two tile jobs: everything right of the next block column,
-// and the block column below the next diagonal block) with the next step's
-// diagonal LU as workgroup 0 of the same launch, so it runs beside the bulk of
-// the update from the start. Every element gets the same MFMA sequence as in
-// the single update, so results are identical.
+// and the block column below the next diagonal block). Every element gets the
+// same MFMA sequence as in the single update, so results are identical.
 template <typename T, int kTile>
 void tail_step_lookahead(hipStream_t st, T* S, int64_t lds, int32_t p, int32_t b, int32_t rest, double limit,
                          int32_t* status, T* inv) {
@@ -1155,7 +1152,6 @@ void tail_step_lookahead(hipStream_t st, T* S, int64_t lds, int32_t p, int32_t b
   const T* L21 = S + int64_t(p) * lds + q;      // rest x b, column stride lds
   const T* U12 = S + int64_t(q) * lds + p;      // b x rest, column stride lds
   TileJob<T> a{nb, nb, b, 0, 1, 1, L21, 1, lds, U12, 1, lds, S + int64_t(q) * lds + q, 1, lds};
-  hipLaunchKernelGGL(k_tail_update_next<T>, dim3(1), dim3(256), 0, st, a, limit, status);
   const int32_t n0 = rest - nb;
   // Job 0: all trailing rows, columns right of the next block column.
   TileJob<T> right{rest, n0, b, 0, tr, (n0 + kTile - 1) / kTile, L21, 1, lds, U12 + int64_t(nb) * lds, 1, lds,
@@ -1165,8 +1161,8 @@ void tail_step_lookahead(hipStream_t st, T* S, int64_t lds, int32_t p, int32_t b
                    S + int64_t(q) * lds + q + nb, 1, lds};
   if (n0 <= 0) right.tiles_m = right.tiles_n = below.tiles_m = below.tiles_n = 0;
   const int32_t g = 1 + right.tiles_m * right.tiles_n + below.tiles_m * below.tiles_n;
-  hipLaunchKernelGGL((k_tail_update_diag<T, kTile>), dim3(g), dim3(256), 0, st, right, below, limit, status, S, lds,
-                     q, nb, inv + 2 * kNb * kNb);
+  hipLaunchKernelGGL((k_tail_update_diag<T, kTile>), dim3(g), dim3(256), 0, st, right, below, a, limit, status, S,
+                     lds, q, nb, inv + 2 * kNb * kNb);
 }
 
 template <typename T>
