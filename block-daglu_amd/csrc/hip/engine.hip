@@ -127,9 +127,9 @@ struct Plan {
   int64_t lu_len = 0;
   double* lu64 = nullptr;
   float* lu32 = nullptr;
-  // Explicit inverses of the dense factors at the same offsets: L^{-1} and U^{-1}.
-  double *invl64 = nullptr, *invu64 = nullptr;
-  float *invl32 = nullptr, *invu32 = nullptr;
+  // Explicit inverses M = U^{-1} L^{-1} of the dense blocks, at the factor offsets.
+  double* minv64 = nullptr;
+  float* minv32 = nullptr;
   int32_t* piv = nullptr;  // per row: block-local source row of the pivot gather
   int32_t* block_status = nullptr;  // per block (dense status)
   int32_t* prec_of_block = nullptr; // per block: 0 fp64 factors, 1 fp32
@@ -409,10 +409,8 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
   P.lu0_of_block = A.copy(lu0_of_block, st);
   P.lu64 = A.get<double>(lu);
   P.lu32 = A.get<float>(lu);
-  P.invl64 = A.get<double>(lu);
-  P.invu64 = A.get<double>(lu);
-  P.invl32 = A.get<float>(lu);
-  P.invu32 = A.get<float>(lu);
+  P.minv64 = A.get<double>(lu);
+  P.minv32 = A.get<float>(lu);
   P.piv = A.get<int32_t>(n);
   P.block_status = A.get<int32_t>(nb);
   P.prec_of_block = A.get<int32_t>(nb);
@@ -707,8 +705,8 @@ void block_solve_launches(Plan& P, hipStream_t st, const double* b, double* x, i
   for (const Phase& ph : P.phases) {
     if (!ph.sparse) {
       // Each dense block's factors are read in the precision it was factored in.
-      bdk::launch_dense_seg_solve(st, kt, ph.ntasks, ph.tasks, P.R, P.lu0_of_block, P.invu64, P.invu32, P.invl64,
-                                  P.invl32, P.prec_of_block, P.piv, P.off_ptr, P.off_col, P.off_val, P.tickets + tix,
+      bdk::launch_dense_seg_solve(st, kt, ph.ntasks, ph.tasks, P.R, P.lu0_of_block, P.minv64, P.minv32,
+                                  P.prec_of_block, P.piv, P.off_ptr, P.off_col, P.off_val, P.tickets + tix,
                                   P.bperm, P.xperm, ld, P.err, P.sched.seg_grid);
       ++tix;
       continue;
@@ -993,11 +991,11 @@ int bdlu_gpu_refactor(bdlu_ctx* c, uint64_t id, const double* values, int where,
       if (fp32) {
         bdk::launch_dense_lu<float>(ds, P.ndense, P.d_size, P.d_start, P.d_val0, P.d_lu0, P.arena, P.lu32, P.piv,
                                     P.block_status, P.d_block, order);
-        bdk::launch_dense_inv<float>(ds, P.ndense, P.d_size, P.d_lu0, P.lu32, P.invl32, P.invu32);
+        bdk::launch_dense_inv<float>(ds, P.ndense, P.d_size, P.d_lu0, P.lu32, P.minv32);
       } else {
         bdk::launch_dense_lu<double>(ds, P.ndense, P.d_size, P.d_start, P.d_val0, P.d_lu0, P.arena, P.lu64, P.piv,
                                      P.block_status, P.d_block, order);
-        bdk::launch_dense_inv<double>(ds, P.ndense, P.d_size, P.d_lu0, P.lu64, P.invl64, P.invu64);
+        bdk::launch_dense_inv<double>(ds, P.ndense, P.d_size, P.d_lu0, P.lu64, P.minv64);
       }
       if (pe) pe->mark(BDLU_PROF_DENSE_LU, st);
       if (fork) HIPCHECK(hipEventRecord(c->event(1), ds));
@@ -1083,8 +1081,7 @@ int bdlu_gpu_refactor(bdlu_ctx* c, uint64_t id, const double* values, int where,
       int64_t* dl0 = tmp.copy(l0, st);
       bdk::launch_dense_lu<double>(st, static_cast<int32_t>(redo.size()), dsz, dst_, dv0, dl0, P.arena, P.lu64,
                                    P.piv, P.block_status, dbl);
-      bdk::launch_dense_inv<double>(st, static_cast<int32_t>(redo.size()), dsz, dl0, P.lu64, P.invl64,
-                                    P.invu64);
+      bdk::launch_dense_inv<double>(st, static_cast<int32_t>(redo.size()), dsz, dl0, P.lu64, P.minv64);
       HIPCHECK(hipStreamSynchronize(st));
       tmp.release();
       fs.fp64_fallback_blocks += static_cast<int32_t>(redo.size());

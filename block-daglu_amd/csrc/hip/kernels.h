@@ -96,11 +96,10 @@ template <typename T>
 void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, int32_t* status, T* inv,
                     TailTimer* timer = nullptr, int32_t small_tile_below = 0, bool lookahead = false);
 
-// Explicit inverses of the dense blocks' factors at the blocks' factor offsets:
-// invl = L^{-1} (unit diagonal stored, zeros above), invu = U^{-1} (zeros below).
+// Explicit inverse M = U^{-1} L^{-1} of each dense block's factors, s x s
+// column-major at the block's factor offset (x = M (P b) solves the block).
 template <typename T>
-void launch_dense_inv(hipStream_t st, int32_t nd, const int32_t* sizes, const int64_t* lu0, const T* lu, T* invl,
-                      T* invu);
+void launch_dense_inv(hipStream_t st, int32_t nd, const int32_t* sizes, const int64_t* lu0, const T* lu, T* minv);
 // ---- solve ----
 // Vector sets hold kc columns (right-hand sides), column c at base + c * ld.
 // Kernels with per-column registers are instantiated for 1, 2, 4 and 8 columns
@@ -112,8 +111,8 @@ void launch_unpermute(hipStream_t st, int32_t n, int32_t k, const int32_t* Q, co
 // Dense-block solve of one segment (persistent, sync-free). ticket starts at 0;
 // xperm entries of this segment's blocks start as the unpublished pattern.
 void launch_dense_seg_solve(hipStream_t st, int32_t kc, int32_t ntasks, const int32_t* tasks, const int32_t* R,
-                            const int64_t* lu0_of_block, const double* invu64, const float* invu32,
-                            const double* invl64, const float* invl32, const int32_t* prec_of_block,
+                            const int64_t* lu0_of_block, const double* minv64, const float* minv32,
+                            const int32_t* prec_of_block,
                             const int32_t* gsrc, const int32_t* off_ptr, const int32_t* off_col,
                             const double* off_val, int32_t* ticket, const double* bperm, double* xperm, int64_t ldv,
                             int32_t* err, int32_t grid = 1024);

@@ -248,21 +248,23 @@ template void launch_dense_lu<float>(hipStream_t, int32_t, const int32_t*, const
                                      const int64_t*, const double*, float*, int32_t*, int32_t*, const int32_t*,
                                      const int32_t*);
 
-// Explicit inverses of each dense block's factors, for the solve, as two full
-// s x s column-major matrices at the block's factor offset: L^{-1} (unit
-// diagonal stored, zeros above) in invl and U^{-1} (zeros below) in invu, so
-// the solve applies them without masks. One workgroup of
-// two wavefronts per block: wave 0 forms the columns of L^{-1}, wave 1 those of
-// U^{-1}, lane j column j, right-looking (63-k independent FMAs per step) over
-// a column-major LDS copy of the factor read at wave-uniform addresses.
-// Applying them in the solve replaces two chains of s dependent substitution
-// steps per block by two matrix-vector products.
+// The explicit inverse of each dense block's factors, for the solve:
+// M = U^{-1} L^{-1} (so x = M (P b) solves the block), s x s column-major at the
+// block's factor offset. One workgroup of two wavefronts per block: wave 0
+// forms the columns of L^{-1}, wave 1 those of U^{-1} (lane j column j,
+// right-looking, 63-k independent FMAs per step, over a column-major LDS copy
+// of the factor read at wave-uniform addresses); wave 1 leaves U^{-1} in LDS
+// and wave 0 multiplies it into its column: M(i,j) = sum_{k>=i} U^{-1}(i,k)
+// L^{-1}(k,j), k ascending, in place for i ascending (row i of the product
+// reads only entries k >= i). Applying M in the solve replaces two chains of s
+// dependent substitution steps per block by one matrix-vector product.
 template <typename T>
 __global__ __launch_bounds__(128) void k_dense_inv(int32_t nd, const int32_t* __restrict__ sizes,
                                                    const int64_t* __restrict__ lu0, const T* __restrict__ lu,
-                                                   T* __restrict__ invl, T* __restrict__ invu) {
+                                                   T* __restrict__ minv) {
   constexpr int LC = 66;
   __shared__ T fc[64 * LC];
+  __shared__ T ui[64 * LC];  // ui[k*66 + i] = U^{-1}(i, k)
   const int32_t t = static_cast<int32_t>(blockIdx.x);
   if (t >= nd) return;
   const int s = __builtin_amdgcn_readfirstlane(sizes[t]);
@@ -290,12 +292,6 @@ __global__ __launch_bounds__(128) void k_dense_inv(int32_t nd, const int32_t* __
         for (int i = k + 1; i < 64; ++i) x[i] = tfma(-fc[k * LC + i], x[k], x[i]);
       }
     }
-    if (j < s) {
-      T* dst = invl + lu0[t];
-#pragma unroll
-      for (int i = 0; i < 64; ++i)
-        if (i < s) dst[int64_t(j) * s + i] = i > j ? x[i] : (i == j ? T(1) : T(0));
-    }
   } else {
 #pragma unroll
     for (int k = 63; k >= 0; --k) {
@@ -305,25 +301,36 @@ __global__ __launch_bounds__(128) void k_dense_inv(int32_t nd, const int32_t* __
         for (int i = 0; i < k; ++i) x[i] = tfma(-fc[k * LC + i], x[k], x[i]);
       }
     }
+#pragma unroll
+    for (int i = 0; i < 64; ++i) ui[j * LC + i] = i <= j ? x[i] : T(0);
+  }
+  __syncthreads();
+  if (w == 0) {
+#pragma unroll
+    for (int i = 0; i < 64; ++i) {
+      if (i < s) {
+        T acc = T(0);
+#pragma unroll
+        for (int k = i; k < 64; ++k) acc = tfma(ui[k * LC + i], x[k], acc);
+        x[i] = acc;
+      }
+    }
     if (j < s) {
-      T* dst = invu + lu0[t];
+      T* dst = minv + lu0[t] + int64_t(j) * s;
 #pragma unroll
       for (int i = 0; i < 64; ++i)
-        if (i < s) dst[int64_t(j) * s + i] = i <= j ? x[i] : T(0);
+        if (i < s) dst[i] = x[i];
     }
   }
 }
 
 template <typename T>
-void launch_dense_inv(hipStream_t st, int32_t nd, const int32_t* sizes, const int64_t* lu0, const T* lu, T* invl,
-                      T* invu) {
+void launch_dense_inv(hipStream_t st, int32_t nd, const int32_t* sizes, const int64_t* lu0, const T* lu, T* minv) {
   if (nd <= 0) return;
-  hipLaunchKernelGGL(k_dense_inv<T>, dim3(nd), dim3(128), 0, st, nd, sizes, lu0, lu, invl, invu);
+  hipLaunchKernelGGL(k_dense_inv<T>, dim3(nd), dim3(128), 0, st, nd, sizes, lu0, lu, minv);
 }
-template void launch_dense_inv<double>(hipStream_t, int32_t, const int32_t*, const int64_t*, const double*, double*,
-                                       double*);
-template void launch_dense_inv<float>(hipStream_t, int32_t, const int32_t*, const int64_t*, const float*, float*,
-                                      float*);
+template void launch_dense_inv<double>(hipStream_t, int32_t, const int32_t*, const int64_t*, const double*, double*);
+template void launch_dense_inv<float>(hipStream_t, int32_t, const int32_t*, const int64_t*, const float*, float*);
 
 // ---------------------------------------------------------------------------
 // Sparse blocks: KLU scale=1 row sums, in a fixed order per row.
@@ -1238,82 +1245,45 @@ __device__ __forceinline__ double readlane_f64(double v, int src) {
   return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
 }
 
-// Stages an s x s column-major factor into LDS with eight loads in flight per
-// lane (the copy is off the dependency chain only if it does not serialize:
-// loads are unconditional with a clamped index, since a conditional load
-// becomes a branch that waits for it).
-template <typename T>
-__device__ __forceinline__ void stage_factor(double* __restrict__ fs, const T* __restrict__ f, int ss, int lane) {
-  for (int base = 0; base < ss; base += 8 * kWave) {
-    T v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int idx = base + u * kWave + lane;
-      v[u] = f[idx < ss ? idx : ss - 1];
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int idx = base + u * kWave + lane;
-      if (idx < ss) fs[idx] = static_cast<double>(v[u]);
-    }
-  }
-}
-
 // Persistent solve of one segment of dense diagonal blocks (a segment is a
 // maximal run of dense blocks between sparse blocks in the solve order). Each
 // workgroup (one wavefront) claims block tickets in order (a topological order
-// of the block DAG: solve level, then descending block id) and solves block b:
-// rhs = bperm - A_off * xperm (ubtf:2269-2286), then the pivot gather and the
-// unit-L and U solves (ubtf:2071-2097), applied as products with the block's
-// explicit inverse factors (lu64/lu32 hold U^{-1}, linv64/linv32 L^{-1}, both
+// of the block DAG: solve level, then the longest dependent chain first) and
+// solves block b: rhs = bperm - A_off * xperm (ubtf:2269-2286), then the pivot
+// gather and the unit-L and U solves (ubtf:2071-2097), applied as one product
+// with the block's explicit inverse M = U^{-1} L^{-1} (minv64/minv32,
 // column-major s x s at the block's factor offset). x values are published
 // values (kXSentinel protocol): everything that does not need other blocks' x
-// (factor to LDS, row gather, the first off-diagonal entries, the next ticket)
-// is issued first, then the block polls exactly the x entries its rows use.
+// (the rows of M into registers, row gather, the first off-diagonal entries,
+// the next ticket) is issued first, then the block polls exactly the x entries
+// its rows use.
 // Deadlock-free: the lowest unfinished ticket is always the current block of
 // some workgroup (a claimed, unstarted ticket is higher than its holder's
 // current one), and every x entry it reads belongs to a lower ticket.
-// The two products of a block solve over the first N columns (N >= s; entries
-// past s are zero on one side of each product): y = L^{-1} v from the rows in
-// registers, then x = U^{-1} y from the LDS tile, the input vector broadcast
-// from LDS at wave-uniform addresses. No masks and no per-step branches.
+// x = M v over the first N columns (N >= s; entries of M past s are zero): the
+// rows of M in registers, the input vector broadcast from LDS at wave-uniform
+// addresses. No masks and no per-step branches.
 template <int N, int K>
-__device__ __forceinline__ void block_inverse_products(const double (&rowl)[64], const double* __restrict__ fs,
-                                                       int s, int li, double* __restrict__ vb, double (&v)[K],
-                                                       int lane) {
+__device__ __forceinline__ void block_inverse_product(const double (&rowm)[64], double* __restrict__ vb,
+                                                      double (&v)[K], int lane) {
 #pragma unroll
   for (int c = 0; c < K; ++c) vb[c * 64 + lane] = v[c];
-  __syncthreads();
-  double y[K];
-#pragma unroll
-  for (int c = 0; c < K; ++c) y[c] = 0.0;
-#pragma unroll
-  for (int j = 0; j < N; ++j)
-#pragma unroll
-    for (int c = 0; c < K; ++c) y[c] = __builtin_fma(rowl[j], vb[c * 64 + j], y[c]);
-  __syncthreads();
-#pragma unroll
-  for (int c = 0; c < K; ++c) vb[c * 64 + lane] = y[c];
   __syncthreads();
 #pragma unroll
   for (int c = 0; c < K; ++c) v[c] = 0.0;
 #pragma unroll
-  for (int j = 0; j < N; ++j) {
-    const double u = fs[(j < s ? j : s - 1) * s + li];  // past s: a finite entry times a zero
+  for (int j = 0; j < N; ++j)
 #pragma unroll
-    for (int c = 0; c < K; ++c) v[c] = __builtin_fma(u, vb[c * 64 + j], v[c]);
-  }
+    for (int c = 0; c < K; ++c) v[c] = __builtin_fma(rowm[j], vb[c * 64 + j], v[c]);
 }
 
 template <int K>
 __global__ __launch_bounds__(64) void k_dense_seg_solve(
     int32_t ntasks, const int32_t* __restrict__ tasks, const int32_t* __restrict__ R,
-    const int64_t* __restrict__ lu0_of_block, const double* __restrict__ lu64, const float* __restrict__ lu32,
-    const double* __restrict__ linv64, const float* __restrict__ linv32,
+    const int64_t* __restrict__ lu0_of_block, const double* __restrict__ minv64, const float* __restrict__ minv32,
     const int32_t* __restrict__ prec_of_block, const int32_t* __restrict__ gsrc, const int32_t* __restrict__ off_ptr,
     const int32_t* __restrict__ off_col, const double* __restrict__ off_val, int32_t* __restrict__ ticket,
     const double* __restrict__ bperm, double* __restrict__ xperm, int64_t ldv, int32_t* __restrict__ err) {
-  __shared__ double fs[64 * 64];
   __shared__ double vb[K * 64];
   constexpr int kPre = 4;
   const int lane = threadIdx.x;
@@ -1343,27 +1313,24 @@ __global__ __launch_bounds__(64) void k_dense_seg_solve(
       oc[u] = ok ? off_col[p0 + u] : -1;
       ov[u] = ok ? off_val[p0 + u] : 0.0;
     }
-    // U^{-1} into LDS; each lane's row of L^{-1} straight into registers
-    // (clamped, unconditional loads; zero past s and on lanes past s).
+    // Each lane's row of M straight into registers (clamped, unconditional
+    // loads; zero past s and on lanes past s), read in the block's precision.
     const int li = lane < s ? lane : s - 1;
     const int64_t o = lu0_of_block[b];
-    double rowl[64];
+    double rowm[64];
     if (__builtin_amdgcn_readfirstlane(prec_of_block[b]) != 0) {
-      stage_factor(fs, lu32 + o, s * s, lane);
 #pragma unroll
       for (int j = 0; j < 64; ++j) {
-        const double l = static_cast<double>(linv32[o + (j < s ? j : s - 1) * s + li]);
-        rowl[j] = (j < s && act) ? l : 0.0;
+        const double m = static_cast<double>(minv32[o + (j < s ? j : s - 1) * s + li]);
+        rowm[j] = (j < s && act) ? m : 0.0;
       }
     } else {
-      stage_factor(fs, lu64 + o, s * s, lane);
 #pragma unroll
       for (int j = 0; j < 64; ++j) {
-        const double l = linv64[o + (j < s ? j : s - 1) * s + li];
-        rowl[j] = (j < s && act) ? l : 0.0;
+        const double m = minv64[o + (j < s ? j : s - 1) * s + li];
+        rowm[j] = (j < s && act) ? m : 0.0;
       }
     }
-    __syncthreads();
     // --- published x of other blocks: all first polls in flight together ---
     uint64_t xu[kPre][K];
 #pragma unroll
@@ -1396,38 +1363,38 @@ __global__ __launch_bounds__(64) void k_dense_seg_solve(
             acc[c] = acc[c] - ov[u] * finish_wait_x(xperm + c * ldv + oc[u], xu[u][c], late);
         }
     }
-    // Row gather (the block's pivoting), then y = L^{-1} (P rhs) and
-    // x = U^{-1} y as matrix-vector products: every step reads the finished
-    // input vector, so there is no dependency chain across the block.
+    // Row gather (the block's pivoting), then x = M (P rhs) as a matrix-vector
+    // product: every step reads the finished input vector, so there is no
+    // dependency chain across the block.
 #pragma unroll
     for (int c = 0; c < K; ++c) {
       acc[c] = __shfl(acc[c], src);
       if (!act) acc[c] = 0.0;
     }
-    if (s <= 16) block_inverse_products<16, K>(rowl, fs, s, li, vb, acc, lane);
-    else if (s <= 32) block_inverse_products<32, K>(rowl, fs, s, li, vb, acc, lane);
-    else block_inverse_products<64, K>(rowl, fs, s, li, vb, acc, lane);
+    if (s <= 16) block_inverse_product<16, K>(rowm, vb, acc, lane);
+    else if (s <= 32) block_inverse_product<32, K>(rowm, vb, acc, lane);
+    else block_inverse_product<64, K>(rowm, vb, acc, lane);
     if (act) {
 #pragma unroll
       for (int c = 0; c < K; ++c) st_x_relaxed(xperm + c * ldv + r, acc[c]);
     }
-    __syncthreads();  // fs is restaged by the next block
+    __syncthreads();  // vb is rewritten by the next block
     t = uniform_from_lane0(tn);
   }
   if (late) atomicOr(err, 1);
 }
 
 void launch_dense_seg_solve(hipStream_t st, int32_t kc, int32_t ntasks, const int32_t* tasks, const int32_t* R,
-                            const int64_t* lu0_of_block, const double* lu64, const float* lu32,
-                            const double* linv64, const float* linv32, const int32_t* prec_of_block,
+                            const int64_t* lu0_of_block, const double* minv64, const float* minv32,
+                            const int32_t* prec_of_block,
                             const int32_t* gsrc, const int32_t* off_ptr, const int32_t* off_col,
                             const double* off_val, int32_t* ticket, const double* bperm,
                             double* xperm, int64_t ldv, int32_t* err, int32_t grid) {
   if (ntasks <= 0) return;
   const int g = ntasks < grid ? ntasks : grid;  // persistent single-wave workgroups
 #define BDLU_SEG(KK)                                                                                              \
-  hipLaunchKernelGGL(k_dense_seg_solve<KK>, dim3(g), dim3(64), 0, st, ntasks, tasks, R, lu0_of_block, lu64, lu32, \
-                     linv64, linv32, prec_of_block, gsrc, off_ptr, off_col, off_val, ticket, bperm, xperm, ldv, err)
+  hipLaunchKernelGGL(k_dense_seg_solve<KK>, dim3(g), dim3(64), 0, st, ntasks, tasks, R, lu0_of_block, minv64,   \
+                     minv32, prec_of_block, gsrc, off_ptr, off_col, off_val, ticket, bperm, xperm, ldv, err)
   switch (kc) {
     case 1: BDLU_SEG(1); break;
     case 2: BDLU_SEG(2); break;
