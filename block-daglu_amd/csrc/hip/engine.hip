@@ -87,6 +87,8 @@ struct SparseDev {
   double* y = nullptr;
   double* inv64 = nullptr;  // 2*64*64 per panel: the tail's diagonal-block inverses
   float* inv32 = nullptr;
+  double* cpl64 = nullptr;  // same layout: the band couplings for the tail solve
+  float* cpl32 = nullptr;
   double* ty = nullptr;  // m: tail forward-solve output (published values)
   bdk::SpTailLong tail_long;  // tail columns with a long sparse reach
   std::vector<bdk::SpColsLong> cols_long;  // per refactor level: columns with a long reach
@@ -576,6 +578,8 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
     const std::size_t npan = (static_cast<std::size_t>(sd.m) + 63) / 64;
     S.inv64 = A.get<double>(2 * 64 * 64 * std::max<std::size_t>(npan, 1));
     S.inv32 = A.get<float>(2 * 64 * 64 * std::max<std::size_t>(npan, 1));
+    S.cpl64 = A.get<double>(2 * 64 * 64 * std::max<std::size_t>(npan, 1));
+    S.cpl32 = A.get<float>(2 * 64 * 64 * std::max<std::size_t>(npan, 1));
     S.ty = A.get<double>(static_cast<std::size_t>(std::max(sd.m, 1)) * kMaxRhs);
     S.status = A.get<int32_t>(1);
     P.sp.push_back(std::move(S));
@@ -716,7 +720,7 @@ void block_solve_launches(Plan& P, hipStream_t st, const double* b, double* x, i
     const int64_t ldy = S.s, ldt = std::max(S.m, 1);
     bdk::launch_sp_gather(st, S.s, S.start, kt, S.rowperm, S.scale ? S.rs : nullptr, P.off_ptr, P.off_col, P.off_val,
                           P.bperm, P.xperm, ld, S.y, ldy);
-    auto run = [&](auto* Lx, auto* Ux, auto* Ud, auto* Sm, auto* inv) {
+    auto run = [&](auto* Lx, auto* Ux, auto* Ud, auto* Sm, auto* inv, auto* cpl) {
       using T = std::remove_pointer_t<decltype(Lx)>;
       for (std::size_t l = 0; l + 1 < S.fl_ptr.size(); ++l)
         bdk::launch_sp_rows<T>(st, kt, S.fl_ptr[l + 1] - S.fl_ptr[l], S.fl_rows + S.fl_ptr[l], S.lr_ptr, S.lr_col,
@@ -724,17 +728,17 @@ void block_solve_launches(Plan& P, hipStream_t st, const double* b, double* x, i
       bdk::launch_sp_rows<T>(st, kt, S.m, S.tl_rows, S.lr_ptr, S.lr_col, S.lr_vidx, Lx, nullptr, S.y, ldy);
       // Tail: forward into ty, backward back into y (each output starts unpublished).
       for (int32_t c = 0; c < kt; ++c) bdk::launch_fill_sentinel(st, S.ty + c * ldt, S.m);
-      bdk::launch_tail_trsv<T>(st, kt, S.m, Sm, S.ldt, inv, S.y + S.k0, ldy, S.ty, ldt, false, P.tickets + tix,
+      bdk::launch_tail_trsv<T>(st, kt, S.m, Sm, S.ldt, inv, cpl, S.y + S.k0, ldy, S.ty, ldt, false, P.tickets + tix,
                                P.err);
       for (int32_t c = 0; c < kt; ++c) bdk::launch_fill_sentinel(st, S.y + c * ldy + S.k0, S.m);
-      bdk::launch_tail_trsv<T>(st, kt, S.m, Sm, S.ldt, inv, S.ty, ldt, S.y + S.k0, ldy, true, P.tickets + tix + 1,
+      bdk::launch_tail_trsv<T>(st, kt, S.m, Sm, S.ldt, inv, cpl, S.ty, ldt, S.y + S.k0, ldy, true, P.tickets + tix + 1,
                                P.err);
       for (std::size_t l = 0; l + 1 < S.bl_ptr.size(); ++l)
         bdk::launch_sp_rows<T>(st, kt, S.bl_ptr[l + 1] - S.bl_ptr[l], S.bl_rows + S.bl_ptr[l], S.ur_ptr, S.ur_col,
                                S.ur_vidx, Ux, Ud, S.y, ldy);
     };
-    if (S.prec == 0) run(S.Lx64, S.Ux64, S.Ud64, S.S64, S.inv64);
-    else run(S.Lx32, S.Ux32, S.Ud32, S.S32, S.inv32);
+    if (S.prec == 0) run(S.Lx64, S.Ux64, S.Ud64, S.S64, S.inv64, S.cpl64);
+    else run(S.Lx32, S.Ux32, S.Ud32, S.S32, S.inv32, S.cpl32);
     tix += 2;
     bdk::launch_sp_scatter_x(st, S.s, S.start, kt, S.q, S.y, ldy, P.xperm, ld);
   }
@@ -822,8 +826,9 @@ void factor_sparse_cols(Plan& P, SparseDev& S, hipStream_t st, T* Lx, T* Ux, T* 
   if (pe) pe->mark(BDLU_PROF_SPARSE, st);
 }
 
-// LU of the dense tail (touches only the block's own tail, inverse scratch and
-// status, so tails of different sparse blocks may run concurrently).
+// LU of the dense tail, then its band couplings for the solve (touches only the
+// block's own tail, inverses, couplings and status, so tails of different
+// sparse blocks may run concurrently).
 template <typename T>
 void factor_sparse_tail(Plan& P, SparseDev& S, hipStream_t st, T* Sm, ProfEvents* pe, double growth_limit,
                         bool lookahead = false) {
@@ -837,6 +842,8 @@ void factor_sparse_tail(Plan& P, SparseDev& S, hipStream_t st, T* Sm, ProfEvents
   T* inv = sizeof(T) == 8 ? reinterpret_cast<T*>(S.inv64) : reinterpret_cast<T*>(S.inv32);
   bdk::launch_tail_lu<T>(st, S.m, Sm, S.ldt, growth_limit, S.status, inv, pe ? &tt : nullptr,
                          P.sched.small_tile_below, lookahead);
+  T* cpl = sizeof(T) == 8 ? reinterpret_cast<T*>(S.cpl64) : reinterpret_cast<T*>(S.cpl32);
+  bdk::launch_tail_coupling<T>(st, S.m, Sm, S.ldt, inv, cpl);
 }
 
 }  // namespace

@@ -123,12 +123,18 @@ void launch_sp_gather(hipStream_t st, int32_t s, int32_t start, int32_t kc, cons
 template <typename T>
 void launch_sp_rows(hipStream_t st, int32_t kc, int32_t nrows, const int32_t* rows, const int32_t* ptr,
                     const int32_t* col, const int32_t* vidx, const T* vals, const T* diag, double* y, int64_t ldy);
-// Dense tail triangular solve over 64-row bands; inv holds the diagonal tiles'
-// inverses as launch_tail_lu leaves them. ticket starts at 0; yout starts as the
-// unpublished pattern and does not alias yin.
+// Couplings of the dense tail's 64-row bands to their neighbours in solve
+// order (D_w^{-1} times the sub- or superdiagonal tile), for launch_tail_trsv;
+// g has the layout of inv (2*64*64 per band).
 template <typename T>
-void launch_tail_trsv(hipStream_t st, int32_t kc, int32_t m, const T* S, int64_t lds, const T* inv, const double* yin,
-                      int64_t ldi, double* yout, int64_t ldo, bool upper, int32_t* ticket, int32_t* err);
+void launch_tail_coupling(hipStream_t st, int32_t m, const T* S, int64_t lds, const T* inv, T* g);
+// Dense tail triangular solve over 64-row bands; inv holds the diagonal tiles'
+// inverses as launch_tail_lu leaves them, g the couplings. ticket starts at 0;
+// yout starts as the unpublished pattern and does not alias yin.
+template <typename T>
+void launch_tail_trsv(hipStream_t st, int32_t kc, int32_t m, const T* S, int64_t lds, const T* inv, const T* g,
+                      const double* yin, int64_t ldi, double* yout, int64_t ldo, bool upper, int32_t* ticket,
+                      int32_t* err);
 void launch_sp_scatter_x(hipStream_t st, int32_t s, int32_t start, int32_t kc, const int32_t* q, const double* x,
                          int64_t ldx, double* xperm, int64_t ldv);
 

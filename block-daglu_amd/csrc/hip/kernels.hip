@@ -1479,26 +1479,80 @@ template void launch_sp_rows<double>(hipStream_t, int32_t, int32_t, const int32_
 template void launch_sp_rows<float>(hipStream_t, int32_t, int32_t, const int32_t*, const int32_t*, const int32_t*,
                                     const int32_t*, const float*, const float*, double*, int64_t);
 
+// Coupling of each 64-row band of the dense tail to its neighbour in solve
+// order, for the tail solve: G_w = L_ww^{-1} L_{w,w-1} (forward, bands w >= 1)
+// and H_w = U_ww^{-1} U_{w,w+1} (backward, bands w < np-1), 64 x 64 row-major
+// at g + (2 w + kind) * 64 * 64 (kind 0 forward, 1 backward), from the
+// diagonal inverses the tail LU leaves in inv (same layout). One 256-thread
+// workgroup per product: thread (i, quarter) forms row i's 16 columns of the
+// quarter, k ascending; pads (rows or columns past the matrix) are zero.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tail_coupling(int32_t m, const T* __restrict__ S, int64_t lds,
+                                                       const T* __restrict__ inv, T* __restrict__ g) {
+  constexpr int kL = kNb + 1;
+  __shared__ T Ds[kNb * kL];  // Ds[i*65 + k] = D^{-1}(i, k)
+  __shared__ T Ts[kNb * kL];  // Ts[j*65 + k] = T(k, j)
+  const int32_t np = (m + kNb - 1) / kNb;
+  const int kind = static_cast<int>(blockIdx.x) / np;
+  const int32_t w = static_cast<int32_t>(blockIdx.x) % np;
+  if (kind == 0 ? w == 0 : w == np - 1) return;  // no neighbour
+  const int32_t r0 = w * kNb;
+  const int32_t rows = (m - r0) < kNb ? (m - r0) : kNb;
+  const int32_t c0 = (kind == 0 ? w - 1 : w + 1) * kNb;
+  const int32_t cols = (m - c0) < kNb ? (m - c0) : kNb;
+  const T* dinv = inv + (int64_t(w) * 2 + kind) * kNb * kNb;
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < kNb * kNb; idx += 256) {
+    Ds[(idx >> 6) * kL + (idx & 63)] = dinv[idx];
+    const int j = idx >> 6, k = idx & 63;  // T(k, j) = S(r0 + k, c0 + j), column-major in S
+    const T v = S[int64_t(c0 + (j < cols ? j : cols - 1)) * lds + r0 + (k < rows ? k : rows - 1)];
+    Ts[j * kL + k] = (j < cols && k < rows) ? v : T(0);
+  }
+  __syncthreads();
+  const int i = tid & 63, jq = tid >> 6;
+  T acc[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) acc[u] = T(0);
+#pragma unroll 8
+  for (int k = 0; k < kNb; ++k) {
+    const T a = Ds[i * kL + k];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) acc[u] = tfma(a, Ts[(jq * 16 + u) * kL + k], acc[u]);
+  }
+  T* dst = g + (int64_t(w) * 2 + kind) * kNb * kNb + i * kNb + jq * 16;
+#pragma unroll
+  for (int u = 0; u < 16; ++u) dst[u] = acc[u];
+}
+template <typename T>
+void launch_tail_coupling(hipStream_t st, int32_t m, const T* S, int64_t lds, const T* inv, T* g) {
+  if (m <= kNb) return;  // one band: nothing to couple
+  const int32_t np = (m + kNb - 1) / kNb;
+  hipLaunchKernelGGL(k_tail_coupling<T>, dim3(2 * np), dim3(256), 0, st, m, S, lds, inv, g);
+}
+template void launch_tail_coupling<double>(hipStream_t, int32_t, const double*, int64_t, const double*, double*);
+template void launch_tail_coupling<float>(hipStream_t, int32_t, const float*, int64_t, const float*, float*);
+
 // Dense tail triangular solve (unit-lower forward or upper backward) on the
-// column-major m x m factor and the inverses of its 64 x 64 diagonal tiles (inv:
-// per band, L^{-1} then U^{-1}, each row-major with an identity pad), one
-// workgroup of four wavefronts per 64-row band,
-// bands claimed in dependency order by ticket. The tiles of the band's earlier
-// bands (in publication order) are dealt round-robin to the four waves; each
-// wave keeps the next half tile's loads in flight while it takes the current
-// one, polls the 64 values of the tile's band as they are published
-// (kXSentinel protocol on yout) and broadcasts them through LDS. The four
-// partial sums are combined in a fixed order, then the first wave applies the
-// inverse of the band's diagonal tile (staged in LDS before any poll) as a
-// matrix-vector product and publishes its 64 values. yin and yout must not alias; yout starts filled
-// with kXSentinel.
+// column-major m x m factor, the inverses D_w^{-1} of its 64 x 64 diagonal
+// tiles (inv) and the band couplings G_w (g, from launch_tail_coupling), one
+// workgroup of four wavefronts per 64-row band, bands claimed in dependency
+// order by ticket. For band w with neighbour band v (w-1 forward, w+1
+// backward): x_w = D_w^{-1} (y_w - sum over the older bands) - G_w x_v, so only
+// the last product waits for the neighbour's values. Waves 1-3 take the older
+// bands' tiles round-robin (each keeping the next tile piece in flight, polling
+// the 64 values of the tile's band as they are published, kXSentinel protocol
+// on yout, and broadcasting them through LDS); their partial sums are combined
+// in a fixed order; wave 0 applies D_w^{-1} (staged in LDS) and then, once the
+// neighbour's values are visible, G_w (its row of G_w in registers, loaded at
+// the start) and publishes the band's 64 values. yin and yout must not alias;
+// yout starts filled with kXSentinel.
 template <typename T, int K>
 __global__ __launch_bounds__(256) void k_tail_trsv(int32_t m, const T* __restrict__ S, int64_t lds,
-                                                   const T* __restrict__ inv, const double* __restrict__ yin,
-                                                   int64_t ldi,
+                                                   const T* __restrict__ inv, const T* __restrict__ g,
+                                                   const double* __restrict__ yin, int64_t ldi,
                                                    double* __restrict__ yout, int64_t ldo, int upper,
                                                    int32_t* __restrict__ ticket, int32_t* __restrict__ err) {
-  constexpr int kW = 4;         // wavefronts per band
+  constexpr int kW = 4;         // wavefronts per band; waves 1..kW-1 take the older tiles
   constexpr int kDL = kNb + 1;  // LDS row length of the diagonal inverse (padded against bank conflicts)
   __shared__ double xs[kW][K][kNb];
   __shared__ double part[kW][K][kNb];
@@ -1521,61 +1575,73 @@ __global__ __launch_bounds__(256) void k_tail_trsv(int32_t m, const T* __restric
   // (a per-element condition becomes a branch that waits on each load). Lanes
   // past the band never store; columns past the matrix meet x = 0.
   const int32_t rowc = live ? row : r0;
-  const T* tinv = inv + (int64_t(w) * 2 + (upper ? 1 : 0)) * kNb * kNb;
+  const int kind = upper ? 1 : 0;
+  const T* tinv = inv + (int64_t(w) * 2 + kind) * kNb * kNb;
   for (int idx = tid; idx < kNb * kNb; idx += kW * kNb) dt[(idx >> 6) * kDL + (idx & (kNb - 1))] = tinv[idx];
   const int32_t nd = upper ? (np - 1 - w) : w;  // earlier bands
+  const int32_t nold = nd > 0 ? nd - 1 : 0;     // all but the neighbour
   bool late = false;
   double acc[K];
 #pragma unroll
   for (int c = 0; c < K; ++c) acc[c] = 0.0;
-  // Tiles are loaded in pieces of kP columns (fewer with more right-hand
-  // sides, for registers), two pieces in flight.
-  constexpr int kP = K >= 4 ? 16 : 32;
-  constexpr int kPieces = kNb / kP;
-  T buf[2][kP];
-  auto load_piece = [&](T(&v)[kP], int32_t q, int h) {
-    const int32_t j = upper ? (np - 1 - q) : q;
-    const int32_t cols = (m - j * kNb) < kNb ? (m - j * kNb) : kNb;
+  if (wv != 0) {
+    // Tiles are loaded in pieces of kP columns (fewer with more right-hand
+    // sides, for registers), two pieces in flight.
+    constexpr int kP = K >= 4 ? 16 : 32;
+    constexpr int kPieces = kNb / kP;
+    T buf[2][kP];
+    auto load_piece = [&](T(&v)[kP], int32_t q, int h) {
+      const int32_t j = upper ? (np - 1 - q) : q;
+      const int32_t cols = (m - j * kNb) < kNb ? (m - j * kNb) : kNb;
 #pragma unroll
-    for (int c = 0; c < kP; ++c) {
-      const int cc = h * kP + c;
-      v[c] = S[int64_t(j * kNb + (cc < cols ? cc : cols - 1)) * lds + rowc];
+      for (int c = 0; c < kP; ++c) {
+        const int cc = h * kP + c;
+        v[c] = S[int64_t(j * kNb + (cc < cols ? cc : cols - 1)) * lds + rowc];
+      }
+    };
+    int32_t q = wv - 1;
+    if (q < nold) load_piece(buf[0], q, 0);
+    while (q < nold) {
+      const int32_t j = upper ? (np - 1 - q) : q;
+      const int32_t cols = (m - j * kNb) < kNb ? (m - j * kNb) : kNb;
+      const int32_t qn = q + (kW - 1);
+      load_piece(buf[1], q, 1);
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        xs[wv][k][lane] = lane < cols ? wait_x(yout + k * ldo + j * kNb + lane, late) : 0.0;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+      for (int h = 0; h < kPieces; ++h) {
+#pragma unroll
+        for (int c = 0; c < kP; ++c)
+#pragma unroll
+          for (int k = 0; k < K; ++k)
+            acc[k] = __builtin_fma(static_cast<double>(buf[h & 1][c]), xs[wv][k][h * kP + c], acc[k]);
+        // The buffer just consumed takes the piece after the one in flight
+        // (unconditionally: after the last tile it reloads this one, which keeps
+        // the loop free of branches around loads and their register copies).
+        if (h + 2 < kPieces)
+          load_piece(buf[h & 1], q, h + 2);
+        else
+          load_piece(buf[h & 1], qn < nold ? qn : q, h + 2 - kPieces);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      q = qn;
     }
-  };
-  int32_t q = wv;
-  if (q < nd) load_piece(buf[0], q, 0);
-  while (q < nd) {
-    const int32_t j = upper ? (np - 1 - q) : q;
-    const int32_t cols = (m - j * kNb) < kNb ? (m - j * kNb) : kNb;
-    const int32_t qn = q + kW;
-    load_piece(buf[1], q, 1);
 #pragma unroll
-    for (int k = 0; k < K; ++k) xs[wv][k][lane] = lane < cols ? wait_x(yout + k * ldo + j * kNb + lane, late) : 0.0;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-    for (int h = 0; h < kPieces; ++h) {
-#pragma unroll
-      for (int c = 0; c < kP; ++c)
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-          acc[k] = __builtin_fma(static_cast<double>(buf[h & 1][c]), xs[wv][k][h * kP + c], acc[k]);
-      // The buffer just consumed takes the piece after the one in flight
-      // (unconditionally: after the last tile it reloads this one, which keeps
-      // the loop free of branches around loads and their register copies).
-      if (h + 2 < kPieces)
-        load_piece(buf[h & 1], q, h + 2);
-      else
-        load_piece(buf[h & 1], qn < nd ? qn : q, h + 2 - kPieces);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    q = qn;
+    for (int k = 0; k < K; ++k) part[wv][k][lane] = acc[k];
   }
+  // Wave 0: its row of G_w (the neighbour coupling) into registers meanwhile.
+  T grow[kNb];
+  if (wv == 0 && nd > 0) {
+    const T* gr = g + (int64_t(w) * 2 + kind) * kNb * kNb + (live ? lane : 0) * kNb;
 #pragma unroll
-  for (int k = 0; k < K; ++k) part[wv][k][lane] = acc[k];
+    for (int c = 0; c < kNb; ++c) grow[c] = gr[c];
+  }
   __syncthreads();
   if (wv != 0) {
     if (late) atomicOr(err, 2);
@@ -1583,17 +1649,15 @@ __global__ __launch_bounds__(256) void k_tail_trsv(int32_t m, const T* __restric
   }
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const double sum = (part[0][k][lane] + part[1][k][lane]) + (part[2][k][lane] + part[3][k][lane]);
-    acc[k] = (live ? yin[k * ldi + row] : 0.0) - sum;
+    const double sum = (part[1][k][lane] + part[2][k][lane]) + part[3][k][lane];
+    xs[0][k][lane] = live ? yin[k * ldi + row] - sum : 0.0;
   }
-  // x = inverse(diagonal tile) * (yin - sum): the right-hand side broadcast
-  // through LDS, row `lane` of the inverse at dt[lane * kDL + c]. Rows past the
-  // band take a zero right-hand side (the inverse's pad is the identity).
-#pragma unroll
-  for (int k = 0; k < K; ++k) xs[0][k][lane] = live ? acc[k] : 0.0;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // D_w^{-1} (yin - sum): row `lane` of the inverse at dt[lane * kDL + c], the
+  // right-hand side broadcast through LDS. Rows past the band take a zero
+  // right-hand side (the inverse's pad is the identity).
 #pragma unroll
   for (int k = 0; k < K; ++k) acc[k] = 0.0;
 #pragma unroll
@@ -1602,6 +1666,25 @@ __global__ __launch_bounds__(256) void k_tail_trsv(int32_t m, const T* __restric
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = __builtin_fma(a, xs[0][k][c], acc[k]);
   }
+  if (nd > 0) {
+    // - G_w x_v, once the neighbour's values are visible.
+    const int32_t j = upper ? (w + 1) : (w - 1);
+    const int32_t cols = (m - j * kNb) < kNb ? (m - j * kNb) : kNb;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int k = 0; k < K; ++k) xs[0][k][lane] = lane < cols ? wait_x(yout + k * ldo + j * kNb + lane, late) : 0.0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int c = 0; c < kNb; ++c) {
+      const double gc = static_cast<double>(grow[c]);
+#pragma unroll
+      for (int k = 0; k < K; ++k) acc[k] = __builtin_fma(-gc, xs[0][k][c], acc[k]);
+    }
+  }
   if (live) {
 #pragma unroll
     for (int k = 0; k < K; ++k) st_x_relaxed(yout + k * ldo + row, acc[k]);
@@ -1609,13 +1692,14 @@ __global__ __launch_bounds__(256) void k_tail_trsv(int32_t m, const T* __restric
   if (late) atomicOr(err, 2);
 }
 template <typename T>
-void launch_tail_trsv(hipStream_t st, int32_t kc, int32_t m, const T* S, int64_t lds, const T* inv, const double* yin,
-                      int64_t ldi, double* yout, int64_t ldo, bool upper, int32_t* ticket, int32_t* err) {
+void launch_tail_trsv(hipStream_t st, int32_t kc, int32_t m, const T* S, int64_t lds, const T* inv, const T* g,
+                      const double* yin, int64_t ldi, double* yout, int64_t ldo, bool upper, int32_t* ticket,
+                      int32_t* err) {
   if (m <= 0) return;
-  const dim3 g((m + kNb - 1) / kNb), b(256);
+  const dim3 gr((m + kNb - 1) / kNb), b(256);
   const int up = upper ? 1 : 0;
 #define BDLU_TRSV(KK)                                                                                        \
-  hipLaunchKernelGGL((k_tail_trsv<T, KK>), g, b, 0, st, m, S, lds, inv, yin, ldi, yout, ldo, up, ticket, err)
+  hipLaunchKernelGGL((k_tail_trsv<T, KK>), gr, b, 0, st, m, S, lds, inv, g, yin, ldi, yout, ldo, up, ticket, err)
   switch (kc) {
     case 1: BDLU_TRSV(1); break;
     case 2: BDLU_TRSV(2); break;
@@ -1625,9 +1709,11 @@ void launch_tail_trsv(hipStream_t st, int32_t kc, int32_t m, const T* S, int64_t
 #undef BDLU_TRSV
 }
 template void launch_tail_trsv<double>(hipStream_t, int32_t, int32_t, const double*, int64_t, const double*,
-                                       const double*, int64_t, double*, int64_t, bool, int32_t*, int32_t*);
+                                       const double*, const double*, int64_t, double*, int64_t, bool, int32_t*,
+                                       int32_t*);
 template void launch_tail_trsv<float>(hipStream_t, int32_t, int32_t, const float*, int64_t, const float*,
-                                      const double*, int64_t, double*, int64_t, bool, int32_t*, int32_t*);
+                                      const float*, const double*, int64_t, double*, int64_t, bool, int32_t*,
+                                      int32_t*);
 
 __global__ void k_sp_scatter_x(int32_t s, int32_t start, int32_t kc, const int32_t* __restrict__ q,
                                const double* __restrict__ x, int64_t ldx, double* __restrict__ xperm, int64_t ldv) {
