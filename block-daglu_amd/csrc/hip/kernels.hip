@@ -1277,23 +1277,31 @@ __device__ __forceinline__ void block_inverse_product(const double (&rowm)[64], 
     for (int c = 0; c < K; ++c) v[c] = __builtin_fma(rowm[j], vb[c * 64 + j], v[c]);
 }
 
-template <int K>
-__global__ __launch_bounds__(64) void k_dense_seg_solve(
+// NW right-hand sides, one wavefront each: the workgroup claims a ticket (thread
+// 0, shared through LDS), and wave c solves the block for column c, so a block
+// costs what it costs for one column however many there are.
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_dense_seg_solve(
     int32_t ntasks, const int32_t* __restrict__ tasks, const int32_t* __restrict__ R,
     const int64_t* __restrict__ lu0_of_block, const double* __restrict__ minv64, const float* __restrict__ minv32,
     const int32_t* __restrict__ prec_of_block, const int32_t* __restrict__ gsrc, const int32_t* __restrict__ off_ptr,
     const int32_t* __restrict__ off_col, const double* __restrict__ off_val, int32_t* __restrict__ ticket,
     const double* __restrict__ bperm, double* __restrict__ xperm, int64_t ldv, int32_t* __restrict__ err) {
-  __shared__ double vb[K * 64];
+  __shared__ double vb[NW * 64];
+  __shared__ int32_t tk_sh[2];  // ticket slots, alternating so one barrier per block suffices
   constexpr int kPre = 4;
-  const int lane = threadIdx.x;
-  int32_t tk = 0;
-  if (lane == 0) tk = atomicAdd(ticket, 1);
-  int32_t t = uniform_from_lane0(tk);
+  const int lane = threadIdx.x & 63;
+  const int col = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const double* bcol = bperm + col * ldv;
+  double* xcol = xperm + col * ldv;
+  double* vbw = vb + col * 64;
+  if (threadIdx.x == 0) tk_sh[0] = atomicAdd(ticket, 1);
+  __syncthreads();
+  int32_t t = __builtin_amdgcn_readfirstlane(tk_sh[0]);
+  int slot = 1;
   bool late = false;
   while (t < ntasks) {
-    int32_t tn = 0;
-    if (lane == 0) tn = atomicAdd(ticket, 1);
+    if (threadIdx.x == 0) tk_sh[slot] = atomicAdd(ticket, 1);  // the next ticket, claimed ahead
     const int32_t b = __builtin_amdgcn_readfirstlane(tasks[t]);
     const int32_t r0 = __builtin_amdgcn_readfirstlane(R[b]);
     const int s = __builtin_amdgcn_readfirstlane(R[b + 1]) - r0;
@@ -1301,9 +1309,7 @@ __global__ __launch_bounds__(64) void k_dense_seg_solve(
     const int32_t r = r0 + lane;
     // --- independent of other blocks ---
     const int32_t src = act ? gsrc[r] : 0;
-    double acc[K];
-#pragma unroll
-    for (int c = 0; c < K; ++c) acc[c] = act ? bperm[c * ldv + r] : 0.0;
+    double acc[1] = {act ? bcol[r] : 0.0};
     const int32_t p0 = act ? off_ptr[r] : 0, p1 = act ? off_ptr[r + 1] : 0;
     int32_t oc[kPre];
     double ov[kPre];
@@ -1331,55 +1337,36 @@ __global__ __launch_bounds__(64) void k_dense_seg_solve(
         rowm[j] = (j < s && act) ? m : 0.0;
       }
     }
-    // --- published x of other blocks: all first polls in flight together ---
-    uint64_t xu[kPre][K];
+    // --- published x of other blocks: all first polls in flight together, then
+    // the rest in groups of kPre ---
+    for (int32_t pg = p0;;) {
+      uint64_t xu[kPre];
 #pragma unroll
-    for (int u = 0; u < kPre; ++u)
+      for (int u = 0; u < kPre; ++u) xu[u] = oc[u] >= 0 ? ld_x_relaxed(xcol + oc[u]) : 0ull;
 #pragma unroll
-      for (int c = 0; c < K; ++c) xu[u][c] = oc[u] >= 0 ? ld_x_relaxed(xperm + c * ldv + oc[u]) : 0ull;
-#pragma unroll
-    for (int u = 0; u < kPre; ++u)
-      if (oc[u] >= 0) {
-#pragma unroll
-        for (int c = 0; c < K; ++c) acc[c] = acc[c] - ov[u] * finish_wait_x(xperm + c * ldv + oc[u], xu[u][c], late);
-      }
-    // The rest in groups of kPre, each group's first polls issued together.
-    for (int32_t pg = p0 + kPre; pg < p1; pg += kPre) {
+      for (int u = 0; u < kPre; ++u)
+        if (oc[u] >= 0) acc[0] = acc[0] - ov[u] * finish_wait_x(xcol + oc[u], xu[u], late);
+      pg += kPre;
+      if (pg >= p1) break;
 #pragma unroll
       for (int u = 0; u < kPre; ++u) {
         const bool ok = pg + u < p1;
         oc[u] = ok ? off_col[pg + u] : -1;
         ov[u] = ok ? off_val[pg + u] : 0.0;
       }
-#pragma unroll
-      for (int u = 0; u < kPre; ++u)
-#pragma unroll
-        for (int c = 0; c < K; ++c) xu[u][c] = oc[u] >= 0 ? ld_x_relaxed(xperm + c * ldv + oc[u]) : 0ull;
-#pragma unroll
-      for (int u = 0; u < kPre; ++u)
-        if (oc[u] >= 0) {
-#pragma unroll
-          for (int c = 0; c < K; ++c)
-            acc[c] = acc[c] - ov[u] * finish_wait_x(xperm + c * ldv + oc[u], xu[u][c], late);
-        }
     }
     // Row gather (the block's pivoting), then x = M (P rhs) as a matrix-vector
     // product: every step reads the finished input vector, so there is no
     // dependency chain across the block.
-#pragma unroll
-    for (int c = 0; c < K; ++c) {
-      acc[c] = __shfl(acc[c], src);
-      if (!act) acc[c] = 0.0;
-    }
-    if (s <= 16) block_inverse_product<16, K>(rowm, vb, acc, lane);
-    else if (s <= 32) block_inverse_product<32, K>(rowm, vb, acc, lane);
-    else block_inverse_product<64, K>(rowm, vb, acc, lane);
-    if (act) {
-#pragma unroll
-      for (int c = 0; c < K; ++c) st_x_relaxed(xperm + c * ldv + r, acc[c]);
-    }
-    __syncthreads();  // vb is rewritten by the next block
-    t = uniform_from_lane0(tn);
+    acc[0] = __shfl(acc[0], src);
+    if (!act) acc[0] = 0.0;
+    if (s <= 16) block_inverse_product<16, 1>(rowm, vbw, acc, lane);
+    else if (s <= 32) block_inverse_product<32, 1>(rowm, vbw, acc, lane);
+    else block_inverse_product<64, 1>(rowm, vbw, acc, lane);
+    if (act) st_x_relaxed(xcol + r, acc[0]);
+    __syncthreads();  // vb is rewritten by the next block; the next ticket is in its slot
+    t = __builtin_amdgcn_readfirstlane(tk_sh[slot]);
+    slot ^= 1;
   }
   if (late) atomicOr(err, 1);
 }
@@ -1391,9 +1378,9 @@ void launch_dense_seg_solve(hipStream_t st, int32_t kc, int32_t ntasks, const in
                             const double* off_val, int32_t* ticket, const double* bperm,
                             double* xperm, int64_t ldv, int32_t* err, int32_t grid) {
   if (ntasks <= 0) return;
-  const int g = ntasks < grid ? ntasks : grid;  // persistent single-wave workgroups
-#define BDLU_SEG(KK)                                                                                              \
-  hipLaunchKernelGGL(k_dense_seg_solve<KK>, dim3(g), dim3(64), 0, st, ntasks, tasks, R, lu0_of_block, minv64,   \
+  const int g = ntasks < grid ? ntasks : grid;  // persistent workgroups, one wavefront per column
+#define BDLU_SEG(KK)                                                                                               \
+  hipLaunchKernelGGL(k_dense_seg_solve<KK>, dim3(g), dim3(64 * KK), 0, st, ntasks, tasks, R, lu0_of_block, minv64, \
                      minv32, prec_of_block, gsrc, off_ptr, off_col, off_val, ticket, bperm, xperm, ldv, err)
   switch (kc) {
     case 1: BDLU_SEG(1); break;
