@@ -1522,8 +1522,9 @@ template void launch_tail_coupling<float>(hipStream_t, int32_t, const float*, in
 // Dense tail triangular solve (unit-lower forward or upper backward) on the
 // column-major m x m factor, the inverses D_w^{-1} of its 64 x 64 diagonal
 // tiles (inv) and the band couplings G_w (g, from launch_tail_coupling), one
-// workgroup of four wavefronts per 64-row band, bands claimed in dependency
-// order by ticket. For band w with neighbour band v (w-1 forward, w+1
+// workgroup of four wavefronts per 64-row band and right-hand side, claimed by
+// ticket in dependency order (ticket t: band t / kc in solve order, column
+// t % kc; each column is its own solve). For band w with neighbour band v (w-1 forward, w+1
 // backward): x_w = D_w^{-1} (y_w - sum over the older bands) - G_w x_v, so only
 // the last product waits for the neighbour's values. Waves 1-3 take the older
 // bands' tiles round-robin (each keeping the next tile piece in flight, polling
@@ -1533,12 +1534,13 @@ template void launch_tail_coupling<float>(hipStream_t, int32_t, const float*, in
 // neighbour's values are visible, G_w (its row of G_w in registers, loaded at
 // the start) and publishes the band's 64 values. yin and yout must not alias;
 // yout starts filled with kXSentinel.
-template <typename T, int K>
-__global__ __launch_bounds__(256) void k_tail_trsv(int32_t m, const T* __restrict__ S, int64_t lds,
+template <typename T>
+__global__ __launch_bounds__(256) void k_tail_trsv(int32_t m, int32_t kc, const T* __restrict__ S, int64_t lds,
                                                    const T* __restrict__ inv, const T* __restrict__ g,
-                                                   const double* __restrict__ yin, int64_t ldi,
-                                                   double* __restrict__ yout, int64_t ldo, int upper,
+                                                   const double* __restrict__ yin_all, int64_t ldi,
+                                                   double* __restrict__ yout_all, int64_t ldo, int upper,
                                                    int32_t* __restrict__ ticket, int32_t* __restrict__ err) {
+  constexpr int K = 1;
   constexpr int kW = 4;         // wavefronts per band; waves 1..kW-1 take the older tiles
   constexpr int kDL = kNb + 1;  // LDS row length of the diagonal inverse (padded against bank conflicts)
   __shared__ double xs[kW][K][kNb];
@@ -1550,9 +1552,12 @@ __global__ __launch_bounds__(256) void k_tail_trsv(int32_t m, const T* __restric
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   if (tid == 0) tk_sh = atomicAdd(ticket, 1);
   __syncthreads();
-  const int32_t t = __builtin_amdgcn_readfirstlane(tk_sh);
+  const int32_t tk = __builtin_amdgcn_readfirstlane(tk_sh);
   const int32_t np = (m + kNb - 1) / kNb;
-  if (t >= np) return;  // uniform over the workgroup
+  if (tk >= np * kc) return;  // uniform over the workgroup
+  const int32_t t = tk / kc;
+  const double* __restrict__ yin = yin_all + (tk % kc) * ldi;
+  double* __restrict__ yout = yout_all + (tk % kc) * ldo;
   const int32_t w = upper ? (np - 1 - t) : t;
   const int32_t r0 = w * kNb;
   const int32_t rows = (m - r0) < kNb ? (m - r0) : kNb;
@@ -1682,18 +1687,10 @@ template <typename T>
 void launch_tail_trsv(hipStream_t st, int32_t kc, int32_t m, const T* S, int64_t lds, const T* inv, const T* g,
                       const double* yin, int64_t ldi, double* yout, int64_t ldo, bool upper, int32_t* ticket,
                       int32_t* err) {
-  if (m <= 0) return;
-  const dim3 gr((m + kNb - 1) / kNb), b(256);
-  const int up = upper ? 1 : 0;
-#define BDLU_TRSV(KK)                                                                                        \
-  hipLaunchKernelGGL((k_tail_trsv<T, KK>), gr, b, 0, st, m, S, lds, inv, g, yin, ldi, yout, ldo, up, ticket, err)
-  switch (kc) {
-    case 1: BDLU_TRSV(1); break;
-    case 2: BDLU_TRSV(2); break;
-    case 4: BDLU_TRSV(4); break;
-    default: BDLU_TRSV(8); break;
-  }
-#undef BDLU_TRSV
+  if (m <= 0 || kc <= 0) return;
+  const int32_t np = (m + kNb - 1) / kNb;
+  hipLaunchKernelGGL(k_tail_trsv<T>, dim3(np * kc), dim3(256), 0, st, m, kc, S, lds, inv, g, yin, ldi, yout, ldo,
+                     upper ? 1 : 0, ticket, err);
 }
 template void launch_tail_trsv<double>(hipStream_t, int32_t, int32_t, const double*, int64_t, const double*,
                                        const double*, const double*, int64_t, double*, int64_t, bool, int32_t*,
