@@ -971,7 +971,7 @@ constexpr int kUpdKc = 16;
 template <int kTile, typename T>
 constexpr std::size_t update_lds_elems() { return 2 * 2 * kUpdKc * (kTile + 1); }
 
-template <typename T, int kTile>
+template <typename T, int kTile, bool kAllChunks = false>
 __device__ __forceinline__ void tile_update(const TileJob<T>& j0, const TileJob<T>& j1, T* lds, int32_t bid) {
   using V = typename MfmaOps<T>::v4;
   constexpr int kLd = kTile + 1;                // LDS row length (padded against bank conflicts)
@@ -1004,36 +1004,46 @@ __device__ __forceinline__ void tile_update(const TileJob<T>& j0, const TileJob<
       }
   // A chunk: kTile rows x 16 columns with the row index running across
   // threads; B chunk: 16 rows x kTile columns with the row (k) index running
-  // across threads (both unit stride in memory).
-  T ra[kPer], rb[kPer];
-  auto gload = [&](int k0) {
+  // across threads (both unit stride in memory). Register sets: one (the next
+  // chunk's loads in flight while the MFMAs take the current one), or with
+  // kAllChunks one per chunk, all loads issued up front, for kernels whose
+  // register count allows few waves per SIMD, so each wave keeps more of its
+  // own loads in flight.
+  constexpr int kSets = kAllChunks ? kChunks : 1;
+  T ra[kSets][kPer], rb[kSets][kPer];
+  auto gload = [&](int set, int k0) {
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
       const int e = u * 256 + tid;
       const int ii = e % kTile, ka = e / kTile;
-      ra[u] = J.A[int64_t(min(i0 + ii, M - 1)) + int64_t(min(k0 + ka, K - 1)) * J.a_c];
+      ra[set][u] = J.A[int64_t(min(i0 + ii, M - 1)) + int64_t(min(k0 + ka, K - 1)) * J.a_c];
       const int kb = e % kUpdKc, jj = e / kUpdKc;
-      rb[u] = J.B[int64_t(min(k0 + kb, K - 1)) + int64_t(min(c0 + jj, N - 1)) * J.b_c];
+      rb[set][u] = J.B[int64_t(min(k0 + kb, K - 1)) + int64_t(min(c0 + jj, N - 1)) * J.b_c];
     }
   };
-  auto lstore = [&](int slot, int k0) {
+  auto lstore = [&](int slot, int set, int k0) {
     T* As = lds + slot * 2 * kUpdKc * kLd;
     T* Bs = As + kUpdKc * kLd;
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
       const int e = u * 256 + tid;
       const int ii = e % kTile, ka = e / kTile;
-      As[ka * kLd + ii] = (i0 + ii < M && k0 + ka < K) ? ra[u] : T(0);
+      As[ka * kLd + ii] = (i0 + ii < M && k0 + ka < K) ? ra[set][u] : T(0);
       const int kb = e % kUpdKc, jj = e / kUpdKc;
-      Bs[kb * kLd + jj] = (c0 + jj < N && k0 + kb < K) ? -rb[u] : T(0);
+      Bs[kb * kLd + jj] = (c0 + jj < N && k0 + kb < K) ? -rb[set][u] : T(0);
     }
   };
-  gload(0);
-  lstore(0, 0);
+  if constexpr (kSets == 1) {
+    gload(0, 0);
+  } else {
+#pragma unroll
+    for (int c = 0; c < kChunks; ++c) gload(c, c * kUpdKc);
+  }
+  lstore(0, 0, 0);
   __syncthreads();
 #pragma unroll
   for (int ch = 0; ch < kChunks; ++ch) {
-    if (ch + 1 < kChunks) gload((ch + 1) * kUpdKc);
+    if (kSets == 1 && ch + 1 < kChunks) gload(0, (ch + 1) * kUpdKc);
     const T* As = lds + (ch & 1) * 2 * kUpdKc * kLd;
     const T* Bs = As + kUpdKc * kLd;
 #pragma unroll
@@ -1049,7 +1059,7 @@ __device__ __forceinline__ void tile_update(const TileJob<T>& j0, const TileJob<
         for (int a = 0; a < kMB; ++a) acc[c][a] = MfmaOps<T>::mma(bf[c], af[a], acc[c][a]);
     }
     // The other slot was last read before the previous barrier.
-    if (ch + 1 < kChunks) lstore((ch + 1) & 1, (ch + 1) * kUpdKc);
+    if (ch + 1 < kChunks) lstore((ch + 1) & 1, kSets == 1 ? 0 : ch + 1, (ch + 1) * kUpdKc);
     __syncthreads();
   }
 #pragma unroll
@@ -1091,11 +1101,13 @@ void k_tail_update_diag(TileJob<T> j0, TileJob<T> j1, TileJob<T> next, double li
   if (blockIdx.x == 0) {
     // The tile ends on a barrier after its last LDS read; the diagonal body's
     // first barrier makes the block's new values visible to its loads.
-    tile_update<T, 64>(next, next, lds, 0);
+    tile_update<T, 64, sizeof(T) == 8>(next, next, lds, 0);
     diag_lu_inv_body<T>(S, lds_s, q, nb, limit, status, inv, lds);
     return;
   }
-  tile_update<T, kTile>(j0, j1, lds, static_cast<int32_t>(blockIdx.x) - 1);
+  // FP64 64x64 tiles: all K chunks' loads up front (two waves per SIMD here,
+  // set by the diagonal path's registers).
+  tile_update<T, kTile, sizeof(T) == 8 && kTile == 64>(j0, j1, lds, static_cast<int32_t>(blockIdx.x) - 1);
 }
 template <typename T, int kTile>
 __global__ __launch_bounds__(256) void k_tail_panel(TileJob<T> j0, TileJob<T> j1, double limit,
