@@ -217,7 +217,7 @@ def run_gpu(bd, torch, n, r, c, v_sets, b, precision, steps, warmup, profile):
     wall = time.perf_counter() - t_start
     gpu_ms = ev0.elapsed_time(ev1)
     rep = s.report()
-    log(f"  precision={precision} refine={os.environ.get('BLOCKDAGLU_REFINE_ITERS', '0')} steps={steps} "
+    log(f"  precision={precision} refine={os.environ.get('USERALGO_REFINE_ITERS', '0')} steps={steps} "
         f"wall={wall:.3f}s gpu={gpu_ms:.1f}ms p50={1e3 * float(np.median(times)):.2f}ms "
         f"max={1e3 * max(times):.2f}ms ir_iters={rep['ir_iterations']} repivots={rep['repivot_blocks']} "
         f"fp64_fallbacks={rep['fp64_fallback_blocks']} rr_engine={rep['rel_residual']:.2e}")
@@ -471,14 +471,14 @@ def main():
     # The badly scaled stress case stops both refinements on componentwise backward
     # error (scale-invariant); ||r||/||b|| is dominated by the largest rows there.
     ir_stop = args.ir_stop or ("componentwise" if args.config == 3 else "residual")
-    os.environ.pop("BLOCKDAGLU_REFINE_ITERS", None)
+    os.environ.pop("USERALGO_REFINE_ITERS", None)
     os.environ.pop("BLOCKDAGLU_GPU_IR_STOP", None)
     res64p = run_gpu(bd, torch, n, r, c, v_sets, b, 0, args.steps, args.warmup, False)
     if ir_stop != "residual":
         os.environ["BLOCKDAGLU_GPU_IR_STOP"] = ir_stop
-    os.environ["BLOCKDAGLU_REFINE_ITERS"] = "10"
+    os.environ["USERALGO_REFINE_ITERS"] = "10"
     res64 = run_gpu(bd, torch, n, r, c, v_sets, b, 0, args.steps, args.warmup, False)
-    os.environ.pop("BLOCKDAGLU_REFINE_ITERS", None)
+    os.environ.pop("USERALGO_REFINE_ITERS", None)
     res32 = run_gpu(bd, torch, n, r, c, v_sets, b, 1, args.steps, args.warmup, not args.no_profile)
     os.environ.pop("BLOCKDAGLU_GPU_IR_STOP", None)
 

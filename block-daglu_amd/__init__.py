@@ -55,7 +55,8 @@ class _Report(C.Structure):
     _fields_ = [("nblocks", C.c_int32), ("dense_blocks", C.c_int32), ("sparse_blocks", C.c_int32),
                 ("max_block", C.c_int32), ("fp64_fallback_blocks", C.c_int32), ("repivot_blocks", C.c_int32),
                 ("ir_iterations", C.c_int32), ("plan_reused", C.c_int32), ("rel_residual", C.c_double),
-                ("backward_error", C.c_double), ("componentwise_error", C.c_double)]
+                ("backward_error", C.c_double), ("componentwise_error", C.c_double),
+                ("ir_fp64_refactor", C.c_int32)]
 
 
 class GenSpec(C.Structure):
@@ -72,7 +73,7 @@ class _ServeResult(C.Structure):
     _fields_ = [("error", C.c_int32), ("slot", C.c_int32), ("ir_iterations", C.c_int32),
                 ("fp64_fallback_blocks", C.c_int32), ("repivots", C.c_int32), ("rel_residual", C.c_double),
                 ("backward_error", C.c_double), ("componentwise_error", C.c_double), ("t_submit_ms", C.c_double), ("t_start_ms", C.c_double),
-                ("t_done_ms", C.c_double)]
+                ("t_done_ms", C.c_double), ("ir_fp64_refactor", C.c_int32)]
 
 
 _P = C.c_void_p
@@ -126,6 +127,7 @@ HOST_SYMBOLS = [
 # Symbols of include/bdlu_gpu.h (device engine).
 GPU_SYMBOLS = [
     ("bdlu_gpu_device_count", C.c_int, [_I32P]),
+    ("bdlu_gpu_build_info", C.c_char_p, []),
     ("bdlu_gpu_create", C.c_int, [C.c_int, C.c_uint64, C.POINTER(_P)]),
     ("bdlu_gpu_destroy", None, [_P]),
     ("bdlu_gpu_last_error", C.c_char_p, [_P]),
@@ -173,6 +175,11 @@ def build(oracle: bool = False) -> None:
     mod = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(mod)
     mod.build(oracle=oracle)
+
+
+def build_info() -> str:
+    """Source commit and target recorded in libbdlu_gpu.so when it was built."""
+    return lib().gpu.bdlu_gpu_build_info().decode()
 
 
 def gpu_available() -> bool:

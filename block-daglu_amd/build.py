@@ -49,13 +49,27 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"build step failed: {cmd[0]} {cmd[-1]}")
 
 
+def source_commit() -> str:
+    """Commit of the sources being built (with "+dirty" for uncommitted changes),
+    recorded inside libbdlu_gpu.so and returned by bdlu_gpu_build_info()."""
+    try:
+        head = subprocess.run(["git", "-C", str(ROOT), "rev-parse", "--short=12", "HEAD"], capture_output=True,
+                              text=True, check=True).stdout.strip()
+        dirty = subprocess.run(["git", "-C", str(ROOT), "status", "--porcelain", "--untracked-files=no"],
+                               capture_output=True, text=True, check=True).stdout.strip()
+        return head + ("+dirty" if dirty else "")
+    except (OSError, subprocess.CalledProcessError):
+        return "unknown"
+
+
 def build(oracle: bool = False, jobs: int = 8) -> None:
     LIB.mkdir(exist_ok=True)
     OBJ.mkdir(exist_ok=True)
     steps = []
+    stamp = f'-DBDLU_BUILD_INFO="{source_commit()} {ARCH}"'
     for src in HIP_SRCS:
         obj = OBJ / (Path(src).stem + ".hip.o")
-        steps.append([HIPCC, *HIP_FLAGS, "-c", str(PKG / src), "-o", str(obj)])
+        steps.append([HIPCC, *HIP_FLAGS, stamp, "-c", str(PKG / src), "-o", str(obj)])
     for src in HOST_SRCS:
         obj = OBJ / (Path(src).stem + ".o")
         steps.append([CXX, *HOST_FLAGS, "-c", str(PKG / src), "-o", str(obj)])

@@ -35,8 +35,15 @@ struct HipError {
     if (e_ != hipSuccess) throw HipError{std::string(#expr) + ": " + hipGetErrorString(e_)}; \
   } while (0)
 
+// Device allocations of one plan (or of one call). Copies from host memory are
+// asynchronous on the caller's stream: a raw host pointer must stay valid until
+// that stream is synchronized (the borrowed plan arrays do, since the upload
+// synchronizes before returning); a host vector may be a temporary, so its
+// bytes are first copied into `staged`, which lives until drop_staged() after
+// the synchronization.
 struct Alloc {
   std::vector<void*> ptrs;
+  std::vector<std::vector<unsigned char>> staged;
   uint64_t bytes = 0;
   template <typename T>
   T* get(std::size_t count) {
@@ -55,11 +62,17 @@ struct Alloc {
   }
   template <typename T>
   T* copy(const std::vector<T>& v, hipStream_t st) {
-    return copy(v.data(), v.size(), st);
+    if (v.empty()) return get<T>(0);
+    const auto* src = reinterpret_cast<const unsigned char*>(v.data());
+    staged.emplace_back(src, src + v.size() * sizeof(T));
+    return copy(reinterpret_cast<const T*>(staged.back().data()), v.size(), st);
   }
+  // Call after the stream that carried the copies has been synchronized.
+  void drop_staged() { staged.clear(); }
   void release() {
     for (void* p : ptrs) (void)hipFree(p);
     ptrs.clear();
+    staged.clear();
     bytes = 0;
   }
 };
@@ -152,6 +165,7 @@ struct Plan {
   double* partials = nullptr;
   double* norms = nullptr;  // per column c: [c*9 + 0..5] residual norms; [kAnormSlot] ||A||_inf
   bool factored = false;
+  bool has_values = false;  // P.vals holds the values of a previous refactor
   // Launch schedule, picked per plan by the autotuner (every choice gives the
   // same results bit for bit; only the time differs).
   struct Schedule {
@@ -339,9 +353,7 @@ void build_sparse_solve(const bdlu_sparse_desc& d, std::vector<int32_t>& rowperm
   }
 }
 
-std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
-  auto pl = std::make_unique<Plan>();
-  Plan& P = *pl;
+void upload_into(bdlu_ctx* c, const bdlu_plan_desc& d, Plan& P) {
   hipStream_t st = c->stream;
   Alloc& A = P.mem;
   P.n = d.n;
@@ -355,7 +367,7 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
   P.vals = A.get<double>(d.nnz);
   P.csr_ptr = A.copy(d.csr_ptr, n + 1, st);
   P.csr_col = A.copy(d.csr_col, d.nnz, st);
-  // (host vectors live until the upload's final stream synchronization)
+  // (vector copies are staged by Alloc, so temporaries may go out of scope)
   std::vector<uint8_t> res_flag(static_cast<std::size_t>(n), 0);
   std::vector<int32_t> res_rows;
   {
@@ -655,6 +667,20 @@ std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
   P.partials = A.get<double>(512 * 6 * kMaxRhs);
   P.norms = A.get<double>(kAnormSlot + 8);
   HIPCHECK(hipStreamSynchronize(st));
+}
+
+std::unique_ptr<Plan> upload(bdlu_ctx* c, const bdlu_plan_desc& d) {
+  auto pl = std::make_unique<Plan>();
+  try {
+    upload_into(c, d, *pl);
+  } catch (...) {
+    // Copies already queued still read the staged host bytes: wait for them
+    // before those and the device buffers go away.
+    (void)hipStreamSynchronize(c->stream);
+    pl->mem.release();
+    throw;
+  }
+  pl->mem.drop_staged();
   return pl;
 }
 
@@ -846,9 +872,257 @@ void factor_sparse_tail(Plan& P, SparseDev& S, hipStream_t st, T* Sm, ProfEvents
   bdk::launch_tail_coupling<T>(st, S.m, Sm, S.ldt, inv, cpl);
 }
 
+// Refactorization of a resident plan: the body of bdlu_gpu_refactor, run with
+// the context lock held. values == nullptr refactors the values of the
+// previous call again (the refinement uses it to redo FP32 factors in FP64).
+int refactor_impl(bdlu_ctx* c, Plan& P, const double* values, int where, int precision, bdlu_factor_stats& fs) {
+  hipStream_t st = c->stream;
+  fs = bdlu_factor_stats{0, 0, 0, -1, 0.0};
+  if (values) {
+    HIPCHECK(hipMemcpyAsync(P.vals, values, sizeof(double) * P.nnz,
+                            where == BDLU_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+    P.has_values = true;
+  } else if (!P.has_values) {
+    return set_err(c, BDLU_E_INVALID, "no values to refactor");
+  }
+  const bool fp32 = precision == BDLU_FP32;
+  ProfEvents pev;
+  ProfEvents* pe = c->prof ? &pev : nullptr;
+  for (SparseDev& S : P.sp) S.prec = fp32 ? 1 : 0;
+  auto launches = [&] {
+    bdk::launch_scatter(st, P.nnz, P.vals, P.vmap, P.arena);
+    bdk::launch_anorm(st, P.n, P.csr_ptr, P.vals, P.partials, P.norms + kAnormSlot);
+    // Dense blocks (dynamic partial pivoting, reference semantics), on a side
+    // stream unless profiling (the profile events time one stream).
+    const bool fork = c->fork && !pe;
+    hipStream_t ds = st;
+    if (fork) {
+      HIPCHECK(hipEventRecord(c->event(0), st));
+      HIPCHECK(hipStreamWaitEvent(c->side[0], c->event(0), 0));
+      ds = c->side[0];
+    }
+    if (pe) pe->mark(BDLU_PROF_DENSE_LU, st);
+    const int32_t* order = P.sched.dense_lpt ? P.d_lpt : nullptr;
+    if (fp32) {
+      bdk::launch_dense_lu<float>(ds, P.ndense, P.d_size, P.d_start, P.d_val0, P.d_lu0, P.arena, P.lu32, P.piv,
+                                  P.block_status, P.d_block, order);
+      bdk::launch_dense_inv<float>(ds, P.ndense, P.d_size, P.d_lu0, P.lu32, P.minv32);
+    } else {
+      bdk::launch_dense_lu<double>(ds, P.ndense, P.d_size, P.d_start, P.d_val0, P.d_lu0, P.arena, P.lu64, P.piv,
+                                   P.block_status, P.d_block, order);
+      bdk::launch_dense_inv<double>(ds, P.ndense, P.d_size, P.d_lu0, P.lu64, P.minv64);
+    }
+    if (pe) pe->mark(BDLU_PROF_DENSE_LU, st);
+    if (fork) HIPCHECK(hipEventRecord(c->event(1), ds));
+    // Sparse blocks, largest tail first. With several blocks, each tail goes
+    // to the least loaded side stream (load: tail size cubed) once its
+    // columns are done; the column phases stay on the main stream.
+    const bool spread = fork && P.sp.size() > 1;
+    double load[bdlu_ctx::kSide] = {0};
+    bool used[bdlu_ctx::kSide] = {false};
+    for (std::size_t q = 0; q < P.sp_order.size(); ++q) {
+      SparseDev& S = P.sp[P.sp_order[q]];
+      if (fp32) factor_sparse_cols<float>(P, S, st, S.Lx32, S.Ux32, S.Ud32, S.S32, pe, c->fp32_growth);
+      else factor_sparse_cols<double>(P, S, st, S.Lx64, S.Ux64, S.Ud64, S.S64, pe, c->fp64_growth);
+      hipStream_t ts = st;
+      if (spread && S.m > 0) {
+        int pick = 1;
+        for (int k = 2; k < bdlu_ctx::kSide; ++k)
+          if (load[k] < load[pick]) pick = k;
+        load[pick] += double(S.m) * S.m * S.m;
+        used[pick] = true;
+        HIPCHECK(hipEventRecord(c->event(2 + q), st));
+        HIPCHECK(hipStreamWaitEvent(c->side[pick], c->event(2 + q), 0));
+        ts = c->side[pick];
+      }
+      if (fp32) factor_sparse_tail<float>(P, S, ts, S.S32, pe, c->fp32_growth, c->lookahead);
+      else factor_sparse_tail<double>(P, S, ts, S.S64, pe, c->fp64_growth, c->lookahead);
+    }
+    if (fork) {
+      // Join: the main stream waits for the dense-block LU and every side stream used.
+      HIPCHECK(hipStreamWaitEvent(st, c->event(1), 0));
+      for (int k = 1; k < bdlu_ctx::kSide; ++k) {
+        if (!used[k]) continue;
+        const std::size_t e = 2 + P.sp_order.size() + k;
+        HIPCHECK(hipEventRecord(c->event(e), c->side[k]));
+        HIPCHECK(hipStreamWaitEvent(st, c->event(e), 0));
+      }
+    }
+  };
+  const int pi = fp32 ? 1 : 0;
+  if (c->autotune && !pe && !P.sched.factor_tuned[pi]) {
+    autotune_factor(P, st, launches);
+    P.sched.factor_tuned[pi] = true;
+    P.drop_graphs();  // recapture with the chosen schedule
+  }
+  run_captured(st, P.refactor_graph[pi], c->graphs && !pe, launches);
+  // Collect statuses.
+  std::vector<int32_t> bst(P.nblocks, 0);
+  if (P.ndense) HIPCHECK(hipMemcpyAsync(bst.data(), P.block_status, sizeof(int32_t) * P.nblocks,
+                                        hipMemcpyDeviceToHost, st));
+  std::vector<int32_t> sst(P.sp.size(), 0);
+  for (std::size_t i = 0; i < P.sp.size(); ++i)
+    HIPCHECK(hipMemcpyAsync(&sst[i], P.sp[i].status, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  if (pe) pe->collect(c);
+  std::fill(P.h_prec.begin(), P.h_prec.end(), fp32 ? 1 : 0);
+  // FP32 dense blocks that failed the small-pivot check: redo in FP64.
+  std::vector<int32_t> redo;
+  for (int32_t t = 0; t < P.ndense; ++t) {
+    const int32_t b = P.h_dense_block[t];
+    if (bst[b] & bdk::kStZeroPivot) {
+      fs.failed_blocks++;
+      if (fs.first_bad_block < 0) fs.first_bad_block = b;
+    } else if (fp32 && (bst[b] & bdk::kStCheck)) {
+      redo.push_back(t);
+    }
+  }
+  if (!redo.empty()) {
+    std::vector<int32_t> sz, sta, blk;
+    std::vector<int64_t> v0, l0;
+    for (int32_t t : redo) {
+      sz.push_back(P.h_dense_size[t]);
+      sta.push_back(P.h_dense_start[t]);
+      blk.push_back(P.h_dense_block[t]);
+      v0.push_back(P.h_dense_val0[t]);
+      l0.push_back(P.h_dense_lu0[t]);
+      P.h_prec[P.h_dense_block[t]] = 0;
+    }
+    Alloc tmp;
+    int32_t* dsz = tmp.copy(sz, st);
+    int32_t* dst_ = tmp.copy(sta, st);
+    int32_t* dbl = tmp.copy(blk, st);
+    int64_t* dv0 = tmp.copy(v0, st);
+    int64_t* dl0 = tmp.copy(l0, st);
+    bdk::launch_dense_lu<double>(st, static_cast<int32_t>(redo.size()), dsz, dst_, dv0, dl0, P.arena, P.lu64,
+                                 P.piv, P.block_status, dbl);
+    bdk::launch_dense_inv<double>(st, static_cast<int32_t>(redo.size()), dsz, dl0, P.lu64, P.minv64);
+    // The FP64 redo writes its own statuses: a block that is singular in
+    // FP64 (exact zero pivot) fails like any other dense block.
+    HIPCHECK(hipMemcpyAsync(bst.data(), P.block_status, sizeof(int32_t) * P.nblocks, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    tmp.release();
+    fs.fp64_fallback_blocks += static_cast<int32_t>(redo.size());
+    for (int32_t t : redo) {
+      const int32_t b = P.h_dense_block[t];
+      if (bst[b] & bdk::kStZeroPivot) {
+        fs.failed_blocks++;
+        if (fs.first_bad_block < 0) fs.first_bad_block = b;
+      }
+    }
+  }
+  for (std::size_t i = 0; i < P.sp.size(); ++i) {
+    SparseDev& S = P.sp[i];
+    int32_t s0 = sst[i];
+    if (fp32 && s0 != 0) {
+      // FP32 refactor failed its check: redo this block in FP64.
+      factor_sparse_cols<double>(P, S, st, S.Lx64, S.Ux64, S.Ud64, S.S64, nullptr, c->fp64_growth);
+      factor_sparse_tail<double>(P, S, st, S.S64, nullptr, c->fp64_growth);
+      S.prec = 0;
+      HIPCHECK(hipMemcpyAsync(&s0, S.status, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+      HIPCHECK(hipStreamSynchronize(st));
+      fs.fp64_fallback_blocks++;
+      P.h_prec[S.block] = 0;
+    }
+    if (s0 & (bdk::kStCheck | bdk::kStZeroPivot | bdk::kStNonFinite)) {
+      fs.repivot_blocks++;
+      if (fs.first_bad_block < 0) fs.first_bad_block = S.block;
+    }
+  }
+  HIPCHECK(hipMemcpyAsync(P.prec_of_block, P.h_prec.data(), sizeof(int32_t) * P.nblocks, hipMemcpyHostToDevice, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  P.factored = fs.failed_blocks == 0 && fs.repivot_blocks == 0;
+  if (fs.failed_blocks) return set_err(c, BDLU_E_FACTOR, "block factorization failed (zero pivot)");
+  if (fs.repivot_blocks) return set_err(c, BDLU_E_REPIVOT, "fixed pivot order failed the pivot check");
+  return BDLU_OK;
+}
+
+// Solves the chunk of right-hand sides in P.b (kt columns, the first kc real)
+// into P.x and, when asked, refines it in FP64. cs receives the chunk's
+// statistics. Returns true if the stop rule was met or no refinement was
+// asked. With watch_stall, refinement also ends early when a step fails to
+// halve the worst stop measure (FP32 factors too inaccurate for this matrix).
+bool solve_chunk(bdlu_ctx* c, Plan& P, int32_t kt, int32_t kc, const bdlu_ir_opts* ir, double anorm,
+                 bool watch_stall, bdlu_solve_stats& cs) {
+  hipStream_t st = c->stream;
+  const int64_t n = P.n;
+  const int32_t max_iters = ir ? ir->max_iters : 0;
+  const bool want_stats = max_iters > 0 || (ir && ir->stop_rule < 0);
+  ProfEvents pev;
+  if (c->prof) pev.mark(BDLU_PROF_SOLVE, st);
+  block_solve(P, st, P.b, P.x, kt, c->graphs && !c->prof);
+  if (c->prof) {
+    pev.mark(BDLU_PROF_SOLVE, st);
+    HIPCHECK(hipStreamSynchronize(st));
+    pev.collect(c);
+  }
+  cs = bdlu_solve_stats{0, -1.0, -1.0, -1.0, 0};
+  if (!want_stats) return true;
+  double h[9 * kMaxRhs];
+  auto eval = [&]() {
+    bdk::launch_residual(st, P.n, kt, P.csr_ptr, P.csr_col, P.vals, P.x, P.b, P.r, P.cw, n, P.res_is_long,
+                         P.res_nlong, P.res_long_rows);
+    bdk::launch_norms(st, P.n, kt, P.r, P.x, P.b, P.cw, n, P.partials, P.norms);
+    HIPCHECK(hipMemcpyAsync(h, P.norms, sizeof(double) * 9 * kt, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+  };
+  auto rel = [&](int k) {
+    const double bn2 = std::sqrt(h[k * 9 + 3]);
+    return bn2 > 0 ? std::sqrt(h[k * 9 + 0]) / bn2 : 0.0;
+  };
+  auto berr = [&](int k) {
+    const double den = anorm * h[k * 9 + 2] + h[k * 9 + 4];
+    return den > 0 ? h[k * 9 + 1] / den : 0.0;
+  };
+  const int rule = ir->stop_rule;
+  auto measure = [&](int k) { return rule == 1 ? berr(k) : rule == 2 ? h[k * 9 + 5] : rel(k); };
+  // Columns with b = 0 have x = 0 and count as converged.
+  auto worst = [&]() {
+    double w = 0.0;
+    for (int k = 0; k < kc; ++k)
+      if (h[k * 9 + 3] > 0.0) w = std::max(w, measure(k));
+    return w;
+  };
+  auto met = [&]() {
+    for (int k = 0; k < kc; ++k) {
+      const bool stop = rule == 0 ? rel(k) < ir->tol : measure(k) <= ir->tol;
+      if (!stop && h[k * 9 + 3] > 0.0) return false;
+    }
+    return true;
+  };
+  eval();
+  int32_t iters = 0;
+  bool converged = max_iters <= 0 || met();
+  double prev = worst();
+  while (!converged && iters < max_iters) {
+    block_solve(P, st, P.r, P.d, kt, c->graphs && !c->prof);
+    bdk::launch_axpy(st, P.n, kt, P.d, P.x, n);
+    ++iters;
+    eval();
+    converged = met();
+    const double now = worst();
+    if (!converged && watch_stall && !(now <= 0.5 * prev)) break;
+    prev = now;
+  }
+  // Worst right-hand side of the chunk.
+  for (int k = 0; k < kc; ++k) {
+    cs.rel_residual = std::max(cs.rel_residual, rel(k));
+    cs.backward_error = std::max(cs.backward_error, berr(k));
+    cs.componentwise_error = std::max(cs.componentwise_error, h[k * 9 + 5]);
+  }
+  cs.iterations = iters;
+  return converged;
+}
+
 }  // namespace
 
 extern "C" {
+
+
+#ifndef BDLU_BUILD_INFO
+#define BDLU_BUILD_INFO "unknown"
+#endif
+const char* bdlu_gpu_build_info(void) { return BDLU_BUILD_INFO; }
 
 int bdlu_gpu_device_count(int* count) {
   int c = 0;
@@ -966,158 +1240,16 @@ int bdlu_gpu_plan_bytes(bdlu_ctx* c, uint64_t id, uint64_t* bytes) {
 
 int bdlu_gpu_refactor(bdlu_ctx* c, uint64_t id, const double* values, int where, int precision,
                       bdlu_factor_stats* stats) {
-  if (!c || !values) return BDLU_E_INVALID;
+  if (!c) return BDLU_E_INVALID;
   std::lock_guard<std::mutex> g(c->mu);
   auto it = c->plans.find(id);
   if (it == c->plans.end()) return set_err(c, BDLU_E_INVALID, "unknown plan");
-  Plan& P = *it->second;
-  hipStream_t st = c->stream;
   bdlu_factor_stats fs{0, 0, 0, -1, 0.0};
   try {
     HIPCHECK(hipSetDevice(c->device));
-    HIPCHECK(hipMemcpyAsync(P.vals, values, sizeof(double) * P.nnz,
-                            where == BDLU_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
-    const bool fp32 = precision == BDLU_FP32;
-    ProfEvents pev;
-    ProfEvents* pe = c->prof ? &pev : nullptr;
-    for (SparseDev& S : P.sp) S.prec = fp32 ? 1 : 0;
-    auto launches = [&] {
-      bdk::launch_scatter(st, P.nnz, P.vals, P.vmap, P.arena);
-      bdk::launch_anorm(st, P.n, P.csr_ptr, P.vals, P.partials, P.norms + kAnormSlot);
-      // Dense blocks (dynamic partial pivoting, reference semantics), on a side
-      // stream unless profiling (the profile events time one stream).
-      const bool fork = c->fork && !pe;
-      hipStream_t ds = st;
-      if (fork) {
-        HIPCHECK(hipEventRecord(c->event(0), st));
-        HIPCHECK(hipStreamWaitEvent(c->side[0], c->event(0), 0));
-        ds = c->side[0];
-      }
-      if (pe) pe->mark(BDLU_PROF_DENSE_LU, st);
-      const int32_t* order = P.sched.dense_lpt ? P.d_lpt : nullptr;
-      if (fp32) {
-        bdk::launch_dense_lu<float>(ds, P.ndense, P.d_size, P.d_start, P.d_val0, P.d_lu0, P.arena, P.lu32, P.piv,
-                                    P.block_status, P.d_block, order);
-        bdk::launch_dense_inv<float>(ds, P.ndense, P.d_size, P.d_lu0, P.lu32, P.minv32);
-      } else {
-        bdk::launch_dense_lu<double>(ds, P.ndense, P.d_size, P.d_start, P.d_val0, P.d_lu0, P.arena, P.lu64, P.piv,
-                                     P.block_status, P.d_block, order);
-        bdk::launch_dense_inv<double>(ds, P.ndense, P.d_size, P.d_lu0, P.lu64, P.minv64);
-      }
-      if (pe) pe->mark(BDLU_PROF_DENSE_LU, st);
-      if (fork) HIPCHECK(hipEventRecord(c->event(1), ds));
-      // Sparse blocks, largest tail first. With several blocks, each tail goes
-      // to the least loaded side stream (load: tail size cubed) once its
-      // columns are done; the column phases stay on the main stream.
-      const bool spread = fork && P.sp.size() > 1;
-      double load[bdlu_ctx::kSide] = {0};
-      bool used[bdlu_ctx::kSide] = {false};
-      for (std::size_t q = 0; q < P.sp_order.size(); ++q) {
-        SparseDev& S = P.sp[P.sp_order[q]];
-        if (fp32) factor_sparse_cols<float>(P, S, st, S.Lx32, S.Ux32, S.Ud32, S.S32, pe, c->fp32_growth);
-        else factor_sparse_cols<double>(P, S, st, S.Lx64, S.Ux64, S.Ud64, S.S64, pe, c->fp64_growth);
-        hipStream_t ts = st;
-        if (spread && S.m > 0) {
-          int pick = 1;
-          for (int k = 2; k < bdlu_ctx::kSide; ++k)
-            if (load[k] < load[pick]) pick = k;
-          load[pick] += double(S.m) * S.m * S.m;
-          used[pick] = true;
-          HIPCHECK(hipEventRecord(c->event(2 + q), st));
-          HIPCHECK(hipStreamWaitEvent(c->side[pick], c->event(2 + q), 0));
-          ts = c->side[pick];
-        }
-        if (fp32) factor_sparse_tail<float>(P, S, ts, S.S32, pe, c->fp32_growth, c->lookahead);
-        else factor_sparse_tail<double>(P, S, ts, S.S64, pe, c->fp64_growth, c->lookahead);
-      }
-      if (fork) {
-        // Join: the main stream waits for the dense-block LU and every side stream used.
-        HIPCHECK(hipStreamWaitEvent(st, c->event(1), 0));
-        for (int k = 1; k < bdlu_ctx::kSide; ++k) {
-          if (!used[k]) continue;
-          const std::size_t e = 2 + P.sp_order.size() + k;
-          HIPCHECK(hipEventRecord(c->event(e), c->side[k]));
-          HIPCHECK(hipStreamWaitEvent(st, c->event(e), 0));
-        }
-      }
-    };
-    const int pi = fp32 ? 1 : 0;
-    if (c->autotune && !pe && !P.sched.factor_tuned[pi]) {
-      autotune_factor(P, st, launches);
-      P.sched.factor_tuned[pi] = true;
-      P.drop_graphs();  // recapture with the chosen schedule
-    }
-    run_captured(st, P.refactor_graph[pi], c->graphs && !pe, launches);
-    // Collect statuses.
-    std::vector<int32_t> bst(P.nblocks, 0);
-    if (P.ndense) HIPCHECK(hipMemcpyAsync(bst.data(), P.block_status, sizeof(int32_t) * P.nblocks,
-                                          hipMemcpyDeviceToHost, st));
-    std::vector<int32_t> sst(P.sp.size(), 0);
-    for (std::size_t i = 0; i < P.sp.size(); ++i)
-      HIPCHECK(hipMemcpyAsync(&sst[i], P.sp[i].status, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipStreamSynchronize(st));
-    if (pe) pe->collect(c);
-    std::fill(P.h_prec.begin(), P.h_prec.end(), fp32 ? 1 : 0);
-    // FP32 dense blocks that failed the small-pivot check: redo in FP64.
-    std::vector<int32_t> redo;
-    for (int32_t t = 0; t < P.ndense; ++t) {
-      const int32_t b = P.h_dense_block[t];
-      if (bst[b] & bdk::kStZeroPivot) {
-        fs.failed_blocks++;
-        if (fs.first_bad_block < 0) fs.first_bad_block = b;
-      } else if (fp32 && (bst[b] & bdk::kStCheck)) {
-        redo.push_back(t);
-      }
-    }
-    if (!redo.empty()) {
-      std::vector<int32_t> sz, sta, blk;
-      std::vector<int64_t> v0, l0;
-      for (int32_t t : redo) {
-        sz.push_back(P.h_dense_size[t]);
-        sta.push_back(P.h_dense_start[t]);
-        blk.push_back(P.h_dense_block[t]);
-        v0.push_back(P.h_dense_val0[t]);
-        l0.push_back(P.h_dense_lu0[t]);
-        P.h_prec[P.h_dense_block[t]] = 0;
-      }
-      Alloc tmp;
-      int32_t* dsz = tmp.copy(sz, st);
-      int32_t* dst_ = tmp.copy(sta, st);
-      int32_t* dbl = tmp.copy(blk, st);
-      int64_t* dv0 = tmp.copy(v0, st);
-      int64_t* dl0 = tmp.copy(l0, st);
-      bdk::launch_dense_lu<double>(st, static_cast<int32_t>(redo.size()), dsz, dst_, dv0, dl0, P.arena, P.lu64,
-                                   P.piv, P.block_status, dbl);
-      bdk::launch_dense_inv<double>(st, static_cast<int32_t>(redo.size()), dsz, dl0, P.lu64, P.minv64);
-      HIPCHECK(hipStreamSynchronize(st));
-      tmp.release();
-      fs.fp64_fallback_blocks += static_cast<int32_t>(redo.size());
-    }
-    for (std::size_t i = 0; i < P.sp.size(); ++i) {
-      SparseDev& S = P.sp[i];
-      int32_t s0 = sst[i];
-      if (fp32 && s0 != 0) {
-        // FP32 refactor failed its check: redo this block in FP64.
-        factor_sparse_cols<double>(P, S, st, S.Lx64, S.Ux64, S.Ud64, S.S64, nullptr, c->fp64_growth);
-        factor_sparse_tail<double>(P, S, st, S.S64, nullptr, c->fp64_growth);
-        S.prec = 0;
-        HIPCHECK(hipMemcpyAsync(&s0, S.status, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipStreamSynchronize(st));
-        fs.fp64_fallback_blocks++;
-        P.h_prec[S.block] = 0;
-      }
-      if (s0 & (bdk::kStCheck | bdk::kStZeroPivot | bdk::kStNonFinite)) {
-        fs.repivot_blocks++;
-        if (fs.first_bad_block < 0) fs.first_bad_block = S.block;
-      }
-    }
-    HIPCHECK(hipMemcpyAsync(P.prec_of_block, P.h_prec.data(), sizeof(int32_t) * P.nblocks, hipMemcpyHostToDevice, st));
-    HIPCHECK(hipStreamSynchronize(st));
-    P.factored = fs.failed_blocks == 0 && fs.repivot_blocks == 0;
+    const int rc = refactor_impl(c, *it->second, values, where, precision, fs);
     if (stats) *stats = fs;
-    if (fs.failed_blocks) return set_err(c, BDLU_E_FACTOR, "block factorization failed (zero pivot)");
-    if (fs.repivot_blocks) return set_err(c, BDLU_E_REPIVOT, "fixed pivot order failed the pivot check");
-    return BDLU_OK;
+    return rc;
   } catch (const HipError& e) {
     return set_err(c, BDLU_E_BACKEND, e.msg);
   }
@@ -1135,7 +1267,7 @@ int bdlu_gpu_solve(bdlu_ctx* c, uint64_t id, double* B, int32_t nrhs, int64_t ld
   hipStream_t st = c->stream;
   const int32_t max_iters = ir ? ir->max_iters : 0;
   const bool want_stats = max_iters > 0 || (ir && ir->stop_rule < 0);
-  bdlu_solve_stats ss{0, -1.0, -1.0, -1.0};
+  bdlu_solve_stats ss{0, -1.0, -1.0, -1.0, 0};
   const int64_t n = P.n;
   try {
     HIPCHECK(hipSetDevice(c->device));
@@ -1151,60 +1283,29 @@ int bdlu_gpu_solve(bdlu_ctx* c, uint64_t id, double* B, int32_t nrhs, int64_t ld
       for (int32_t k = 0; k < kc; ++k)
         HIPCHECK(hipMemcpyAsync(P.b + k * n, B + (c0 + k) * ldb, sizeof(double) * n, in, st));
       if (kt > kc) bdk::launch_fill_f64(st, P.b + kc * n, (kt - kc) * n, 0.0);
-      ProfEvents pev;
       if (c->autotune && !c->prof && !P.sched.solve_tuned) {
         autotune_solve(P, st, kt);
         P.sched.solve_tuned = true;
         P.drop_graphs();
       }
-      if (c->prof) pev.mark(BDLU_PROF_SOLVE, st);
-      block_solve(P, st, P.b, P.x, kt, c->graphs && !c->prof);
-      if (c->prof) {
-        pev.mark(BDLU_PROF_SOLVE, st);
-        HIPCHECK(hipStreamSynchronize(st));
-        pev.collect(c);
+      const bool fp32_factors = std::any_of(P.h_prec.begin(), P.h_prec.end(), [](int32_t p) { return p != 0; });
+      const bool fallback = ir && ir->fp64_fallback && max_iters > 0 && fp32_factors;
+      bdlu_solve_stats cs{};
+      if (!solve_chunk(c, P, kt, kc, ir, anorm, fallback, cs) && fallback) {
+        // FP32 factors did not reach the stop rule: refactor the same values in
+        // FP64 (P.b still holds this chunk) and solve the chunk again.
+        bdlu_factor_stats fs{};
+        const int rc = refactor_impl(c, P, nullptr, BDLU_MEM_DEVICE, BDLU_FP64, fs);
+        if (rc != BDLU_OK) return rc;
+        ss.fp64_refactor = 1;
+        const int32_t first_iters = cs.iterations;
+        solve_chunk(c, P, kt, kc, ir, anorm, false, cs);
+        cs.iterations += first_iters;
       }
-      if (want_stats) {
-        double h[9 * kMaxRhs];
-        auto eval = [&]() {
-          bdk::launch_residual(st, P.n, kt, P.csr_ptr, P.csr_col, P.vals, P.x, P.b, P.r, P.cw, n, P.res_is_long,
-                               P.res_nlong, P.res_long_rows);
-          bdk::launch_norms(st, P.n, kt, P.r, P.x, P.b, P.cw, n, P.partials, P.norms);
-          HIPCHECK(hipMemcpyAsync(h, P.norms, sizeof(double) * 9 * kt, hipMemcpyDeviceToHost, st));
-          HIPCHECK(hipStreamSynchronize(st));
-        };
-        auto rel = [&](int k) {
-          const double bn2 = std::sqrt(h[k * 9 + 3]);
-          return bn2 > 0 ? std::sqrt(h[k * 9 + 0]) / bn2 : 0.0;
-        };
-        auto berr = [&](int k) {
-          const double den = anorm * h[k * 9 + 2] + h[k * 9 + 4];
-          return den > 0 ? h[k * 9 + 1] / den : 0.0;
-        };
-        eval();
-        int32_t iters = 0;
-        for (int it2 = 0; it2 < max_iters; ++it2) {
-          bool done = true;
-          for (int k = 0; k < kc; ++k) {
-            const bool stop = ir->stop_rule == 1   ? berr(k) <= ir->tol
-                              : ir->stop_rule == 2 ? h[k * 9 + 5] <= ir->tol
-                                                   : rel(k) < ir->tol;
-            if (!stop && h[k * 9 + 3] > 0.0) done = false;
-          }
-          if (done) break;
-          block_solve(P, st, P.r, P.d, kt, c->graphs && !c->prof);
-          bdk::launch_axpy(st, P.n, kt, P.d, P.x, n);
-          ++iters;
-          eval();
-        }
-        // Worst right-hand side of the call.
-        for (int k = 0; k < kc; ++k) {
-          ss.rel_residual = std::max(ss.rel_residual, rel(k));
-          ss.backward_error = std::max(ss.backward_error, berr(k));
-          ss.componentwise_error = std::max(ss.componentwise_error, h[k * 9 + 5]);
-        }
-        ss.iterations = std::max(ss.iterations, iters);
-      }
+      ss.rel_residual = std::max(ss.rel_residual, cs.rel_residual);
+      ss.backward_error = std::max(ss.backward_error, cs.backward_error);
+      ss.componentwise_error = std::max(ss.componentwise_error, cs.componentwise_error);
+      ss.iterations = std::max(ss.iterations, cs.iterations);
       for (int32_t k = 0; k < kc; ++k)
         HIPCHECK(hipMemcpyAsync(B + (c0 + k) * ldb, P.x + k * n, sizeof(double) * n, out, st));
     }

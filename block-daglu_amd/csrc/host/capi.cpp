@@ -103,6 +103,7 @@ void bdlu_solver_report(void* s, bdlu_report* out) {
   out->rel_residual = r.rel_residual;
   out->backward_error = r.backward_error;
   out->componentwise_error = r.componentwise_error;
+  out->ir_fp64_refactor = r.ir_fp64_refactor ? 1 : 0;
 }
 
 int32_t bdlu_solver_nblocks(void* s) {
@@ -311,6 +312,7 @@ int bdlu_serve_wait(void* s, int64_t request, double* X, bdlu_serve_result* out)
     out->t_submit_ms = r.t_submit_ms;
     out->t_start_ms = r.t_start_ms;
     out->t_done_ms = r.t_done_ms;
+    out->ir_fp64_refactor = r.ir_fp64_refactor;
   }
   return static_cast<int>(r.error);
 }

@@ -1,7 +1,8 @@
-// Serving loop: registered patterns, plans resident per slot, FIFO admission of
-// requests onto slots (one engine context and HIP stream per slot, so requests
-// in different slots overlap on the device). No counterpart in the reference;
-// each request runs the same refactor + solve (+ refinement) as Solver does.
+// Serving loop: registered patterns, device plans resident per slot under one
+// HBM budget for all slots, near-FIFO admission of requests onto slots (one
+// engine context and HIP stream per slot, so requests in different slots
+// overlap on the device). No counterpart in the reference; each request runs
+// the same refactor + solve (+ refinement) as Solver does.
 #include "blockdaglu/serve.h"
 
 #include <algorithm>
@@ -10,7 +11,7 @@
 #include <condition_variable>
 #include <cstdlib>
 #include <deque>
-#include <list>
+#include <map>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -30,10 +31,18 @@ using Clock = std::chrono::steady_clock;
 struct Pattern {
   int32_t n = 0;
   int64_t nnz_in = 0;
-  std::vector<int64_t> slot;  // input entry -> canonical index, -1 if not part of the plan
+  // Input entry -> canonical entry, or -1 when its (row, col) group summed to
+  // exactly zero at registration (then zgroup names the group).
+  std::vector<int64_t> slot;
+  std::vector<int32_t> zgroup;
+  int32_t nzgroups = 0;
+  // Canonical entries the plan left out under the drop rule (|v| < drop_abs_lt).
+  std::vector<uint8_t> dropped;
+  double drop_abs_lt = 0.0;
   bdlu::Canonical canon;
   std::unique_ptr<bdlu::HostPlan> plan;
   bool singular = false;
+  uint64_t bytes = 0;  // device bytes of one resident plan, known after its first upload
 };
 
 struct Request {
@@ -43,6 +52,14 @@ struct Request {
   int32_t nrhs = 1;
   ServeResult res;
   bool done = false;
+};
+
+// One pattern's plan on one slot.
+struct Resident {
+  uint64_t id = 0;
+  uint64_t bytes = 0;
+  bool in_use = false;
+  uint64_t last_use = 0;
 };
 
 int refine_cap(Precision p) {
@@ -62,6 +79,11 @@ class ServeEngine {
  public:
   ServeEngine(int device, int slots, uint64_t budget, Precision p)
       : device_(device), slots_(std::max(1, slots)), budget_(budget), precision_(p), t0_(Clock::now()) {
+    // Contexts are created here, one after another, before any worker starts.
+    ctx_.assign(slots_, nullptr);
+    for (int w = 0; w < slots_; ++w)
+      if (bdlu_gpu_create(device_, 0, &ctx_[w]) != BDLU_OK) ctx_[w] = nullptr;
+    resident_.resize(slots_);
     for (int w = 0; w < slots_; ++w) workers_.emplace_back([this, w] { worker(w); });
   }
   ~ServeEngine() {
@@ -71,6 +93,11 @@ class ServeEngine {
     }
     cv_.notify_all();
     for (auto& t : workers_) t.join();
+    for (int w = 0; w < slots_; ++w) {
+      if (!ctx_[w]) continue;
+      for (auto& kv : resident_[w]) bdlu_gpu_plan_release(ctx_[w], kv.second.id);
+      bdlu_gpu_destroy(ctx_[w]);
+    }
   }
 
   int32_t add_pattern(const CooMatrix& m, int32_t dimension) {
@@ -85,8 +112,8 @@ class ServeEngine {
     pt->n = pt->canon.n;
     pt->nnz_in = static_cast<int64_t>(m.size());
     // Input entry -> canonical entry: entries sorted by (row, col) (stable), each
-    // (row, col) group mapped to the canonical entry with that key if it exists
-    // (groups whose registration sum was exactly zero are not in the pattern).
+    // (row, col) group mapped to the canonical entry with that key if it exists;
+    // groups whose registration sum was exactly zero get a group id instead.
     std::vector<int64_t> ord(m.size());
     std::iota(ord.begin(), ord.end(), 0);
     std::stable_sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) {
@@ -94,14 +121,20 @@ class ServeEngine {
                                                      : std::get<1>(m[a]) < std::get<1>(m[b]);
     });
     pt->slot.assign(m.size(), -1);
+    pt->zgroup.assign(m.size(), -1);
     std::size_t ci = 0;
     for (std::size_t k = 0; k < ord.size(); ++k) {
       const auto& e = m[ord[k]];
       while (ci < pt->canon.nnz() && (pt->canon.row[ci] < std::get<0>(e) ||
                                       (pt->canon.row[ci] == std::get<0>(e) && pt->canon.col[ci] < std::get<1>(e))))
         ++ci;
-      if (ci < pt->canon.nnz() && pt->canon.row[ci] == std::get<0>(e) && pt->canon.col[ci] == std::get<1>(e))
+      if (ci < pt->canon.nnz() && pt->canon.row[ci] == std::get<0>(e) && pt->canon.col[ci] == std::get<1>(e)) {
         pt->slot[ord[k]] = static_cast<int64_t>(ci);
+      } else {
+        const bool same_group = k > 0 && pt->slot[ord[k - 1]] < 0 && std::get<0>(m[ord[k - 1]]) == std::get<0>(e) &&
+                                std::get<1>(m[ord[k - 1]]) == std::get<1>(e);
+        pt->zgroup[ord[k]] = same_group ? pt->zgroup[ord[k - 1]] : pt->nzgroups++;
+      }
     }
     pt->plan = std::make_unique<bdlu::HostPlan>();
     try {
@@ -111,6 +144,10 @@ class ServeEngine {
       last_error_ = SolverError::AnalysisFailed;
       return -1;
     }
+    pt->drop_abs_lt = pt->plan->dec.drop_abs_lt;
+    pt->dropped.assign(pt->canon.nnz(), 0);
+    for (std::size_t e = 0; e < pt->plan->vmap.size() && e < pt->dropped.size(); ++e)
+      pt->dropped[e] = pt->plan->vmap[e] < 0 ? 1 : 0;
     std::lock_guard<std::mutex> g(mu_);
     patterns_.push_back(std::move(pt));
     return static_cast<int32_t>(patterns_.size()) - 1;
@@ -135,16 +172,39 @@ class ServeEngine {
       last_error_ = SolverError::InvalidRightHandSide;
       return -1;
     }
+    for (int64_t k = 0; k < nvals; ++k)
+      if (!std::isfinite(values[k])) {
+        last_error_ = SolverError::NonFiniteValue;
+        return -1;
+      }
     auto r = std::make_unique<Request>();
     r->pattern = pattern;
     r->nrhs = nrhs;
     // Canonical values: duplicates summed in long double (as the canonical form).
     std::vector<long double> acc(pt->canon.nnz(), 0.0L);
-    for (int64_t k = 0; k < nvals; ++k)
+    std::vector<long double> zacc(static_cast<std::size_t>(pt->nzgroups), 0.0L);
+    for (int64_t k = 0; k < nvals; ++k) {
       if (pt->slot[k] >= 0) acc[pt->slot[k]] += static_cast<long double>(values[k]);
+      else zacc[pt->zgroup[k]] += static_cast<long double>(values[k]);
+    }
+    // Values where the registered pattern has no entry make another pattern.
+    bool outside = false;
+    for (long double z : zacc) outside = outside || z != 0.0L;
     r->vals.resize(acc.size());
-    for (std::size_t k = 0; k < acc.size(); ++k) r->vals[k] = static_cast<double>(acc[k]);
+    for (std::size_t k = 0; k < acc.size(); ++k) {
+      r->vals[k] = static_cast<double>(acc[k]);
+      if (pt->dropped[k] && !(std::abs(r->vals[k]) < pt->drop_abs_lt)) outside = true;
+    }
+    if (outside) {
+      last_error_ = SolverError::AnalysisFailed;
+      return -1;
+    }
     r->B.assign(B, B + static_cast<int64_t>(pt->n) * nrhs);
+    for (double v : r->B)
+      if (!std::isfinite(v)) {
+        last_error_ = SolverError::NonFiniteValue;
+        return -1;
+      }
     std::lock_guard<std::mutex> g(mu_);
     r->res.t_submit_ms = now_ms();
     const int64_t id = next_id_++;
@@ -177,21 +237,115 @@ class ServeEngine {
  private:
   double now_ms() const { return std::chrono::duration<double, std::milli>(Clock::now() - t0_).count(); }
 
-  // Uploads a plan into this slot's context, evicting its least recently used
-  // resident plans while the slot's share of the budget is exceeded.
-  int upload(bdlu_ctx* ctx, const bdlu::HostPlan& hp, std::unordered_map<int32_t, uint64_t>& resident,
-             std::list<int32_t>& lru, uint64_t* id) {
-    int rc;
-    while ((rc = bdlu_gpu_plan_upload(ctx, &hp.desc, id)) == BDLU_E_BUDGET && !lru.empty()) {
-      const int32_t victim = lru.back();
-      lru.pop_back();
-      bdlu_gpu_plan_release(ctx, resident[victim]);
-      resident.erase(victim);
-    }
-    return rc;
+  // Picks the least recently used idle plan of any slot, removes it from the
+  // books and returns it (slot, id); false if no plan is idle. Caller holds mu_.
+  bool take_victim(int* slot, uint64_t* id) {
+    int vs = -1;
+    int32_t vp = -1;
+    uint64_t oldest = ~uint64_t(0);
+    for (int w = 0; w < slots_; ++w)
+      for (auto& kv : resident_[w])
+        if (!kv.second.in_use && kv.second.last_use < oldest) {
+          oldest = kv.second.last_use;
+          vs = w;
+          vp = kv.first;
+        }
+    if (vs < 0) return false;
+    *slot = vs;
+    *id = resident_[vs][vp].id;
+    used_ -= resident_[vs][vp].bytes;
+    resident_[vs].erase(vp);
+    return true;
   }
 
-  void run(bdlu_ctx* ctx, std::unordered_map<int32_t, uint64_t>& resident, std::list<int32_t>& lru, Request& r) {
+  bool any_in_use() const {
+    for (const auto& m : resident_)
+      for (const auto& kv : m)
+        if (kv.second.in_use) return true;
+    return false;
+  }
+
+  // Releases a plan taken by take_victim (without mu_: it waits for that slot's
+  // context, which may be finishing another request).
+  void release_plan(int slot, uint64_t id) {
+    bdlu_gpu_plan_release(ctx_[slot], id);
+    std::lock_guard<std::mutex> g(mu_);
+    mem_cv_.notify_all();
+  }
+
+  // The resident plan of pattern p on slot w, uploaded if needed. While the
+  // budget cannot take it, idle plans (any slot, least recently used first) are
+  // evicted, and if none is idle the request waits for requests in flight to
+  // finish. Returns a SolverError; on success the plan is marked in use.
+  SolverError acquire(int w, int32_t p, uint64_t* id) {
+    bdlu_ctx* ctx = ctx_[w];
+    std::unique_lock<std::mutex> lk(mu_);
+    auto it = resident_[w].find(p);
+    if (it != resident_[w].end()) {
+      it->second.in_use = true;
+      it->second.last_use = ++tick_;
+      *id = it->second.id;
+      return SolverError::None;
+    }
+    Pattern& pt = *patterns_[p];
+    for (;;) {
+      const uint64_t need = pt.bytes;  // 0 until the pattern's first upload measured it
+      if (budget_ == 0 || used_ + need <= budget_) break;
+      int vs;
+      uint64_t vid;
+      if (take_victim(&vs, &vid)) {
+        lk.unlock();
+        release_plan(vs, vid);
+        lk.lock();
+        continue;
+      }
+      if (!any_in_use()) return SolverError::BackendUnavailable;  // larger than the whole budget
+      mem_cv_.wait(lk);
+    }
+    used_ += pt.bytes;
+    const uint64_t reserved = pt.bytes;
+    lk.unlock();
+    uint64_t bytes = 0;
+    int rc = bdlu_gpu_plan_upload(ctx, &pt.plan->desc, id);
+    if (rc == BDLU_OK) rc = bdlu_gpu_plan_bytes(ctx, *id, &bytes);
+    lk.lock();
+    used_ -= reserved;
+    if (rc != BDLU_OK) {
+      mem_cv_.notify_all();
+      return SolverError::BackendUnavailable;
+    }
+    pt.bytes = bytes;
+    if (budget_ != 0 && bytes > budget_) {
+      // Larger than the whole budget: this pattern cannot be served here.
+      lk.unlock();
+      bdlu_gpu_plan_release(ctx, *id);
+      lk.lock();
+      mem_cv_.notify_all();
+      return SolverError::BackendUnavailable;
+    }
+    used_ += bytes;
+    resident_[w][p] = Resident{*id, bytes, true, ++tick_};
+    // A pattern's first upload is measured only after the fact: make room now.
+    while (budget_ != 0 && used_ > budget_) {
+      int vs;
+      uint64_t vid;
+      if (!take_victim(&vs, &vid)) break;
+      lk.unlock();
+      release_plan(vs, vid);
+      lk.lock();
+    }
+    return SolverError::None;
+  }
+
+  void unuse(int w, int32_t p) {
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = resident_[w].find(p);
+    if (it != resident_[w].end()) it->second.in_use = false;
+    mem_cv_.notify_all();
+  }
+
+  void run(int w, Request& r) {
+    bdlu_ctx* ctx = ctx_[w];
     const Pattern* pt;
     {
       std::lock_guard<std::mutex> g(mu_);
@@ -202,29 +356,24 @@ class ServeEngine {
       return;
     }
     uint64_t id = 0;
-    auto it = resident.find(r.pattern);
-    if (it == resident.end()) {
-      if (upload(ctx, *pt->plan, resident, lru, &id) != BDLU_OK) {
-        r.res.error = SolverError::BackendUnavailable;
-        return;
-      }
-      resident[r.pattern] = id;
-      lru.push_front(r.pattern);
-    } else {
-      id = it->second;
-      lru.remove(r.pattern);
-      lru.push_front(r.pattern);
+    const SolverError ae = acquire(w, r.pattern, &id);
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      r.res.t_start_ms = now_ms();
+    }
+    if (ae != SolverError::None) {
+      r.res.error = ae;
+      return;
     }
     const int prec = precision_ == Precision::Fp32Ir ? BDLU_FP32 : BDLU_FP64;
     bdlu_factor_stats fs{};
     int rc = bdlu_gpu_refactor(ctx, id, r.vals.data(), BDLU_MEM_HOST, prec, &fs);
     uint64_t use = id;
-    bool tmp_plan = false;
-    std::unique_ptr<bdlu::HostPlan> tmp;
+    uint64_t tmp_bytes = 0;
     if (rc == BDLU_E_REPIVOT) {
       // This request's values fail the fixed pivot order: re-analyse a private
       // copy with its values (the registered plan stays as it is).
-      tmp = std::make_unique<bdlu::HostPlan>(*pt->plan);
+      auto tmp = std::make_unique<bdlu::HostPlan>(*pt->plan);
       bdlu::Canonical c2 = pt->canon;
       c2.val = r.vals;
       bdlu::PlanStatus ps;
@@ -234,14 +383,20 @@ class ServeEngine {
         ps.ok = false;
       }
       if (!ps.ok) {
+        unuse(w, r.pattern);
         r.res.error = SolverError::FactorizationFailed;
         return;
       }
-      if (upload(ctx, *tmp, resident, lru, &use) != BDLU_OK) {
+      if (bdlu_gpu_plan_upload(ctx, &tmp->desc, &use) != BDLU_OK) {
+        unuse(w, r.pattern);
         r.res.error = SolverError::BackendUnavailable;
         return;
       }
-      tmp_plan = true;
+      bdlu_gpu_plan_bytes(ctx, use, &tmp_bytes);
+      {
+        std::lock_guard<std::mutex> g(mu_);
+        used_ += tmp_bytes;
+      }
       r.res.repivots = 1;
       rc = bdlu_gpu_refactor(ctx, use, r.vals.data(), BDLU_MEM_HOST, prec, &fs);
     }
@@ -251,9 +406,11 @@ class ServeEngine {
       ir.max_iters = refine_cap(precision_);
       ir.stop_rule = ir.max_iters > 0 ? 0 : -1;  // -1: report the residual without refining
       ir.tol = 1e-12;
+      ir.fp64_fallback = precision_ == Precision::Fp32Ir ? 1 : 0;
       bdlu_solve_stats ss{};
       rc = bdlu_gpu_solve(ctx, use, r.B.data(), r.nrhs, pt->n, BDLU_MEM_HOST, &ir, &ss);
       r.res.ir_iterations = ss.iterations;
+      r.res.ir_fp64_refactor = ss.fp64_refactor;
       r.res.rel_residual = ss.rel_residual;
       r.res.backward_error = ss.backward_error;
       r.res.componentwise_error = ss.componentwise_error;
@@ -261,37 +418,44 @@ class ServeEngine {
     } else {
       r.res.error = rc == BDLU_E_BACKEND ? SolverError::BackendUnavailable : SolverError::FactorizationFailed;
     }
-    if (tmp_plan) bdlu_gpu_plan_release(ctx, use);
+    if (use != id) {
+      bdlu_gpu_plan_release(ctx, use);
+      std::lock_guard<std::mutex> g(mu_);
+      used_ -= tmp_bytes;
+    }
+    unuse(w, r.pattern);
   }
 
   void worker(int w) {
-    bdlu_ctx* ctx = nullptr;
-    if (bdlu_gpu_create(device_, budget_ / static_cast<uint64_t>(slots_), &ctx) != BDLU_OK) ctx = nullptr;
-    std::unordered_map<int32_t, uint64_t> resident;
-    std::list<int32_t> lru;
     for (;;) {
       Request* r = nullptr;
       {
         std::unique_lock<std::mutex> lk(mu_);
         cv_.wait(lk, [&] { return stop_ || !queue_.empty(); });
         if (queue_.empty()) break;  // stop requested and nothing left
-        r = queue_.front();
-        queue_.pop_front();
-        r->res.t_start_ms = now_ms();
+        // Oldest request among the next few whose plan this slot holds, else the oldest.
+        std::size_t pick = 0;
+        const std::size_t window = std::min<std::size_t>(queue_.size(), 2 * static_cast<std::size_t>(slots_));
+        for (std::size_t i = 0; i < window; ++i)
+          if (resident_[w].count(queue_[i]->pattern)) {
+            pick = i;
+            break;
+          }
+        r = queue_[pick];
+        queue_.erase(queue_.begin() + static_cast<std::ptrdiff_t>(pick));
       }
       r->res.slot = w;
-      if (ctx) run(ctx, resident, lru, *r);
-      else r->res.error = SolverError::BackendUnavailable;
+      if (ctx_[w]) {
+        run(w, *r);
+      } else {
+        r->res.error = SolverError::BackendUnavailable;
+      }
       {
         std::lock_guard<std::mutex> g(mu_);
         r->res.t_done_ms = now_ms();
         r->done = true;
       }
       done_cv_.notify_all();
-    }
-    if (ctx) {
-      for (auto& kv : resident) bdlu_gpu_plan_release(ctx, kv.second);
-      bdlu_gpu_destroy(ctx);
     }
   }
 
@@ -301,7 +465,11 @@ class ServeEngine {
   const Precision precision_;
   const Clock::time_point t0_;
   mutable std::mutex mu_;
-  std::condition_variable cv_, done_cv_;
+  std::condition_variable cv_, done_cv_, mem_cv_;
+  std::vector<bdlu_ctx*> ctx_;
+  std::vector<std::map<int32_t, Resident>> resident_;  // per slot: pattern -> plan
+  uint64_t used_ = 0;  // device bytes of all resident plans (plus private re-analysed ones in flight)
+  uint64_t tick_ = 0;
   std::vector<std::unique_ptr<Pattern>> patterns_;
   std::deque<Request*> queue_;
   std::unordered_map<int64_t, std::unique_ptr<Request>> requests_;

@@ -377,6 +377,8 @@ bdlu_ir_opts ir_options(Precision p) {
     int cap = 30;
     if (const char* e = std::getenv("BLOCKDAGLU_GPU_IR_MAX")) cap = std::max(1, std::atoi(e));
     iters = std::max(iters, cap);
+    // FP32 factors that cannot reach the stop rule are redone in FP64.
+    o.fp64_fallback = 1;
   }
   o.max_iters = iters;
   return o;
@@ -407,6 +409,7 @@ bool Solver::solve_multi_in_place(std::vector<double>& B, int32_t nrhs) {
   const int rc = bdlu_gpu_solve(ctx, s.plan->id, B.data(), nrhs, n_, BDLU_MEM_HOST, &ir, &ss);
   if (rc != BDLU_OK) return fail(SolverError::SolveFailed, std::string("block solve failed: ") + bdlu_gpu_last_error(ctx));
   report_.ir_iterations = ss.iterations;
+  report_.ir_fp64_refactor = ss.fp64_refactor != 0;
   report_.rel_residual = ss.rel_residual;
   report_.backward_error = ss.backward_error;
   report_.componentwise_error = ss.componentwise_error;
@@ -428,6 +431,7 @@ bool Solver::solve_device(double* d_B, int32_t nrhs, int64_t ldb) {
   const int rc = bdlu_gpu_solve(ctx, s.plan->id, d_B, nrhs, ldb, BDLU_MEM_DEVICE, &ir, &ss);
   if (rc != BDLU_OK) return fail(SolverError::SolveFailed, std::string("block solve failed: ") + bdlu_gpu_last_error(ctx));
   report_.ir_iterations = ss.iterations;
+  report_.ir_fp64_refactor = ss.fp64_refactor != 0;
   report_.rel_residual = ss.rel_residual;
   report_.backward_error = ss.backward_error;
   report_.componentwise_error = ss.componentwise_error;

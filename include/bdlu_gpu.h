@@ -105,6 +105,10 @@ typedef struct bdlu_ir_opts {
                             // 1: normwise backward error <= tol
                             // 2: componentwise backward error <= tol
   double tol;
+  int32_t fp64_fallback;    // 1: when the factors in use include FP32 ones and the stop rule is not met
+                            // within max_iters (or a step fails to halve the worst stop measure), the
+                            // plan is refactored in FP64 from the same values and the columns are
+                            // solved and refined again (bdlu_solve_stats.fp64_refactor = 1)
 } bdlu_ir_opts;
 
 typedef struct bdlu_solve_stats {
@@ -112,9 +116,13 @@ typedef struct bdlu_solve_stats {
   double rel_residual;      // ||b - A x||_2 / ||b||_2 (FP64)
   double backward_error;    // ||r||_inf / (||A||_inf ||x||_inf + ||b||_inf)
   double componentwise_error;  // max_i |r_i| / (|A| |x| + |b|)_i (Oettli-Prager)
+  int32_t fp64_refactor;    // 1: FP32 factors did not converge and the plan was refactored in FP64
 } bdlu_solve_stats;
 
 int bdlu_gpu_device_count(int* count);
+// Source commit and target of this build, e.g. "1a2b3c4d5e6f gfx950" (no
+// counterpart in the reference).
+const char* bdlu_gpu_build_info(void);
 int bdlu_gpu_create(int device, uint64_t hbm_budget_bytes, bdlu_ctx** out);
 void bdlu_gpu_destroy(bdlu_ctx* ctx);
 const char* bdlu_gpu_last_error(bdlu_ctx* ctx);
@@ -131,8 +139,9 @@ int bdlu_gpu_plan_bytes(bdlu_ctx* ctx, uint64_t plan_id, uint64_t* bytes);
 // candidate gives bit-identical results.
 int bdlu_gpu_plan_schedule(bdlu_ctx* ctx, uint64_t plan_id, int32_t* out4);
 
-// New values for the plan's pattern, canonical order. precision: BDLU_FP64 or
-// BDLU_FP32. Sparse blocks refactor on the analysis' pivot order and check the
+// New values for the plan's pattern, canonical order (values == NULL: refactor
+// the values of the plan's previous refactor again, e.g. in the other
+// precision). precision: BDLU_FP64 or BDLU_FP32. Sparse blocks refactor on the analysis' pivot order and check the
 // growth of L: an FP32 block with |L| above BLOCKDAGLU_GPU_FP32_GROWTH (1e4)
 // is refactored in FP64; an FP64 block with |L| above
 // BLOCKDAGLU_GPU_REPIVOT_GROWTH (1e8), a zero or a non-finite pivot returns

@@ -42,6 +42,7 @@ typedef struct bdlu_report {
   int32_t fp64_fallback_blocks, repivot_blocks, ir_iterations;
   int32_t plan_reused;
   double rel_residual, backward_error, componentwise_error;
+  int32_t ir_fp64_refactor;  // 1: FP32 refinement did not converge; refactored in FP64 and solved again
 } bdlu_report;
 
 void* bdlu_solver_create(void);
@@ -78,7 +79,8 @@ int64_t bdlu_solver_export(void* s, int which, int32_t* rows, int32_t* cols, dou
 // No counterpart in the reference: for callers that serve many refactor+solve
 // requests on registered patterns (Newton-like loops, circuit simulators). Slots
 // run concurrently on the device, each with its own stream and resident plans;
-// requests are admitted in arrival order as slots free up, within an HBM budget.
+// requests are admitted near arrival order as slots free up, and the resident
+// plans of all slots together stay within one HBM budget (blockdaglu/serve.h).
 typedef struct bdlu_serve_result {
   int32_t error;                 // SolverError value (0 = success)
   int32_t slot;
@@ -89,6 +91,7 @@ typedef struct bdlu_serve_result {
   double backward_error;         // worst right-hand side of the request
   double componentwise_error;    // worst right-hand side of the request
   double t_submit_ms, t_start_ms, t_done_ms;  // server clock
+  int32_t ir_fp64_refactor;      // 1: FP32 refinement did not converge; refactored in FP64
 } bdlu_serve_result;
 void* bdlu_serve_create(int device, int slots, uint64_t hbm_budget_bytes, int precision /* 0 FP64, 1 FP32+IR */);
 void bdlu_serve_destroy(void* s);

@@ -1,7 +1,11 @@
 #pragma once
 // Environment lookup with the reference's naming rule
-// (blockdaglu/include/blockdaglu/env.h:15-25): a request for USERALGO_FOO first
-// tries BLOCKDAGLU_FOO, then USERALGO_FOO. Every other name is read as given.
+// (blockdaglu/include/blockdaglu/env.h:15-25). A request whose name starts with
+// the 8 characters "USERALGO" first tries "BLOCKDAGLU_" followed by the rest of
+// the name after those 8 characters, then the name itself. The rest keeps the
+// underscore that follows "USERALGO", so USERALGO_FOO is looked up as
+// BLOCKDAGLU__FOO (two underscores) and then as USERALGO_FOO; BLOCKDAGLU_FOO
+// is never read for it. Every other name is read as given.
 
 #include <cstdlib>
 #include <cstring>
@@ -9,11 +13,10 @@
 
 inline const char* blockdaglu_getenv_compat(const char* name) {
   if (name == nullptr) return nullptr;
-  static const char kLegacy[] = "USERALGO_";
-  const std::size_t legacy_len = sizeof(kLegacy) - 1;
-  if (std::strncmp(name, kLegacy, legacy_len) == 0) {
-    const std::string modern = std::string("BLOCKDAGLU_") + (name + legacy_len);
-    if (const char* v = std::getenv(modern.c_str())) return v;
+  constexpr std::size_t kPrefixLen = 8;  // "USERALGO", as the reference compares it
+  if (std::strncmp(name, "USERALGO", kPrefixLen) == 0) {
+    const std::string alias = std::string("BLOCKDAGLU_") + (name + kPrefixLen);
+    if (const char* v = std::getenv(alias.c_str())) return v;
   }
   return std::getenv(name);
 }

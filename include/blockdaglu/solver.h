@@ -38,9 +38,14 @@ enum class SolverError {
 };
 
 // Additive: arithmetic used by factor()/solve_in_place().
-//   Fp64   — plain FP64 factors and solves (default).
-//   Fp32Ir — FP32 factors, FP64 residual and correction accumulation until the
-//            normwise backward error reaches `ir_tol` (or `ir_max_iters`).
+//   Fp64   — plain FP64 factors and solves (default); USERALGO_REFINE_ITERS
+//            steps of FP64 refinement as in the reference.
+//   Fp32Ir — FP32 factors; FP64 residuals and FP64 correction accumulation
+//            until ||b - A x||_2 / ||b||_2 < 1e-12 (BLOCKDAGLU_GPU_IR_STOP
+//            selects the normwise or componentwise backward error instead),
+//            at most BLOCKDAGLU_GPU_IR_MAX (30) steps. If the stop rule is not
+//            met, or a step fails to halve the stop measure, the matrix is
+//            refactored in FP64 and solved again (report().ir_fp64_refactor).
 enum class Precision { Fp64 = 0, Fp32Ir = 1 };
 
 // Additive: what the last factor()/solve_in_place() did.
@@ -52,6 +57,7 @@ struct EngineReport {
   int32_t fp64_fallback_blocks = 0;  // FP32 blocks refactored in FP64 after a pivot check
   int32_t repivot_blocks = 0;        // blocks re-analysed because fixed pivots failed the check
   int32_t ir_iterations = 0;
+  bool ir_fp64_refactor = false;     // FP32 refinement did not converge; refactored in FP64
   double rel_residual = -1.0;        // ||b - A x||_2 / ||b||_2 (FP64, on device)
   double backward_error = -1.0;      // ||r||_inf / (||A||_inf ||x||_inf + ||b||_inf)
   double componentwise_error = -1.0; // max_i |r_i| / (|A| |x| + |b|)_i

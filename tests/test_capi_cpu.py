@@ -90,3 +90,46 @@ def test_error_enum_matches_reference_order(bd):
     names = [e.name.rstrip("_") for e in sorted(bd.SolverError, key=int)]
     assert names == ref
     assert [int(e) for e in sorted(bd.SolverError, key=int)] == list(range(len(ref)))
+
+
+def _submit_error(srv, pid, vals, n):
+    try:
+        srv.submit(pid, vals, np.ones(n))
+        return None
+    except RuntimeError as e:
+        return str(e)
+
+
+def test_serve_rejects_values_outside_the_registered_pattern(bd):
+    """Serving maps request values onto the registered pattern's canonical
+    entries (host side, no device work). A duplicate group that summed to zero
+    at registration has no entry in the pattern, so a request with a nonzero sum
+    there is another sparsity pattern and is rejected; a zero sum is accepted."""
+    srv = bd.Server(slots=1, hbm_budget_gb=1.0)
+    r = np.array([0, 1, 2, 0, 0], np.int32)
+    c = np.array([0, 1, 2, 1, 1], np.int32)
+    v = np.array([4.0, 5.0, 6.0, 1.0, -1.0])  # (0, 1) sums to zero at registration
+    pid = srv.add_pattern(r, c, v, 3)
+    assert _submit_error(srv, pid, np.array([4.0, 5.0, 6.0, 2.0, -2.0]), 3) is None
+    assert "AnalysisFailed" in _submit_error(srv, pid, np.array([4.0, 5.0, 6.0, 1.0, -0.5]), 3)
+    assert "NonFiniteValue" in _submit_error(srv, pid, np.array([4.0, np.nan, 6.0, 1.0, -1.0]), 3)
+    srv.close()
+
+
+def test_serve_rejects_values_at_dropped_entries(bd):
+    """For n >= 50000 the analysis drops entries with |v| < 1e-20
+    (ubtf_block_solver.cpp:582-589); a request with a larger value there is
+    another sparsity pattern and is rejected."""
+    n = 50000
+    r = np.concatenate([np.arange(n), [0]]).astype(np.int32)
+    c = np.concatenate([np.arange(n), [1]]).astype(np.int32)
+    v = np.concatenate([np.full(n, 2.0), [1e-25]])
+    srv = bd.Server(slots=1, hbm_budget_gb=1.0)
+    pid = srv.add_pattern(r, c, v, n)
+    keep = v.copy()
+    keep[-1] = 3e-21
+    assert _submit_error(srv, pid, keep, n) is None
+    bad = v.copy()
+    bad[-1] = 1e-3
+    assert "AnalysisFailed" in _submit_error(srv, pid, bad, n)
+    srv.close()

@@ -13,6 +13,7 @@ here (no SuiteSparse build): those stay parity unpinned beyond the oracle."""
 from __future__ import annotations
 
 import numpy as np
+import pytest
 import scipy.sparse as sp
 
 
@@ -77,7 +78,7 @@ def test_invariants_on_generated_workload(bd):
 def test_row_scaling(bd, monkeypatch):
     """KLU scale = 2 (row maxima): rearranged = P (R \\ A) Q with R by original
     row, every row of R \\ A has maximum magnitude 1, and it still equals L U."""
-    monkeypatch.setenv("BLOCKDAGLU_KLU_SCALE", "2")
+    monkeypatch.setenv("USERALGO_KLU_SCALE", "2")
     sp_ = bd.config_spec(3)
     sp_.n_small_blocks, sp_.n_small_total = 30, 600
     sp_.n_large_blocks = 0
@@ -90,6 +91,28 @@ def test_row_scaling(bd, monkeypatch):
     _check_invariants(n, r, c, v, ex, scaled=rmax)
     Rm = ex["get_rearranged_matrix"]
     np.testing.assert_allclose(abs(Rm).max(axis=1).toarray().ravel(), 1.0, rtol=1e-15)
+    s.close()
+
+
+@pytest.mark.parametrize("name,scaled", [("USERALGO_KLU_SCALE", True), ("BLOCKDAGLU__KLU_SCALE", True),
+                                         ("BLOCKDAGLU_KLU_SCALE", False)])
+def test_env_alias_rule(bd, monkeypatch, name, scaled):
+    """The reference reads USERALGO_X, then BLOCKDAGLU_ + X with X keeping its
+    leading underscore (include/blockdaglu/env.h:19-21 compares 8 characters),
+    so BLOCKDAGLU__KLU_SCALE is an alias and BLOCKDAGLU_KLU_SCALE is ignored."""
+    for k in ("USERALGO_KLU_SCALE", "BLOCKDAGLU__KLU_SCALE", "BLOCKDAGLU_KLU_SCALE"):
+        monkeypatch.delenv(k, raising=False)
+    monkeypatch.setenv(name, "2")
+    n = 6
+    r = np.array([0, 1, 2, 3, 4, 5, 0, 2, 4], np.int32)
+    c = np.array([0, 1, 2, 3, 4, 5, 1, 3, 5], np.int32)
+    v = np.array([4.0, 8.0, 2.0, 5.0, 3.0, 7.0, 1.0, 1.0, 1.0])
+    s = bd.Solver()
+    assert s.set_matrix(r, c, v, n)
+    Rm = s.get_rearranged_matrix()
+    peak = max(abs(x[2]) for x in Rm)
+    # Row maxima scaling makes every rearranged entry at most 1 in magnitude.
+    assert (peak <= 1.0) == scaled
     s.close()
 
 

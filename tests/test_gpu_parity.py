@@ -162,7 +162,7 @@ def test_fp64_refinement_reaches_1e12(gpu, monkeypatch):
     # reference stop rule ||r||_2/||b||_2 < 1e-12 (src/solver.cpp:1508-1521).
     _say("case: FP64 + refinement")
     bd = gpu
-    monkeypatch.setenv("BLOCKDAGLU_REFINE_ITERS", "4")
+    monkeypatch.setenv("USERALGO_REFINE_ITERS", "4")
     n, r, c, v, B = bd.generate(small_cfg1(bd))
     b = B[:, 0].copy()
     s, x = solve_gpu(bd, n, r, c, v, b)
