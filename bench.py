@@ -168,7 +168,7 @@ def componentwise_error(n, r, c, v, x, b):
     return worst
 
 
-def run_gpu(bd, torch, n, r, c, v_sets, b, precision, steps, warmup, profile):
+def run_gpu(bd, n, r, c, v_sets, b, precision, steps, warmup, profile):
     """Returns (seconds per step list, final x of the last step, report, prof)."""
     L = bd.lib()
     s = bd.Solver()
@@ -179,49 +179,45 @@ def run_gpu(bd, torch, n, r, c, v_sets, b, precision, steps, warmup, profile):
     log(f"analyze (plan build + upload) {time.time() - t0:.2f}s precision={precision}")
     assert s.factor(), s.last_error_message()
     order = bd.canonical_order(r, c)
-    dev = torch.device("cuda:0")
-    vals_dev = torch.tensor(np.stack([vs[order] for vs in v_sets]), dtype=torch.float64, device=dev)
-    # right-hand sides as a (k, n) row-major tensor = n x k column-major
+    # Value sets and right-hand sides resident in HBM before the timed region.
+    vals_dev = [bd.DeviceArray.from_numpy(vs[order]) for vs in v_sets]
     bk = b.reshape(n, -1)
     nrhs = bk.shape[1]
-    b_dev = torch.tensor(np.ascontiguousarray(bk.T), dtype=torch.float64, device=dev)
-    x_dev = torch.empty_like(b_dev)
-    torch.cuda.synchronize()
-    ext = torch.cuda.ExternalStream(s.device_stream())
+    b_dev = bd.DeviceArray.from_numpy(bk.ravel(order="F"))  # n x k column-major
+    x_dev = bd.DeviceArray(n * nrhs)
+    stream = s.device_stream()
     ctx = C.c_void_p(s.device_context())
 
     def step(i):
-        with torch.cuda.stream(ext):
-            x_dev.copy_(b_dev)
-        ok = s.factor_device(vals_dev[i % len(v_sets)].data_ptr())
+        x_dev.copy_from(b_dev, stream)  # on the engine stream, ahead of the refactor
+        ok = s.factor_device(vals_dev[i % len(v_sets)].address)
         assert ok, s.last_error_message()
-        ok = s.solve_device(x_dev.data_ptr(), nrhs, n)
+        ok = s.solve_device(x_dev.address, nrhs, n)
         assert ok, s.last_error_message()
 
     for i in range(warmup):
         t = time.perf_counter()
         step(i)
         log(f"  warmup step {i}: {1e3 * (time.perf_counter() - t):.2f} ms")
-    torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
+    bd.hip().hipDeviceSynchronize()
+    ev0, ev1 = bd.Event(), bd.Event()
     times = []
-    ev0.record(ext)
+    ev0.record(stream)
     t_start = time.perf_counter()
     for i in range(steps):
         t = time.perf_counter()
         step(i)
         times.append(time.perf_counter() - t)
-    ev1.record(ext)
+    ev1.record(stream)
     ev1.synchronize()
     wall = time.perf_counter() - t_start
-    gpu_ms = ev0.elapsed_time(ev1)
+    gpu_ms = ev0.elapsed_ms(ev1)
     rep = s.report()
     log(f"  precision={precision} refine={os.environ.get('USERALGO_REFINE_ITERS', '0')} steps={steps} "
         f"wall={wall:.3f}s gpu={gpu_ms:.1f}ms p50={1e3 * float(np.median(times)):.2f}ms "
         f"max={1e3 * max(times):.2f}ms ir_iters={rep['ir_iterations']} repivots={rep['repivot_blocks']} "
         f"fp64_fallbacks={rep['fp64_fallback_blocks']} rr_engine={rep['rel_residual']:.2e}")
-    x = x_dev.cpu().numpy().T  # n x k
+    x = x_dev.numpy().reshape(nrhs, n).T  # n x k
     last_vals = v_sets[(steps - 1) % len(v_sets)]
     prof = None
     if profile:
@@ -389,7 +385,8 @@ def bench_serving(bd, args):
             "config": {"workload": f"serving: 16 patterns n=1e3..1e5, {nreq} Poisson requests at "
                                    f"{lam:.1f}/s, 1-8 RHS, {args.slots} slots, 64 GB budget"},
             "fp32ir": out["FP32_IR"], "fp64": out["FP64"],
-            "fp32ir_over_fp64_capacity": out["FP32_IR"]["capacity_req_per_s"] / out["FP64"]["capacity_req_per_s"]}
+            "fp32ir_over_fp64_capacity": out["FP32_IR"]["capacity_req_per_s"] / out["FP64"]["capacity_req_per_s"],
+            "build": bd.build_info()}
     print(json.dumps(line))
 
 
@@ -432,18 +429,18 @@ def main():
         print("bench.py measures one GPU: this build's path has no multi-GPU mode (see the sharding note "
               "in the design document); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    import torch
     bd = load_package()
     if not bd.gpu_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     if args.copy_calibration:
-        x = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda:0")
-        y = torch.empty_like(x)
+        x = bd.DeviceArray.from_numpy(np.ones(COPY_BYTES // 8))
+        y = bd.DeviceArray(COPY_BYTES // 8)
         for _ in range(3):
-            y.copy_(x)
-        torch.cuda.synchronize()
-        del x, y
+            y.copy_from(x)
+        bd.hip().hipDeviceSynchronize()
+        x.free()
+        y.free()
         log(f"copy calibration: 3 x {COPY_BYTES} B device-to-device")
     if args.config == 4:
         bench_serving(bd, args)
@@ -473,13 +470,13 @@ def main():
     ir_stop = args.ir_stop or ("componentwise" if args.config == 3 else "residual")
     os.environ.pop("USERALGO_REFINE_ITERS", None)
     os.environ.pop("BLOCKDAGLU_GPU_IR_STOP", None)
-    res64p = run_gpu(bd, torch, n, r, c, v_sets, b, 0, args.steps, args.warmup, False)
+    res64p = run_gpu(bd, n, r, c, v_sets, b, 0, args.steps, args.warmup, False)
     if ir_stop != "residual":
         os.environ["BLOCKDAGLU_GPU_IR_STOP"] = ir_stop
     os.environ["USERALGO_REFINE_ITERS"] = "10"
-    res64 = run_gpu(bd, torch, n, r, c, v_sets, b, 0, args.steps, args.warmup, False)
+    res64 = run_gpu(bd, n, r, c, v_sets, b, 0, args.steps, args.warmup, False)
     os.environ.pop("USERALGO_REFINE_ITERS", None)
-    res32 = run_gpu(bd, torch, n, r, c, v_sets, b, 1, args.steps, args.warmup, not args.no_profile)
+    res32 = run_gpu(bd, n, r, c, v_sets, b, 1, args.steps, args.warmup, not args.no_profile)
     os.environ.pop("BLOCKDAGLU_GPU_IR_STOP", None)
 
     def summarize(res):
@@ -553,6 +550,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernel_profile": prof,
+        "build": bd.build_info(),
     }
     print(json.dumps(out))
 

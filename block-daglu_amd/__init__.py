@@ -99,6 +99,7 @@ HOST_SYMBOLS = [
     ("bdlu_solver_solve_device", C.c_int, [_P, C.c_void_p, C.c_int32, C.c_int64]),
     ("bdlu_solver_device_context", _P, [_P]),
     ("bdlu_solver_device_stream", _P, [_P]),
+    ("bdlu_solver_device_plan", C.c_uint64, [_P]),
     ("bdlu_solver_nblocks", C.c_int32, [_P]),
     ("bdlu_solver_copy_perm", None, [_P, _I32P, _I32P, _I32P]),
     ("bdlu_solver_export", C.c_int64, [_P, C.c_int, _I32P, _I32P, _F64P]),
@@ -145,6 +146,119 @@ GPU_SYMBOLS = [
 ]
 
 PROF_TAIL_GEMM, PROF_TAIL_PANEL, PROF_DENSE_LU, PROF_SPARSE, PROF_SOLVE = range(5)
+
+
+class IrOpts(C.Structure):
+    """bdlu_ir_opts (include/bdlu_gpu.h)."""
+    _fields_ = [("max_iters", C.c_int32), ("stop_rule", C.c_int32), ("tol", C.c_double),
+                ("fp64_fallback", C.c_int32)]
+
+
+class SolveStats(C.Structure):
+    """bdlu_solve_stats (include/bdlu_gpu.h)."""
+    _fields_ = [("iterations", C.c_int32), ("rel_residual", C.c_double), ("backward_error", C.c_double),
+                ("componentwise_error", C.c_double), ("fp64_refactor", C.c_int32)]
+
+
+_hip = None
+
+
+def hip() -> C.CDLL:
+    """The HIP runtime the engine library is linked against (already loaded by
+    lib()), for device buffers and events in callers such as the benchmark."""
+    global _hip
+    if _hip is None:
+        lib()
+        H = C.CDLL("libamdhip64.so.7")
+        for name, args in (("hipMalloc", [C.POINTER(C.c_void_p), C.c_size_t]), ("hipFree", [C.c_void_p]),
+                           ("hipMemcpy", [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
+                           ("hipMemcpyAsync", [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]),
+                           ("hipDeviceSynchronize", []), ("hipStreamSynchronize", [C.c_void_p]),
+                           ("hipEventCreate", [C.POINTER(C.c_void_p)]), ("hipEventDestroy", [C.c_void_p]),
+                           ("hipEventRecord", [C.c_void_p, C.c_void_p]), ("hipEventSynchronize", [C.c_void_p]),
+                           ("hipEventElapsedTime", [C.POINTER(C.c_float), C.c_void_p, C.c_void_p])):
+            f = getattr(H, name)
+            f.restype = C.c_int
+            f.argtypes = args
+        _hip = H
+    return _hip
+
+
+def _hipcheck(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed with HIP error {rc}")
+
+
+class DeviceArray:
+    """A float64 buffer in device memory (HBM) with host copies in and out."""
+    H2D, D2H, D2D = 1, 2, 3
+
+    def __init__(self, count: int):
+        self.count = int(count)
+        self.ptr = C.c_void_p()
+        _hipcheck(hip().hipMalloc(C.byref(self.ptr), max(1, self.count) * 8), "hipMalloc")
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        d = cls(a.size)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.size <= self.count
+        _hipcheck(hip().hipMemcpy(self.ptr, a.ctypes.data, a.size * 8, self.H2D), "hipMemcpy")
+
+    def copy_from(self, other: "DeviceArray", stream: int = 0) -> None:
+        assert other.count <= self.count
+        _hipcheck(hip().hipMemcpyAsync(self.ptr, other.ptr, other.count * 8, self.D2D, C.c_void_p(stream)),
+                  "hipMemcpyAsync")
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.count, np.float64)
+        _hipcheck(hip().hipMemcpy(out.ctypes.data, self.ptr, self.count * 8, self.D2H), "hipMemcpy")
+        return out
+
+    @property
+    def address(self) -> int:
+        return int(self.ptr.value or 0)
+
+    def free(self) -> None:
+        if getattr(self, "ptr", None) is not None and self.ptr.value:
+            hip().hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Event:
+    """A HIP event (timing) on a given stream."""
+
+    def __init__(self):
+        self.e = C.c_void_p()
+        _hipcheck(hip().hipEventCreate(C.byref(self.e)), "hipEventCreate")
+
+    def record(self, stream: int) -> None:
+        _hipcheck(hip().hipEventRecord(self.e, C.c_void_p(stream)), "hipEventRecord")
+
+    def synchronize(self) -> None:
+        _hipcheck(hip().hipEventSynchronize(self.e), "hipEventSynchronize")
+
+    def elapsed_ms(self, later: "Event") -> float:
+        ms = C.c_float()
+        _hipcheck(hip().hipEventElapsedTime(C.byref(ms), self.e, later.e), "hipEventElapsedTime")
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            hip().hipEventDestroy(self.e)
+        except Exception:
+            pass
 
 
 def lib() -> C.CDLL:
@@ -261,6 +375,9 @@ class Solver:
 
     def device_context(self) -> int:
         return self._L.bdlu_solver_device_context(self._h) or 0
+
+    def device_plan(self) -> int:
+        return int(self._L.bdlu_solver_device_plan(self._h))
 
     def last_error(self) -> SolverError:
         return SolverError(self._L.bdlu_solver_last_error(self._h))

@@ -109,6 +109,18 @@ struct SparseDev {
   int prec = 0;
 };
 
+// Per-call vectors of a solve (n x kMaxRhs sets, column stride n) and its
+// counters. A plan owns one set; the no-allocation entry point carves one out
+// of the library's static scratch array instead.
+struct Work {
+  double *b = nullptr, *x = nullptr, *r = nullptr, *d = nullptr, *bperm = nullptr, *xperm = nullptr;
+  double* cw = nullptr;        // per-row componentwise backward-error ratios of the last residual
+  double* partials = nullptr;  // 512 * 6 * kMaxRhs
+  double* norms = nullptr;     // per column c: [c*9 + 0..5] residual norms
+  int32_t* tickets = nullptr;  // the plan's nticket counters
+  int32_t* err = nullptr;
+};
+
 struct Phase {
   bool sparse = false;
   int32_t index = 0;   // sparse block index or segment id
@@ -157,13 +169,9 @@ struct Plan {
   int32_t nslots = 0;
   // solve
   std::vector<Phase> phases;
-  int32_t* tickets = nullptr;
   int32_t nticket = 0;
-  int32_t* err = nullptr;
-  double *b = nullptr, *x = nullptr, *r = nullptr, *d = nullptr, *bperm = nullptr, *xperm = nullptr;
-  double* cw = nullptr;  // per-row componentwise backward-error ratios of the last residual
-  double* partials = nullptr;
-  double* norms = nullptr;  // per column c: [c*9 + 0..5] residual norms; [kAnormSlot] ||A||_inf
+  Work work;                // the plan's own solve vectors (work.norms aliases norms)
+  double* norms = nullptr;  // [kAnormSlot]: ||A||_inf of the current values
   bool factored = false;
   bool has_values = false;  // P.vals holds the values of a previous refactor
   // Launch schedule, picked per plan by the autotuner (every choice gives the
@@ -654,18 +662,20 @@ void upload_into(bdlu_ctx* c, const bdlu_plan_desc& d, Plan& P) {
   }
   flush();
   P.nticket = static_cast<int32_t>(P.phases.size()) + 2 * d.nsparse;
-  P.tickets = A.get<int32_t>(P.nticket);
-  P.err = A.get<int32_t>(1);
+  Work& W = P.work;
+  W.tickets = A.get<int32_t>(P.nticket);
+  W.err = A.get<int32_t>(1);
   const std::size_t nk = static_cast<std::size_t>(n) * kMaxRhs;
-  P.b = A.get<double>(nk);
-  P.x = A.get<double>(nk);
-  P.r = A.get<double>(nk);
-  P.d = A.get<double>(nk);
-  P.bperm = A.get<double>(nk);
-  P.xperm = A.get<double>(nk);
-  P.cw = A.get<double>(nk);
-  P.partials = A.get<double>(512 * 6 * kMaxRhs);
+  W.b = A.get<double>(nk);
+  W.x = A.get<double>(nk);
+  W.r = A.get<double>(nk);
+  W.d = A.get<double>(nk);
+  W.bperm = A.get<double>(nk);
+  W.xperm = A.get<double>(nk);
+  W.cw = A.get<double>(nk);
+  W.partials = A.get<double>(512 * 6 * kMaxRhs);
   P.norms = A.get<double>(kAnormSlot + 8);
+  W.norms = P.norms;
   HIPCHECK(hipStreamSynchronize(st));
 }
 
@@ -715,29 +725,32 @@ void run_captured(hipStream_t st, hipGraphExec_t& exec, bool use, F&& body) {
   HIPCHECK(hipGraphLaunch(exec, st));
 }
 
-// Solves kt columns at once: b and x are n x kt sets (column stride n).
-void block_solve_launches(Plan& P, hipStream_t st, const double* b, double* x, int32_t kt);
+// Solves kt columns at once: b and x are n x kt sets (column stride n); W
+// supplies the permuted vectors and the counters.
+void block_solve_launches(Plan& P, const Work& W, hipStream_t st, const double* b, double* x, int32_t kt);
+// With graphs (the plan's own work set only), the launches are captured once
+// per key and replayed.
 void block_solve(Plan& P, hipStream_t st, const double* b, double* x, int32_t kt, bool graphs) {
   // The launch sequence depends on the column count, the vector pair and the
   // precision each sparse block was last factored in (its factor arrays).
-  std::vector<int32_t> key{kt, b == P.b ? 0 : 1};
+  std::vector<int32_t> key{kt, b == P.work.b ? 0 : 1};
   for (const SparseDev& S : P.sp) key.push_back(S.prec);
   hipGraphExec_t& exec = P.solve_graph[key];
-  run_captured(st, exec, graphs, [&] { block_solve_launches(P, st, b, x, kt); });
+  run_captured(st, exec, graphs, [&] { block_solve_launches(P, P.work, st, b, x, kt); });
 }
 
-void block_solve_launches(Plan& P, hipStream_t st, const double* b, double* x, int32_t kt) {
+void block_solve_launches(Plan& P, const Work& W, hipStream_t st, const double* b, double* x, int32_t kt) {
   const int64_t ld = P.n;
-  bdk::launch_fill_i32(st, P.tickets, P.nticket, 0);
-  bdk::launch_fill_sentinel(st, P.xperm, int64_t(P.n) * kt);
-  bdk::launch_permute(st, P.n, kt, P.P, b, ld, P.bperm, ld);
+  bdk::launch_fill_i32(st, W.tickets, P.nticket, 0);
+  bdk::launch_fill_sentinel(st, W.xperm, int64_t(P.n) * kt);
+  bdk::launch_permute(st, P.n, kt, P.P, b, ld, W.bperm, ld);
   int32_t tix = 0;
   for (const Phase& ph : P.phases) {
     if (!ph.sparse) {
       // Each dense block's factors are read in the precision it was factored in.
       bdk::launch_dense_seg_solve(st, kt, ph.ntasks, ph.tasks, P.R, P.lu0_of_block, P.minv64, P.minv32,
-                                  P.prec_of_block, P.piv, P.off_ptr, P.off_col, P.off_val, P.tickets + tix,
-                                  P.bperm, P.xperm, ld, P.err, P.sched.seg_grid);
+                                  P.prec_of_block, P.piv, P.off_ptr, P.off_col, P.off_val, W.tickets + tix,
+                                  W.bperm, W.xperm, ld, W.err, P.sched.seg_grid);
       ++tix;
       continue;
     }
@@ -745,7 +758,7 @@ void block_solve_launches(Plan& P, hipStream_t st, const double* b, double* x, i
     ++tix;
     const int64_t ldy = S.s, ldt = std::max(S.m, 1);
     bdk::launch_sp_gather(st, S.s, S.start, kt, S.rowperm, S.scale ? S.rs : nullptr, P.off_ptr, P.off_col, P.off_val,
-                          P.bperm, P.xperm, ld, S.y, ldy);
+                          W.bperm, W.xperm, ld, S.y, ldy);
     auto run = [&](auto* Lx, auto* Ux, auto* Ud, auto* Sm, auto* inv, auto* cpl) {
       using T = std::remove_pointer_t<decltype(Lx)>;
       for (std::size_t l = 0; l + 1 < S.fl_ptr.size(); ++l)
@@ -754,11 +767,11 @@ void block_solve_launches(Plan& P, hipStream_t st, const double* b, double* x, i
       bdk::launch_sp_rows<T>(st, kt, S.m, S.tl_rows, S.lr_ptr, S.lr_col, S.lr_vidx, Lx, nullptr, S.y, ldy);
       // Tail: forward into ty, backward back into y (each output starts unpublished).
       for (int32_t c = 0; c < kt; ++c) bdk::launch_fill_sentinel(st, S.ty + c * ldt, S.m);
-      bdk::launch_tail_trsv<T>(st, kt, S.m, Sm, S.ldt, inv, cpl, S.y + S.k0, ldy, S.ty, ldt, false, P.tickets + tix,
-                               P.err);
+      bdk::launch_tail_trsv<T>(st, kt, S.m, Sm, S.ldt, inv, cpl, S.y + S.k0, ldy, S.ty, ldt, false, W.tickets + tix,
+                               W.err);
       for (int32_t c = 0; c < kt; ++c) bdk::launch_fill_sentinel(st, S.y + c * ldy + S.k0, S.m);
-      bdk::launch_tail_trsv<T>(st, kt, S.m, Sm, S.ldt, inv, cpl, S.ty, ldt, S.y + S.k0, ldy, true, P.tickets + tix + 1,
-                               P.err);
+      bdk::launch_tail_trsv<T>(st, kt, S.m, Sm, S.ldt, inv, cpl, S.ty, ldt, S.y + S.k0, ldy, true, W.tickets + tix + 1,
+                               W.err);
       for (std::size_t l = 0; l + 1 < S.bl_ptr.size(); ++l)
         bdk::launch_sp_rows<T>(st, kt, S.bl_ptr[l + 1] - S.bl_ptr[l], S.bl_rows + S.bl_ptr[l], S.ur_ptr, S.ur_col,
                                S.ur_vidx, Ux, Ud, S.y, ldy);
@@ -766,9 +779,9 @@ void block_solve_launches(Plan& P, hipStream_t st, const double* b, double* x, i
     if (S.prec == 0) run(S.Lx64, S.Ux64, S.Ud64, S.S64, S.inv64, S.cpl64);
     else run(S.Lx32, S.Ux32, S.Ud32, S.S32, S.inv32, S.cpl32);
     tix += 2;
-    bdk::launch_sp_scatter_x(st, S.s, S.start, kt, S.q, S.y, ldy, P.xperm, ld);
+    bdk::launch_sp_scatter_x(st, S.s, S.start, kt, S.q, S.y, ldy, W.xperm, ld);
   }
-  bdk::launch_unpermute(st, P.n, kt, P.Q, P.xperm, ld, x, ld);
+  bdk::launch_unpermute(st, P.n, kt, P.Q, W.xperm, ld, x, ld);
 }
 
 // Autotuner: times each candidate schedule once (after one warm-up run) with
@@ -815,7 +828,7 @@ void autotune_factor(Plan& P, hipStream_t st, const std::function<void()>& launc
 
 void autotune_solve(Plan& P, hipStream_t st, int32_t kt) {
   std::vector<int32_t> grids{256, 512, 1024};
-  auto run = [&] { block_solve_launches(P, st, P.b, P.x, kt); };
+  auto run = [&] { block_solve_launches(P, P.work, st, P.work.b, P.work.x, kt); };
   run();  // warm-up
   float best = 1e30f;
   int32_t pick = P.sched.seg_grid;
@@ -891,7 +904,7 @@ int refactor_impl(bdlu_ctx* c, Plan& P, const double* values, int where, int pre
   for (SparseDev& S : P.sp) S.prec = fp32 ? 1 : 0;
   auto launches = [&] {
     bdk::launch_scatter(st, P.nnz, P.vals, P.vmap, P.arena);
-    bdk::launch_anorm(st, P.n, P.csr_ptr, P.vals, P.partials, P.norms + kAnormSlot);
+    bdk::launch_anorm(st, P.n, P.csr_ptr, P.vals, P.work.partials, P.norms + kAnormSlot);
     // Dense blocks (dynamic partial pivoting, reference semantics), on a side
     // stream unless profiling (the profile events time one stream).
     const bool fork = c->fork && !pe;
@@ -1037,21 +1050,29 @@ int refactor_impl(bdlu_ctx* c, Plan& P, const double* values, int where, int pre
   return BDLU_OK;
 }
 
-// Solves the chunk of right-hand sides in P.b (kt columns, the first kc real)
-// into P.x and, when asked, refines it in FP64. cs receives the chunk's
+// Solves the chunk of right-hand sides in W.b (kt columns, the first kc real)
+// into W.x and, when asked, refines it in FP64. cs receives the chunk's
 // statistics. Returns true if the stop rule was met or no refinement was
 // asked. With watch_stall, refinement also ends early when a step fails to
 // halve the worst stop measure (FP32 factors too inaccurate for this matrix).
-bool solve_chunk(bdlu_ctx* c, Plan& P, int32_t kt, int32_t kc, const bdlu_ir_opts* ir, double anorm,
+// The plan's own work set replays captured graphs; any other set (the
+// no-allocation entry point's) is launched directly, without profiling.
+bool solve_chunk(bdlu_ctx* c, Plan& P, const Work& W, int32_t kt, int32_t kc, const bdlu_ir_opts* ir, double anorm,
                  bool watch_stall, bdlu_solve_stats& cs) {
   hipStream_t st = c->stream;
+  const bool own = &W == &P.work;
+  const bool prof = own && c->prof;
+  auto solve = [&](const double* b, double* x) {
+    if (own) block_solve(P, st, b, x, kt, c->graphs && !c->prof);
+    else block_solve_launches(P, W, st, b, x, kt);
+  };
   const int64_t n = P.n;
   const int32_t max_iters = ir ? ir->max_iters : 0;
   const bool want_stats = max_iters > 0 || (ir && ir->stop_rule < 0);
   ProfEvents pev;
-  if (c->prof) pev.mark(BDLU_PROF_SOLVE, st);
-  block_solve(P, st, P.b, P.x, kt, c->graphs && !c->prof);
-  if (c->prof) {
+  if (prof) pev.mark(BDLU_PROF_SOLVE, st);
+  solve(W.b, W.x);
+  if (prof) {
     pev.mark(BDLU_PROF_SOLVE, st);
     HIPCHECK(hipStreamSynchronize(st));
     pev.collect(c);
@@ -1060,10 +1081,10 @@ bool solve_chunk(bdlu_ctx* c, Plan& P, int32_t kt, int32_t kc, const bdlu_ir_opt
   if (!want_stats) return true;
   double h[9 * kMaxRhs];
   auto eval = [&]() {
-    bdk::launch_residual(st, P.n, kt, P.csr_ptr, P.csr_col, P.vals, P.x, P.b, P.r, P.cw, n, P.res_is_long,
+    bdk::launch_residual(st, P.n, kt, P.csr_ptr, P.csr_col, P.vals, W.x, W.b, W.r, W.cw, n, P.res_is_long,
                          P.res_nlong, P.res_long_rows);
-    bdk::launch_norms(st, P.n, kt, P.r, P.x, P.b, P.cw, n, P.partials, P.norms);
-    HIPCHECK(hipMemcpyAsync(h, P.norms, sizeof(double) * 9 * kt, hipMemcpyDeviceToHost, st));
+    bdk::launch_norms(st, P.n, kt, W.r, W.x, W.b, W.cw, n, W.partials, W.norms);
+    HIPCHECK(hipMemcpyAsync(h, W.norms, sizeof(double) * 9 * kt, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
   };
   auto rel = [&](int k) {
@@ -1095,8 +1116,8 @@ bool solve_chunk(bdlu_ctx* c, Plan& P, int32_t kt, int32_t kc, const bdlu_ir_opt
   bool converged = max_iters <= 0 || met();
   double prev = worst();
   while (!converged && iters < max_iters) {
-    block_solve(P, st, P.r, P.d, kt, c->graphs && !c->prof);
-    bdk::launch_axpy(st, P.n, kt, P.d, P.x, n);
+    solve(W.r, W.d);
+    bdk::launch_axpy(st, P.n, kt, W.d, W.x, n);
     ++iters;
     eval();
     converged = met();
@@ -1273,7 +1294,8 @@ int bdlu_gpu_solve(bdlu_ctx* c, uint64_t id, double* B, int32_t nrhs, int64_t ld
     HIPCHECK(hipSetDevice(c->device));
     const hipMemcpyKind in = where == BDLU_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     const hipMemcpyKind out = where == BDLU_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-    bdk::launch_fill_i32(st, P.err, 1, 0);
+    const Work& W = P.work;
+    bdk::launch_fill_i32(st, W.err, 1, 0);
     double anorm = 0.0;
     if (want_stats) HIPCHECK(hipMemcpyAsync(&anorm, P.norms + kAnormSlot, sizeof(double), hipMemcpyDeviceToHost, st));
     // Columns in chunks of up to kMaxRhs, each solved (and refined) together.
@@ -1281,8 +1303,8 @@ int bdlu_gpu_solve(bdlu_ctx* c, uint64_t id, double* B, int32_t nrhs, int64_t ld
       const int32_t kc = std::min(kMaxRhs, nrhs - c0);
       const int32_t kt = kc <= 1 ? 1 : kc <= 2 ? 2 : kc <= 4 ? 4 : 8;
       for (int32_t k = 0; k < kc; ++k)
-        HIPCHECK(hipMemcpyAsync(P.b + k * n, B + (c0 + k) * ldb, sizeof(double) * n, in, st));
-      if (kt > kc) bdk::launch_fill_f64(st, P.b + kc * n, (kt - kc) * n, 0.0);
+        HIPCHECK(hipMemcpyAsync(W.b + k * n, B + (c0 + k) * ldb, sizeof(double) * n, in, st));
+      if (kt > kc) bdk::launch_fill_f64(st, W.b + kc * n, (kt - kc) * n, 0.0);
       if (c->autotune && !c->prof && !P.sched.solve_tuned) {
         autotune_solve(P, st, kt);
         P.sched.solve_tuned = true;
@@ -1291,15 +1313,15 @@ int bdlu_gpu_solve(bdlu_ctx* c, uint64_t id, double* B, int32_t nrhs, int64_t ld
       const bool fp32_factors = std::any_of(P.h_prec.begin(), P.h_prec.end(), [](int32_t p) { return p != 0; });
       const bool fallback = ir && ir->fp64_fallback && max_iters > 0 && fp32_factors;
       bdlu_solve_stats cs{};
-      if (!solve_chunk(c, P, kt, kc, ir, anorm, fallback, cs) && fallback) {
+      if (!solve_chunk(c, P, W, kt, kc, ir, anorm, fallback, cs) && fallback) {
         // FP32 factors did not reach the stop rule: refactor the same values in
-        // FP64 (P.b still holds this chunk) and solve the chunk again.
+        // FP64 (W.b still holds this chunk) and solve the chunk again.
         bdlu_factor_stats fs{};
         const int rc = refactor_impl(c, P, nullptr, BDLU_MEM_DEVICE, BDLU_FP64, fs);
         if (rc != BDLU_OK) return rc;
         ss.fp64_refactor = 1;
         const int32_t first_iters = cs.iterations;
-        solve_chunk(c, P, kt, kc, ir, anorm, false, cs);
+        solve_chunk(c, P, W, kt, kc, ir, anorm, false, cs);
         cs.iterations += first_iters;
       }
       ss.rel_residual = std::max(ss.rel_residual, cs.rel_residual);
@@ -1307,10 +1329,10 @@ int bdlu_gpu_solve(bdlu_ctx* c, uint64_t id, double* B, int32_t nrhs, int64_t ld
       ss.componentwise_error = std::max(ss.componentwise_error, cs.componentwise_error);
       ss.iterations = std::max(ss.iterations, cs.iterations);
       for (int32_t k = 0; k < kc; ++k)
-        HIPCHECK(hipMemcpyAsync(B + (c0 + k) * ldb, P.x + k * n, sizeof(double) * n, out, st));
+        HIPCHECK(hipMemcpyAsync(B + (c0 + k) * ldb, W.x + k * n, sizeof(double) * n, out, st));
     }
     int32_t herr = 0;
-    HIPCHECK(hipMemcpyAsync(&herr, P.err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(&herr, W.err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     if (stats) *stats = ss;
     if (herr) return set_err(c, BDLU_E_SOLVE, "solve wait bound exceeded");

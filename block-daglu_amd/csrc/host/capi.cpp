@@ -73,6 +73,7 @@ int bdlu_solver_solve_device(void* s, double* d_B, int32_t nrhs, int64_t ldb) {
 }
 void* bdlu_solver_device_context(void* s) { return s ? static_cast<Solver*>(s)->device_context() : nullptr; }
 void* bdlu_solver_device_stream(void* s) { return s ? static_cast<Solver*>(s)->device_stream() : nullptr; }
+uint64_t bdlu_solver_device_plan(void* s) { return s ? static_cast<Solver*>(s)->device_plan() : 0; }
 
 int bdlu_solver_last_error(void* s) { return s ? static_cast<int>(static_cast<Solver*>(s)->last_error()) : -1; }
 

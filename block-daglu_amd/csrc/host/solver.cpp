@@ -443,6 +443,8 @@ void* Solver::device_context() const {
   return context(msg);
 }
 
+uint64_t Solver::device_plan() const { return impl_->plan ? impl_->plan->id : 0; }
+
 void* Solver::device_stream() const {
   std::string msg;
   bdlu_ctx* c = context(msg);

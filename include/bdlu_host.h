@@ -64,8 +64,11 @@ void bdlu_solver_report(void* s, bdlu_report* out);
 // memory (HBM); same return convention.
 int bdlu_solver_factor_device(void* s, const double* d_values);
 int bdlu_solver_solve_device(void* s, double* d_B, int32_t nrhs, int64_t ldb);
-// Engine context (bdlu_ctx*, see bdlu_gpu.h) and its HIP stream.
+// Engine context (bdlu_ctx*, see bdlu_gpu.h), its HIP stream, and the id of the
+// resident device plan of the last analyze() (0 if none), for callers of the
+// device C-ABI.
 void* bdlu_solver_device_context(void* s);
+uint64_t bdlu_solver_device_plan(void* s);
 void* bdlu_solver_device_stream(void* s);
 // Structural permutations of the last analyze(); arrays of n (n+1 / nblocks+1 for R).
 int32_t bdlu_solver_nblocks(void* s);

@@ -106,6 +106,8 @@ class Solver final {
   // or time device work around these calls.
   void* device_context() const;
   void* device_stream() const;
+  // Id of the resident device plan of the last analyze() in that context (0 if none).
+  uint64_t device_plan() const;
   // Structural permutations of the BLOCK decomposition (perm index -> original).
   const std::vector<int32_t>& block_row_perm() const;
   const std::vector<int32_t>& block_col_perm() const;
