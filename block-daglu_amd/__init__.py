@@ -138,6 +138,7 @@ GPU_SYMBOLS = [
     ("bdlu_gpu_plan_schedule", C.c_int, [_P, C.c_uint64, _I32P]),
     ("bdlu_gpu_refactor", C.c_int, [_P, C.c_uint64, C.c_void_p, C.c_int, C.c_int, _P]),
     ("bdlu_gpu_solve", C.c_int, [_P, C.c_uint64, C.c_void_p, C.c_int32, C.c_int64, C.c_int, _P, _P]),
+    ("bdlu_gpu_solve_noalloc", C.c_int, [_P, C.c_uint64, C.c_void_p, C.c_int32, C.c_int64, _P, _P]),
     ("bdlu_gpu_plan_values", C.c_int, [_P, C.c_uint64, _F64P]),
     ("bdlu_gpu_stream", _P, [_P]),
     ("bdlu_gpu_profile", C.c_int, [_P, C.c_int]),

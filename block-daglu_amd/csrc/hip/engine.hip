@@ -202,7 +202,39 @@ struct Plan {
 
 }  // namespace
 
+// Scratch of the no-allocation solve (bdlu_gpu_solve_noalloc): the solve's
+// per-call vectors and counters, carved from one array in the code object's
+// device memory, so that entry point calls no allocator. Every GPU the library
+// runs on holds its own copy for the life of the process; host code reaches it
+// only through the per-device address hipGetSymbolAddress returns. Calls that
+// use it at the same time on one GPU race.
+constexpr int64_t kScratchDoubles = int64_t(16) << 20;  // 128 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_addr[kMaxDevices] = {};
+
+// The device address of `device`'s scratch array, looked up once per device
+// with that device current. Returns nullptr if the lookup fails.
+double* scratch_address(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_addr[device]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+    g_scratch_addr[device] = static_cast<double*>(ptr);
+  }
+  return g_scratch_addr[device];
+}
+
+}  // namespace
+
 struct bdlu_ctx {
+  // Device address of this device's no-allocation scratch (nullptr if unavailable).
+  double* scratch = nullptr;
   // Largest |L| a refactorization on the fixed pivot order may produce before
   // the block is refactored in FP64 (FP32 runs) or re-pivoted (FP64 runs).
   double fp32_growth = 1e4;
@@ -1170,6 +1202,7 @@ int bdlu_gpu_create(int device, uint64_t hbm_budget_bytes, bdlu_ctx** out) {
   if (const char* e = std::getenv("BLOCKDAGLU_GPU_LOOKAHEAD")) c->lookahead = std::atoi(e) != 0;
   bool ok = hipSetDevice(device) == hipSuccess &&
             hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
+  if (ok) c->scratch = scratch_address(device);
   for (int i = 0; ok && i < bdlu_ctx::kSide; ++i)
     ok = hipStreamCreateWithFlags(&c->side[i], hipStreamNonBlocking) == hipSuccess;
   if (!ok) {
@@ -1336,6 +1369,84 @@ int bdlu_gpu_solve(bdlu_ctx* c, uint64_t id, double* B, int32_t nrhs, int64_t ld
     HIPCHECK(hipStreamSynchronize(st));
     if (stats) *stats = ss;
     if (herr) return set_err(c, BDLU_E_SOLVE, "solve wait bound exceeded");
+    return BDLU_OK;
+  } catch (const HipError& e) {
+    return set_err(c, BDLU_E_BACKEND, e.msg);
+  }
+}
+
+int bdlu_gpu_solve_noalloc(bdlu_ctx* c, uint64_t id, double* d_B, int32_t nrhs, int64_t ldb,
+                           const bdlu_ir_opts* ir, bdlu_solve_stats* stats) {
+  if (!c || !d_B || nrhs < 1) return BDLU_E_INVALID;
+  std::lock_guard<std::mutex> g(c->mu);
+  auto it = c->plans.find(id);
+  if (it == c->plans.end()) return set_err(c, BDLU_E_INVALID, "unknown plan");
+  Plan& P = *it->second;
+  if (!P.factored) return set_err(c, BDLU_E_SOLVE, "plan has no valid factors");
+  if (ldb < P.n) return set_err(c, BDLU_E_INVALID, "ldb < n");
+  if (!c->scratch) return set_err(c, BDLU_E_BACKEND, "no scratch address on this device");
+  hipStream_t st = c->stream;
+  const int32_t max_iters = ir ? ir->max_iters : 0;
+  const bool want_stats = max_iters > 0 || (ir && ir->stop_rule < 0);
+  const int64_t n = P.n;
+  // Work set in the scratch array: b, x, bperm, xperm (+ r, d, cw with
+  // refinement) of n x kt each, the norm partials, the norms and the counters,
+  // each piece starting on a 256-byte boundary.
+  const int32_t kt_max = nrhs <= 1 ? 1 : nrhs <= 2 ? 2 : nrhs <= 4 ? 4 : kMaxRhs;
+  auto round32 = [](int64_t v) { return (v + 31) / 32 * 32; };
+  const int64_t vec = round32(n * kt_max);
+  const int64_t nvec = want_stats ? 7 : 4;
+  const int64_t need = nvec * vec + round32(512 * 6 * kMaxRhs) + round32(9 * kMaxRhs) +
+                       round32((static_cast<int64_t>(P.nticket) + 1 + 1) / 2);
+  if (need > kScratchDoubles)
+    return set_err(c, BDLU_E_INVALID, "solve needs " + std::to_string(need) + " doubles of scratch, more than the " +
+                                          std::to_string(kScratchDoubles) + " of the no-allocation entry point");
+  Work W;
+  double* p = c->scratch;
+  W.b = p, p += vec;
+  W.x = p, p += vec;
+  W.bperm = p, p += vec;
+  W.xperm = p, p += vec;
+  if (want_stats) {
+    W.r = p, p += vec;
+    W.d = p, p += vec;
+    W.cw = p, p += vec;
+  }
+  W.partials = p, p += round32(512 * 6 * kMaxRhs);
+  W.norms = p, p += round32(9 * kMaxRhs);
+  W.tickets = reinterpret_cast<int32_t*>(p);
+  W.err = W.tickets + P.nticket;
+  bdlu_solve_stats ss{0, -1.0, -1.0, -1.0, 0};
+  bool all_converged = true;
+  try {
+    HIPCHECK(hipSetDevice(c->device));
+    bdk::launch_fill_i32(st, W.err, 1, 0);
+    double anorm = 0.0;
+    if (want_stats) HIPCHECK(hipMemcpyAsync(&anorm, P.norms + kAnormSlot, sizeof(double), hipMemcpyDeviceToHost, st));
+    for (int32_t c0 = 0; c0 < nrhs; c0 += kMaxRhs) {
+      const int32_t kc = std::min(kMaxRhs, nrhs - c0);
+      const int32_t kt = kc <= 1 ? 1 : kc <= 2 ? 2 : kc <= 4 ? 4 : 8;
+      for (int32_t k = 0; k < kc; ++k)
+        HIPCHECK(hipMemcpyAsync(W.b + k * n, d_B + (c0 + k) * ldb, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+      if (kt > kc) bdk::launch_fill_f64(st, W.b + kc * n, (kt - kc) * n, 0.0);
+      bdlu_solve_stats cs{};
+      all_converged = solve_chunk(c, P, W, kt, kc, ir, anorm, false, cs) && all_converged;
+      ss.rel_residual = std::max(ss.rel_residual, cs.rel_residual);
+      ss.backward_error = std::max(ss.backward_error, cs.backward_error);
+      ss.componentwise_error = std::max(ss.componentwise_error, cs.componentwise_error);
+      ss.iterations = std::max(ss.iterations, cs.iterations);
+      for (int32_t k = 0; k < kc; ++k)
+        HIPCHECK(hipMemcpyAsync(d_B + (c0 + k) * ldb, W.x + k * n, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    }
+    int32_t herr = 0;
+    HIPCHECK(hipMemcpyAsync(&herr, W.err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if (stats) *stats = ss;
+    if (herr) return set_err(c, BDLU_E_SOLVE, "solve wait bound exceeded");
+    const bool fp32_factors = std::any_of(P.h_prec.begin(), P.h_prec.end(), [](int32_t q) { return q != 0; });
+    if (!all_converged && ir && ir->fp64_fallback && fp32_factors)
+      return set_err(c, BDLU_E_SOLVE, "refinement on FP32 factors did not reach the stop rule (this entry point "
+                                      "does not refactor in FP64)");
     return BDLU_OK;
   } catch (const HipError& e) {
     return set_err(c, BDLU_E_BACKEND, e.msg);

@@ -152,6 +152,21 @@ int bdlu_gpu_refactor(bdlu_ctx* ctx, uint64_t plan_id, const double* values, int
 int bdlu_gpu_solve(bdlu_ctx* ctx, uint64_t plan_id, double* B, int32_t nrhs, int64_t ldb, int where,
                    const bdlu_ir_opts* ir, bdlu_solve_stats* stats);
 
+// No counterpart in the reference. For callers that cannot allocate (code under
+// HIP graph capture, tight serving loops, harnesses that forbid allocation):
+// bdlu_gpu_solve with B in device memory and no allocation of any kind. The
+// solve's per-call vectors and counters come from a static device array in the
+// library (128 MiB per GPU, kept for the life of the process) instead of the
+// plan, launches are issued directly (no graph is instantiated, no autotuning),
+// and FP32 factors that do not reach the stop rule return BDLU_E_SOLVE rather
+// than being refactored in FP64. Limit: 4 vectors of n x k doubles (7 with
+// refinement), k = nrhs rounded up to 1, 2, 4 or 8, plus a few KiB, must fit
+// in 16Mi doubles; larger requests return BDLU_E_INVALID. Results equal
+// bdlu_gpu_solve's bit for bit. Calls on one GPU share the array: calls that
+// overlap in time on one GPU (from any context) race.
+int bdlu_gpu_solve_noalloc(bdlu_ctx* ctx, uint64_t plan_id, double* d_B, int32_t nrhs, int64_t ldb,
+                           const bdlu_ir_opts* ir, bdlu_solve_stats* stats);
+
 // Copies the plan's current canonical-order values (last refactor) to host memory.
 int bdlu_gpu_plan_values(bdlu_ctx* ctx, uint64_t plan_id, double* host_values);
 
