@@ -1,19 +1,29 @@
 #!/usr/bin/env python3
 """Refactor+solve throughput of the MI355X BLOCK engine.
 
-Workload (one GPU): the circuit-like system — n = 200,000, ~1.29M nonzeros,
-one 20,000 power-law block plus 9,000 blocks of size 1-40, coupling 0.5/row,
-hidden by random row/column permutations (seeded generator). A step is one
-refactorization of the resident pattern with new values (10 pre-generated
-+-5% perturbations, cycled, already in HBM) followed by one solve of a 1-RHS
-system (RHS in HBM). Reported:
-  value            FP32 factors + FP64 refinement to ||b-Ax||/||b|| < 1e-12
-  fp64             the plain FP64 GPU path (no refinement, the reference's default)
-  latency p50/p99  per step, host clock around the synchronous calls
-  roofline         the dominant kernel (dense-tail trailing update), per-launch
-                   algorithmic bytes/flops over its average HIP-event duration
-  cpu_baseline     the CPU restatement of the reference BLOCK route (oracle/),
-                   one request = set_matrix -> analyze -> factor -> solve
+Default workload (one GPU): the serving stream. 16 seeded sparsity patterns
+built like the circuit-like system (n log-uniform in 1e3..1e5, one giant block
+of 10% of n, blocks of size 1-40, coupling 0.5/row, hidden by random row and
+column permutations); each request brings +-5% new values for one pattern and
+1-8 right-hand sides. Requests run on 4 slots (engine contexts) under one 64 GB
+HBM budget, with symbolic state and factors resident. Reported:
+  value            closed-loop capacity (refactor+solve requests/s) with FP32
+                   factors + FP64 refinement to ||b-Ax||/||b|| < 1e-12; a step
+                   is a batch of 100 requests
+  p50/p99          arrival-to-result latency of 2000 Poisson arrivals at 0.8 x
+                   the FP64 capacity (same rate for both precisions)
+  fp64_plain       the same for the plain FP64 GPU path (no refinement, the
+                   reference's default)
+  closed_loop      refactor+solve steps on the circuit-like n = 200,000 system
+                   (one 20,000 power-law block + 9,000 blocks of size 1-40):
+                   plain FP64, FP64 + refinement, FP32 + FP64 refinement
+  roofline         the dominant kernel of that system (the fused dense-tail
+                   trailing update), per-launch algorithmic bytes/flops over its
+                   average HIP-event duration in the timed launch structure
+  cpu_baseline     the CPU restatement of the reference BLOCK route (oracle/)
+                   on complete requests of the serving stream, each
+                   set_matrix -> analyze -> factor -> solve
+--config 0..3 runs the closed loop on that configuration instead.
 """
 from __future__ import annotations
 
@@ -36,14 +46,15 @@ COPY_BYTES = 1 << 30  # the plain copy that calibrates the HBM byte counters
 MFMA_PEAK = {"f64": 78.6, "f32": 157.3}  # TFLOP/s dense
 
 
-TRAFFIC_FILE = ROOT / "profiles" / "traffic_tail_update_circuit200k.json"
-ROOFLINE_KERNEL = "k_tail_update<float, *>"
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_tail_update_diag_circuit200k.json"
+ROOFLINE_KERNEL = "k_tail_update_diag<float, *>"
 
 
 def is_roofline_kernel(name: str) -> bool:
-    """Both tile variants of the FP32 trailing update, by demangled or mangled
-    name (not the fused launch k_tail_update_diag of the timed steps)."""
-    return "k_tail_update<float," in name or "k_tail_updateIfLi" in name
+    """Both tile variants of the FP32 fused trailing update (the launch the timed
+    steps run: the trailing update plus the next diagonal block's LU), by
+    demangled or mangled name."""
+    return "k_tail_update_diag<float," in name or "k_tail_update_diagIfLi" in name
 
 
 def _db_view(db, name):
@@ -239,235 +250,267 @@ def run_gpu(bd, n, r, c, v_sets, b, precision, steps, warmup, profile):
 CPU_SCRIPT = r"""
 import sys, time
 sys.path.insert(0, sys.argv[1])
-import importlib.util
-spec = importlib.util.spec_from_file_location("blockdaglu_amd", sys.argv[1] + "/block-daglu_amd/__init__.py")
-bd = importlib.util.module_from_spec(spec); sys.modules["blockdaglu_amd"] = bd; spec.loader.exec_module(bd)
-from oracle import oracle as orc
-n, r, c, v, B = bd.generate(int(sys.argv[2]), keep_order=int(sys.argv[3]))
-o = orc.OracleSolver(r, c, v)
-t = time.perf_counter(); assert o.analyze() == 0; print("analyze", time.perf_counter() - t, flush=True)
-t = time.perf_counter(); assert o.factor() == 0; print("factor", time.perf_counter() - t, o.info()["threads"], flush=True)
-t = time.perf_counter(); rc, x = o.solve(B[:, 0]); print("solve", time.perf_counter() - t, flush=True)
+import bench
+bench.cpu_child(sys.argv[2:])
 """
 
 
-def cpu_baseline(config, budget_s, keep_order=0):
-    """The oracle (CPU restatement of the reference BLOCK route) on one request of
-    the same workload: set_matrix -> analyze -> factor -> solve (the reference API
-    has no pattern-only refactor), reference thread policy (src/solver.cpp:366-390),
-    in a child process stopped after budget_s seconds. If the request completes,
-    its throughput is measured; otherwise 1/budget_s is an upper bound on it and
-    the sample says which phases completed."""
-    import subprocess
-    cmd = [sys.executable, "-c", CPU_SCRIPT, str(ROOT), str(config), str(keep_order)]
-    t0 = time.perf_counter()
-    proc = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True)
+def cpu_info() -> dict:
+    """The host CPU: model, physical cores of the machine and the CPUs this
+    process may run on."""
+    info = {"model": "unknown", "physical_cores": None, "allowed_cpus": len(os.sched_getaffinity(0))}
     try:
-        out, _ = proc.communicate(timeout=budget_s)
-        complete = proc.returncode == 0
-    except subprocess.TimeoutExpired:
-        proc.kill()  # the child started here, by its own PID
-        out, _ = proc.communicate()
-        complete = False
-    wall = time.perf_counter() - t0
-    phases = {}
-    threads = 1
-    for line in out.splitlines():
-        f = line.split()
-        if len(f) >= 2 and f[0] in ("analyze", "factor", "solve"):
-            phases[f[0]] = float(f[1])
-            if f[0] == "factor" and len(f) > 2:
-                threads = int(f[2])
-    done = ", ".join(f"{k} {v:.2f}s" for k, v in phases.items()) or "none"
-    if complete and len(phases) == 3:
-        total = sum(phases.values())
-        return {"value": 1.0 / total, "unit": "refactor+solve/s", "cores": threads, "kind": "port",
-                "sample": f"one complete request of the same workload ({done}); factor threads per the "
-                          f"reference policy", "measured": True}
-    return {"value": 1.0 / wall, "unit": "refactor+solve/s", "cores": max(threads, 1), "kind": "port",
-            "sample": f"one request of the same workload stopped at the {budget_s:.0f} s sample limit; phases "
-                      f"completed: {done}; value = 1/{wall:.1f} s is an upper bound on CPU throughput",
-            "measured": False}
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = {}
+        for line in out.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        info["model"] = kv.get("Model name", "unknown")
+        if "Core(s) per socket" in kv and "Socket(s)" in kv:
+            info["physical_cores"] = int(kv["Core(s) per socket"]) * int(kv["Socket(s)"])
+    except Exception:
+        pass
+    return info
 
 
-def bench_serving(bd, args):
-    """Serving stream: 16 seeded patterns (config 4), 2000 requests with Poisson
-    arrivals at a fixed rate (0.8 x the measured FP64 capacity, the same rate for
-    both precisions), each a +-5% value perturbation of its pattern with 1-8
-    right-hand sides; 4 slots, 64 GB HBM budget."""
-    import threading
-    rng = np.random.default_rng(4)
+def serving_stream(bd, nreq, seed=4):
+    """The serving workload: 16 seeded patterns built like the circuit-like
+    system (n log-uniform in 1e3..1e5), six +-5% value sets per pattern, and a
+    request stream (pattern, value set, right-hand-side count 1..8, exponential
+    inter-arrival gaps for unit rate) from one seeded generator."""
+    rng = np.random.default_rng(seed)
     pats = []
-    t = time.time()
     for p in range(16):
         n, r, c, v, _ = bd.generate(4, pattern=p)
         vals = [bd.perturb(v, seed=40_000 + 16 * p + i) for i in range(6)]
         rhs = np.asfortranarray(rng.uniform(-1, 1, (n, 8)))
         pats.append((n, r, c, v, vals, rhs))
-    log(f"serving: 16 patterns n={[p[0] for p in pats]} generated in {time.time() - t:.1f}s")
-    nreq = args.requests
     req_pat = rng.integers(0, 16, nreq)
     req_val = rng.integers(0, 6, nreq)
     req_k = rng.integers(1, 9, nreq)
-    gaps = rng.exponential(1.0, nreq)  # scaled by 1/lambda below
-    out = {}
+    gaps = rng.exponential(1.0, nreq)
+    return pats, req_pat, req_val, req_k, gaps
+
+
+def cpu_child(argv):
+    """Child process of the CPU baseline: the oracle (CPU restatement of the
+    reference BLOCK route) runs complete requests one after another, each
+    set_matrix -> analyze -> factor -> solve of every right-hand side (the
+    reference API has no pattern-only refactor), and prints one line per
+    completed request. argv: config, keep_order, threads (0: the reference's
+    thread policy), requests."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("blockdaglu_amd", ROOT / "block-daglu_amd" / "__init__.py")
+    bd = importlib.util.module_from_spec(spec)
+    sys.modules["blockdaglu_amd"] = bd
+    spec.loader.exec_module(bd)
+    from oracle import oracle as orc
+    config, keep_order, threads, nreq = (int(a) for a in argv[:4])
+    if config == 4:
+        pats, req_pat, req_val, req_k, _ = serving_stream(bd, nreq)
+        items = [(pats[req_pat[i]], pats[req_pat[i]][4][req_val[i]], pats[req_pat[i]][5][:, :req_k[i]])
+                 for i in range(nreq)]
+        items = ((p[0], p[1], p[2], vals, B) for p, vals, B in items)
+    else:
+        n, r, c, v, B = bd.generate(config, keep_order=keep_order)
+        items = ((n, r, c, bd.perturb(v, seed=1000 + i) if i else v, B) for i in range(nreq))
+    for (n, r, c, vals, B) in items:
+        t0 = time.perf_counter()
+        o = orc.OracleSolver(r, c, vals, n)
+        assert o.analyze() == 0
+        t1 = time.perf_counter()
+        assert o.factor(threads) == 0
+        t2 = time.perf_counter()
+        for j in range(B.shape[1]):
+            rc, _ = o.solve(np.ascontiguousarray(B[:, j]))
+            assert rc == 0
+        t3 = time.perf_counter()
+        print(f"request {t3 - t0:.6f} analyze {t1 - t0:.6f} factor {t2 - t1:.6f} solve {t3 - t2:.6f} "
+              f"threads {o.info()['threads']}", flush=True)
+
+
+def cpu_baseline(config, budget_s, keep_order=0, nreq=1000):
+    """The oracle on complete requests of the same workload, in a child process
+    on this host, run twice: with the reference's thread policy
+    (src/solver.cpp:341-390) and with one factor thread per allowed physical
+    core. Each run keeps issuing requests in stream order until budget_s
+    seconds have passed (then the child is stopped) and reports completed
+    requests per second over the completed requests only, with the median and
+    p95 request time; a run that completes none reports nothing measured."""
+    import subprocess
+    cpu = cpu_info()
+    cores = min(x for x in (cpu["physical_cores"], cpu["allowed_cpus"]) if x)
+    runs = {}
+    for label, threads in (("reference_policy", 0), ("physical_cores", cores)):
+        cmd = [sys.executable, "-c", CPU_SCRIPT, str(ROOT), str(config), str(keep_order), str(threads), str(nreq)]
+        proc = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True)
+        try:
+            out, _ = proc.communicate(timeout=budget_s)
+        except subprocess.TimeoutExpired:
+            proc.kill()  # the child started here, by its own PID
+            out, _ = proc.communicate()
+        reqs = [dict(zip(f.split()[0::2], map(float, f.split()[1::2]))) for f in out.splitlines()
+                if f.startswith("request ")]
+        if not reqs:
+            runs[label] = {"measured": False, "completed": 0, "budget_s": budget_s}
+            continue
+        t = np.array([q["request"] for q in reqs])
+        runs[label] = {"measured": True, "completed": len(reqs), "value": len(reqs) / t.sum(),
+                       "median_ms": float(np.median(t) * 1e3), "p95_ms": float(np.percentile(t, 95) * 1e3),
+                       "threads": int(reqs[0]["threads"]),
+                       "analyze_ms": float(np.median([q["analyze"] for q in reqs]) * 1e3),
+                       "factor_ms": float(np.median([q["factor"] for q in reqs]) * 1e3),
+                       "solve_ms": float(np.median([q["solve"] for q in reqs]) * 1e3)}
+        log(f"cpu baseline ({label}): {runs[label]}")
+    best = max((r for r in runs.values() if r["measured"]), key=lambda r: r["value"], default=None)
+    if best is None:
+        return {"value": None, "unit": "refactor+solve/s", "cores": cores, "kind": "port", "measured": False,
+                "sample": f"no request of the workload completed within {budget_s:.0f} s", "cpu": cpu, "runs": runs}
+    return {"value": best["value"], "unit": "refactor+solve/s", "cores": best["threads"], "kind": "port",
+            "measured": True,
+            "sample": f"complete requests of the same workload in stream order for up to {budget_s:.0f} s per "
+                      f"thread setting (set_matrix -> analyze -> factor -> solve each); value = the faster "
+                      f"setting, completed requests / their total time",
+            "cpu": cpu, "runs": runs}
+
+
+REQS_PER_STEP = 100  # serving stream: one step = one batch of 100 requests
+
+
+def bench_serving(bd, args):
+    """Serving stream: 16 seeded patterns (n 1e3..1e5), each request a +-5%
+    value perturbation of its pattern with 1-8 right-hand sides; 4 slots, one
+    64 GB HBM budget. Per precision:
+      warm-up    every pattern once on every slot, then W x 100 requests;
+      capacity   K x 100 requests queued at once (closed loop), timed from the
+                 first submission to the last result: the refactor+solve/s value;
+      open loop  the --requests stream with Poisson arrivals at a fixed rate
+                 (0.8 x the measured FP64 capacity, the same rate for both
+                 precisions): arrival-to-result p50/p99 latency.
+    FP32+IR solutions of the first 100 open-loop requests are compared with the
+    FP64 ones of the same requests (forward difference)."""
+    import threading
+    t = time.time()
+    nstream = max(args.requests, (args.steps + args.warmup) * REQS_PER_STEP)
+    pats, req_pat, req_val, req_k, gaps = serving_stream(bd, nstream)
+    log(f"serving: 16 patterns n={[p[0] for p in pats]} generated in {time.time() - t:.1f}s")
+    out, keep = {}, {}
     lam = None
+    ncap = args.steps * REQS_PER_STEP
+    nwarm = args.warmup * REQS_PER_STEP
+
+    def submit(srv, pid, i):
+        p = req_pat[i]
+        return srv.submit(pid[p], pats[p][4][req_val[i]], pats[p][5][:, :req_k[i]])
+
     for prec in (bd.Precision.FP64, bd.Precision.FP32_IR):
         srv = bd.Server(slots=args.slots, hbm_budget_gb=64.0, precision=prec)
         t = time.time()
         pid = []
         for k, p in enumerate(pats):
             pid.append(srv.add_pattern(p[1], p[2], p[3], p[0]))
-            log(f"  {prec.name}: pattern {k} (n={p[0]}) registered, {time.time() - t:.1f}s")
         t_an = time.time() - t
-        # warm every slot on every pattern (plan upload, first factor)
+        log(f"  {prec.name}: 16 patterns registered (host analysis) in {t_an:.1f}s")
         ids = [srv.submit(pid[p], pats[p][4][0], pats[p][5][:, :1]) for p in range(16) for _ in range(args.slots)]
-        for k, i in enumerate(ids):
+        for i in ids:
             srv.wait(i)
-            if (k + 1) % args.slots == 0:
-                log(f"  {prec.name}: warm-up, pattern {k // args.slots} on every slot, {time.time() - t:.1f}s")
+        ids = [submit(srv, pid, i) for i in range(nwarm)]
+        for i in ids:
+            srv.wait(i)
+        log(f"  {prec.name}: warm-up done ({16 * args.slots} + {nwarm} requests), {time.time() - t:.1f}s")
         # capacity: closed loop, all requests queued at once
-        ncap = min(nreq, 400)
-        t = time.perf_counter()
-        ids = [srv.submit(pid[req_pat[i]], pats[req_pat[i]][4][req_val[i]], pats[req_pat[i]][5][:, :req_k[i]])
-               for i in range(ncap)]
+        t0 = time.perf_counter()
+        ids = [submit(srv, pid, i % nstream) for i in range(nwarm, nwarm + ncap)]
+        step_ends = []
         for k, i in enumerate(ids):
             srv.wait(i)
-            if (k + 1) % 100 == 0:
-                log(f"  {prec.name}: capacity run {k + 1}/{ncap}")
-        cap = ncap / (time.perf_counter() - t)
+            if (k + 1) % REQS_PER_STEP == 0:
+                step_ends.append(time.perf_counter() - t0)
+        wall_cap = time.perf_counter() - t0
+        cap = ncap / wall_cap
         if lam is None:
             lam = 0.8 * cap
-        log(f"  {prec.name}: analysis {t_an:.1f}s, closed-loop capacity {cap:.1f} req/s; open loop at {lam:.1f} req/s")
+        step_ms = np.diff(np.concatenate([[0.0], step_ends])) * 1e3
+        log(f"  {prec.name}: closed-loop capacity {cap:.1f} req/s over {ncap} requests; open loop at {lam:.1f} req/s")
         # open loop
-        arrivals = np.cumsum(gaps / lam)
-        ids = [None] * nreq
+        nreq = args.requests
+        arrivals = np.cumsum(gaps[:nreq] / lam)
+        oids = [None] * nreq
 
         def feeder():
-            t0 = time.perf_counter()
+            t_feed = time.perf_counter()
             for i in range(nreq):
-                d = arrivals[i] - (time.perf_counter() - t0)
+                d = arrivals[i] - (time.perf_counter() - t_feed)
                 if d > 0:
                     time.sleep(d)
-                p = req_pat[i]
-                ids[i] = srv.submit(pid[p], pats[p][4][req_val[i]], pats[p][5][:, :req_k[i]])
+                oids[i] = submit(srv, pid, i)
 
         th = threading.Thread(target=feeder)
-        t = time.perf_counter()
+        t0 = time.perf_counter()
         th.start()
-        res = []
+        res, xs = [], []
         for i in range(nreq):
-            while ids[i] is None:
+            while oids[i] is None:
                 time.sleep(0.0002)
-            X, rr = srv.wait(ids[i])
+            X, rr = srv.wait(oids[i])
             res.append(rr)
+            if i < 100:
+                xs.append(X)
             if (i + 1) % 500 == 0:
                 log(f"    {i + 1}/{nreq} done")
         th.join()
-        wall = time.perf_counter() - t
+        wall = time.perf_counter() - t0
+        keep[prec.name] = xs
         lat = np.array([r["t_done_ms"] - r["t_submit_ms"] for r in res])
+        qwait = np.array([r["t_start_ms"] - r["t_submit_ms"] for r in res])
         errs = [r for r in res if r["error"] != bd.SolverError.None_]
         out[prec.name] = {
-            "capacity_req_per_s": cap, "offered_req_per_s": lam, "served_req_per_s": nreq / wall,
-            "solves_per_s": float(req_k.sum()) / wall, "p50_ms": float(np.percentile(lat, 50)),
-            "p99_ms": float(np.percentile(lat, 99)), "max_rel_residual": max(r["rel_residual"] for r in res),
+            "capacity_req_per_s": cap, "capacity_ms_per_step": float(wall_cap / args.steps * 1e3),
+            "step_ms_p50": float(np.percentile(step_ms, 50)), "step_ms_p99": float(np.percentile(step_ms, 99)),
+            "offered_req_per_s": lam, "served_req_per_s": nreq / wall,
+            "solves_per_s": float(req_k[:nreq].sum()) / wall, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "queue_wait_p99_ms": float(np.percentile(qwait, 99)),
+            "max_rel_residual": max(r["rel_residual"] for r in res),
             "max_backward_error": max(r["backward_error"] for r in res),
             "max_ir_iterations": max(r["ir_iterations"] for r in res),
             "fp64_fallback_blocks": int(sum(r["fp64_fallback_blocks"] for r in res)),
+            "fp64_refactors": int(sum(r["ir_fp64_refactor"] for r in res)),
             "repivots": int(sum(r["repivots"] for r in res)), "errors": len(errs), "analysis_s": t_an}
         log(f"  {prec.name}: p50 {out[prec.name]['p50_ms']:.2f} ms p99 {out[prec.name]['p99_ms']:.2f} ms "
-            f"served {out[prec.name]['served_req_per_s']:.1f} req/s errors {len(errs)}")
+            f"served {out[prec.name]['served_req_per_s']:.1f} req/s errors {len(errs)} "
+            f"max rel residual {out[prec.name]['max_rel_residual']:.2e}")
         srv.close()
-    line = {"metric": "served refactor+solve/s, p50/p99 latency (serving stream, 16 patterns)",
-            "value": out["FP32_IR"]["capacity_req_per_s"], "unit": "refactor+solve/s (closed-loop capacity)",
-            "n_gpus": 1, "higher_is_better": True, "dtype": "f32 factors + f64 refinement",
-            "data": "synthetic (seeded generator)",
-            "config": {"workload": f"serving: 16 patterns n=1e3..1e5, {nreq} Poisson requests at "
-                                   f"{lam:.1f}/s, 1-8 RHS, {args.slots} slots, 64 GB budget"},
-            "fp32ir": out["FP32_IR"], "fp64": out["FP64"],
-            "fp32ir_over_fp64_capacity": out["FP32_IR"]["capacity_req_per_s"] / out["FP64"]["capacity_req_per_s"],
-            "build": bd.build_info()}
-    print(json.dumps(line))
+    fwd = max(float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(keep["FP32_IR"], keep["FP64"]))
+    return out, lam, fwd
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", type=int, default=1,
-                    help="workload: 1 circuit-like n=200,000 (default); 4 serving stream")
-    ap.add_argument("--requests", type=int, default=2000, help="serving stream length (config 4)")
-    ap.add_argument("--slots", type=int, default=4, help="serving slots (config 4)")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=25.0, help="seconds of CPU baseline sample")
-    ap.add_argument("--no-profile", action="store_true")
-    ap.add_argument("--keep-order", action="store_true",
-                    help="skip the generator's structure-hiding permutation (same blocks and values); the "
-                         "batched-blocks workload needs it for a host analysis of seconds")
-    ap.add_argument("--ir-stop", choices=["residual", "backward", "componentwise"],
-                    help="refinement stop rule at 1e-12 (default: componentwise for the scaled stress case, "
-                         "the reference's ||r||/||b|| otherwise)")
-    ap.add_argument("--profile-summary", metavar="DB", help="print the kernel table of a rocprofv3 database and exit")
-    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
-                    help="compute HBM bytes per launch of the roofline kernel from two --pmc runs, write "
-                         "profiles/traffic_tail_update_circuit200k.json and exit")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="first run a plain 1 GiB device-to-device copy (the known-size reference kernel for "
-                         "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs)")
-    args = ap.parse_args()
-    if args.profile_summary:
-        print(profile_summary(args.profile_summary))
-        return
-    if args.traffic:
-        t = traffic_from_counters(*args.traffic)
-        t["kernel"] = ROOFLINE_KERNEL
-        TRAFFIC_FILE.write_text(json.dumps(t, indent=1) + "\n")
-        print(json.dumps(t, indent=1))
-        return
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build's path has no multi-GPU mode (see the sharding note "
-              "in the design document); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
-    bd = load_package()
-    if not bd.gpu_available():
-        print("no GPU visible", file=sys.stderr)
-        sys.exit(1)
-    if args.copy_calibration:
-        x = bd.DeviceArray.from_numpy(np.ones(COPY_BYTES // 8))
-        y = bd.DeviceArray(COPY_BYTES // 8)
-        for _ in range(3):
-            y.copy_from(x)
-        bd.hip().hipDeviceSynchronize()
-        x.free()
-        y.free()
-        log(f"copy calibration: 3 x {COPY_BYTES} B device-to-device")
-    if args.config == 4:
-        bench_serving(bd, args)
-        return
+def closed_loop(bd, args, config):
+    """Closed-loop refactor+solve steps on one system (configs 0-3): plain FP64,
+    FP64 + FP64 refinement, and FP32 factors + FP64 refinement, with the
+    dominant kernel's roofline from a profiling pass over the same steps."""
     t = time.time()
-    n, r, c, v, B = bd.generate(args.config, keep_order=int(args.keep_order))
+    n, r, c, v, B = bd.generate(config, keep_order=int(args.keep_order))
     b = np.asfortranarray(B)  # n x k right-hand sides
     v_sets = [v] + [bd.perturb(v, seed=1000 + i) for i in range(1, 10)]
-    log(f"workload n={n} nnz={len(r)} rhs={b.shape[1]} generated in {time.time() - t:.1f}s")
+    log(f"workload config {config}: n={n} nnz={len(r)} rhs={b.shape[1]} generated in {time.time() - t:.1f}s")
     workload = {
+        0: f"n={n} nnz={len(r)}: 200 hidden blocks of geometric size (mean 25), 10 refactors at +-5%, 1 RHS",
         1: f"circuit-like n={n} nnz={len(r)}: one 20000 power-law block + 9000 blocks of size 1-40, "
            f"10 refactors at +-5%, 1 RHS",
         2: f"batched blocks n={n} nnz={len(r)}: 50000 blocks log-uniform 1-64 + 8 dense-ish 512-2048 (30% fill), "
            f"10 refactors at +-5%, {b.shape[1]} RHS",
         3: f"mixed-precision stress n={n} nnz={len(r)}: circuit-like pattern, rows/cols scaled by 10^U(-4,4), "
            f"10 refactors at +-5%, 1 RHS, FP32+IR to backward error <= 1e-12",
-    }.get(args.config, f"config {args.config} n={n} nnz={len(r)}")
+    }.get(config, f"config {config} n={n} nnz={len(r)}")
     if args.keep_order:
         workload += (" (generated order kept: the random permutation that hides the block structure is skipped, "
                      "so the one-time host analysis is seconds instead of a long maximum-transversal search; "
                      "blocks, fills, coupling and values are the same)")
-
-    # Plain FP64 (the reference default, no refinement), FP64 + FP64 refinement to
-    # ||r||/||b|| < 1e-12 (REFINE_ITERS semantics), and FP32 factors + FP64 refinement.
-    # The badly scaled stress case stops both refinements on componentwise backward
-    # error (scale-invariant); ||r||/||b|| is dominated by the largest rows there.
-    ir_stop = args.ir_stop or ("componentwise" if args.config == 3 else "residual")
+    # The badly scaled stress case stops both refinements on componentwise
+    # backward error (scale-invariant); ||r||/||b|| is dominated by the largest rows there.
+    ir_stop = args.ir_stop or ("componentwise" if config == 3 else "residual")
     os.environ.pop("USERALGO_REFINE_ITERS", None)
     os.environ.pop("BLOCKDAGLU_GPU_IR_STOP", None)
     res64p = run_gpu(bd, n, r, c, v_sets, b, 0, args.steps, args.warmup, False)
@@ -488,7 +531,8 @@ def main():
                 "rel_residual": rel_residual(n, r, c, last_vals, x, b),
                 "backward_error": backward_error(n, r, c, last_vals, x, b),
                 "componentwise_error": componentwise_error(n, r, c, last_vals, x, b),
-                "ir_iterations": rep["ir_iterations"], "fp64_fallback_blocks": rep["fp64_fallback_blocks"]}
+                "ir_iterations": rep["ir_iterations"], "fp64_fallback_blocks": rep["fp64_fallback_blocks"],
+                "ir_fp64_refactor": rep["ir_fp64_refactor"]}
 
     s64p, s64, s32 = summarize(res64p), summarize(res64), summarize(res32)
     prof = res32[6]
@@ -500,9 +544,8 @@ def main():
         byts = g["bytes"] / g["launches"]
         tf = flops / avg_s / 1e12
         gbs = byts / avg_s / 1e9
-        intensity = flops / byts
         ridge = MFMA_PEAK["f32"] * 1e12 / (HBM_PEAK_GBS * 1e9)
-        if intensity < ridge:
+        if flops / byts < ridge:
             roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
         else:
             roof = {"bound": "mfma", "achieved": tf, "peak": MFMA_PEAK["f32"], "unit": "TFLOP/s",
@@ -510,33 +553,17 @@ def main():
         traffic = None
         if TRAFFIC_FILE.exists():
             traffic = json.loads(TRAFFIC_FILE.read_text())["bytes_per_launch"]
-        roof.update({"traffic": traffic, "kernel": ROOFLINE_KERNEL, "avg_launch_us": avg_s * 1e6,
-                     "launches_per_step": g["launches"] / args.steps, "algorithmic_bytes_per_launch": byts,
-                     "algorithmic_flops_per_launch": flops, "tflops": tf, "gbs": gbs})
-    cpu = None
-    if not args.no_cpu_baseline:
-        log("cpu baseline (oracle) ...")
-        cpu = cpu_baseline(args.config, args.cpu_budget, int(args.keep_order))
-        log(f"cpu baseline: {cpu['sample']}")
-    out = {
-        "metric": METRIC,
-        "value": s32["value"],
-        "unit": "refactor+solve/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": s32["ms_per_step"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 factors + f64 refinement",
-        "data": "synthetic (seeded generator)",
-        "config": {"workload": workload, "parallelism": "single GPU", "keep_order": bool(args.keep_order)},
-        "p50_ms": s32["p50_ms"], "p99_ms": s32["p99_ms"], "rel_residual": s32["rel_residual"],
-        "backward_error": s32["backward_error"], "componentwise_error": s32["componentwise_error"],
-        "ir_stop_rule": ir_stop,
+        roof.update({"traffic": traffic, "kernel": ROOFLINE_KERNEL, "workload": f"config {config}",
+                     "avg_launch_us": avg_s * 1e6, "launches_per_step": g["launches"] / args.steps,
+                     "algorithmic_bytes_per_launch": byts, "algorithmic_flops_per_launch": flops, "tflops": tf,
+                     "tflops_frac_of_mfma_peak": tf / MFMA_PEAK["f32"], "gbs": gbs})
+    return {
+        "workload": workload, "value": s32["value"], "ms_per_step": s32["ms_per_step"], "p50_ms": s32["p50_ms"],
+        "p99_ms": s32["p99_ms"], "rel_residual": s32["rel_residual"], "backward_error": s32["backward_error"],
+        "componentwise_error": s32["componentwise_error"], "ir_stop_rule": ir_stop,
         "forward_error_vs_fp64": float(np.linalg.norm(res32[3] - res64[3]) / np.linalg.norm(res64[3])),
         "ir_iterations": s32["ir_iterations"], "fp64_fallback_blocks": s32["fp64_fallback_blocks"],
+        "ir_fp64_refactor": s32["ir_fp64_refactor"],
         "fp64": {"value": s64["value"], "ms_per_step": s64["ms_per_step"], "p50_ms": s64["p50_ms"],
                  "p99_ms": s64["p99_ms"], "rel_residual": s64["rel_residual"], "ir_iterations": s64["ir_iterations"],
                  "backward_error": s64["backward_error"], "componentwise_error": s64["componentwise_error"],
@@ -547,12 +574,107 @@ def main():
                        "componentwise_error": s64p["componentwise_error"], "dtype": "f64"},
         "fp32ir_over_fp64": s32["value"] / s64["value"],
         "fp32ir_over_fp64_plain": s32["value"] / s64p["value"],
-        "roofline": roof,
-        "cpu_baseline": cpu,
-        "kernel_profile": prof,
-        "build": bd.build_info(),
+        "roofline": roof, "kernel_profile": prof}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20,
+                    help="timed steps: closed-loop refactor+solve steps (configs 0-3) or batches of 100 requests "
+                         "(serving stream)")
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=4,
+                    help="workload: 4 serving stream (default, with the circuit-like closed loop beside it); "
+                         "0-3 closed loop on one system")
+    ap.add_argument("--requests", type=int, default=2000, help="open-loop serving stream length")
+    ap.add_argument("--slots", type=int, default=4, help="serving slots")
+    ap.add_argument("--no-closed-loop", action="store_true", help="serving stream only")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline sample per thread setting")
+    ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--keep-order", action="store_true",
+                    help="skip the generator's structure-hiding permutation (same blocks and values); the "
+                         "batched-blocks workload needs it for a host analysis of seconds")
+    ap.add_argument("--ir-stop", choices=["residual", "backward", "componentwise"],
+                    help="refinement stop rule at 1e-12 (default: componentwise for the scaled stress case, "
+                         "the reference's ||r||/||b|| otherwise)")
+    ap.add_argument("--profile-summary", metavar="DB", help="print the kernel table of a rocprofv3 database and exit")
+    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
+                    help="compute HBM bytes per launch of the roofline kernel from two --pmc runs, write "
+                         f"{TRAFFIC_FILE.relative_to(ROOT)} and exit")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 1 GiB device-to-device copy (the known-size reference kernel for "
+                         "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs)")
+    args = ap.parse_args()
+    if args.profile_summary:
+        print(profile_summary(args.profile_summary))
+        return
+    if args.traffic:
+        t = traffic_from_counters(*args.traffic)
+        TRAFFIC_FILE.write_text(json.dumps(t, indent=1) + "\n")
+        print(json.dumps(t, indent=1))
+        return
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build's path has no multi-GPU mode (see the sharding note "
+              "in the design document); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    bd = load_package()
+    if not bd.gpu_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    log(f"library build {bd.build_info()}")
+    if args.copy_calibration:
+        x = bd.DeviceArray.from_numpy(np.ones(COPY_BYTES // 8))
+        y = bd.DeviceArray(COPY_BYTES // 8)
+        for _ in range(3):
+            y.copy_from(x)
+        bd.hip().hipDeviceSynchronize()
+        x.free()
+        y.free()
+        log(f"copy calibration: 3 x {COPY_BYTES} B device-to-device")
+    if args.config != 4:
+        cl = closed_loop(bd, args, args.config)
+        cpu = None
+        if not args.no_cpu_baseline:
+            cpu = cpu_baseline(args.config, args.cpu_budget, int(args.keep_order), nreq=10)
+        line = {"metric": METRIC, "value": cl["value"], "unit": "refactor+solve/s", "n_gpus": 1,
+                "steps": args.steps, "warmup": args.warmup, "ms_per_step": cl["ms_per_step"],
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": "f32 factors + f64 refinement", "data": "synthetic (seeded generator)",
+                "config": {"workload": cl["workload"], "parallelism": "single GPU",
+                           "keep_order": bool(args.keep_order)}}
+        line.update({k: v for k, v in cl.items() if k not in ("workload", "value", "ms_per_step")})
+        line.update({"cpu_baseline": cpu, "build": bd.build_info()})
+        print(json.dumps(line))
+        return
+    srv, lam, fwd = bench_serving(bd, args)
+    f32, f64 = srv["FP32_IR"], srv["FP64"]
+    line = {
+        "metric": METRIC, "value": f32["capacity_req_per_s"], "unit": "refactor+solve/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": f32["capacity_ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 factors + f64 refinement", "data": "synthetic (seeded generator)",
+        "config": {"workload": f"serving stream: 16 patterns n=1e3..1e5 (circuit-like, giant block 10% of n), "
+                               f"+-5% value perturbations, 1-8 RHS per request, {args.slots} slots, one 64 GB "
+                               f"HBM budget; a step is a batch of {REQS_PER_STEP} requests queued at once "
+                               f"(closed loop); latency from {args.requests} Poisson arrivals at {lam:.1f}/s",
+                   "parallelism": "single GPU"},
+        "p50_ms": f32["p50_ms"], "p99_ms": f32["p99_ms"], "latency": "arrival to result, open loop",
+        "rel_residual": f32["max_rel_residual"], "backward_error": f32["max_backward_error"],
+        "forward_error_vs_fp64": fwd, "ir_iterations": f32["max_ir_iterations"],
+        "fp32ir": f32, "fp64_plain": f64,
+        "fp32ir_over_fp64_plain": f32["capacity_req_per_s"] / f64["capacity_req_per_s"],
     }
-    print(json.dumps(out))
+    if not args.no_closed_loop:
+        cl = closed_loop(bd, args, 1)
+        line["roofline"] = cl.pop("roofline")
+        line["closed_loop"] = cl
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(4, args.cpu_budget, 0, nreq=args.requests)
+    line.update({"cpu_baseline": cpu, "build": bd.build_info()})
+    print(json.dumps(line))
 
 
 if __name__ == "__main__":

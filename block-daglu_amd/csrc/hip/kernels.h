@@ -91,7 +91,9 @@ struct TailTimer {
 // inv: 2*64*64 per 64-column panel (ceil(m/64) panels): the inverses of the
 // diagonal blocks, L^{-1} then U^{-1}, row-major, kept for the tail solve.
 // lookahead: each step's next diagonal LU runs as one workgroup of the bulk
-// trailing-update launch (ignored while timing); results are identical either way.
+// trailing-update launch; results are identical either way. With a timer,
+// events bracket every panel launch and every trailing-update launch (the
+// fused one with lookahead) on the same launch structure.
 template <typename T>
 void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, int32_t* status, T* inv,
                     TailTimer* timer = nullptr, int32_t small_tile_below = 0, bool lookahead = false);

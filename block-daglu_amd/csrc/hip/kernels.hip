@@ -1158,14 +1158,16 @@ void tail_step(hipStream_t st, T* S, int64_t lds, int32_t p, int32_t b, int32_t 
 // same MFMA sequence as in the single update, so results are identical.
 template <typename T, int kTile>
 void tail_step_lookahead(hipStream_t st, T* S, int64_t lds, int32_t p, int32_t b, int32_t rest, double limit,
-                         int32_t* status, T* inv) {
+                         int32_t* status, T* inv, TailTimer* timer) {
   // inv: this panel's pair of diagonal-block inverses; the next panel's follows.
   const int32_t tr = (rest + kTile - 1) / kTile;
   TileJob<T> l21{rest, b, b, 2, tr, 1, S + int64_t(p) * lds + p + b, 1, lds, inv + kNb * kNb, kNb, 1,
                  S + int64_t(p) * lds + p + b, 1, lds};
   TileJob<T> u12{b, rest, b, 1, 1, tr, inv, kNb, 1, S + int64_t(p + b) * lds + p, 1, lds,
                  S + int64_t(p + b) * lds + p, 1, lds};
+  if (timer && timer->panel_events) timer_event(timer->panel_events, st);
   hipLaunchKernelGGL((k_tail_panel<T, kTile>), dim3(2 * tr), dim3(256), 0, st, l21, u12, limit, status);
+  if (timer && timer->panel_events) timer_event(timer->panel_events, st);
   const int32_t q = p + b;                      // next diagonal block
   const int32_t nb = rest < kNb ? rest : kNb;   // its size
   const T* L21 = S + int64_t(p) * lds + q;      // rest x b, column stride lds
@@ -1180,14 +1182,24 @@ void tail_step_lookahead(hipStream_t st, T* S, int64_t lds, int32_t p, int32_t b
                    S + int64_t(q) * lds + q + nb, 1, lds};
   if (n0 <= 0) right.tiles_m = right.tiles_n = below.tiles_m = below.tiles_n = 0;
   const int32_t g = 1 + right.tiles_m * right.tiles_n + below.tiles_m * below.tiles_n;
+  if (timer && timer->gemm_events) timer_event(timer->gemm_events, st);
   hipLaunchKernelGGL((k_tail_update_diag<T, kTile>), dim3(g), dim3(256), 0, st, right, below, a, limit, status, S,
                      lds, q, nb, inv + 2 * kNb * kNb);
+  if (timer && timer->gemm_events) {
+    timer_event(timer->gemm_events, st);
+    // Algorithmic work of the launch: the whole rest x rest trailing update
+    // (C read and written once, the two panels read once) plus the next
+    // diagonal block's LU and its two triangular inverses.
+    const double r = rest, k = b, d = nb;
+    timer->gemm_flops->push_back(2.0 * r * r * k + (2.0 / 3.0 + 2.0 / 3.0) * d * d * d);
+    timer->gemm_bytes->push_back(sizeof(T) * (2.0 * r * r + 2.0 * r * k));
+  }
 }
 
 template <typename T>
 void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, int32_t* status, T* inv,
                     TailTimer* timer, int32_t small_tile_below, bool lookahead) {
-  if (lookahead && !timer) {
+  if (lookahead) {
     if (m <= 0) return;
     hipLaunchKernelGGL(k_diag_lu_inv<T>, dim3(1), dim3(kDiagThreads), 0, st, S, lds, 0, m < kNb ? m : kNb, limit,
                        status, inv);
@@ -1197,9 +1209,9 @@ void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, 
       const int32_t rest = m - p - b;
       if (rest <= 0) break;
       if (rest < small_tile_below)
-        tail_step_lookahead<T, 64>(st, S, lds, p, b, rest, limit, status, inv + p / kNb * kInvPair);
+        tail_step_lookahead<T, 64>(st, S, lds, p, b, rest, limit, status, inv + p / kNb * kInvPair, timer);
       else
-        tail_step_lookahead<T, 128>(st, S, lds, p, b, rest, limit, status, inv + p / kNb * kInvPair);
+        tail_step_lookahead<T, 128>(st, S, lds, p, b, rest, limit, status, inv + p / kNb * kInvPair, timer);
     }
     return;
   }

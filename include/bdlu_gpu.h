@@ -178,7 +178,8 @@ void* bdlu_gpu_stream(bdlu_ctx* ctx);
 // Enabling resets the counters. flops/bytes are algorithmic (from the plan's
 // shapes), accumulated over the timed launches.
 enum {
-  BDLU_PROF_TAIL_GEMM = 0,   // trailing MFMA update of the dense Schur tails
+  BDLU_PROF_TAIL_GEMM = 0,   // trailing MFMA update of the dense Schur tails (with lookahead: the fused
+                             // launch that also factors the next diagonal block)
   BDLU_PROF_TAIL_PANEL = 1,  // diagonal LU + both triangular solves of a tail panel
   BDLU_PROF_DENSE_LU = 2,    // batched dense LU of the blocks <= 64
   BDLU_PROF_SPARSE = 3,      // sparse leading columns + Schur columns
