@@ -3,6 +3,7 @@
 Produces, in-tree:
   block-daglu_amd/lib/libbdlu_gpu.so     HIP kernels + device C-ABI (hipcc, gfx950)
   block-daglu_amd/lib/libblockdaglu.so   host C++ (Solver, analysis, planner, C-ABI)
+  block-daglu_amd/lib/bdlu_api_check     tests/cpp/api_check.cpp against the public C++ API
   oracle/liboracle.so                    the oracle (test infrastructure; built by
                                          oracle/Makefile, only when asked)
 Usage: python block-daglu_amd/build.py [--oracle] [--jobs N]
@@ -84,6 +85,12 @@ def build(oracle: bool = False, jobs: int = 8) -> None:
           *[str(OBJ / (Path(s).stem + ".o")) for s in HOST_SRCS],
           f"-L{LIB}", "-lbdlu_gpu", "-Wl,-rpath,$ORIGIN", "-lpthread"])
     os.replace(str(host) + ".tmp", host)
+    # A program written only against the public C++ header, linked like a
+    # caller of the reference library (the drop-in check the tests run).
+    check = LIB / "bdlu_api_check"
+    _run([CXX, "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(ROOT / "tests" / "cpp" / "api_check.cpp"),
+          "-o", str(check) + ".tmp", f"-L{LIB}", "-lblockdaglu", "-Wl,-rpath,$ORIGIN", "-lpthread"])
+    os.replace(str(check) + ".tmp", check)
     if oracle:
         _run(["make", "-s", "-C", str(ROOT / "oracle")])
     shutil.rmtree(OBJ, ignore_errors=True)

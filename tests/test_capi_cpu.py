@@ -133,3 +133,16 @@ def test_serve_rejects_values_at_dropped_entries(bd):
     bad[-1] = 1e-3
     assert "AnalysisFailed" in _submit_error(srv, pid, bad, n)
     srv.close()
+
+
+def test_cpp_api_program_host_checks(bd):
+    """tests/cpp/api_check.cpp is compiled against include/blockdaglu/solver.h
+    alone and linked to libblockdaglu.so; its host-side checks (validation error
+    codes and the 4x4 export identity of apps/tests/main.cpp:93-109, :170-195)
+    pass without a GPU."""
+    import subprocess
+    exe = ROOT / "block-daglu_amd" / "lib" / "bdlu_api_check"
+    assert exe.exists(), "build() compiles the drop-in check program"
+    r = subprocess.run([str(exe), "--host-only"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "checks passed" in r.stdout
