@@ -116,6 +116,7 @@ HOST_SYMBOLS = [
     ("bdlu_host_block_kinds", None, [_P, _I32P, _I32P]),
     ("bdlu_host_sparse_stats", C.c_int, [_P, C.c_int32, _I32P, _F64P]),
     ("bdlu_host_sparse_order", C.c_int, [_P, C.c_int32, _I32P, _I32P]),
+    ("bdlu_host_colamd", None, [C.c_int32, _I32P, _I32P, _I32P]),
     ("bdlu_host_signatures", None, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("bdlu_gen_config_spec", None, [C.c_int, C.c_int, C.POINTER(GenSpec)]),
     ("bdlu_gen_create", _P, [C.POINTER(GenSpec)]),
@@ -548,6 +549,15 @@ class HostAnalysis:
         a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
         self._L.bdlu_host_signatures(self._h, C.byref(a), C.byref(b), C.byref(c))
         return a.value, b.value, c.value
+
+
+def colamd_order(n: int, indptr, indices) -> np.ndarray:
+    """Column order the host analysis gives an n x n block with this CSC
+    pattern (order[k] = column at position k)."""
+    ip, ix = _i32(indptr), _i32(indices)
+    out = np.zeros(n, np.int32)
+    lib().bdlu_host_colamd(n, _ptr(ip, _I32P), _ptr(ix, _I32P), _ptr(out, _I32P))
+    return out
 
 
 def config_spec(cfg: int, pattern: int = 0) -> GenSpec:

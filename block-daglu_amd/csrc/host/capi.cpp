@@ -318,4 +318,10 @@ int bdlu_serve_wait(void* s, int64_t request, double* X, bdlu_serve_result* out)
   return static_cast<int>(r.error);
 }
 
+void bdlu_host_colamd(int32_t n, const int32_t* Ap, const int32_t* Ai, int32_t* order) {
+  if (n <= 0 || !Ap || !Ai || !order) return;
+  const std::vector<int32_t> q = bdlu::colamd_order(n, Ap, Ai);
+  std::copy(q.begin(), q.end(), order);
+}
+
 }  // extern "C"

@@ -1,154 +1,291 @@
-// Column approximate-minimum-degree ordering for the diagonal blocks larger than
-// the dense threshold. In the reference this is SuiteSparse COLAMD, reached
-// through klu_analyze with ordering = 1 (ubtf:493-499, :1804). This is a
-// restatement of the published COLAMD method (Davis, Gilbert, Larimore, Ng,
-// ACM TOMS 30(3), 2004) in a compact form:
-//   * dense columns (> max(16, 10*sqrt(s)) entries) and empty columns are
-//     ordered last, ascending; dense rows (> max(16, 10*sqrt(s)) live columns)
-//     and empty rows are ignored;
-//   * the pivot is a live column of minimum approximate score, ties broken by
-//     the smallest column index;
-//   * eliminating column c merges every row of c into one new row (the pivot
-//     row), rows wholly contained in it are absorbed (aggressive absorption),
-//     and every column of the pivot row is re-scored with the approximate
-//     external degree  (|pivot row| - 1) + sum over its other rows of
-//     |row \ pivot row|, capped by the number of live columns.
-// Supercolumn detection and COLAMD's degree-list tie order are not reproduced,
-// so the sequence can differ from SuiteSparse's on ties.
+// Column approximate minimum degree ordering of the diagonal blocks larger
+// than the dense threshold: the role SuiteSparse COLAMD plays inside
+// klu_analyze with ordering = 1 (ubtf_block_solver.cpp:493-499, :1804), called
+// with default knobs (dense row/column cut 10, aggressive absorption on).
+//
+// Restated from the published method (Davis, Gilbert, Larimore, Ng, "A column
+// approximate minimum degree ordering algorithm", ACM TOMS 30(3), 2004, and
+// the COLAMD 2.x distribution it describes), including the details that fix
+// the order on ties:
+//   * columns are taken with sorted, duplicate-free row lists; rows list their
+//     columns in ascending order;
+//   * empty columns, then dense columns (more than max(16, 10 sqrt(min(m,n)))
+//     entries), then columns left empty by dense rows (more than
+//     max(16, 10 sqrt(n)) live columns) are ordered last, each group scanned
+//     from the highest column index down;
+//   * initial score of a column = sum over its live rows of (row degree - 1),
+//     capped at n; degree lists are filled from the highest column index down,
+//     inserting at the head, and the pivot is the head of the lowest non-empty
+//     list;
+//   * the pivot row is the union of the pivot column's live rows, columns in
+//     the order met; all those rows die and the pivot row takes the index of
+//     the first row stored in the pivot column;
+//   * set differences |row \ pivot row| with aggressive absorption of rows
+//     whose difference reaches zero, columns compacted in place, mass
+//     elimination of columns left with no row;
+//   * supercolumns: columns of the pivot row hashed by (sum of row indices)
+//     mod (n + 1) into buckets (each insertion at the front), identical columns
+//     (same length, same partial score, same row sequence) merged into the
+//     first of the bucket that precedes them;
+//   * final score = partial score + pivot row degree - thickness, capped at
+//     n - k - thickness, columns re-inserted at the head of their lists;
+//   * absorbed (non-principal) columns take the positions of their principal
+//     column's group in ascending column order, the principal the last one.
+// The workspace layout and garbage collection of the original only move
+// storage and never reorder entries, so separate per-row and per-column lists
+// give the same order.
 #include <algorithm>
 #include <cmath>
-#include <queue>
 
 #include "host.h"
 
 namespace bdlu {
 
-std::vector<int32_t> colamd_order(int32_t s, const int32_t* Ap, const int32_t* Ai) {
-  std::vector<int32_t> order;
-  order.reserve(s);
-  if (s <= 0) return order;
-  const int32_t dense_cut = std::max<int32_t>(16, static_cast<int32_t>(10.0 * std::sqrt(static_cast<double>(s))));
+namespace {
 
-  // Column -> rows, row -> columns.
-  std::vector<char> col_alive(s, 1);
-  std::vector<int32_t> col_cnt(s);
-  for (int32_t c = 0; c < s; ++c) col_cnt[c] = Ap[c + 1] - Ap[c];
-  std::vector<char> col_killed(s, 0);
-  for (int32_t c = 0; c < s; ++c)
-    if (col_cnt[c] == 0 || col_cnt[c] > dense_cut) col_killed[c] = 1;
+constexpr int32_t kNone = -1;
 
-  std::vector<std::vector<int32_t>> rows_of;  // row/element -> live columns
-  rows_of.resize(s);
-  for (int32_t c = 0; c < s; ++c) {
-    if (col_killed[c]) continue;
-    for (int32_t p = Ap[c]; p < Ap[c + 1]; ++p) rows_of[Ai[p]].push_back(c);
+struct ColamdState {
+  int32_t n = 0;
+  // columns
+  std::vector<std::vector<int32_t>> col_rows;
+  std::vector<int32_t> thickness, score, order, parent, prev, next, hash, hash_next;
+  std::vector<uint8_t> col_dead, col_principal;  // dead: out of the live set; principal: not absorbed
+  // rows
+  std::vector<std::vector<int32_t>> row_cols;
+  std::vector<int32_t> row_degree;
+  std::vector<uint8_t> row_dead;
+  // degree lists
+  std::vector<int32_t> head;
+
+  void list_insert(int32_t c, int32_t sc) {
+    const int32_t nx = head[sc];
+    next[c] = nx;
+    prev[c] = kNone;
+    if (nx != kNone) prev[nx] = c;
+    head[sc] = c;
   }
-  std::vector<char> row_alive(s, 1);
-  for (int32_t r = 0; r < s; ++r)
-    if (rows_of[r].empty() || static_cast<int32_t>(rows_of[r].size()) > dense_cut) row_alive[r] = 0;
-
-  std::vector<std::vector<int32_t>> cols_of(s);  // column -> live rows/elements
-  for (int32_t c = 0; c < s; ++c) {
-    if (col_killed[c]) continue;
-    for (int32_t p = Ap[c]; p < Ap[c + 1]; ++p)
-      if (row_alive[Ai[p]]) cols_of[c].push_back(Ai[p]);
+  void list_remove(int32_t c) {
+    const int32_t pv = prev[c], nx = next[c];
+    if (pv == kNone) head[score[c]] = nx;
+    else next[pv] = nx;
+    if (nx != kNone) prev[nx] = pv;
   }
-
-  int32_t live_cols = 0;
-  for (int32_t c = 0; c < s; ++c) live_cols += col_killed[c] ? 0 : 1;
-
-  using Item = std::pair<int64_t, int32_t>;  // (score, column)
-  std::priority_queue<Item, std::vector<Item>, std::greater<Item>> heap;
-  std::vector<int64_t> score(s, 0);
-  for (int32_t c = 0; c < s; ++c) {
-    if (col_killed[c]) continue;
-    int64_t sc = 0;
-    for (int32_t r : cols_of[c]) sc += static_cast<int64_t>(rows_of[r].size()) - 1;
-    score[c] = std::min<int64_t>(sc, live_cols - 1);
-    heap.push({score[c], c});
+  void kill_principal(int32_t c) {
+    col_dead[c] = 1;
+    col_principal[c] = 1;
   }
+};
 
-  std::vector<int32_t> col_mark(s, -1);
-  std::vector<int32_t> row_overlap;  // indexed by row/element id
-  std::vector<int32_t> row_stamp;
-  row_overlap.assign(rows_of.size(), 0);
-  row_stamp.assign(rows_of.size(), -1);
-  std::vector<int32_t> touched;
-  std::vector<int32_t> pivot_cols;
+}  // namespace
 
-  for (int32_t step = 0; !heap.empty();) {
-    const Item top = heap.top();
-    heap.pop();
-    const int32_t c = top.second;
-    if (col_killed[c] || !col_alive[c] || top.first != score[c]) continue;
-    col_alive[c] = 0;
-    order.push_back(c);
-    --live_cols;
+std::vector<int32_t> colamd_order(int32_t n, const int32_t* Ap, const int32_t* Ai) {
+  std::vector<int32_t> perm;
+  if (n <= 0) return perm;
+  ColamdState S;
+  S.n = n;
+  S.col_rows.assign(n, {});
+  S.row_cols.assign(n, {});
+  // Sorted, duplicate-free column lists; row lists in ascending column order.
+  for (int32_t c = 0; c < n; ++c) {
+    std::vector<int32_t>& rows = S.col_rows[c];
+    rows.assign(Ai + Ap[c], Ai + Ap[c + 1]);
+    std::sort(rows.begin(), rows.end());
+    rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
+    for (int32_t r : rows) S.row_cols[r].push_back(c);
+  }
+  S.thickness.assign(n, 1);
+  S.score.assign(n, 0);
+  S.order.assign(n, kNone);
+  S.parent.assign(n, kNone);
+  S.prev.assign(n, kNone);
+  S.next.assign(n, kNone);
+  S.hash.assign(n, 0);
+  S.hash_next.assign(n, kNone);
+  S.col_dead.assign(n, 0);
+  S.col_principal.assign(n, 1);
+  S.row_degree.assign(n, 0);
+  S.row_dead.assign(n, 0);
+  for (int32_t r = 0; r < n; ++r) S.row_degree[r] = static_cast<int32_t>(S.row_cols[r].size());
 
-    // Pivot row: union of the live rows of c, minus c.
-    pivot_cols.clear();
-    for (int32_t r : cols_of[c]) {
-      if (!row_alive[r]) continue;
-      for (int32_t c2 : rows_of[r]) {
-        if (c2 == c || !col_alive[c2] || col_mark[c2] == step) continue;
-        col_mark[c2] = step;
-        pivot_cols.push_back(c2);
-      }
-      row_alive[r] = 0;
+  // ---- initial scoring ------------------------------------------------------
+  const auto dense_cut = [](double alpha, int32_t m) {
+    return static_cast<int32_t>(std::max(16.0, alpha * std::sqrt(static_cast<double>(m))));
+  };
+  const int32_t dense_row = dense_cut(10.0, n);
+  const int32_t dense_col = dense_cut(10.0, n);  // min(rows, cols) = n for a square block
+  int32_t n_col2 = n;
+  for (int32_t c = n - 1; c >= 0; --c)
+    if (S.col_rows[c].empty()) {
+      S.order[c] = --n_col2;
+      S.kill_principal(c);
     }
-    cols_of[c].clear();
-    cols_of[c].shrink_to_fit();
-    const int32_t e = static_cast<int32_t>(rows_of.size());
-    rows_of.push_back(pivot_cols);
-    row_alive.push_back(pivot_cols.empty() ? 0 : 1);
-    row_overlap.push_back(0);
-    row_stamp.push_back(-1);
+  for (int32_t c = n - 1; c >= 0; --c) {
+    if (S.col_dead[c]) continue;
+    if (static_cast<int32_t>(S.col_rows[c].size()) > dense_col) {
+      S.order[c] = --n_col2;
+      for (int32_t r : S.col_rows[c]) S.row_degree[r]--;
+      S.kill_principal(c);
+    }
+  }
+  for (int32_t r = 0; r < n; ++r)
+    if (S.row_degree[r] > dense_row || S.row_degree[r] == 0) S.row_dead[r] = 1;
+  for (int32_t c = n - 1; c >= 0; --c) {
+    if (S.col_dead[c]) continue;
+    std::vector<int32_t>& rows = S.col_rows[c];
+    int32_t sc = 0, w = 0;
+    for (int32_t r : rows) {
+      if (S.row_dead[r]) continue;
+      rows[w++] = r;
+      sc = std::min(sc + S.row_degree[r] - 1, n);
+    }
+    rows.resize(w);
+    if (w == 0) {
+      S.order[c] = --n_col2;
+      S.kill_principal(c);
+    } else {
+      S.score[c] = sc;
+    }
+  }
+  S.head.assign(static_cast<std::size_t>(n) + 1, kNone);
+  int32_t min_score = n;
+  for (int32_t c = n - 1; c >= 0; --c) {
+    if (S.col_dead[c]) continue;
+    S.list_insert(c, S.score[c]);
+    min_score = std::min(min_score, S.score[c]);
+  }
 
-    // |r intersect pivot row| for the other live rows of the pivot columns.
-    touched.clear();
-    for (int32_t c2 : pivot_cols) {
-      for (int32_t r : cols_of[c2]) {
-        if (!row_alive[r]) continue;
-        if (row_stamp[r] != step) {
-          row_stamp[r] = step;
-          row_overlap[r] = 0;
-          touched.push_back(r);
+  // ---- elimination ------------------------------------------------------------
+  std::vector<int32_t> in_pivot_row(n, 0);   // column already in this pivot row
+  std::vector<int32_t> seen(n, -1);          // row -> step at which its set difference was started
+  std::vector<int32_t> setdiff(n, 0);        // row -> |row \ pivot row| (thickness-weighted)
+  std::vector<int32_t> bucket(static_cast<std::size_t>(n) + 1, kNone);
+  std::vector<int32_t> prow;
+  min_score = 0;
+  for (int32_t k = 0, step = 0; k < n_col2; ++step) {
+    while (min_score < n && S.head[min_score] == kNone) ++min_score;
+    const int32_t pc = S.head[min_score];
+    {
+      const int32_t nx = S.next[pc];
+      S.head[min_score] = nx;
+      if (nx != kNone) S.prev[nx] = kNone;
+    }
+    S.order[pc] = k;
+    k += S.thickness[pc];
+
+    // Pivot row: union of the live rows of the pivot column.
+    prow.clear();
+    int32_t prow_degree = 0;
+    in_pivot_row[pc] = 1;
+    for (int32_t r : S.col_rows[pc]) {
+      if (S.row_dead[r]) continue;
+      for (int32_t c : S.row_cols[r]) {
+        if (S.col_dead[c] || in_pivot_row[c]) continue;
+        in_pivot_row[c] = 1;
+        prow.push_back(c);
+        prow_degree += S.thickness[c];
+      }
+    }
+    in_pivot_row[pc] = 0;
+    for (int32_t r : S.col_rows[pc]) S.row_dead[r] = 1;
+    const int32_t pivot_row = prow.empty() ? kNone : S.col_rows[pc].front();
+
+    // Set differences, with aggressive absorption.
+    for (int32_t c : prow) {
+      in_pivot_row[c] = 0;
+      S.list_remove(c);
+      for (int32_t r : S.col_rows[c]) {
+        if (S.row_dead[r]) continue;
+        if (seen[r] != step) {
+          seen[r] = step;
+          setdiff[r] = S.row_degree[r];
         }
-        row_overlap[r]++;
+        setdiff[r] -= S.thickness[c];
+        if (setdiff[r] == 0) S.row_dead[r] = 1;
       }
     }
-    // Aggressive absorption of rows contained in the pivot row.
-    for (int32_t r : touched)
-      if (static_cast<int32_t>(rows_of[r].size()) == row_overlap[r]) row_alive[r] = 0;
+    // Partial scores, column compaction, mass elimination, hash buckets.
+    for (int32_t c : prow) {
+      std::vector<int32_t>& rows = S.col_rows[c];
+      uint32_t h = 0;  // unsigned 32-bit sum, wrapping as the original's
+      int32_t sc = 0, w = 0;
+      for (int32_t r : rows) {
+        if (S.row_dead[r]) continue;
+        rows[w++] = r;
+        h += static_cast<uint32_t>(r);
+        sc = std::min(sc + setdiff[r], n);
+      }
+      rows.resize(w);
+      if (w == 0) {
+        S.kill_principal(c);
+        prow_degree -= S.thickness[c];
+        S.order[c] = k;
+        k += S.thickness[c];
+      } else {
+        S.score[c] = sc;
+        const int32_t hv = static_cast<int32_t>(h % (static_cast<uint32_t>(n) + 1u));
+        S.hash[c] = hv;
+        S.hash_next[c] = bucket[hv];
+        bucket[hv] = c;
+      }
+    }
+    // Supercolumn detection.
+    for (int32_t c0 : prow) {
+      if (S.col_dead[c0]) continue;
+      const int32_t hv = S.hash[c0];
+      for (int32_t sup = bucket[hv]; sup != kNone; sup = S.hash_next[sup]) {
+        const std::vector<int32_t>& ref = S.col_rows[sup];
+        int32_t before = sup;
+        for (int32_t c = S.hash_next[sup]; c != kNone; c = S.hash_next[c]) {
+          if (S.col_rows[c].size() != ref.size() || S.score[c] != S.score[sup] || S.col_rows[c] != ref) {
+            before = c;
+            continue;
+          }
+          S.thickness[sup] += S.thickness[c];
+          S.parent[c] = sup;
+          S.col_dead[c] = 1;
+          S.col_principal[c] = 0;
+          S.order[c] = kNone;
+          S.hash_next[before] = S.hash_next[c];
+        }
+      }
+      bucket[hv] = kNone;
+    }
+    S.kill_principal(pc);
 
-    const int64_t esize = static_cast<int64_t>(pivot_cols.size());
-    for (int32_t c2 : pivot_cols) {
-      auto& lst = cols_of[c2];
-      std::size_t w = 0;
-      int64_t sc = esize - 1;
-      for (std::size_t t = 0; t < lst.size(); ++t) {
-        const int32_t r = lst[t];
-        if (!row_alive[r]) continue;
-        lst[w++] = r;
-        sc += static_cast<int64_t>(rows_of[r].size()) - row_overlap[r];
-      }
-      lst.resize(w);
-      lst.push_back(e);
-      sc = std::min<int64_t>(sc, std::max<int32_t>(live_cols - 1, 0));
-      score[c2] = sc;
-      heap.push({sc, c2});
+    // Final scores; the pivot row joins every remaining column.
+    int32_t w = 0;
+    for (int32_t c : prow) {
+      if (S.col_dead[c]) continue;
+      prow[w++] = c;
+      S.col_rows[c].push_back(pivot_row);
+      const int32_t max_score = n - k - S.thickness[c];
+      const int32_t sc = std::min(S.score[c] + prow_degree - S.thickness[c], max_score);
+      S.score[c] = sc;
+      S.list_insert(c, sc);
+      min_score = std::min(min_score, sc);
     }
-    // Absorbed rows' column lists are no longer needed.
-    for (int32_t r : touched)
-      if (!row_alive[r]) {
-        rows_of[r].clear();
-        rows_of[r].shrink_to_fit();
-      }
-    ++step;
+    prow.resize(w);
+    if (prow_degree > 0) {
+      S.row_cols[pivot_row] = prow;
+      S.row_degree[pivot_row] = prow_degree;
+      S.row_dead[pivot_row] = 0;
+    }
   }
-  for (int32_t c = 0; c < s; ++c)
-    if (col_killed[c]) order.push_back(c);
-  return order;
+
+  // ---- absorbed columns: the slots of their principal's group -------------------
+  for (int32_t i = 0; i < n; ++i) {
+    if (S.col_principal[i] || S.order[i] != kNone) continue;
+    int32_t p = i;
+    do p = S.parent[p];
+    while (!S.col_principal[p]);
+    S.order[i] = S.order[p];
+    S.parent[i] = p;
+    S.order[p] += 1;
+  }
+  perm.assign(n, 0);
+  for (int32_t c = 0; c < n; ++c) perm[S.order[c]] = c;
+  return perm;
 }
 
 }  // namespace bdlu

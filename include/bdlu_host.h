@@ -123,6 +123,9 @@ void bdlu_host_block_kinds(void* h, int32_t* kind, int32_t* size);
 int bdlu_host_sparse_stats(void* h, int32_t b, int32_t* stats6, double* fstats4);
 // Column order (s) and sparse-part pivot rows (k0) of sparse block b (local indices).
 int bdlu_host_sparse_order(void* h, int32_t b, int32_t* q, int32_t* prow);
+// Column order (n) that the analysis gives an n x n block with this CSC pattern
+// (the COLAMD role of klu_analyze, ordering = 1): order[k] = column at position k.
+void bdlu_host_colamd(int32_t n, const int32_t* Ap, const int32_t* Ai, int32_t* order);
 // Signatures (reference src/solver.cpp:90-136) of the canonical matrix.
 void bdlu_host_signatures(void* h, uint64_t* sig, uint64_t* sig2, uint64_t* value_sig);
 

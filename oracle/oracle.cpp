@@ -9,9 +9,9 @@
 // reference takes from SuiteSparse (vcpkg tag 2024.05.24, not vendored, not
 // present here) are restated from their published algorithms:
 //   btf_order   (maxtrans + Tarjan SCC)            -> oracle_btf_order
-//   klu_analyze ordering=1 (COLAMD)                -> the column AMD ordering
-//               compiled from block-daglu_amd/csrc/host/colamd.cpp (shared with
-//               the product host; parity unpinned: only libklu pins COLAMD)
+//   klu_analyze ordering=1 (COLAMD)                -> oracle::colamd
+//               (oracle/colamd_oracle.cpp, the oracle's own restatement; pinned
+//               against the COLAMD that SuperLU bundles, through scipy)
 //   klu_factor  (left-looking Gilbert-Peierls, threshold partial pivoting with
 //               diagonal preference and diagonal tracking, tol 0.001, scale=1
 //               row sums)                         -> klu_like_factor
@@ -33,8 +33,8 @@
 #include <tuple>
 #include <vector>
 
-namespace bdlu {
-std::vector<int32_t> colamd_order(int32_t s, const int32_t* Ap, const int32_t* Ai);
+namespace oracle {
+std::vector<int32_t> colamd(int32_t n, const int32_t* Ap, const int32_t* Ai);
 }
 
 namespace oracle {
@@ -365,7 +365,7 @@ bool klu_like_factor(BlockFactor& bf, const std::vector<std::tuple<int, int, dou
       Ax[p] = std::get<2>(t);
     }
   }
-  bf.q = bdlu::colamd_order(s, Ap.data(), Ai.data());
+  bf.q = oracle::colamd(s, Ap.data(), Ai.data());
   bf.rs.assign(s, 1.0);
   if (s >= 4096) {
     std::fill(bf.rs.begin(), bf.rs.end(), 0.0);
@@ -758,6 +758,11 @@ int32_t oracle_btf_order(int32_t n, const int32_t* Ap, const int32_t* Ai, int32_
   std::copy(r.begin(), r.end(), R);
   return nb;
 }
+void oracle_colamd(int32_t n, const int32_t* Ap, const int32_t* Ai, int32_t* order) {
+  const std::vector<int32_t> q = oracle::colamd(n, Ap, Ai);
+  std::copy(q.begin(), q.end(), order);
+}
+
 int32_t oracle_merge(const int32_t* R, int32_t nb, int32_t n, int32_t merge_max, int32_t* out) {
   std::vector<int32_t> r(R, R + nb + 1);
   const auto m = oracle::merge_blocks(r, n, merge_max);

@@ -50,6 +50,7 @@ def lib():
                                          C.POINTER(C.c_uint64)]),
             "oracle_btf_order": (C.c_int32, [C.c_int32, _I32P, _I32P, _I32P, _I32P, _I32P, _I32P]),
             "oracle_merge": (C.c_int32, [_I32P, C.c_int32, C.c_int32, C.c_int32, _I32P]),
+            "oracle_colamd": (None, [C.c_int32, _I32P, _I32P, _I32P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -160,3 +161,12 @@ def merge(R, n, merge_max=64):
     out = np.zeros(len(R), np.int32)
     nb = L.oracle_merge(_p(R, _I32P), len(R) - 1, n, merge_max, _p(out, _I32P))
     return out[: nb + 1]
+
+
+def colamd(n, indptr, indices):
+    """The oracle's COLAMD order of an n x n CSC pattern (order[k] = column at position k)."""
+    ip = np.ascontiguousarray(indptr, np.int32)
+    ix = np.ascontiguousarray(indices, np.int32)
+    out = np.zeros(n, np.int32)
+    lib().oracle_colamd(n, _p(ip, _I32P), _p(ix, _I32P), _p(out, _I32P))
+    return out
