@@ -541,9 +541,9 @@ class HostAnalysis:
     def sparse_order(self, b):
         st = self.sparse_stats(b)
         q = np.zeros(st["s"], np.int32)
-        prow = np.zeros(max(st["k0"], 1), np.int32)
+        prow = np.zeros(st["s"], np.int32)
         self._L.bdlu_host_sparse_order(self._h, b, _ptr(q, _I32P), _ptr(prow, _I32P))
-        return q, prow[: st["k0"]]
+        return q, prow
 
     def signatures(self):
         a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()

@@ -241,7 +241,10 @@ int bdlu_host_sparse_order(void* hp, int32_t b, int32_t* q, int32_t* prow) {
   const bdlu::SparseBlockPlan* sp = find_sparse(hp, b);
   if (!sp) return 0;
   if (q) std::copy(sp->q.begin(), sp->q.end(), q);
-  if (prow) std::copy(sp->prow.begin(), sp->prow.end(), prow);
+  if (prow) {
+    std::copy(sp->prow.begin(), sp->prow.end(), prow);
+    for (int32_t t = 0; t < sp->m; ++t) prow[sp->k0 + t] = sp->trow[sp->tperm[t]];
+  }
   return 1;
 }
 

@@ -264,7 +264,8 @@ bool analyze_sparse_block(int32_t s, const std::vector<int32_t>& rows, const std
     }
     for (int32_t k : reach) x[out.prow[k]] = 0.0;
   }
-  if (out.m > 0 && !dense_threshold_lu(out.m, S.data(), out.tdiag, opt.tail_tol, out.tperm)) return false;
+  const double tail_tol = opt.tail_tol > 0.0 ? opt.tail_tol : opt.tol;
+  if (out.m > 0 && !dense_threshold_lu(out.m, S.data(), out.tdiag, tail_tol, out.tperm)) return false;
 
   // Refactor levels of the sparse columns.
   std::vector<int32_t> lev(k0, 0);

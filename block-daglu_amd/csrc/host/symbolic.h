@@ -42,9 +42,13 @@ struct SparseBlockPlan {
 
 struct SymbolicOptions {
   double tol = 0.001;            // KLU default pivot tolerance (sparse leading columns)
-  // Pivot tolerance inside the dense Schur tail. KLU's diagonal preference
-  // protects sparsity; the tail is dense, so it only costs stability there.
-  // 1.0 = partial pivoting (the diagonal wins only ties), which bounds |L| by 1.
+  // Pivot tolerance inside the dense Schur tail. 1.0 (default) = partial
+  // pivoting (the diagonal wins only ties), which bounds |L| by 1: the device
+  // factors the tail with explicit 64x64 triangular inverses in its MFMA panel
+  // products, which amplify |L| growth (with KLU's tol 0.001 the plain FP64
+  // residual of the circuit-like system grows from 7e-11 to 1.8e-6).
+  // tol (BLOCKDAGLU_GPU_TAIL_TOL=0.001) gives klu_factor's pivot rows over the
+  // whole block; <= 0 means the same as tol.
   double tail_tol = 1.0;
   double tail_density = 0.2;     // switch to the dense tail when |L(:,j)| > density*(s-j)
   int32_t tail_min = 32;         // ... and at least this many columns remain

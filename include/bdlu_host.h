@@ -121,7 +121,8 @@ void bdlu_host_block_kinds(void* h, int32_t* kind, int32_t* size);
 // Sparse block statistics for block b (returns 0 if b is not a sparse block):
 // stats = {s, k0, m, levels, offdiag_pivots, scale}, fstats = {nnz_l, nnz_u, updates, flops}.
 int bdlu_host_sparse_stats(void* h, int32_t b, int32_t* stats6, double* fstats4);
-// Column order (s) and sparse-part pivot rows (k0) of sparse block b (local indices).
+// Column order (s) and pivot rows (s: the sparse part's, then the dense tail's
+// in its pivot order) of sparse block b (local indices).
 int bdlu_host_sparse_order(void* h, int32_t b, int32_t* q, int32_t* prow);
 // Column order (n) that the analysis gives an n x n block with this CSC pattern
 // (the COLAMD role of klu_analyze, ordering = 1): order[k] = column at position k.
