@@ -102,6 +102,7 @@ HOST_SYMBOLS = [
     ("bdlu_solver_device_plan", C.c_uint64, [_P]),
     ("bdlu_solver_nblocks", C.c_int32, [_P]),
     ("bdlu_solver_copy_perm", None, [_P, _I32P, _I32P, _I32P]),
+    ("bdlu_solver_sparse_order", C.c_int32, [_P, C.c_int32, _I32P, _I32P]),
     ("bdlu_solver_export", C.c_int64, [_P, C.c_int, _I32P, _I32P, _F64P]),
     ("bdlu_serve_create", _P, [C.c_int, C.c_int, C.c_uint64, C.c_int]),
     ("bdlu_serve_destroy", None, [_P]),
@@ -409,6 +410,13 @@ class Solver:
         R = np.zeros(nb + 1, np.int32)
         self._L.bdlu_solver_copy_perm(self._h, _ptr(P, _I32P), _ptr(Q, _I32P), _ptr(R, _I32P))
         return P, Q, R
+
+    def sparse_order(self, b: int, size: int):
+        """(q, prow, k0) of block b if it is above the dense threshold, else None."""
+        q = np.zeros(size, np.int32)
+        prow = np.zeros(size, np.int32)
+        k0 = self._L.bdlu_solver_sparse_order(self._h, b, _ptr(q, _I32P), _ptr(prow, _I32P))
+        return None if k0 < 0 else (q, prow, k0)
 
     def _export(self, which):
         cnt = self._L.bdlu_solver_export(self._h, which, None, None, None)

@@ -1,4 +1,5 @@
 // C-ABI of the host library (include/bdlu_host.h).
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <tuple>
@@ -105,6 +106,16 @@ void bdlu_solver_report(void* s, bdlu_report* out) {
   out->backward_error = r.backward_error;
   out->componentwise_error = r.componentwise_error;
   out->ir_fp64_refactor = r.ir_fp64_refactor ? 1 : 0;
+}
+
+int32_t bdlu_solver_sparse_order(void* s, int32_t b, int32_t* q, int32_t* prow) {
+  if (!s) return -1;
+  std::vector<int32_t> qv, pv;
+  const int32_t k0 = static_cast<Solver*>(s)->sparse_block_order(b, qv, pv);
+  if (k0 < 0) return -1;
+  if (q) std::copy(qv.begin(), qv.end(), q);
+  if (prow) std::copy(pv.begin(), pv.end(), prow);
+  return k0;
 }
 
 int32_t bdlu_solver_nblocks(void* s) {

@@ -525,6 +525,20 @@ CooMatrix Solver::getU() {
   return impl_->exported->U;
 }
 
+int32_t Solver::sparse_block_order(int32_t block, std::vector<int32_t>& q, std::vector<int32_t>& prow) const {
+  if (!impl_->plan) return -1;
+  const bdlu::HostPlan& hp = *impl_->plan->host;
+  for (std::size_t i = 0; i < hp.sparse_blocks.size(); ++i) {
+    if (hp.sparse_blocks[i] != block) continue;
+    const bdlu::SparseBlockPlan& sp = hp.sparse[i];
+    q = sp.q;
+    prow = sp.prow;
+    for (int32_t t = 0; t < sp.m; ++t) prow.push_back(sp.trow[sp.tperm[t]]);
+    return sp.k0;
+  }
+  return -1;
+}
+
 const std::vector<int32_t>& Solver::block_row_perm() const {
   static const std::vector<int32_t> empty;
   return impl_->plan ? impl_->plan->host->dec.P : empty;

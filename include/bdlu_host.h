@@ -73,6 +73,10 @@ void* bdlu_solver_device_stream(void* s);
 // Structural permutations of the last analyze(); arrays of n (n+1 / nblocks+1 for R).
 int32_t bdlu_solver_nblocks(void* s);
 void bdlu_solver_copy_perm(void* s, int32_t* P, int32_t* Q, int32_t* R);
+// Column order and pivot rows (size arrays of the block size) of block b of the
+// last analyze() when it is above the dense threshold; returns the number of
+// sparse leading columns k0 (the rest is the dense tail), or -1.
+int32_t bdlu_solver_sparse_order(void* s, int32_t b, int32_t* q, int32_t* prow);
 // Exports (reference Solver::getP/getQ/getL/getU/get_rearranged_matrix). Two-call
 // protocol: pass nullptr arrays to get the entry count, then arrays of that size.
 // which: 0 rearranged, 1 P, 2 Q, 3 L, 4 U. Returns entry count or -1 on failure.

@@ -112,6 +112,11 @@ class Solver final {
   const std::vector<int32_t>& block_row_perm() const;
   const std::vector<int32_t>& block_col_perm() const;
   const std::vector<int32_t>& block_bounds() const;
+  // Column order and pivot rows (local indices) of diagonal block `block` if it
+  // is above the dense threshold: q[k] = column at position k, prow[k] = row
+  // pivoted there (the sparse part's first, then the dense tail's). Returns the
+  // number of sparse leading columns k0, or -1 if the block is dense or unknown.
+  int32_t sparse_block_order(int32_t block, std::vector<int32_t>& q, std::vector<int32_t>& prow) const;
 
  private:
   bool clear_error_ok();
