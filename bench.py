@@ -577,6 +577,65 @@ def closed_loop(bd, args, config):
         "roofline": roof, "kernel_profile": prof}
 
 
+def plugin_evidence(bd, args):
+    """The competition plugin path end to end: writes cases from the seeded
+    workloads (the 5,000-unknown system and three serving patterns, b = A x*
+    with x* ~ U(-1, 1)) in the plugin benchmark's file layout, runs the oracle
+    (the CPU restatement of the reference BLOCK route) as the baseline column
+    with the reference's policy (1 warm-up + repetitions, one fresh solver per
+    run: set_matrix -> analyze -> factor -> solve), then runs
+    bdlu_plugin_bench, which loads libblockdaglu.so through CreateAlgorithm /
+    DestroyAlgorithm and writes runs.csv, margin.csv, timing_summary.csv and
+    summary.txt into the output directory."""
+    import subprocess
+    from oracle import oracle as orc
+    out = Path(args.plugin_evidence).resolve()
+    cases = out / "cases"
+    cases.mkdir(parents=True, exist_ok=True)
+    picks = [("5000-unknown system", bd.config_spec(0))] + [
+        (f"serving pattern {p}", bd.config_spec(4, p)) for p in (11, 8, 12)]
+    reps = args.plugin_reps
+    base_rows = ["alg,case,repetition,strategy,n,nnz,reorder_ms,lu_ms,solve_ms,total_ms,rel_err,rel_residual,ok"]
+    for cid, (label, spec) in enumerate(picks, start=1):
+        n, r, c, v, _ = bd.generate(spec)
+        xs = np.random.default_rng(900 + cid).uniform(-1, 1, n)
+        b = np.zeros(n)
+        np.add.at(b, r, v * xs[c])
+        d = cases / str(cid)
+        d.mkdir(exist_ok=True)
+        with open(d / f"{cid}_matrix.coo", "w") as f:
+            f.write(f"{n} {len(r)}\n")
+            f.writelines(f"{i} {j} {x!r}\n" for i, j, x in zip(r.tolist(), c.tolist(), v.tolist()))
+        np.savetxt(d / f"{cid}_rhs.csv", b, fmt="%.17g")
+        np.savetxt(d / f"{cid}_xstar.csv", xs, fmt="%.17g")
+        for rep in range(reps + 1):
+            t0 = time.perf_counter()
+            o = orc.OracleSolver(r, c, v, n)
+            ok = o.analyze() == 0
+            t1 = time.perf_counter()
+            ok = ok and o.factor() == 0
+            t2 = time.perf_counter()
+            rc, x = o.solve(b) if ok else (1, np.full(n, np.inf))
+            t3 = time.perf_counter()
+            ok = ok and rc == 0
+            if rep == 0:
+                continue  # warm-up
+            rel_err = float(np.linalg.norm(x - xs) / np.linalg.norm(xs))
+            rr = rel_residual(n, r, c, v, x, b)
+            base_rows.append(f"CPUReference,{cid},{rep},BLOCK,{n},{len(r)},{1e3 * (t1 - t0):.3f},"
+                             f"{1e3 * (t2 - t1):.3f},{1e3 * (t3 - t2):.3f},{1e3 * (t3 - t0):.3f},"
+                             f"{rel_err:.6e},{rr:.6e},{int(ok)}")
+        log(f"plugin evidence: case {cid} ({label}, n={n}) written, baseline timed")
+    (cases / "baseline_runs.csv").write_text("\n".join(base_rows) + "\n")
+    env = dict(os.environ, BLOCKDAGLU_BENCH_OUT=str(out), BLOCKDAGLU_BENCH_REPETITIONS=str(reps),
+               BLOCKDAGLU_BENCH_WARMUP="1")
+    exe = ROOT / "block-daglu_amd" / "lib" / "bdlu_plugin_bench"
+    lib = ROOT / "block-daglu_amd" / "lib" / "libblockdaglu.so"
+    rc = subprocess.run([str(exe), "--lib", str(lib), str(cases)], env=env).returncode
+    print((out / "summary.txt").read_text() if (out / "summary.txt").exists() else "no summary written")
+    sys.exit(rc)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -603,6 +662,10 @@ def main():
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
                     help="compute HBM bytes per launch of the roofline kernel from two --pmc runs, write "
                          f"{TRAFFIC_FILE.relative_to(ROOT)} and exit")
+    ap.add_argument("--plugin-evidence", metavar="DIR",
+                    help="run the competition plugin (CreateAlgorithm) on seeded cases against the CPU "
+                         "reference restatement and write the evidence CSVs into DIR, then exit")
+    ap.add_argument("--plugin-reps", type=int, default=10, help="repetitions per case for --plugin-evidence")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB device-to-device copy (the known-size reference kernel for "
                          "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs)")
@@ -624,6 +687,8 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     log(f"library build {bd.build_info()}")
+    if args.plugin_evidence:
+        plugin_evidence(bd, args)
     if args.copy_calibration:
         x = bd.DeviceArray.from_numpy(np.ones(COPY_BYTES // 8))
         y = bd.DeviceArray(COPY_BYTES // 8)

@@ -4,6 +4,7 @@ Produces, in-tree:
   block-daglu_amd/lib/libbdlu_gpu.so     HIP kernels + device C-ABI (hipcc, gfx950)
   block-daglu_amd/lib/libblockdaglu.so   host C++ (Solver, analysis, planner, C-ABI)
   block-daglu_amd/lib/bdlu_api_check     tests/cpp/api_check.cpp against the public C++ API
+  block-daglu_amd/lib/bdlu_plugin_bench  tests/cpp/plugin_bench.cpp, loads the plugin exports at run time
   oracle/liboracle.so                    the oracle (test infrastructure; built by
                                          oracle/Makefile, only when asked)
 Usage: python block-daglu_amd/build.py [--oracle] [--jobs N]
@@ -32,6 +33,7 @@ HOST_SRCS = [
     "csrc/host/canon.cpp", "csrc/host/btf.cpp", "csrc/host/decomp.cpp", "csrc/host/colamd.cpp",
     "csrc/host/symbolic.cpp", "csrc/host/denselu.cpp", "csrc/host/plan.cpp", "csrc/host/solver.cpp",
     "csrc/host/capi.cpp", "csrc/host/gen.cpp", "csrc/host/export.cpp", "csrc/host/serve.cpp",
+    "csrc/host/plugin.cpp",
 ]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
@@ -91,6 +93,11 @@ def build(oracle: bool = False, jobs: int = 8) -> None:
     _run([CXX, "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(ROOT / "tests" / "cpp" / "api_check.cpp"),
           "-o", str(check) + ".tmp", f"-L{LIB}", "-lblockdaglu", "-Wl,-rpath,$ORIGIN", "-lpthread"])
     os.replace(str(check) + ".tmp", check)
+    # The plugin benchmark loads libblockdaglu.so at run time (no link to it).
+    pbench = LIB / "bdlu_plugin_bench"
+    _run([CXX, "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(ROOT / "tests" / "cpp" / "plugin_bench.cpp"),
+          "-o", str(pbench) + ".tmp", "-ldl"])
+    os.replace(str(pbench) + ".tmp", pbench)
     if oracle:
         _run(["make", "-s", "-C", str(ROOT / "oracle")])
     shutil.rmtree(OBJ, ignore_errors=True)

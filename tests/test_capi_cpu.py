@@ -146,3 +146,17 @@ def test_cpp_api_program_host_checks(bd):
     r = subprocess.run([str(exe), "--host-only"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert "checks passed" in r.stdout
+
+
+def test_plugin_exports_and_host_stages(bd):
+    """The competition plugin boundary (src/exports.cpp:14-24): libblockdaglu.so
+    exports CreateAlgorithm / DestroyAlgorithm with C linkage; the plugin
+    benchmark loads it at run time, stores a matrix (prepareData is storage
+    only) and sees rearrangeMatrix reject a negative index, without a GPU."""
+    import subprocess
+    exe = ROOT / "block-daglu_amd" / "lib" / "bdlu_plugin_bench"
+    lib = ROOT / "block-daglu_amd" / "lib" / "libblockdaglu.so"
+    L = bd.lib()
+    assert hasattr(L, "CreateAlgorithm") and hasattr(L, "DestroyAlgorithm")
+    r = subprocess.run([str(exe), "--lib", str(lib), "--selftest"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
