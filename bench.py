@@ -186,7 +186,11 @@ def run_gpu(bd, n, r, c, v_sets, b, precision, steps, warmup, profile):
     s.set_precision(bd.Precision(precision))
     assert s.set_matrix(r, c, v_sets[0], n), s.last_error_message()
     t0 = time.time()
+    # The BLOCK route is the path measured (AUTO would send systems below
+    # 50,000 unknowns to the global-KLU route).
+    os.environ["USERALGO_MODE"] = "block"
     assert s.analyze(), s.last_error_message()
+    assert s.selected_mode() == "BLOCK"
     log(f"analyze (plan build + upload) {time.time() - t0:.2f}s precision={precision}")
     assert s.factor(), s.last_error_message()
     order = bd.canonical_order(r, c)

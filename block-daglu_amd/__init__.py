@@ -111,6 +111,9 @@ HOST_SYMBOLS = [
     ("bdlu_serve_submit", C.c_int64, [_P, C.c_int32, _F64P, C.c_int64, _F64P, C.c_int32]),
     ("bdlu_serve_wait", C.c_int, [_P, C.c_int64, _F64P, C.POINTER(_ServeResult)]),
     ("bdlu_host_analyze", _P, [C.c_int64, _I32P, _I32P, _F64P, C.c_int32, C.POINTER(C.c_int)]),
+    ("bdlu_host_analyze_klu", _P, [C.c_int64, _I32P, _I32P, _F64P, C.c_int32, C.c_int, C.c_int, C.c_double,
+                                   C.POINTER(C.c_int)]),
+    ("bdlu_route_for", C.c_int, [C.c_int64, _I32P, _I32P, _F64P, C.c_int32, _I32P, _F64P]),
     ("bdlu_host_free", None, [_P]),
     ("bdlu_host_info", None, [_P, _I32P, _I32P, _I32P, _I32P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _I32P]),
     ("bdlu_host_copy_perm", None, [_P, _I32P, _I32P, _I32P]),
@@ -143,6 +146,7 @@ GPU_SYMBOLS = [
     ("bdlu_gpu_solve_noalloc", C.c_int, [_P, C.c_uint64, C.c_void_p, C.c_int32, C.c_int64, _P, _P]),
     ("bdlu_gpu_plan_values", C.c_int, [_P, C.c_uint64, _F64P]),
     ("bdlu_gpu_stream", _P, [_P]),
+    ("bdlu_gpu_plan_dense_pivots", C.c_int, [_P, C.c_uint64, _I32P]),
     ("bdlu_gpu_profile", C.c_int, [_P, C.c_int]),
     ("bdlu_gpu_profile_read", C.c_int, [_P, C.c_int, _F64P, C.POINTER(C.c_int64), _F64P, _F64P]),
     ("bdlu_gpu_debug_dense_lu", C.c_int, [_P, C.c_int32, _F64P, C.c_int]),
@@ -411,6 +415,15 @@ class Solver:
         self._L.bdlu_solver_copy_perm(self._h, _ptr(P, _I32P), _ptr(Q, _I32P), _ptr(R, _I32P))
         return P, Q, R
 
+    def dense_pivots(self):
+        """Row pivots of the dense blocks from the last factor(): for each permuted
+        row i inside a dense block, the block-local row the pivoting brought there."""
+        piv = np.zeros(self.n(), np.int32)
+        rc = lib().gpu.bdlu_gpu_plan_dense_pivots(self.device_context(), self.device_plan(), _ptr(piv, _I32P))
+        if rc != 0:
+            raise RuntimeError("bdlu_gpu_plan_dense_pivots failed")
+        return piv
+
     def sparse_order(self, b: int, size: int):
         """(q, prow, k0) of block b if it is above the dense threshold, else None."""
         q = np.zeros(size, np.int32)
@@ -501,13 +514,20 @@ class Server:
 class HostAnalysis:
     """Read-only view of the host analysis (canonicalize, BTF, merge, per-block symbolic). CPU only."""
 
-    def __init__(self, rows, cols, vals, dimension: int = -1):
+    def __init__(self, rows, cols, vals, dimension: int = -1, klu=None):
+        """klu: None for the BLOCK route's analysis, or (btf, scale, tol) for
+        the global-KLU route's."""
         L = lib()
         r, c, v = _i32(rows), _i32(cols), _f64(vals)
         err = C.c_int(0)
         self._L = L
-        self._h = L.bdlu_host_analyze(len(r), _ptr(r, _I32P), _ptr(c, _I32P), _ptr(v, _F64P), dimension,
-                                      C.byref(err))
+        if klu is None:
+            self._h = L.bdlu_host_analyze(len(r), _ptr(r, _I32P), _ptr(c, _I32P), _ptr(v, _F64P), dimension,
+                                          C.byref(err))
+        else:
+            btf, scale, tol = klu
+            self._h = L.bdlu_host_analyze_klu(len(r), _ptr(r, _I32P), _ptr(c, _I32P), _ptr(v, _F64P), dimension,
+                                              int(btf), int(scale), float(tol), C.byref(err))
         self.error = SolverError(err.value)
         if not self._h:
             return
@@ -557,6 +577,21 @@ class HostAnalysis:
         a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
         self._L.bdlu_host_signatures(self._h, C.byref(a), C.byref(b), C.byref(c))
         return a.value, b.value, c.value
+
+
+
+def route_for(rows, cols, vals, dimension: int = -1) -> dict:
+    """The route Solver.analyze would take for this matrix under the current
+    environment: {"klu": bool, "btf", "scale", "ordering", "tol"} (CPU only)."""
+    r, c, v = _i32(rows), _i32(cols), _f64(vals)
+    out = np.zeros(4, np.int32)
+    tol = C.c_double()
+    rc = lib().bdlu_route_for(len(r), _ptr(r, _I32P), _ptr(c, _I32P), _ptr(v, _F64P), dimension, _ptr(out, _I32P),
+                              C.byref(tol))
+    if rc != 0:
+        raise ValueError("matrix fails validation")
+    return {"klu": bool(out[0]), "btf": int(out[1]), "scale": int(out[2]), "ordering": int(out[3]),
+            "tol": tol.value}
 
 
 def colamd_order(n: int, indptr, indices) -> np.ndarray:

@@ -33,7 +33,7 @@ HOST_SRCS = [
     "csrc/host/canon.cpp", "csrc/host/btf.cpp", "csrc/host/decomp.cpp", "csrc/host/colamd.cpp",
     "csrc/host/symbolic.cpp", "csrc/host/denselu.cpp", "csrc/host/plan.cpp", "csrc/host/solver.cpp",
     "csrc/host/capi.cpp", "csrc/host/gen.cpp", "csrc/host/export.cpp", "csrc/host/serve.cpp",
-    "csrc/host/plugin.cpp",
+    "csrc/host/plugin.cpp", "csrc/host/route.cpp",
 ]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

@@ -157,6 +157,16 @@ struct Plan {
   // Explicit inverses M = U^{-1} L^{-1} of the dense blocks, at the factor offsets.
   double* minv64 = nullptr;
   float* minv32 = nullptr;
+  // Pivot rule of the dense blocks (0 partial pivoting; 1 KLU threshold rule
+  // with tolerance dense_tol and per-row tie keys dense_key) and KLU's whole-row
+  // scaling (row_scale 1 sums / 2 maxima; rs per original row, svals the
+  // scaled canonical values the refactor scatters).
+  int32_t dense_rule = 0;
+  double dense_tol = 1.0;
+  int32_t* dense_key = nullptr;
+  int32_t row_scale = 0;
+  double* rs = nullptr;
+  double* svals = nullptr;
   int32_t* piv = nullptr;  // per row: block-local source row of the pivot gather
   int32_t* block_status = nullptr;  // per block (dense status)
   int32_t* prec_of_block = nullptr; // per block: 0 fp64 factors, 1 fp32
@@ -430,6 +440,19 @@ void upload_into(bdlu_ctx* c, const bdlu_plan_desc& d, Plan& P) {
   const int64_t noff = d.off_ptr[n];
   P.off_col = A.copy(d.off_col, noff, st);
   P.off_val = P.arena + d.off_val0;
+  if (d.dense_rule != 0 && d.dense_rule != 1) throw HipError{"unknown dense pivot rule"};
+  if (d.row_scale < 0 || d.row_scale > 2) throw HipError{"unknown row scaling"};
+  P.dense_rule = d.dense_rule;
+  P.dense_tol = d.dense_tol;
+  if (d.dense_rule == 1) {
+    if (!d.dense_key) throw HipError{"KLU pivot rule without tie keys"};
+    P.dense_key = A.copy(d.dense_key, n, st);
+  }
+  P.row_scale = d.row_scale;
+  if (d.row_scale > 0) {
+    P.rs = A.get<double>(n);
+    P.svals = A.get<double>(d.nnz);
+  }
 
   // Dense blocks.
   P.ndense = d.ndense;
@@ -775,7 +798,7 @@ void block_solve_launches(Plan& P, const Work& W, hipStream_t st, const double* 
   const int64_t ld = P.n;
   bdk::launch_fill_i32(st, W.tickets, P.nticket, 0);
   bdk::launch_fill_sentinel(st, W.xperm, int64_t(P.n) * kt);
-  bdk::launch_permute(st, P.n, kt, P.P, b, ld, W.bperm, ld);
+  bdk::launch_permute(st, P.n, kt, P.P, b, ld, W.bperm, ld, P.row_scale ? P.rs : nullptr);
   int32_t tix = 0;
   for (const Phase& ph : P.phases) {
     if (!ph.sparse) {
@@ -935,7 +958,12 @@ int refactor_impl(bdlu_ctx* c, Plan& P, const double* values, int where, int pre
   ProfEvents* pe = c->prof ? &pev : nullptr;
   for (SparseDev& S : P.sp) S.prec = fp32 ? 1 : 0;
   auto launches = [&] {
-    bdk::launch_scatter(st, P.nnz, P.vals, P.vmap, P.arena);
+    if (P.row_scale) {
+      bdk::launch_row_scale(st, P.n, P.csr_ptr, P.vals, P.row_scale, P.rs, P.svals);
+      bdk::launch_scatter(st, P.nnz, P.svals, P.vmap, P.arena);
+    } else {
+      bdk::launch_scatter(st, P.nnz, P.vals, P.vmap, P.arena);
+    }
     bdk::launch_anorm(st, P.n, P.csr_ptr, P.vals, P.work.partials, P.norms + kAnormSlot);
     // Dense blocks (dynamic partial pivoting, reference semantics), on a side
     // stream unless profiling (the profile events time one stream).
@@ -950,11 +978,11 @@ int refactor_impl(bdlu_ctx* c, Plan& P, const double* values, int where, int pre
     const int32_t* order = P.sched.dense_lpt ? P.d_lpt : nullptr;
     if (fp32) {
       bdk::launch_dense_lu<float>(ds, P.ndense, P.d_size, P.d_start, P.d_val0, P.d_lu0, P.arena, P.lu32, P.piv,
-                                  P.block_status, P.d_block, order);
+                                  P.block_status, P.d_block, order, P.dense_tol, P.dense_key);
       bdk::launch_dense_inv<float>(ds, P.ndense, P.d_size, P.d_lu0, P.lu32, P.minv32);
     } else {
       bdk::launch_dense_lu<double>(ds, P.ndense, P.d_size, P.d_start, P.d_val0, P.d_lu0, P.arena, P.lu64, P.piv,
-                                   P.block_status, P.d_block, order);
+                                   P.block_status, P.d_block, order, P.dense_tol, P.dense_key);
       bdk::launch_dense_inv<double>(ds, P.ndense, P.d_size, P.d_lu0, P.lu64, P.minv64);
     }
     if (pe) pe->mark(BDLU_PROF_DENSE_LU, st);
@@ -1040,7 +1068,7 @@ int refactor_impl(bdlu_ctx* c, Plan& P, const double* values, int where, int pre
     int64_t* dv0 = tmp.copy(v0, st);
     int64_t* dl0 = tmp.copy(l0, st);
     bdk::launch_dense_lu<double>(st, static_cast<int32_t>(redo.size()), dsz, dst_, dv0, dl0, P.arena, P.lu64,
-                                 P.piv, P.block_status, dbl);
+                                 P.piv, P.block_status, dbl, nullptr, P.dense_tol, P.dense_key);
     bdk::launch_dense_inv<double>(st, static_cast<int32_t>(redo.size()), dsz, dl0, P.lu64, P.minv64);
     // The FP64 redo writes its own statuses: a block that is singular in
     // FP64 (exact zero pivot) fails like any other dense block.
@@ -1463,6 +1491,22 @@ int bdlu_gpu_plan_values(bdlu_ctx* c, uint64_t id, double* host_values) {
       hipMemcpyAsync(host_values, P.vals, sizeof(double) * P.nnz, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     return set_err(c, BDLU_E_BACKEND, "value copy failed");
+  return BDLU_OK;
+}
+
+int bdlu_gpu_plan_dense_pivots(bdlu_ctx* c, uint64_t id, int32_t* host_piv) {
+  if (!c || !host_piv) return BDLU_E_INVALID;
+  std::lock_guard<std::mutex> g(c->mu);
+  auto it = c->plans.find(id);
+  if (it == c->plans.end()) return set_err(c, BDLU_E_INVALID, "unknown plan");
+  Plan& P = *it->second;
+  try {
+    HIPCHECK(hipSetDevice(c->device));
+    HIPCHECK(hipMemcpyAsync(host_piv, P.piv, sizeof(int32_t) * P.n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  } catch (const HipError& e) {
+    return set_err(c, BDLU_E_BACKEND, e.msg);
+  }
   return BDLU_OK;
 }
 

@@ -37,10 +37,17 @@ void launch_fill_f32(hipStream_t st, float* p, int64_t n, float v);
 // Fills with the solve's "not yet published" pattern (see kernels.hip).
 void launch_fill_sentinel(hipStream_t st, double* p, int64_t n);
 
+// key == nullptr: partial pivoting (BLOCK route); else KLU's threshold rule
+// with tolerance tol and per-row tie keys (global-KLU route).
 template <typename T>
 void launch_dense_lu(hipStream_t st, int32_t nd, const int32_t* sizes, const int32_t* starts, const int64_t* val0,
                      const int64_t* lu0, const double* arena, T* lu, int32_t* piv, int32_t* status,
-                     const int32_t* block_of, const int32_t* order = nullptr);
+                     const int32_t* block_of, const int32_t* order = nullptr, double tol = 1.0,
+                     const int32_t* key = nullptr);
+// KLU whole-row scaling: rs[i] = sum (kind 1) or max (kind 2) of |vals| in CSR
+// row i (1 for a row without a nonzero), svals[e] = vals[e] / rs[row of e].
+void launch_row_scale(hipStream_t st, int32_t n, const int64_t* ptr, const double* vals, int32_t kind, double* rs,
+                      double* svals);
 
 void launch_rowscale(hipStream_t st, const SpView& v, const double* arena, double* rs);
 // Columns of one refactor level whose reach is long, done row-oriented by
@@ -107,7 +114,7 @@ void launch_dense_inv(hipStream_t st, int32_t nd, const int32_t* sizes, const in
 // Kernels with per-column registers are instantiated for 1, 2, 4 and 8 columns
 // (kc is rounded up by the caller, padding with zero columns).
 void launch_permute(hipStream_t st, int32_t n, int32_t k, const int32_t* P, const double* src, int64_t lds,
-                    double* dst, int64_t ldd);  // dst[c][i] = src[c][P[i]]
+                    double* dst, int64_t ldd, const double* rs = nullptr);  // dst[c][i] = src[c][P[i]] (/ rs[P[i]])
 void launch_unpermute(hipStream_t st, int32_t n, int32_t k, const int32_t* Q, const double* src, int64_t lds,
                       double* dst, int64_t ldd);  // dst[c][Q[i]] = src[c][i]
 // Dense-block solve of one segment (persistent, sync-free). ticket starts at 0;

@@ -142,18 +142,27 @@ void launch_fill_sentinel(hipStream_t st, double* p, int64_t n) {
 }
 
 // ---------------------------------------------------------------------------
-// K1: batched dense LU with partial pivoting, one wavefront per block (s <= 64).
-// Same arithmetic as the reference dense branch (ubtf:1465-1494): pivot = first
-// row with the largest |a_ik| (ties -> smaller row), fail iff that is 0, full
-// row swap, l_ik = a_ik / a_kk, a_ij -= l_ik * a_kj. piv out: for each row
-// position of the block, the block-local row that the swaps bring there.
+// K1: batched dense LU with row pivoting, one wavefront per block (s <= 64).
+// Same arithmetic as the reference dense branch (ubtf:1465-1494): fail iff the
+// largest candidate |a_ik| is 0, full row swap, l_ik = a_ik / a_kk,
+// a_ij -= l_ik * a_kj. Pivot choice:
+//   key == nullptr: partial pivoting, the first row with the largest |a_ik|
+//     (ties -> smaller row position), as the reference's dense branch;
+//   key != nullptr: KLU's threshold rule (global-KLU route): the row at
+//     position k (the column's diagonal; the physical swaps carry KLU's
+//     diagonal tracking) is kept when |a_kk| >= tol * max_i |a_ik|, else the
+//     largest, ties to the smaller key (each row's id before the block's column
+//     ordering, key[starts + position], moved with its row).
+// piv out: for each row position of the block, the block-local row that the
+// swaps bring there.
 template <typename T>
 __global__ __launch_bounds__(64) void k_dense_lu(int32_t nd, const int32_t* __restrict__ sizes,
                                                  const int32_t* __restrict__ starts, const int64_t* __restrict__ val0,
                                                  const int64_t* __restrict__ lu0, const double* __restrict__ arena,
                                                  T* __restrict__ lu, int32_t* __restrict__ piv,
                                                  int32_t* __restrict__ status, const int32_t* __restrict__ block_of,
-                                                 const int32_t* __restrict__ order) {
+                                                 const int32_t* __restrict__ order, double tol,
+                                                 const int32_t* __restrict__ key) {
   __shared__ T a[64 * kLdsPad];
   const int lane = threadIdx.x;
   // order: which dense block each workgroup takes (e.g. largest first), or none.
@@ -168,15 +177,31 @@ __global__ __launch_bounds__(64) void k_dense_lu(int32_t nd, const int32_t* __re
   __syncthreads();
   int32_t mypiv = lane;
   int32_t st = 0;
+  const bool klu = key != nullptr;
+  int32_t mykey = (klu && lane < s) ? key[starts[t] + lane] : lane;
   for (int k = 0; k < s; ++k) {
     T v = (lane >= k && lane < s) ? tabs(a[k * kLdsPad + lane]) : T(-1);
     int idx = lane;
-    for (int off = 32; off > 0; off >>= 1) {
-      const T ov = __shfl_xor(v, off);
-      const int oi = __shfl_xor(idx, off);
-      if (ov > v || (ov == v && oi < idx)) {
-        v = ov;
-        idx = oi;
+    if (!klu) {
+      for (int off = 32; off > 0; off >>= 1) {
+        const T ov = __shfl_xor(v, off);
+        const int oi = __shfl_xor(idx, off);
+        if (ov > v || (ov == v && oi < idx)) {
+          v = ov;
+          idx = oi;
+        }
+      }
+    } else {
+      int kk = mykey;
+      for (int off = 32; off > 0; off >>= 1) {
+        const T ov = __shfl_xor(v, off);
+        const int oi = __shfl_xor(idx, off);
+        const int ok = __shfl_xor(kk, off);
+        if (ov > v || (ov == v && ok < kk)) {
+          v = ov;
+          idx = oi;
+          kk = ok;
+        }
       }
     }
     const int32_t zero = uniform_from_lane0(!(v > T(0)) ? 1 : 0);
@@ -184,7 +209,15 @@ __global__ __launch_bounds__(64) void k_dense_lu(int32_t nd, const int32_t* __re
       st |= kStZeroPivot;
       break;
     }
-    const int p = uniform_from_lane0(idx);
+    int p = idx;
+    // Diagonal preference (KLU's test: |diag| >= tol * largest); every lane
+    // holds the same largest value after the butterfly.
+    if (klu && tabs(a[k * kLdsPad + k]) >= static_cast<T>(tol) * v) p = k;
+    p = uniform_from_lane0(p);
+    if (klu) {
+      const int32_t kk_k = __shfl(mykey, k), kk_p = __shfl(mykey, p);
+      mykey = (lane == k) ? kk_p : ((lane == p) ? kk_k : mykey);
+    }
     mypiv = (lane == k) ? p : mypiv;
     if (p != k) {
       if (lane < s) {
@@ -236,17 +269,41 @@ __global__ __launch_bounds__(64) void k_dense_lu(int32_t nd, const int32_t* __re
 template <typename T>
 void launch_dense_lu(hipStream_t st, int32_t nd, const int32_t* sizes, const int32_t* starts, const int64_t* val0,
                      const int64_t* lu0, const double* arena, T* lu, int32_t* piv, int32_t* status,
-                     const int32_t* block_of, const int32_t* order) {
+                     const int32_t* block_of, const int32_t* order, double tol, const int32_t* key) {
   if (nd <= 0) return;
   hipLaunchKernelGGL(k_dense_lu<T>, dim3(nd), dim3(64), 0, st, nd, sizes, starts, val0, lu0, arena, lu, piv,
-                     status, block_of, order);
+                     status, block_of, order, tol, key);
 }
 template void launch_dense_lu<double>(hipStream_t, int32_t, const int32_t*, const int32_t*, const int64_t*,
                                       const int64_t*, const double*, double*, int32_t*, int32_t*, const int32_t*,
-                                      const int32_t*);
+                                      const int32_t*, double, const int32_t*);
 template void launch_dense_lu<float>(hipStream_t, int32_t, const int32_t*, const int32_t*, const int64_t*,
                                      const int64_t*, const double*, float*, int32_t*, int32_t*, const int32_t*,
-                                     const int32_t*);
+                                     const int32_t*, double, const int32_t*);
+
+// KLU row scaling over whole rows (klu_scale, scale 1 sums / 2 maxima of |a_ij|
+// in column order; a row without a nonzero gets 1), and the scaled values
+// a_ij / rs_i that the blocks are factored from. One thread per row of the
+// canonical CSR.
+__global__ void k_row_scale(int32_t n, const int64_t* __restrict__ ptr, const double* __restrict__ vals, int32_t kind,
+                            double* __restrict__ rs, double* __restrict__ svals) {
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int64_t e0 = ptr[i], e1 = ptr[i + 1];
+    double r = 0.0;
+    for (int64_t e = e0; e < e1; ++e) {
+      const double v = vals[e] < 0.0 ? -vals[e] : vals[e];
+      r = kind == 1 ? r + v : (r < v ? v : r);
+    }
+    if (r == 0.0) r = 1.0;
+    rs[i] = r;
+    for (int64_t e = e0; e < e1; ++e) svals[e] = vals[e] / r;
+  }
+}
+void launch_row_scale(hipStream_t st, int32_t n, const int64_t* ptr, const double* vals, int32_t kind, double* rs,
+                      double* svals) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_row_scale, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, n, ptr, vals, kind, rs, svals);
+}
 
 // The explicit inverse of each dense block's factors, for the solve:
 // M = U^{-1} L^{-1} (so x = M (P b) solves the block), s x s column-major at the
@@ -1239,11 +1296,17 @@ template void launch_tail_lu<float>(hipStream_t, int32_t, float*, int64_t, doubl
 // ---------------------------------------------------------------------------
 // Solve.
 // Multi-column vectors: column c of a vector set lives at base + c * ld.
+// dst(i) = src(P[i]), divided by rs[P[i]] when row scale factors are given.
 __global__ void k_permute(int32_t n, int32_t k, const int32_t* __restrict__ P, const double* __restrict__ src,
-                          int64_t lds, double* __restrict__ dst, int64_t ldd) {
+                          int64_t lds, double* __restrict__ dst, int64_t ldd, const double* __restrict__ rs) {
   for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int32_t pi = P[i];
-    for (int32_t c = 0; c < k; ++c) dst[c * ldd + i] = src[c * lds + pi];
+    if (rs) {
+      const double r = rs[pi];
+      for (int32_t c = 0; c < k; ++c) dst[c * ldd + i] = src[c * lds + pi] / r;
+    } else {
+      for (int32_t c = 0; c < k; ++c) dst[c * ldd + i] = src[c * lds + pi];
+    }
   }
 }
 __global__ void k_unpermute(int32_t n, int32_t k, const int32_t* __restrict__ Q, const double* __restrict__ src,
@@ -1254,8 +1317,8 @@ __global__ void k_unpermute(int32_t n, int32_t k, const int32_t* __restrict__ Q,
   }
 }
 void launch_permute(hipStream_t st, int32_t n, int32_t k, const int32_t* P, const double* src, int64_t lds,
-                    double* dst, int64_t ldd) {
-  hipLaunchKernelGGL(k_permute, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, n, k, P, src, lds, dst, ldd);
+                    double* dst, int64_t ldd, const double* rs) {
+  hipLaunchKernelGGL(k_permute, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, n, k, P, src, lds, dst, ldd, rs);
 }
 void launch_unpermute(hipStream_t st, int32_t n, int32_t k, const int32_t* Q, const double* src, int64_t lds,
                       double* dst, int64_t ldd) {

@@ -191,6 +191,54 @@ void* bdlu_host_analyze(int64_t nnz, const int32_t* rows, const int32_t* cols, c
   return h;
 }
 
+void* bdlu_host_analyze_klu(int64_t nnz, const int32_t* rows, const int32_t* cols, const double* vals,
+                            int32_t dimension, int btf, int scale, double tol, int* error_code) {
+  auto* h = new HostAnalysis;
+  std::vector<std::tuple<int, int, double>> coo(static_cast<std::size_t>(nnz));
+  for (int64_t k = 0; k < nnz; ++k) coo[k] = std::make_tuple(rows[k], cols[k], vals[k]);
+  std::string msg;
+  int code = bdlu::canonicalize(coo, dimension, h->canon, msg) == bdlu::CanonStatus::Ok ? 0 : BDLU_ERR_INVALID_INDEX;
+  if (code == 0) {
+    bdlu::KluRoute klu;
+    klu.btf = btf;
+    klu.scale = scale;
+    klu.tol = tol;
+    try {
+      h->plan_ok = bdlu::build_host_plan(h->canon, h->plan, &klu).ok;
+      if (!h->plan_ok) code = BDLU_ERR_FACTORIZATION_FAILED;
+    } catch (...) {
+      code = BDLU_ERR_ANALYSIS_FAILED;
+    }
+  }
+  if (error_code) *error_code = code;
+  if (code != 0 && code != BDLU_ERR_FACTORIZATION_FAILED) {
+    delete h;
+    return nullptr;
+  }
+  return h;
+}
+
+int bdlu_route_for(int64_t nnz, const int32_t* rows, const int32_t* cols, const double* vals, int32_t dimension,
+                   int32_t* out4, double* tol) {
+  if (!out4) return -1;
+  std::vector<std::tuple<int, int, double>> coo(static_cast<std::size_t>(nnz));
+  for (int64_t k = 0; k < nnz; ++k) coo[k] = std::make_tuple(rows[k], cols[k], vals[k]);
+  std::string msg;
+  bdlu::Canonical a;
+  if (bdlu::canonicalize(coo, dimension, a, msg) != bdlu::CanonStatus::Ok) return BDLU_ERR_INVALID_INDEX;
+  const bdlu::RouteMode mode = bdlu::route_mode_from_env();
+  bool klu = mode == bdlu::RouteMode::Klu;
+  if (mode == bdlu::RouteMode::Auto)
+    klu = !bdlu::auto_choose_block(a, bdlu::matrix_sig(a.n, a), bdlu::matrix_sig2(a.n, a), bdlu::matrix_value_sig(a),
+                                   0);
+  out4[0] = klu ? 1 : 0;
+  out4[1] = bdlu::klu_btf_for(a.n, a.nnz());
+  out4[2] = bdlu::klu_scale_for(a.n, a.nnz(), out4[1]);
+  out4[3] = bdlu::klu_ordering();
+  if (tol) *tol = bdlu::klu_tol_for(a.n, a.nnz(), out4[1]);
+  return 0;
+}
+
 void bdlu_host_free(void* h) { delete static_cast<HostAnalysis*>(h); }
 
 void bdlu_host_info(void* hp, int32_t* n, int32_t* nblocks_raw, int32_t* nblocks, int32_t* nmatch, int64_t* nnz,

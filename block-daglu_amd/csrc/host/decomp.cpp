@@ -109,17 +109,26 @@ Decomposition build_decomposition(const Canonical& a, const DecompOptions& opt) 
   }
 
   std::vector<int32_t> P(n), Q(n), R(static_cast<std::size_t>(n) + 1);
-  double work = 0.0;
-  int32_t nmatch = 0;
-  int32_t nb = btf_order(n, Ap.data(), Ai.data(), opt.btf_maxwork, &work, P.data(), Q.data(), R.data(), &nmatch);
-  if (nb <= 0) throw std::runtime_error("btf_order failed");
-  if (opt.btf_maxwork > 0.0 && nmatch < n) {
-    nb = btf_order(n, Ap.data(), Ai.data(), 0.0, &work, P.data(), Q.data(), R.data(), &nmatch);
-    if (nb <= 0) throw std::runtime_error("btf_order retry failed");
+  int32_t nb = 1;
+  if (opt.btf) {
+    double work = 0.0;
+    int32_t nmatch = 0;
+    nb = btf_order(n, Ap.data(), Ai.data(), opt.btf_maxwork, &work, P.data(), Q.data(), R.data(), &nmatch);
+    if (nb <= 0) throw std::runtime_error("btf_order failed");
+    if (opt.btf_maxwork > 0.0 && nmatch < n) {
+      nb = btf_order(n, Ap.data(), Ai.data(), 0.0, &work, P.data(), Q.data(), R.data(), &nmatch);
+      if (nb <= 0) throw std::runtime_error("btf_order retry failed");
+    }
+    d.nmatch = nmatch;
+    for (int32_t k = 0; k < n; ++k) Q[k] = btf_unflip(Q[k]);
+  } else {
+    // KLU with btf = 0: the matrix is one block in its own order.
+    for (int32_t k = 0; k < n; ++k) P[k] = Q[k] = k;
+    R[0] = 0;
+    R[1] = n;
+    d.nmatch = n;
   }
   d.nblocks_raw = nb;
-  d.nmatch = nmatch;
-  for (int32_t k = 0; k < n; ++k) Q[k] = btf_unflip(Q[k]);
 
   std::vector<int32_t> Rraw(R.begin(), R.begin() + nb + 1);
   d.R = opt.merge ? merge_block_boundaries(Rraw, n, opt.merge_max) : Rraw;
@@ -135,6 +144,56 @@ Decomposition build_decomposition(const Canonical& a, const DecompOptions& opt) 
   std::vector<int32_t> blk(n);
   for (int32_t b = 0; b < d.nblocks; ++b)
     for (int32_t k = d.R[b]; k < d.R[b + 1]; ++k) blk[k] = b;
+
+  if (opt.order_small_max >= 2) {
+    // Column order of each block of 2..order_small_max (klu_analyze's block
+    // ordering): the block's local CSC takes the rows of each column in the
+    // order of the matrix's CSC (ascending original row, which is canonical
+    // order), then the order is applied to rows and columns alike.
+    auto target = [&](int32_t b) {
+      const int32_t s = d.R[b + 1] - d.R[b];
+      return s >= 2 && s <= opt.order_small_max;
+    };
+    std::vector<int32_t> cptr(static_cast<std::size_t>(n) + 1, 0);
+    for (std::size_t k = 0; k < nnz; ++k) {
+      if (!kept(k)) continue;
+      const int32_t rp = d.invP[a.row[k]], cp = d.invQ[a.col[k]];
+      if (blk[rp] == blk[cp] && target(blk[cp])) cptr[cp + 1]++;
+    }
+    for (int32_t j = 0; j < n; ++j) cptr[j + 1] += cptr[j];
+    std::vector<int32_t> crow(static_cast<std::size_t>(cptr[n]));
+    {
+      std::vector<int32_t> next(cptr.begin(), cptr.end() - 1);
+      for (std::size_t k = 0; k < nnz; ++k) {
+        if (!kept(k)) continue;
+        const int32_t rp = d.invP[a.row[k]], cp = d.invQ[a.col[k]];
+        if (blk[rp] == blk[cp] && target(blk[cp])) crow[next[cp]++] = rp - d.R[blk[cp]];
+      }
+    }
+    d.tie_key.resize(n);
+    std::vector<int32_t> P2(d.P), Q2(d.Q), lp;
+    for (int32_t b = 0; b < d.nblocks; ++b) {
+      const int32_t lo = d.R[b], s = d.R[b + 1] - lo;
+      if (!target(b)) {
+        for (int32_t k = 0; k < s; ++k) d.tie_key[lo + k] = k;
+        continue;
+      }
+      lp.assign(static_cast<std::size_t>(s) + 1, 0);
+      for (int32_t j = 0; j <= s; ++j) lp[j] = cptr[lo + j] - cptr[lo];
+      const std::vector<int32_t> q = colamd_order(s, lp.data(), crow.data() + cptr[lo]);
+      for (int32_t k = 0; k < s; ++k) {
+        P2[lo + k] = d.P[lo + q[k]];
+        Q2[lo + k] = d.Q[lo + q[k]];
+        d.tie_key[lo + k] = q[k];
+      }
+    }
+    d.P = std::move(P2);
+    d.Q = std::move(Q2);
+    for (int32_t k = 0; k < n; ++k) {
+      d.invP[d.P[k]] = k;
+      d.invQ[d.Q[k]] = k;
+    }
+  }
 
   // Pass 1: counts (and the UBTF check, ubtf:739-804).
   std::vector<int32_t> rowcnt(n, 0);

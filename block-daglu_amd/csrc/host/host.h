@@ -77,6 +77,10 @@ struct Decomposition {
   std::vector<int32_t> level;
   int32_t max_level = 0;
   std::vector<int32_t> level_ptr, level_blocks;
+  // With DecompOptions::order_small_max: for each permuted position, the
+  // block-local index it had before its block's column order was applied (the
+  // row id KLU's pivot search breaks ties on); empty otherwise.
+  std::vector<int32_t> tie_key;
 };
 
 struct DecompOptions {
@@ -85,6 +89,13 @@ struct DecompOptions {
   bool merge = true;
   int32_t merge_max = 64;
   bool build_levels = true;
+  // Global-KLU settings (reference src/solver.cpp:878-968 through klu_analyze):
+  // btf=false keeps the whole matrix as one block (P = Q = identity), and
+  // blocks of 2..order_small_max get their COLAMD column order applied
+  // symmetrically to P and Q (KLU combines its block ordering with the BTF
+  // permutations that way); larger blocks are ordered by their own analysis.
+  bool btf = true;
+  int32_t order_small_max = 0;
 };
 DecompOptions decomp_options_from_env();
 
@@ -92,6 +103,23 @@ DecompOptions decomp_options_from_env();
 Decomposition build_decomposition(const Canonical& a, const DecompOptions& opt);
 
 std::vector<int32_t> merge_block_boundaries(const std::vector<int32_t>& R, int32_t n, int32_t merge_max);
+
+// ---- route selection (reference src/solver.cpp:182-556, :820-860) ------------------
+enum class RouteMode { Auto, Klu, Block };
+RouteMode route_mode_from_env();
+struct ProfileLite {
+  double avg_deg = 0.0, diag_cov = 0.0;
+  int32_t row_max = 0, col_max = 0, row_p95 = 0, col_p95 = 0;
+};
+ProfileLite profile_lite(const Canonical& a);
+// true: BLOCK route, false: global KLU (AUTO mode).
+bool auto_choose_block(const Canonical& a, uint64_t sig, uint64_t sig2, uint64_t value_sig, uint64_t config);
+void note_block_analysis(const Canonical& a, uint64_t sig, uint64_t sig2, uint64_t value_sig, uint64_t config,
+                         double build_ms, int32_t max_block);
+int klu_btf_for(int32_t n, std::size_t nnz);
+int klu_scale_for(int32_t n, std::size_t nnz, int btf);
+double klu_tol_for(int32_t n, std::size_t nnz, int btf);
+int klu_ordering();
 
 // ---- per-block ordering ---------------------------------------------------------
 // Column approximate-minimum-degree ordering of an s x s pattern in CSC

@@ -7,7 +7,9 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdlib>
+#include <limits>
 #include <stdexcept>
 #include <mutex>
 #include <thread>
@@ -28,11 +30,40 @@ int32_t dense_block_max_from_env() {
   return std::min<int32_t>(v, 64);
 }
 
+std::vector<double> klu_row_scale(const Canonical& a, int scale) {
+  std::vector<double> rs;
+  if (scale <= 0) return rs;
+  rs.assign(static_cast<std::size_t>(a.n), 0.0);
+  // Canonical order is row-major with ascending columns, so each row's terms
+  // are combined in the column order klu_scale visits them.
+  for (std::size_t k = 0; k < a.nnz(); ++k) {
+    const double v = std::abs(a.val[k]);
+    double& r = rs[a.row[k]];
+    r = scale == 1 ? r + v : std::max(r, v);
+  }
+  for (double& r : rs)
+    if (r == 0.0) r = 1.0;
+  return rs;
+}
+
 namespace {
+// Pivoting options of the sparse-block analysis: the BLOCK route's (env), or
+// the global-KLU route's tolerance with the row scaling applied to the whole
+// matrix beforehand (so no per-block scaling).
+SymbolicOptions sparse_options(const HostPlan& p) {
+  SymbolicOptions so = symbolic_options_from_env();
+  if (p.route == 1) {
+    so.tol = p.klu.tol;
+    so.scale_min_n = std::numeric_limits<int32_t>::max();
+  }
+  return so;
+}
+
 // Symbolic analysis of the sparse blocks, largest first, on worker threads.
 bool analyze_all_sparse(const Canonical& a, HostPlan& p, PlanStatus& status) {
   const Decomposition& d = p.dec;
-  const SymbolicOptions so = symbolic_options_from_env();
+  const SymbolicOptions so = sparse_options(p);
+  const bool scaled = p.route == 1 && !p.rs.empty();
   p.sparse.assign(p.sparse_blocks.size(), SparseBlockPlan{});
   std::vector<int32_t> order(p.sparse_blocks.size());
   for (std::size_t i = 0; i < order.size(); ++i) order[i] = static_cast<int32_t>(i);
@@ -59,7 +90,8 @@ bool analyze_all_sparse(const Canonical& a, HostPlan& p, PlanStatus& status) {
         for (int64_t q = d.dblk_ptr[b]; q < d.dblk_ptr[b + 1]; ++q) {
           rr.push_back(d.dblk_r[q]);
           cc.push_back(d.dblk_c[q]);
-          vv.push_back(a.val[d.dblk_src[q]]);
+          const int64_t e = d.dblk_src[q];
+          vv.push_back(scaled ? a.val[e] / p.rs[a.row[e]] : a.val[e]);
         }
         ok[i] = analyze_sparse_block(s, rr, cc, vv, so, p.sparse[i]) ? 1 : 0;
       }
@@ -91,6 +123,7 @@ PlanStatus reanalyze_sparse_blocks(const Canonical& a, HostPlan& p) {
   PlanStatus status;
   std::vector<std::vector<int32_t>> old_q(p.sparse.size());
   for (std::size_t i = 0; i < p.sparse.size(); ++i) old_q[i] = p.sparse[i].q;
+  if (p.route == 1) p.rs = klu_row_scale(a, p.klu.scale);  // the scale factors follow the values
   if (!analyze_all_sparse(a, p, status)) return status;
   for (std::size_t i = 0; i < p.sparse.size(); ++i)
     if (p.sparse[i].q != old_q[i]) throw std::runtime_error("column order changed on re-analysis");
@@ -98,15 +131,33 @@ PlanStatus reanalyze_sparse_blocks(const Canonical& a, HostPlan& p) {
   return status;
 }
 
-PlanStatus build_host_plan(const Canonical& a, HostPlan& p) {
+PlanStatus build_host_plan(const Canonical& a, HostPlan& p, const KluRoute* klu) {
   const auto t0 = std::chrono::steady_clock::now();
   PlanStatus status;
   p = HostPlan{};
   p.n = a.n;
   p.nnz = static_cast<int64_t>(a.nnz());
-  p.dec = build_decomposition(a, decomp_options_from_env());
+  if (klu) {
+    // Global KLU: the full pattern (no drop rule, coo_to_csc_i32 at
+    // src/solver.cpp:896), KLU's BTF or none, no merging, every block of
+    // 2..64 in its COLAMD order (those blocks are factored by the dense kernel
+    // under KLU's pivot rule; larger ones by the sparse-block analysis).
+    p.route = 1;
+    p.klu = *klu;
+    p.rs = klu_row_scale(a, klu->scale);
+    DecompOptions o;
+    o.drop_abs_lt = 0.0;
+    o.btf_maxwork = 0.0;
+    o.merge = false;
+    o.btf = klu->btf != 0;
+    o.order_small_max = 64;
+    p.dec = build_decomposition(a, o);
+    p.dense_max = 64;
+  } else {
+    p.dec = build_decomposition(a, decomp_options_from_env());
+    p.dense_max = dense_block_max_from_env();
+  }
   const Decomposition& d = p.dec;
-  p.dense_max = dense_block_max_from_env();
   const int32_t nb = d.nblocks;
   p.kind.assign(nb, 0);
   for (int32_t b = 0; b < nb; ++b) {
@@ -185,7 +236,7 @@ void finalize_desc(HostPlan& p) {
     s.k0 = sp.k0;
     s.m = sp.m;
     s.scale = sp.scale ? 1 : 0;
-    s.tol = symbolic_options_from_env().tol;
+    s.tol = sparse_options(p).tol;
     s.a_val0 = p.sparse_val0[i];
     s.a_ptr = sp.a_ptr.data();
     s.a_row = sp.a_row.data();
@@ -222,6 +273,10 @@ void finalize_desc(HostPlan& p) {
   q.dense_val0 = p.dense_val0.data();
   q.nsparse = static_cast<int32_t>(p.sdesc.size());
   q.sparse = p.sdesc.data();
+  q.dense_rule = p.route == 1 ? 1 : 0;
+  q.dense_tol = p.route == 1 ? p.klu.tol : 1.0;
+  q.dense_key = p.route == 1 ? d.tie_key.data() : nullptr;
+  q.row_scale = p.route == 1 ? std::max(0, p.klu.scale) : 0;
 }
 
 }  // namespace bdlu

@@ -11,7 +11,19 @@
 
 namespace bdlu {
 
+// Settings of the global-KLU route (reference src/solver.cpp:878-968,
+// :1190-1242: klu_analyze + klu_factor on the whole matrix, btf/scale/tol from
+// pick_klu_btf_for / pick_klu_scale_for / pick_klu_tol_for).
+struct KluRoute {
+  int btf = 0;         // 1: KLU's BTF blocks (no merge); 0: the matrix is one block
+  int scale = 0;       // KLU scale: <= 0 none, 1 row sums, 2 row maxima of |a| over whole rows
+  double tol = 0.001;  // threshold pivoting tolerance (diagonal preference)
+};
+
 struct HostPlan {
+  int32_t route = 0;   // 0: BLOCK route, 1: global-KLU route
+  KluRoute klu;        // route 1 only
+  std::vector<double> rs;  // route 1 with scale > 0: row scale factor of each original row
   int32_t n = 0;
   int64_t nnz = 0;
   Decomposition dec;
@@ -43,9 +55,14 @@ struct PlanStatus {
   int32_t bad_block = -1;  // sparse block whose analysis failed (singular)
 };
 
-// Builds the full host plan for a canonical matrix. Throws std::runtime_error
-// on decomposition failure; returns ok=false if a sparse block is singular.
-PlanStatus build_host_plan(const Canonical& a, HostPlan& out);
+// Builds the full host plan for a canonical matrix: the BLOCK route (klu ==
+// nullptr) or the global-KLU route with the given settings. Throws
+// std::runtime_error on decomposition failure; returns ok=false if a sparse
+// block is singular.
+PlanStatus build_host_plan(const Canonical& a, HostPlan& out, const KluRoute* klu = nullptr);
+// KLU's row scale factors (klu_scale semantics): scale 1 sums, scale 2 maxima
+// of |a_ij| over each row, in column order; rows without a nonzero get 1.
+std::vector<double> klu_row_scale(const Canonical& a, int scale);
 // Re-runs the per-block symbolic analysis (pivot choice) of every sparse block
 // with the current values, keeping the decomposition, the column orders and the
 // value routing (the column order depends on the pattern only).

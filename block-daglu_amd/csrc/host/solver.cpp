@@ -14,6 +14,7 @@
 // time; a Solver returns it to the cache on set_matrix or destruction.
 #include "blockdaglu/solver.h"
 
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -36,8 +37,10 @@ struct PlanKey {
   int32_t n = 0;
   int64_t nnz = 0;
   uint64_t sig = 0, sig2 = 0, drop = 0, config = 0;
+  int32_t route = 0;  // 0 BLOCK, 1 global KLU
   bool operator==(const PlanKey& o) const {
-    return n == o.n && nnz == o.nnz && sig == o.sig && sig2 == o.sig2 && drop == o.drop && config == o.config;
+    return n == o.n && nnz == o.nnz && sig == o.sig && sig2 == o.sig2 && drop == o.drop && config == o.config &&
+           route == o.route;
   }
 };
 
@@ -153,6 +156,11 @@ struct SolverImpl {
   PlanKey key;
   std::unique_ptr<Resident> plan;
   bool singular_at_analysis = false;
+  // Route of the last analyze (src/solver.cpp:820-860): the requested mode and
+  // whether the global-KLU route was selected.
+  bdlu::RouteMode mode = bdlu::RouteMode::Auto;
+  bool use_klu = false;
+  uint64_t sig = 0, sig2 = 0;
   std::unique_ptr<bdlu::ExportFactors> exported;  // factor export, built on first request
   ~SolverImpl() { give_back(std::move(plan)); }
 };
@@ -184,6 +192,7 @@ bool Solver::set_matrix(CooMatrix m, int32_t dimension) {
   s.prepared = true;
   s.validated = s.decomposed = s.factored = false;
   s.singular_at_analysis = false;
+  s.use_klu = false;
   s.exported.reset();
   s.canon = bdlu::Canonical{};
   n_ = dimension >= 0 ? dimension : -1;
@@ -215,17 +224,87 @@ bool Solver::validate() {
   return true;
 }
 
+const char* Solver::selected_mode() const { return impl_->use_klu ? "KLU" : "BLOCK"; }
+
+namespace {
+// Resident plan for key k: the Solver's own if it matches, a cached one, or a
+// new one built by `build` and uploaded (evicting cached plans while the HBM
+// budget cannot take it). Returns BDLU_OK, or the build/upload failure with a
+// message; *singular is set when the host analysis found a sparse block with
+// no admissible pivot (the plan is then not uploaded).
+template <class Build>
+int acquire_plan(bdlu_ctx* ctx, SolverImpl& s, const PlanKey& k, Build&& build, bool* reused, bool* singular,
+                 double* build_ms, std::string& msg) {
+  *singular = false;
+  *build_ms = 0.0;
+  if (s.plan && s.plan->key == k) {
+    *reused = true;
+    return BDLU_OK;
+  }
+  give_back(std::move(s.plan));
+  s.plan = lease(k);
+  *reused = static_cast<bool>(s.plan);
+  if (s.plan) return BDLU_OK;
+  auto r = std::make_unique<Resident>();
+  r->key = k;
+  r->host = std::make_unique<bdlu::HostPlan>();
+  bdlu::PlanStatus ps;
+  const auto t0 = std::chrono::steady_clock::now();
+  try {
+    ps = build(*r->host);
+  } catch (const std::exception& e) {
+    msg = e.what();
+    return BDLU_E_ANALYSIS;
+  }
+  *build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (!ps.ok) {
+    *singular = true;
+    return BDLU_OK;
+  }
+  int rc = BDLU_OK;
+  while ((rc = bdlu_gpu_plan_upload(ctx, &r->host->desc, &r->id)) == BDLU_E_BUDGET) {
+    if (!evict_one()) break;
+  }
+  if (rc != BDLU_OK) {
+    msg = bdlu_gpu_last_error(ctx);
+    return rc;
+  }
+  s.plan = std::move(r);
+  return BDLU_OK;
+}
+}  // namespace
+
 bool Solver::analyze() {
   clear_error_ok();
   SolverImpl& s = *impl_;
   if (!s.prepared) return fail(SolverError::NotPrepared, "set_matrix must be called before analyze");
   if (!validate()) return false;
+  s.sig = bdlu::matrix_sig(s.canon.n, s.canon);
+  s.sig2 = bdlu::matrix_sig2(s.canon.n, s.canon);
+  std::string msg;
+  bdlu_ctx* ctx = context(msg);
+  if (!ctx) return fail(SolverError::BackendUnavailable, msg);
+  report_ = EngineReport{};
+  s.singular_at_analysis = false;
+  s.decomposed = s.factored = false;
+  // Mode (src/solver.cpp:820-860): USERALGO_MODE forces a route; AUTO decides
+  // from the matrix (src/solver.cpp:432-515).
+  s.mode = bdlu::route_mode_from_env();
+  s.use_klu = s.mode == bdlu::RouteMode::Klu;
+  const uint64_t config = gpu_config_sig();
+  uint64_t value_sig = 0;
+  if (s.mode == bdlu::RouteMode::Auto) {
+    value_sig = bdlu::matrix_value_sig(s.canon);
+    s.use_klu = !bdlu::auto_choose_block(s.canon, s.sig, s.sig2, value_sig, config);
+  }
+  if (s.use_klu) return analyze_klu();
+
   PlanKey k;
   k.n = s.canon.n;
   k.nnz = static_cast<int64_t>(s.canon.nnz());
-  k.sig = bdlu::matrix_sig(k.n, s.canon);
-  k.sig2 = bdlu::matrix_sig2(k.n, s.canon);
-  k.config = gpu_config_sig();
+  k.sig = s.sig;
+  k.sig2 = s.sig2;
+  k.config = config;
   {
     // Values below the drop threshold change the pattern (ubtf:582-589).
     double drop = bdlu::decomp_options_from_env().drop_abs_lt;
@@ -236,51 +315,99 @@ bool Solver::analyze() {
         if (std::abs(s.canon.val[e]) < drop) h = (h ^ (e + 1)) * 1099511628211ull;
     k.drop = h;
   }
-  std::string msg;
-  bdlu_ctx* ctx = context(msg);
-  if (!ctx) return fail(SolverError::BackendUnavailable, msg);
-  report_ = EngineReport{};
-  if (s.plan && s.plan->key == k) {
-    report_.plan_reused = true;
-  } else {
-    give_back(std::move(s.plan));
-    s.plan = lease(k);
-    report_.plan_reused = static_cast<bool>(s.plan);
+  bool reused = false, singular = false;
+  double build_ms = 0.0;
+  const int rc = acquire_plan(
+      ctx, s, k, [&](bdlu::HostPlan& hp) { return bdlu::build_host_plan(s.canon, hp); }, &reused, &singular,
+      &build_ms, msg);
+  if (rc == BDLU_E_ANALYSIS) {
+    // AUTO: a failed block analysis falls back to global KLU (src/solver.cpp:1026-1070).
+    if (s.mode == bdlu::RouteMode::Auto) {
+      s.use_klu = true;
+      return analyze_klu();
+    }
+    return fail(SolverError::AnalysisFailed, msg);
   }
-  s.singular_at_analysis = false;
-  if (!s.plan) {
-    auto r = std::make_unique<Resident>();
-    r->key = k;
-    r->host = std::make_unique<bdlu::HostPlan>();
-    bdlu::PlanStatus ps;
-    try {
-      ps = bdlu::build_host_plan(s.canon, *r->host);
-    } catch (const std::exception& e) {
-      return fail(SolverError::AnalysisFailed, e.what());
-    }
-    if (!ps.ok) {
-      // No admissible pivot sequence for a sparse block: the reference's KLU
-      // factorization of that block fails, which surfaces in factor().
-      s.singular_at_analysis = true;
-      s.decomposed = true;
-      s.factored = false;
-      return true;
-    }
-    int rc = BDLU_OK;
-    while ((rc = bdlu_gpu_plan_upload(ctx, &r->host->desc, &r->id)) == BDLU_E_BUDGET) {
-      if (!evict_one()) break;
-    }
-    if (rc != BDLU_OK) return fail(SolverError::BackendUnavailable, bdlu_gpu_last_error(ctx));
-    s.plan = std::move(r);
-  }
+  if (rc != BDLU_OK) return fail(SolverError::BackendUnavailable, msg);
+  report_.plan_reused = reused;
   s.key = k;
+  if (singular) {
+    // No admissible pivot sequence for a sparse block: the reference's KLU
+    // factorization of that block fails, which surfaces in factor().
+    s.singular_at_analysis = true;
+    s.decomposed = true;
+    return true;
+  }
   const bdlu::HostPlan& hp = *s.plan->host;
+  if (s.mode == bdlu::RouteMode::Auto) {
+    // An incomplete structural match makes AUTO take global KLU
+    // (src/solver.cpp:1078-1131); an expensive analysis that left one block
+    // above 80% of n makes it prefer global KLU next time (:1133-1145).
+    if (hp.dec.nmatch < k.n) {
+      s.use_klu = true;
+      return analyze_klu();
+    }
+    if (!reused) bdlu::note_block_analysis(s.canon, s.sig, s.sig2, value_sig, config, build_ms, hp.max_block);
+  }
   report_.nblocks = hp.dec.nblocks;
   report_.sparse_blocks = static_cast<int32_t>(hp.sparse_blocks.size());
   report_.dense_blocks = report_.nblocks - report_.sparse_blocks;
   report_.max_block = hp.max_block;
   s.decomposed = true;
+  return true;
+}
+
+// The global-KLU route's analysis (src/solver.cpp:878-968): KLU settings from
+// pick_klu_btf_for / pick_klu_scale_for / pick_klu_tol_for, then KLU's BTF (or
+// none), per-block COLAMD, and the pivot rows of a first factorization, as a
+// device plan like the BLOCK route's.
+bool Solver::analyze_klu() {
+  SolverImpl& s = *impl_;
+  s.use_klu = true;
+  std::string msg;
+  bdlu_ctx* ctx = context(msg);
+  if (!ctx) return fail(SolverError::BackendUnavailable, msg);
+  if (bdlu::klu_ordering() != 1)
+    return fail(SolverError::BackendUnavailable,
+                "KLU ordering other than COLAMD (USERALGO_KLU_ORDERING=1) is not available in this build");
+  bdlu::KluRoute klu;
+  const std::size_t nnz = s.canon.nnz();
+  klu.btf = bdlu::klu_btf_for(s.canon.n, nnz);
+  klu.scale = bdlu::klu_scale_for(s.canon.n, nnz, klu.btf);
+  klu.tol = bdlu::klu_tol_for(s.canon.n, nnz, klu.btf);
+  PlanKey k;
+  k.n = s.canon.n;
+  k.nnz = static_cast<int64_t>(nnz);
+  k.sig = s.sig;
+  k.sig2 = s.sig2;
+  k.route = 1;
+  {
+    uint64_t h = gpu_config_sig();
+    uint64_t tb = 0;
+    std::memcpy(&tb, &klu.tol, sizeof(tb));
+    for (uint64_t x : {static_cast<uint64_t>(klu.btf), static_cast<uint64_t>(klu.scale + 8), tb})
+      h = (h ^ x) * 1099511628211ull;
+    k.config = h;
+  }
+  bool reused = false, singular = false;
+  double build_ms = 0.0;
+  const int rc = acquire_plan(
+      ctx, s, k, [&](bdlu::HostPlan& hp) { return bdlu::build_host_plan(s.canon, hp, &klu); }, &reused, &singular,
+      &build_ms, msg);
+  if (rc == BDLU_E_ANALYSIS) return fail(SolverError::AnalysisFailed, "global KLU symbolic analysis failed: " + msg);
+  if (rc != BDLU_OK) return fail(SolverError::BackendUnavailable, msg);
+  report_.plan_reused = reused;
+  s.key = k;
+  s.singular_at_analysis = singular;
+  s.decomposed = true;
   s.factored = false;
+  if (!singular) {
+    const bdlu::HostPlan& hp = *s.plan->host;
+    report_.nblocks = hp.dec.nblocks;
+    report_.sparse_blocks = static_cast<int32_t>(hp.sparse_blocks.size());
+    report_.dense_blocks = report_.nblocks - report_.sparse_blocks;
+    report_.max_block = hp.max_block;
+  }
   return true;
 }
 
@@ -295,63 +422,98 @@ bool Solver::factor_device(const double* d_values) {
   return factor_values(d_values, true);
 }
 
+// One refactorization on the resident plan, with the re-analysis the fixed
+// pivot order may need (BDLU_E_REPIVOT: re-analyse this pattern with the
+// current values and retry once). Returns the engine status.
+int Solver::refactor_plan(const double* values, bool on_device, int prec, bdlu_factor_stats& fs) {
+  SolverImpl& s = *impl_;
+  std::string msg;
+  bdlu_ctx* ctx = context(msg);
+  if (!ctx) return BDLU_E_BACKEND;
+  const int where = on_device ? BDLU_MEM_DEVICE : BDLU_MEM_HOST;
+  int rc = bdlu_gpu_refactor(ctx, s.plan->id, values, where, prec, &fs);
+  if (rc != BDLU_E_REPIVOT) return rc;
+  if (on_device && bdlu_gpu_plan_values(ctx, s.plan->id, s.canon.val.data()) != BDLU_OK) return BDLU_E_BACKEND;
+  std::unique_ptr<bdlu::HostPlan> host = std::move(s.plan->host);
+  bdlu::PlanStatus ps;
+  try {
+    ps = bdlu::reanalyze_sparse_blocks(s.canon, *host);
+  } catch (const std::exception&) {
+    ps.ok = false;
+  }
+  // The device copy of the old analysis is stale either way.
+  bdlu_gpu_plan_release(ctx, s.plan->id);
+  s.plan.reset();
+  if (!ps.ok) {
+    fs.failed_blocks = std::max(1, fs.failed_blocks);
+    return BDLU_E_FACTOR;
+  }
+  auto r = std::make_unique<Resident>();
+  r->key = s.key;
+  r->host = std::move(host);
+  s.plan = std::move(r);
+  int up = BDLU_OK;
+  while ((up = bdlu_gpu_plan_upload(ctx, &s.plan->host->desc, &s.plan->id)) == BDLU_E_BUDGET) {
+    if (!evict_one()) break;
+  }
+  if (up != BDLU_OK) {
+    s.plan.reset();
+    return up;
+  }
+  const int32_t repivots = fs.repivot_blocks;
+  rc = bdlu_gpu_refactor(ctx, s.plan->id, s.canon.val.data(), BDLU_MEM_HOST, prec, &fs);
+  fs.repivot_blocks = repivots;
+  return rc;
+}
+
 bool Solver::factor_values(const double* values, bool on_device) {
   SolverImpl& s = *impl_;
   if (!s.decomposed) return fail(SolverError::NotPrepared, "analyze must succeed before factor");
-  if (s.singular_at_analysis || !s.plan)
-    return fail(SolverError::FactorizationFailed, "block factorization failed for 1 diagonal block(s)");
   std::string msg;
   bdlu_ctx* ctx = context(msg);
   if (!ctx) return fail(SolverError::BackendUnavailable, msg);
   const int prec = precision_ == Precision::Fp32Ir ? BDLU_FP32 : BDLU_FP64;
-  const int where = on_device ? BDLU_MEM_DEVICE : BDLU_MEM_HOST;
   bdlu_factor_stats fs{};
   s.factored = false;
-  int rc = bdlu_gpu_refactor(ctx, s.plan->id, values, where, prec, &fs);
-  if (rc == BDLU_E_REPIVOT) {
-    // The fixed pivot order no longer passes the threshold check for these
-    // values: re-analyse this pattern with the current values and retry once.
-    if (on_device && bdlu_gpu_plan_values(ctx, s.plan->id, s.canon.val.data()) != BDLU_OK)
-      return fail(SolverError::BackendUnavailable, bdlu_gpu_last_error(ctx));
-    std::unique_ptr<bdlu::HostPlan> host = std::move(s.plan->host);
-    bdlu::PlanStatus ps;
-    try {
-      ps = bdlu::reanalyze_sparse_blocks(s.canon, *host);
-    } catch (const std::exception&) {
-      ps.ok = false;
-    }
-    // The device copy of the old analysis is stale either way.
-    bdlu_gpu_plan_release(ctx, s.plan->id);
-    s.plan.reset();
-    if (!ps.ok) {
-      s.decomposed = false;
-      return fail(SolverError::FactorizationFailed, "block factorization failed for 1 diagonal block(s)");
-    }
-    auto r = std::make_unique<Resident>();
-    r->key = s.key;
-    r->host = std::move(host);
-    s.plan = std::move(r);
-    int up = BDLU_OK;
-    while ((up = bdlu_gpu_plan_upload(ctx, &s.plan->host->desc, &s.plan->id)) == BDLU_E_BUDGET) {
-      if (!evict_one()) break;
-    }
-    if (up != BDLU_OK) {
-      s.plan.reset();
-      s.decomposed = false;
-      return fail(SolverError::BackendUnavailable, bdlu_gpu_last_error(ctx));
-    }
-    const int32_t repivots = fs.repivot_blocks;
-    rc = bdlu_gpu_refactor(ctx, s.plan->id, s.canon.val.data(), BDLU_MEM_HOST, prec, &fs);
-    fs.repivot_blocks = repivots;
-  }
+  int rc = BDLU_E_FACTOR;
+  if (s.plan && !s.singular_at_analysis) rc = refactor_plan(values, on_device, prec, fs);
+  else fs.failed_blocks = 1;
   report_.fp64_fallback_blocks = fs.fp64_fallback_blocks;
   report_.repivot_blocks = fs.repivot_blocks;
-  if (rc == BDLU_E_FACTOR || rc == BDLU_E_REPIVOT) {
+  const bool failed = rc == BDLU_E_FACTOR || rc == BDLU_E_REPIVOT;
+  if (failed && !s.use_klu && s.mode == bdlu::RouteMode::Auto) {
+    // AUTO stays robust: the same canonical matrix is refactored on the
+    // global-KLU route (src/solver.cpp:1264-1316).
+    if (on_device && s.plan && !s.singular_at_analysis &&
+        bdlu_gpu_plan_values(ctx, s.plan->id, s.canon.val.data()) != BDLU_OK)
+      return fail(SolverError::BackendUnavailable, bdlu_gpu_last_error(ctx));
+    if (!analyze_klu()) {
+      if (last_error_ == SolverError::AnalysisFailed)
+        last_error_message_ = "global KLU analysis failed during AUTO fallback";
+      return false;
+    }
+    bdlu_factor_stats kfs{};
+    int krc = BDLU_E_FACTOR;
+    if (s.plan && !s.singular_at_analysis) krc = refactor_plan(s.canon.val.data(), false, prec, kfs);
+    report_.fp64_fallback_blocks = kfs.fp64_fallback_blocks;
+    report_.repivot_blocks = kfs.repivot_blocks;
+    if (krc == BDLU_E_FACTOR || krc == BDLU_E_REPIVOT)
+      return fail(SolverError::FactorizationFailed, "global KLU factorization failed during AUTO fallback");
+    if (krc != BDLU_OK) return fail(SolverError::BackendUnavailable, bdlu_gpu_last_error(ctx));
+    s.factored = true;
+    return true;
+  }
+  if (failed) {
+    if (!s.plan && !s.singular_at_analysis) s.decomposed = false;
+    if (s.use_klu) return fail(SolverError::FactorizationFailed, "global KLU numeric factorization failed");
     std::ostringstream m;
     m << "block factorization failed for " << std::max(1, fs.failed_blocks) << " diagonal block(s)";
     return fail(SolverError::FactorizationFailed, m.str());
   }
-  if (rc != BDLU_OK) return fail(SolverError::BackendUnavailable, bdlu_gpu_last_error(ctx));
+  if (rc != BDLU_OK) {
+    if (!s.plan) s.decomposed = false;
+    return fail(SolverError::BackendUnavailable, bdlu_gpu_last_error(ctx));
+  }
   s.factored = true;
   return true;
 }

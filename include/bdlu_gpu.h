@@ -89,6 +89,18 @@ typedef struct bdlu_plan_desc {
   const int64_t* dense_val0;   // arena offset of its s*s column-major values
   int32_t nsparse;
   const bdlu_sparse_desc* sparse;
+  // Pivot rule of the dense blocks: 0 partial pivoting, first largest |a| (the
+  // BLOCK route's dense branch, ubtf:1465-1477); 1 KLU's threshold rule (the
+  // global-KLU route, src/solver.cpp:1198): the diagonal row is kept when
+  // |a_kk| >= dense_tol * max_i |a_ik|, else the largest, ties to the smaller
+  // dense_key (n values, one per permuted row; NULL with rule 0).
+  int32_t dense_rule;
+  double dense_tol;
+  const int32_t* dense_key;
+  // 0: none; 1 / 2: every row of A is divided by the sum / maximum of its |a_ij|
+  // (KLU scale 1 / 2 over whole rows) before the blocks are factored, and the
+  // right-hand side by the same factors before the block solve (klu_solve).
+  int32_t row_scale;
 } bdlu_plan_desc;
 
 typedef struct bdlu_factor_stats {
@@ -189,6 +201,11 @@ enum {
 int bdlu_gpu_profile(bdlu_ctx* ctx, int enable);
 int bdlu_gpu_profile_read(bdlu_ctx* ctx, int kind, double* total_ms, int64_t* launches, double* flops,
                           double* bytes);
+
+// Row pivots of the dense blocks from the last refactor (test hook, no
+// counterpart in the reference): host_piv[i] (n values) = the block-local row
+// that the pivoting brought to permuted row i (i inside a dense block).
+int bdlu_gpu_plan_dense_pivots(bdlu_ctx* ctx, uint64_t plan_id, int32_t* host_piv);
 
 // Test hook: no-pivot dense LU of an m x m column-major matrix on the device
 // through the same kernels the tail factorization uses; A is overwritten by L\U.

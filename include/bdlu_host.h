@@ -115,6 +115,18 @@ int bdlu_serve_wait(void* s, int64_t request, double* X, bdlu_serve_result* out)
 // ---- host analysis queries (CPU only) ---------------------------------------
 void* bdlu_host_analyze(int64_t nnz, const int32_t* rows, const int32_t* cols, const double* vals,
                         int32_t dimension, int* error_code);
+// The same for the global-KLU route with KLU's settings (btf 0/1, scale
+// <= 0 / 1 / 2, pivot tolerance): KLU's BTF or one block, no merging, blocks of
+// 2..64 in their COLAMD order (baked into the permutations), whole-row
+// scaling (src/solver.cpp:878-968).
+void* bdlu_host_analyze_klu(int64_t nnz, const int32_t* rows, const int32_t* cols, const double* vals,
+                            int32_t dimension, int btf, int scale, double tol, int* error_code);
+// Route Solver::analyze would take for this COO matrix under the current
+// environment (src/solver.cpp:820-860, :432-565): out4 = {1 global KLU / 0
+// BLOCK, KLU btf, KLU scale, KLU ordering}, *tol = KLU pivot tolerance.
+// Returns 0, or an error code if the matrix fails validation.
+int bdlu_route_for(int64_t nnz, const int32_t* rows, const int32_t* cols, const double* vals, int32_t dimension,
+                   int32_t* out4, double* tol);
 void bdlu_host_free(void* h);
 // n, raw/merged block counts, nmatch, nnz (canonical), off-diagonal nnz, levels.
 void bdlu_host_info(void* h, int32_t* n, int32_t* nblocks_raw, int32_t* nblocks, int32_t* nmatch, int64_t* nnz,

@@ -20,6 +20,8 @@
 
 #include "blockdaglu/types.h"
 
+struct bdlu_factor_stats;  // include/bdlu_gpu.h
+
 namespace blockdaglu {
 
 // Same enumerators, same order as the reference (solver.h:19-31).
@@ -89,7 +91,8 @@ class Solver final {
   CooMatrix getU();
 
   int32_t n() const { return n_; }
-  const char* selected_mode() const { return "BLOCK"; }
+  // "KLU" when the last analyze() selected the global-KLU route, else "BLOCK".
+  const char* selected_mode() const;
 
   // ---- additive ----------------------------------------------------------
   // k right-hand sides, column-major with leading dimension n (B.size() == n*k).
@@ -122,6 +125,8 @@ class Solver final {
   bool clear_error_ok();
   bool validate();
   bool ensure_export();
+  bool analyze_klu();
+  int refactor_plan(const double* values, bool on_device, int precision, bdlu_factor_stats& stats);
   bool factor_values(const double* values, bool on_device);
   bool fail(SolverError error, std::string message);
 

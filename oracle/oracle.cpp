@@ -351,7 +351,8 @@ bool dense_solve(const BlockFactor& bf, double* rhs) {
 // Updates of column j are applied in ascending pivot position; candidates are
 // scanned in ascending row order and the first largest |x| is the partial-pivot
 // choice; the diagonal is taken when |x_diag| >= tol * max.
-bool klu_like_factor(BlockFactor& bf, const std::vector<std::tuple<int, int, double>>& coo, double tol) {
+bool klu_like_factor(BlockFactor& bf, const std::vector<std::tuple<int, int, double>>& coo, double tol,
+                     int32_t scale_min = 4096) {
   const int32_t s = bf.size;
   std::vector<int32_t> Ap(s + 1, 0), Ai(coo.size());
   std::vector<double> Ax(coo.size());
@@ -367,7 +368,7 @@ bool klu_like_factor(BlockFactor& bf, const std::vector<std::tuple<int, int, dou
   }
   bf.q = oracle::colamd(s, Ap.data(), Ai.data());
   bf.rs.assign(s, 1.0);
-  if (s >= 4096) {
+  if (s >= scale_min) {
     std::fill(bf.rs.begin(), bf.rs.end(), 0.0);
     for (std::size_t p = 0; p < Ai.size(); ++p) bf.rs[Ai[p]] += std::abs(Ax[p]);
     for (auto& v : bf.rs)
@@ -501,6 +502,11 @@ struct Solver {
   std::vector<BlockFactor> fac;
   double t_analyze = 0, t_factor = 0, t_solve = 0;
   int threads_used = 1;
+  // Global-KLU route: every block by klu_like_factor with klu_tol, whole rows
+  // scaled by rs (empty: unscaled).
+  int route = 0;
+  double klu_tol = 0.001;
+  std::vector<double> rs;
 };
 
 int analyze(Solver& s) {
@@ -571,6 +577,95 @@ int analyze(Solver& s) {
   return kNone;
 }
 
+// ---- global-KLU route: src/solver.cpp:878-968 (analyze), :1190-1242 (factor),
+// :1472-1479 (solve), through klu_analyze / klu_factor / klu_solve ------------------
+// klu_analyze with ordering = 1: the CSC of the full canonical pattern
+// (coo_to_csc_i32, no drop rule), BTF when btf = 1 (btf_order, Q unflipped, no
+// merging) or the whole matrix as one block, then COLAMD on every block (in
+// klu_like_factor). klu_factor with scale = 1 / 2 divides every row by the sum /
+// maximum of |a_ij| over the whole row (klu_scale; an all-zero row gets 1),
+// including the entries outside the diagonal blocks, and factors each block with
+// the threshold rule at tol; a block of size 1 pivots on its entry.
+int analyze_klu(Solver& s, int btf, int scale, double tol) {
+  using Clock = std::chrono::steady_clock;
+  const auto t0 = Clock::now();
+  if (!s.validated) {
+    const int e = validate_and_canonicalize(s.coo, s.hint, s.n);
+    if (e) return e;
+    s.validated = true;
+  }
+  const int32_t n = s.n;
+  s.route = 1;
+  s.klu_tol = tol;
+  s.rs.clear();
+  if (scale > 0) {
+    s.rs.assign(n, 0.0);
+    for (const auto& t : s.coo) {
+      const double a = std::abs(std::get<2>(t));
+      double& r = s.rs[std::get<0>(t)];
+      r = scale == 1 ? r + a : std::max(r, a);
+    }
+    for (double& r : s.rs)
+      if (r == 0.0) r = 1.0;
+  }
+  auto val = [&](const std::tuple<int, int, double>& t) {
+    return s.rs.empty() ? std::get<2>(t) : std::get<2>(t) / s.rs[std::get<0>(t)];
+  };
+  if (btf) {
+    std::vector<int32_t> Ap(n + 1, 0), Ai(s.coo.size());
+    for (const auto& t : s.coo) Ap[std::get<1>(t) + 1]++;
+    for (int32_t j = 0; j < n; ++j) Ap[j + 1] += Ap[j];
+    std::vector<int32_t> next(Ap.begin(), Ap.end() - 1);
+    for (const auto& t : s.coo) Ai[next[std::get<1>(t)]++] = std::get<0>(t);
+    s.nblocks_raw = btf_order(n, Ap, Ai, s.P, s.Q, s.R, s.nmatch);
+    for (auto& q : s.Q) q = q < -1 ? -q - 2 : q;
+  } else {
+    s.P.resize(n);
+    s.Q.resize(n);
+    for (int32_t k = 0; k < n; ++k) s.P[k] = s.Q[k] = k;
+    s.R = {0, n};
+    s.nblocks_raw = 1;
+    s.nmatch = n;
+  }
+  s.nblocks = static_cast<int32_t>(s.R.size()) - 1;
+  s.invP.assign(n, -1);
+  s.invQ.assign(n, -1);
+  for (int32_t k = 0; k < n; ++k) {
+    s.invP[s.P[k]] = k;
+    s.invQ[s.Q[k]] = k;
+  }
+  std::vector<int32_t> blk(n);
+  for (int32_t b = 0; b < s.nblocks; ++b)
+    for (int32_t k = s.R[b]; k < s.R[b + 1]; ++k) blk[k] = b;
+  std::vector<int32_t> cnt(n + 1, 0);
+  s.blocks.assign(s.nblocks, {});
+  for (const auto& t : s.coo) {
+    const int32_t rp = s.invP[std::get<0>(t)], cp = s.invQ[std::get<1>(t)];
+    if (blk[cp] < blk[rp]) return kAnalysis;
+    if (blk[cp] > blk[rp]) cnt[rp + 1]++;
+  }
+  for (int32_t i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+  s.off_ptr = cnt;
+  s.off_col.resize(cnt[n]);
+  s.off_val.resize(cnt[n]);
+  std::vector<int32_t> next(cnt.begin(), cnt.end() - 1);
+  for (const auto& t : s.coo) {
+    const int32_t rp = s.invP[std::get<0>(t)], cp = s.invQ[std::get<1>(t)];
+    const int32_t br = blk[rp], bc = blk[cp];
+    if (bc == br) {
+      s.blocks[br].emplace_back(rp - s.R[br], cp - s.R[br], val(t));
+    } else {
+      const int32_t p = next[rp]++;
+      s.off_col[p] = cp;
+      s.off_val[p] = val(t);
+    }
+  }
+  s.decomposed = true;
+  s.factored = false;
+  s.t_analyze = std::chrono::duration<double>(Clock::now() - t0).count();
+  return kNone;
+}
+
 // src/solver.cpp:366-390 thread policy (work threads for factor).
 int work_threads(const Solver& s, int hw) {
   const int cap = std::min(hw, 16);
@@ -611,7 +706,10 @@ int factor(Solver& s, int threads) {
         BlockFactor& bf = s.fac[b];
         bf.start = s.R[b];
         bf.size = s.R[b + 1] - s.R[b];
-        bf.ok = bf.size <= 64 ? dense_factor(bf, s.blocks[b]) : klu_like_factor(bf, s.blocks[b], 0.001);
+        if (s.route == 1)
+          bf.ok = klu_like_factor(bf, s.blocks[b], s.klu_tol, std::numeric_limits<int32_t>::max());
+        else
+          bf.ok = bf.size <= 64 ? dense_factor(bf, s.blocks[b]) : klu_like_factor(bf, s.blocks[b], 0.001);
       }
     }
   };
@@ -630,7 +728,7 @@ int factor(Solver& s, int threads) {
 bool solve_ubtf(const Solver& s, std::vector<double>& b) {
   const int32_t n = s.n;
   std::vector<double> bp(n), xp(n, 0.0), rhs;
-  for (int32_t i = 0; i < n; ++i) bp[i] = b[s.P[i]];
+  for (int32_t i = 0; i < n; ++i) bp[i] = s.rs.empty() ? b[s.P[i]] : b[s.P[i]] / s.rs[s.P[i]];
   for (int32_t bid = s.nblocks - 1; bid >= 0; --bid) {
     const BlockFactor& bf = s.fac[bid];
     rhs.resize(bf.size);
@@ -685,6 +783,9 @@ void* oracle_create(int64_t nnz, const int32_t* rows, const int32_t* cols, const
 }
 void oracle_free(void* h) { delete static_cast<oracle::Solver*>(h); }
 int oracle_analyze(void* h) { return oracle::analyze(*static_cast<oracle::Solver*>(h)); }
+int oracle_analyze_klu(void* h, int btf, int scale, double tol) {
+  return oracle::analyze_klu(*static_cast<oracle::Solver*>(h), btf, scale, tol);
+}
 int oracle_factor(void* h, int threads) { return oracle::factor(*static_cast<oracle::Solver*>(h), threads); }
 int oracle_solve(void* h, double* b, int refine_iters) {
   return oracle::solve(*static_cast<oracle::Solver*>(h), b, refine_iters);

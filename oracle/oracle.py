@@ -35,6 +35,7 @@ def lib():
             "oracle_create": (_P, [C.c_int64, _I32P, _I32P, _F64P, C.c_int32]),
             "oracle_free": (None, [_P]),
             "oracle_analyze": (C.c_int, [_P]),
+            "oracle_analyze_klu": (C.c_int, [_P, C.c_int, C.c_int, C.c_double]),
             "oracle_factor": (C.c_int, [_P, C.c_int]),
             "oracle_solve": (C.c_int, [_P, _F64P, C.c_int]),
             "oracle_set_values": (None, [_P, _F64P]),
@@ -81,6 +82,25 @@ class OracleSolver:
 
     def analyze(self) -> int:
         return self.L.oracle_analyze(self.h)
+
+    def analyze_klu(self, btf: int, scale: int, tol: float = 0.001) -> int:
+        """The global-KLU route's analysis with KLU's settings (every block by the
+        KLU rule, whole-row scaling for scale 1 / 2)."""
+        return self.L.oracle_analyze_klu(self.h, btf, scale, tol)
+
+    def klu_permutations(self, small_max: int = 64):
+        """After analyze_klu + factor: the row/column permutations with each
+        block's COLAMD order applied symmetrically to the blocks of
+        2..small_max (KLU's P = Pbtf[Pblk], Q = Qbtf[Pblk]), and R."""
+        P, Q, R = self.permutations()
+        P2, Q2 = P.copy(), Q.copy()
+        for b in range(len(R) - 1):
+            lo, s = int(R[b]), int(R[b + 1] - R[b])
+            if 2 <= s <= small_max:
+                q, _ = self.sparse_order(b, s)
+                P2[lo:lo + s] = P[lo + q]
+                Q2[lo:lo + s] = Q[lo + q]
+        return P2, Q2, R
 
     def factor(self, threads: int = 0) -> int:
         return self.L.oracle_factor(self.h, threads)
@@ -170,3 +190,79 @@ def colamd(n, indptr, indices):
     out = np.zeros(n, np.int32)
     lib().oracle_colamd(n, _p(ip, _I32P), _p(ix, _I32P), _p(out, _I32P))
     return out
+
+
+# ---- route selection, restated from the reference's src/solver.cpp -------------------
+# Decision rules of Solver::analyze in AUTO mode and the KLU settings of the
+# global route. Pure functions of (n, nnz, sampled degree profile, env values);
+# restated here so that the product's choice can be checked against them.
+
+def profile_lite(n, rows, cols):
+    """compute_profile_lite (src/solver.cpp:182-230) over canonical-order entries."""
+    nnz = len(rows)
+    step = max(1, nnz // 200000) if nnz > 200000 else 1
+    r = np.asarray(rows)[::step]
+    c = np.asarray(cols)[::step]
+    rd = np.bincount(r, minlength=n)
+    cd = np.bincount(c, minlength=n)
+    diag = np.zeros(n, bool)
+    diag[r[r == c]] = True
+    k = int(0.95 * (n - 1))
+    return {"row_max": int(rd.max()) * step, "col_max": int(cd.max()) * step,
+            "row_p95": int(np.partition(rd, k)[k]) * step, "col_p95": int(np.partition(cd, k)[k]) * step,
+            "diag_cov": float(diag.sum()) / n}
+
+
+def auto_choose_block(n, nnz, profile=None, auto_n=50000, auto_nnz=2000000):
+    """auto_choose_block (src/solver.cpp:450-515), without the learned override;
+    profile: a callable returning profile_lite's dict (evaluated only when the
+    rules need it)."""
+    if n >= 65000 and nnz >= 3000000:
+        coarse = False
+    elif n <= 25000 and nnz >= 600000:
+        coarse = True
+    elif n >= auto_n or nnz >= auto_nnz:
+        coarse = True
+    else:
+        coarse = False
+    if not coarse:
+        return False
+    if n < 40000 and nnz < 1500000:
+        return True
+    p = profile()
+    dense_thr = max(4096, n // 8)
+    hub = p["row_max"] >= dense_thr or p["col_max"] >= dense_thr
+    broad_thr = max(512, n // 64)
+    broad = p["row_p95"] >= broad_thr and p["col_p95"] >= broad_thr and p["diag_cov"] >= 0.9
+    return not (hub or broad)
+
+
+def klu_btf_for(n, nnz):
+    """pick_klu_btf_for (src/solver.cpp:517-532), no env override."""
+    if n <= 10000:
+        return 1
+    if n <= 25000 and nnz >= 600000:
+        return 1
+    if 35000 <= n <= 50000 and 500000 <= nnz <= 800000:
+        return 1
+    if 30000 <= n <= 38000 and nnz >= 1500000:
+        return 1
+    return 0
+
+
+def klu_scale_for(n, nnz, btf):
+    """pick_klu_scale_for (src/solver.cpp:534-554), no env override."""
+    if btf == 1 and 35000 <= n <= 50000 and 500000 <= nnz <= 800000:
+        return 2
+    if btf == 1 and 30000 <= n <= 38000 and nnz >= 1500000:
+        return 2
+    if n >= 65000 and nnz >= 3000000:
+        return 2
+    return 0
+
+
+def klu_tol_for(n, nnz, btf):
+    """pick_klu_tol_for (src/solver.cpp:556-565), no env override; 0.001 is KLU's default."""
+    if btf == 1 and 30000 <= n <= 38000 and nnz >= 1500000:
+        return 1e-6
+    return 0.001

@@ -26,6 +26,16 @@ def load_package():
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs on the GPU box)")
+    config.addinivalue_line("markers", "any_route: the test chooses the Solver route itself (no BLOCK default)")
+
+
+@pytest.fixture(autouse=True)
+def _block_route(request, monkeypatch):
+    """Solver defaults to AUTO routing like the reference (small systems then
+    take the global-KLU route); the tests of the BLOCK route force it with
+    USERALGO_MODE=block, and tests marked any_route choose for themselves."""
+    if "any_route" not in request.keywords:
+        monkeypatch.setenv("USERALGO_MODE", "block")
 
 
 @pytest.fixture(scope="session")

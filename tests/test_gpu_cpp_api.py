@@ -2,7 +2,8 @@
 include/blockdaglu/solver.h alone and linked to libblockdaglu.so, runs the
 reference's known answers (apps/tests/main.cpp:93-204: error codes, the 3x3
 duplicate system, the n = 1100 cache matrices with x = 1, the export identity,
-two solvers on two threads) as a fresh child process."""
+two solvers on two threads) as a fresh child process, under the reference's
+default AUTO routing (these small systems take the global-KLU route)."""
 from __future__ import annotations
 
 import subprocess
@@ -11,7 +12,7 @@ from pathlib import Path
 
 import pytest
 
-pytestmark = pytest.mark.gpu
+pytestmark = [pytest.mark.gpu, pytest.mark.any_route]
 
 ROOT = Path(__file__).resolve().parents[1]
 
