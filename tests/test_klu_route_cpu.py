@@ -180,7 +180,8 @@ def test_klu_analysis_matches_oracle_btf(bd, orc):
 @pytest.mark.parametrize("scale", [1, 2])
 def test_klu_analysis_matches_oracle_scaled(bd, orc, scale):
     n, r, c, v, _ = bd.generate(0)
-    v = v * np.exp(np.random.default_rng(7).uniform(-4, 4, len(v)))
+    rng = np.random.default_rng(7)
+    v = v * (10.0 ** rng.uniform(-4, 4, n))[r] * (10.0 ** rng.uniform(-4, 4, n))[c]
     _compare_klu_analysis(bd, orc, n, r, c, v, btf=1, scale=scale)
 
 
