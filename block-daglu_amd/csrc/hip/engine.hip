@@ -266,7 +266,10 @@ struct bdlu_ctx {
   static constexpr int kSide = 4;
   hipStream_t side[kSide] = {};
   bool fork = true;  // BLOCKDAGLU_GPU_STREAMS=1 keeps every launch on one stream
-  bool lookahead = true;  // BLOCKDAGLU_GPU_LOOKAHEAD=0: next diagonal LU in its own launch
+  // Dense-tail launch structure (BLOCKDAGLU_GPU_LOOKAHEAD): 2 (default) paired
+  // panels with K = 128 bulk updates and the next diagonal LU fused into them,
+  // 1 one bulk update per panel, 0 the next diagonal LU in its own launch.
+  int lookahead = 2;
   std::vector<hipEvent_t> events;  // fork/join points of a refactor
   hipEvent_t event(std::size_t i) {
     while (events.size() <= i) {
@@ -925,7 +928,7 @@ void factor_sparse_cols(Plan& P, SparseDev& S, hipStream_t st, T* Lx, T* Ux, T* 
 // sparse blocks may run concurrently).
 template <typename T>
 void factor_sparse_tail(Plan& P, SparseDev& S, hipStream_t st, T* Sm, ProfEvents* pe, double growth_limit,
-                        bool lookahead = false) {
+                        int lookahead = 0) {
   bdk::TailTimer tt;
   if (pe) {
     tt.gemm_events = &pe->ev[BDLU_PROF_TAIL_GEMM];
@@ -1227,7 +1230,7 @@ int bdlu_gpu_create(int device, uint64_t hbm_budget_bytes, bdlu_ctx** out) {
   if (const char* e = std::getenv("BLOCKDAGLU_GPU_GRAPH")) c->graphs = std::atoi(e) != 0;
   if (const char* e = std::getenv("BLOCKDAGLU_GPU_AUTOTUNE")) c->autotune = std::atoi(e) != 0;
   if (const char* e = std::getenv("BLOCKDAGLU_GPU_STREAMS")) c->fork = std::atoi(e) > 1;
-  if (const char* e = std::getenv("BLOCKDAGLU_GPU_LOOKAHEAD")) c->lookahead = std::atoi(e) != 0;
+  if (const char* e = std::getenv("BLOCKDAGLU_GPU_LOOKAHEAD")) c->lookahead = std::max(0, std::min(2, std::atoi(e)));
   bool ok = hipSetDevice(device) == hipSuccess &&
             hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
   if (ok) c->scratch = scratch_address(device);
@@ -1533,8 +1536,9 @@ int bdlu_gpu_profile_read(bdlu_ctx* c, int kind, double* total_ms, int64_t* laun
   return BDLU_OK;
 }
 
-int bdlu_gpu_debug_dense_lu(bdlu_ctx* c, int32_t m, double* A, int precision) {
-  if (!c || !A || m <= 0) return BDLU_E_INVALID;
+int bdlu_gpu_debug_dense_lu(bdlu_ctx* c, int32_t m, double* A, int precision, int lookahead,
+                            int32_t small_tile_below) {
+  if (!c || !A || m <= 0 || lookahead < 0 || lookahead > 2) return BDLU_E_INVALID;
   std::lock_guard<std::mutex> g(c->mu);
   hipStream_t st = c->stream;
   Alloc tmp;
@@ -1548,14 +1552,14 @@ int bdlu_gpu_debug_dense_lu(bdlu_ctx* c, int32_t m, double* A, int precision) {
       for (std::size_t i = 0; i < mm; ++i) h[i] = static_cast<float>(A[i]);
       float* d = tmp.copy(h, st);
       float* inv = tmp.get<float>(2 * 64 * 64 * ((static_cast<std::size_t>(m) + 63) / 64));
-      bdk::launch_tail_lu<float>(st, m, d, m, 1e300, status, inv);
+      bdk::launch_tail_lu<float>(st, m, d, m, 1e300, status, inv, nullptr, small_tile_below, lookahead);
       HIPCHECK(hipMemcpyAsync(h.data(), d, sizeof(float) * mm, hipMemcpyDeviceToHost, st));
       HIPCHECK(hipStreamSynchronize(st));
       for (std::size_t i = 0; i < mm; ++i) A[i] = h[i];
     } else {
       double* d = tmp.copy(A, mm, st);
       double* inv = tmp.get<double>(2 * 64 * 64 * ((static_cast<std::size_t>(m) + 63) / 64));
-      bdk::launch_tail_lu<double>(st, m, d, m, 1e300, status, inv);
+      bdk::launch_tail_lu<double>(st, m, d, m, 1e300, status, inv, nullptr, small_tile_below, lookahead);
       HIPCHECK(hipMemcpyAsync(A, d, sizeof(double) * mm, hipMemcpyDeviceToHost, st));
       HIPCHECK(hipStreamSynchronize(st));
     }

@@ -97,13 +97,15 @@ struct TailTimer {
 // placed in the fixed pivot order). Check: |L| <= limit and nonzero pivots.
 // inv: 2*64*64 per 64-column panel (ceil(m/64) panels): the inverses of the
 // diagonal blocks, L^{-1} then U^{-1}, row-major, kept for the tail solve.
-// lookahead: each step's next diagonal LU runs as one workgroup of the bulk
-// trailing-update launch; results are identical either way. With a timer,
-// events bracket every panel launch and every trailing-update launch (the
-// fused one with lookahead) on the same launch structure.
+// lookahead 1: each step's next diagonal LU runs as one workgroup of the bulk
+// trailing-update launch; 2: in addition, panels go in pairs and the bulk
+// update applies both at once (K = 128). Results are identical in every mode.
+// With a timer, events bracket every panel launch (and the strip updates
+// between paired panels) and every bulk trailing-update launch (the fused one
+// with lookahead) on the same launch structure.
 template <typename T>
 void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, int32_t* status, T* inv,
-                    TailTimer* timer = nullptr, int32_t small_tile_below = 0, bool lookahead = false);
+                    TailTimer* timer = nullptr, int32_t small_tile_below = 0, int lookahead = 0);
 
 // Explicit inverse M = U^{-1} L^{-1} of each dense block's factors, s x s
 // column-major at the block's factor offset (x = M (P b) solves the block).

@@ -9,6 +9,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace bdk {
 
 namespace {
@@ -739,7 +741,7 @@ __device__ __forceinline__ double readlane_t(double v, int src) {
   return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
 }
 
-constexpr int kDiagLC = kNb + 2;  // Lc[k*66 + i] = LU(i, k)
+constexpr int kDiagLC = kNb + 4;  // Lc[k*68 + i] = LU(i, k); columns start on 16-byte boundaries
 constexpr int kDiagStep = 8;      // LU steps between barriers
 template <typename T>
 constexpr std::size_t diag_lds_elems() {
@@ -766,18 +768,32 @@ __device__ __forceinline__ void diag_lu_inv_body(T* __restrict__ S, int64_t lds,
     const int i = lane;
     T a[kNb];
     const T* src = S + int64_t(p) * lds + p + (i < b ? i : 0);
+    if constexpr (sizeof(T) == 4) {
+      // Clamped addresses, unconditional loads, all in flight before the first
+      // use (a conditional load branches and waits; a shared destination
+      // register serializes the loads).
 #pragma unroll
-    for (int j = 0; j < kNb; ++j) {
-      // Clamped address, unconditional load (a conditional load branches and waits).
-      const T v = src[int64_t(j < b ? j : 0) * lds];
-      a[j] = (i < b && j < b) ? v : (i == j ? T(1) : T(0));
+      for (int j = 0; j < kNb; ++j) a[j] = __builtin_nontemporal_load(src + int64_t(j < b ? j : 0) * lds);
+#pragma unroll
+      for (int j = 0; j < kNb; ++j) a[j] = (i < b && j < b) ? a[j] : (i == j ? T(1) : T(0));
+    } else {
+#pragma unroll
+      for (int j = 0; j < kNb; ++j) {
+        // Clamped address, unconditional load (a conditional load branches and waits).
+        const T v = src[int64_t(j < b ? j : 0) * lds];
+        a[j] = (i < b && j < b) ? v : (i == j ? T(1) : T(0));
+      }
     }
     // The next pivot's reciprocal is started as soon as that pivot is final
     // (after its own column's update), so the division overlaps the rest of
-    // the step.
+    // the step. FP32 reads the pivot row's entries with readlane in groups of
+    // 8 into distinct scalar registers before their FMAs, so the wait between a
+    // readlane and the VALU op that uses its result is paid once per group
+    // (FP64 keeps one readlane pair per FMA: the groups were slower inside the
+    // fused update kernel).
     T akk = readlane_t(a[0], 0);
     T rp = T(1) / akk;
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int k = 0; k < kNb; ++k) {
       if (akk == T(0)) st |= kStZeroPivot;
       const bool below = i > k;
@@ -790,8 +806,20 @@ __device__ __forceinline__ void diag_lu_inv_body(T* __restrict__ S, int64_t lds,
         akk = readlane_t(a[k + 1], k + 1);
         rp = T(1) / akk;
       }
+      if constexpr (sizeof(T) == 4) {
+#pragma clang loop unroll(full)
+        for (int g = 0; g < kNb / 8; ++g) {
+          T r[8];
+#pragma clang loop unroll(full)
+          for (int u = 0; u < 8; ++u) r[u] = (8 * g + u >= k + 2) ? readlane_t(a[8 * g + u], k) : T(0);
+#pragma clang loop unroll(full)
+          for (int u = 0; u < 8; ++u)
+            if (8 * g + u >= k + 2) a[8 * g + u] = tfma(m, r[u], a[8 * g + u]);
+        }
+      } else {
 #pragma unroll
-      for (int j = k + 2; j < kNb; ++j) a[j] = tfma(m, readlane_t(a[j], k), a[j]);
+        for (int j = k + 2; j < kNb; ++j) a[j] = tfma(m, readlane_t(a[j], k), a[j]);
+      }
       a[k] = below ? lik : a[k];
       Lc[k * LC + i] = a[k];  // column k, final in both factors
       if ((k + 1) % kDiagStep == 0) __syncthreads();
@@ -812,16 +840,30 @@ __device__ __forceinline__ void diag_lu_inv_body(T* __restrict__ S, int64_t lds,
         if (c < b) dst[int64_t(c) * lds] = a[c];
     }
   } else if (w == 1) {
-    // Column j of L11^{-1}.
+    // Column j of L11^{-1}. FP32 reads column k of L as 16-byte broadcast
+    // vectors; FP64 element by element (its vectors in flight would cost the
+    // fused update kernel a wave per SIMD in registers).
     const int j = lane;
     T x[kNb];
 #pragma unroll
     for (int q = 0; q < kNb; ++q) x[q] = (q == j) ? T(1) : T(0);
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int k = 0; k < kNb - 1; ++k) {
       if (k % kDiagStep == 0) __syncthreads();
+      if constexpr (sizeof(T) == 4) {
+        const T xk = x[k];
+#pragma clang loop unroll(full)
+        for (int t = 0; t < kNb / kV; ++t) {
+          if (kV * t + kV - 1 <= k) continue;
+          const VT lv = *reinterpret_cast<const VT*>(Lc + k * LC + kV * t);
+#pragma clang loop unroll(full)
+          for (int e = 0; e < kV; ++e)
+            if (kV * t + e > k) x[kV * t + e] = tfma(-lv[e], xk, x[kV * t + e]);
+        }
+      } else {
 #pragma unroll
-      for (int q = k + 1; q < kNb; ++q) x[q] = tfma(-Lc[k * LC + q], x[k], x[q]);
+        for (int q = k + 1; q < kNb; ++q) x[q] = tfma(-Lc[k * LC + q], x[k], x[q]);
+      }
       __builtin_amdgcn_sched_barrier(0);  // one step's reads in flight at a time (registers)
     }
 #pragma unroll
@@ -833,12 +875,25 @@ __device__ __forceinline__ void diag_lu_inv_body(T* __restrict__ S, int64_t lds,
     T z[kNb];
 #pragma unroll
     for (int q = 0; q < kNb; ++q) z[q] = T(0);
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int k = 0; k < kNb; ++k) {
       if (k % kDiagStep == 0) __syncthreads();
       T ps[4] = {T(0), T(0), T(0), T(0)};
+      if constexpr (sizeof(T) == 4) {
+#pragma clang loop unroll(full)
+        for (int t = 0; t < kNb / kV; ++t) {
+          if (kV * t >= k) continue;
+          const VT lv = *reinterpret_cast<const VT*>(Lc + k * LC + kV * t);
+#pragma clang loop unroll(full)
+          for (int e = 0; e < kV; ++e) {
+            const int mm = kV * t + e;
+            if (mm < k) ps[mm & 3] = tfma(z[mm], lv[e], ps[mm & 3]);
+          }
+        }
+      } else {
 #pragma unroll
-      for (int m = 0; m < k; ++m) ps[m & 3] = tfma(z[m], Lc[k * LC + m], ps[m & 3]);
+        for (int m = 0; m < k; ++m) ps[m & 3] = tfma(z[m], Lc[k * LC + m], ps[m & 3]);
+      }
       const T sum = (ps[0] + ps[1]) + (ps[2] + ps[3]);
       z[k] = (i < k ? -sum : (i == k ? T(1) : T(0))) / Lc[k * LC + k];
       __builtin_amdgcn_sched_barrier(0);
@@ -1028,14 +1083,17 @@ constexpr int kUpdKc = 16;
 template <int kTile, typename T>
 constexpr std::size_t update_lds_elems() { return 2 * 2 * kUpdKc * (kTile + 1); }
 
-template <typename T, int kTile, bool kAllChunks = false>
+// kK: the largest K the tile takes (64, or 128 for two panels applied at once;
+// the chunks past J.K stage zeros).
+template <typename T, int kTile, bool kAllChunks = false, int kK = kNb>
 __device__ __forceinline__ void tile_update(const TileJob<T>& j0, const TileJob<T>& j1, T* lds, int32_t bid) {
   using V = typename MfmaOps<T>::v4;
   constexpr int kLd = kTile + 1;                // LDS row length (padded against bank conflicts)
   constexpr int kWT = kTile / 2;                // wavefront sub-tile edge
   constexpr int kMB = kTile / 32;               // 16x16 MFMA blocks per wavefront and dimension
   constexpr int kPer = kTile * kUpdKc / 256;    // staged elements per thread, operand and chunk
-  constexpr int kChunks = kNb / kUpdKc;
+  constexpr int kChunks = kK / kUpdKc;
+  static_assert(!kAllChunks || kK == kNb, "one register set per chunk only for K = 64");
   // Ring slot s: As = lds + s * 2 * kUpdKc * kLd (As[k*kLd + i] = A(i0 + i, k0 + k)),
   // Bs = As + kUpdKc * kLd (Bs[k*kLd + j] = -B(k0 + k, c0 + j)).
   const int32_t n0 = j0.tiles_m * j0.tiles_n;
@@ -1147,7 +1205,10 @@ __global__ __launch_bounds__(256) void k_tail_update(TileJob<T> j0, TileJob<T> j
 // and diagonal code share one LDS buffer.
 // FP32: two waves per SIMD, as for the plain update (the diagonal path would
 // otherwise raise the register count of the whole kernel).
-template <typename T, int kTile>
+// kK = 128: the jobs apply two consecutive panels at once (K = 128), which
+// reads and writes the trailing matrix once per two panels; each element still
+// gets the MFMA sequence of the two single-panel updates in turn.
+template <typename T, int kTile, int kK = kNb>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1)))
 void k_tail_update_diag(TileJob<T> j0, TileJob<T> j1, TileJob<T> next, double limit,
                         int32_t* __restrict__ status, T* __restrict__ S, int64_t lds_s, int32_t q, int32_t nb,
@@ -1158,13 +1219,14 @@ void k_tail_update_diag(TileJob<T> j0, TileJob<T> j1, TileJob<T> next, double li
   if (blockIdx.x == 0) {
     // The tile ends on a barrier after its last LDS read; the diagonal body's
     // first barrier makes the block's new values visible to its loads.
-    tile_update<T, 64, sizeof(T) == 8>(next, next, lds, 0);
+    tile_update<T, 64, sizeof(T) == 8 && kK == kNb, kK>(next, next, lds, 0);
     diag_lu_inv_body<T>(S, lds_s, q, nb, limit, status, inv, lds);
     return;
   }
-  // FP64 64x64 tiles: all K chunks' loads up front (two waves per SIMD here,
-  // set by the diagonal path's registers).
-  tile_update<T, kTile, sizeof(T) == 8 && kTile == 64>(j0, j1, lds, static_cast<int32_t>(blockIdx.x) - 1);
+  // FP64 64x64 tiles with K = 64: all K chunks' loads up front (two waves per
+  // SIMD here, set by the diagonal path's registers).
+  tile_update<T, kTile, sizeof(T) == 8 && kTile == 64 && kK == kNb, kK>(j0, j1, lds,
+                                                                      static_cast<int32_t>(blockIdx.x) - 1);
 }
 template <typename T, int kTile>
 __global__ __launch_bounds__(256) void k_tail_panel(TileJob<T> j0, TileJob<T> j1, double limit,
@@ -1253,22 +1315,112 @@ void tail_step_lookahead(hipStream_t st, T* S, int64_t lds, int32_t p, int32_t b
   }
 }
 
+// Panel products of the panel at p (L21 = A21 U11^{-1} over the rest rows
+// below it, U12 = L11^{-1} A12 over the rest columns right of it).
+template <typename T, int kTile>
+void tail_panel(hipStream_t st, T* S, int64_t lds, int32_t p, int32_t b, int32_t rest, double limit,
+                int32_t* status, const T* inv, TailTimer* timer) {
+  const int32_t tr = (rest + kTile - 1) / kTile;
+  TileJob<T> l21{rest, b, b, 2, tr, 1, S + int64_t(p) * lds + p + b, 1, lds, inv + kNb * kNb, kNb, 1,
+                 S + int64_t(p) * lds + p + b, 1, lds};
+  TileJob<T> u12{b, rest, b, 1, 1, tr, inv, kNb, 1, S + int64_t(p + b) * lds + p, 1, lds,
+                 S + int64_t(p + b) * lds + p, 1, lds};
+  if (timer && timer->panel_events) timer_event(timer->panel_events, st);
+  hipLaunchKernelGGL((k_tail_panel<T, kTile>), dim3(2 * tr), dim3(256), 0, st, l21, u12, limit, status);
+  if (timer && timer->panel_events) timer_event(timer->panel_events, st);
+}
+
+// Two panels per trailing update. Panel s (at p, width 64, rest > 64 rows
+// below it): its products, then one launch that applies it only to what panel
+// s+1 needs: workgroup 0 updates the next diagonal block and factors it, the
+// others the block column below it and the block row right of it (64-wide
+// strips). Panel s+1 (at q = p + 64): its products, then one launch that
+// applies both panels (K = 128) to the rest of the trailing matrix, with
+// workgroup 0 updating and factoring the diagonal block after them. Every
+// element gets the same MFMA sequence as with one update per panel, so the
+// factors are identical; the trailing matrix is read and written once per two
+// panels.
+template <typename T>
+void tail_pair(hipStream_t st, T* S, int64_t lds, int32_t p, int32_t rest, double limit, int32_t* status, T* inv,
+               TailTimer* timer, int32_t small_tile_below) {
+  const int64_t kInvPair = 2 * kNb * kNb;
+  const int32_t b = kNb;
+  if (rest < small_tile_below) tail_panel<T, 64>(st, S, lds, p, b, rest, limit, status, inv, timer);
+  else tail_panel<T, 128>(st, S, lds, p, b, rest, limit, status, inv, timer);
+  const int32_t q = p + b;   // panel s+1 (64 columns, since rest > 64)
+  const int32_t n0 = rest - kNb;
+  {
+    const T* L21 = S + int64_t(p) * lds + q;  // rest x 64
+    const T* U12 = S + int64_t(q) * lds + p;  // 64 x rest
+    TileJob<T> a{kNb, kNb, b, 0, 1, 1, L21, 1, lds, U12, 1, lds, S + int64_t(q) * lds + q, 1, lds};
+    const int32_t t0 = (n0 + kNb - 1) / kNb;
+    TileJob<T> row{kNb, n0, b, 0, 1, t0, L21, 1, lds, U12 + int64_t(kNb) * lds, 1, lds,
+                   S + int64_t(q + kNb) * lds + q, 1, lds};
+    TileJob<T> col{n0, kNb, b, 0, t0, 1, L21 + kNb, 1, lds, U12, 1, lds, S + int64_t(q) * lds + q + kNb, 1, lds};
+    if (timer && timer->panel_events) timer_event(timer->panel_events, st);
+    hipLaunchKernelGGL((k_tail_update_diag<T, 64, kNb>), dim3(1 + 2 * t0), dim3(256), 0, st, row, col, a, limit,
+                       status, S, lds, q, kNb, inv + kInvPair);
+    if (timer && timer->panel_events) timer_event(timer->panel_events, st);
+  }
+  // Panel s+1.
+  const int32_t rest1 = n0;
+  if (rest1 < small_tile_below) tail_panel<T, 64>(st, S, lds, q, kNb, rest1, limit, status, inv + kInvPair, timer);
+  else tail_panel<T, 128>(st, S, lds, q, kNb, rest1, limit, status, inv + kInvPair, timer);
+  const int32_t q2 = q + kNb;                        // next diagonal block
+  const int32_t nb2 = rest1 < kNb ? rest1 : kNb;     // its size
+  const int32_t n02 = rest1 - nb2;
+  const int32_t K = 2 * kNb;
+  const T* L21 = S + int64_t(p) * lds + q2;          // rest1 x 128 (panels s and s+1)
+  const T* U12 = S + int64_t(q2) * lds + p;          // 128 x rest1
+  TileJob<T> a{nb2, nb2, K, 0, 1, 1, L21, 1, lds, U12, 1, lds, S + int64_t(q2) * lds + q2, 1, lds};
+  auto launch = [&](auto tile) {
+    constexpr int kTile = decltype(tile)::value;
+    const int32_t tr = (rest1 + kTile - 1) / kTile;
+    TileJob<T> right{rest1, n02, K, 0, tr, (n02 + kTile - 1) / kTile, L21, 1, lds, U12 + int64_t(nb2) * lds, 1, lds,
+                     S + int64_t(q2 + nb2) * lds + q2, 1, lds};
+    TileJob<T> below{n02, nb2, K, 0, (n02 + kTile - 1) / kTile, 1, L21 + nb2, 1, lds, U12, 1, lds,
+                     S + int64_t(q2) * lds + q2 + nb2, 1, lds};
+    if (n02 <= 0) right.tiles_m = right.tiles_n = below.tiles_m = below.tiles_n = 0;
+    const int32_t g = 1 + right.tiles_m * right.tiles_n + below.tiles_m * below.tiles_n;
+    if (timer && timer->gemm_events) timer_event(timer->gemm_events, st);
+    hipLaunchKernelGGL((k_tail_update_diag<T, kTile, 2 * kNb>), dim3(g), dim3(256), 0, st, right, below, a, limit,
+                       status, S, lds, q2, nb2, inv + 2 * kInvPair);
+  };
+  if (rest1 < small_tile_below) launch(std::integral_constant<int, 64>{});
+  else launch(std::integral_constant<int, 128>{});
+  if (timer && timer->gemm_events) {
+    timer_event(timer->gemm_events, st);
+    // Algorithmic work of the launch: the rest1 x rest1 trailing update with
+    // K = 128 (C read and written once, both panels read once) plus the next
+    // diagonal block's LU and its two triangular inverses.
+    const double r = rest1, k = K, d = nb2;
+    timer->gemm_flops->push_back(2.0 * r * r * k + (2.0 / 3.0 + 2.0 / 3.0) * d * d * d);
+    timer->gemm_bytes->push_back(sizeof(T) * (2.0 * r * r + 2.0 * r * k));
+  }
+}
+
 template <typename T>
 void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, int32_t* status, T* inv,
-                    TailTimer* timer, int32_t small_tile_below, bool lookahead) {
+                    TailTimer* timer, int32_t small_tile_below, int lookahead) {
   if (lookahead) {
     if (m <= 0) return;
     hipLaunchKernelGGL(k_diag_lu_inv<T>, dim3(1), dim3(kDiagThreads), 0, st, S, lds, 0, m < kNb ? m : kNb, limit,
                        status, inv);
     const int64_t kInvPair = 2 * kNb * kNb;
-    for (int32_t p = 0; p < m; p += kNb) {
+    for (int32_t p = 0; p < m;) {
       const int32_t b = (m - p) < kNb ? (m - p) : kNb;
       const int32_t rest = m - p - b;
       if (rest <= 0) break;
+      if (lookahead >= 2 && rest > kNb) {
+        tail_pair<T>(st, S, lds, p, rest, limit, status, inv + p / kNb * kInvPair, timer, small_tile_below);
+        p += 2 * kNb;
+        continue;
+      }
       if (rest < small_tile_below)
         tail_step_lookahead<T, 64>(st, S, lds, p, b, rest, limit, status, inv + p / kNb * kInvPair, timer);
       else
         tail_step_lookahead<T, 128>(st, S, lds, p, b, rest, limit, status, inv + p / kNb * kInvPair, timer);
+      p += b;
     }
     return;
   }
@@ -1289,9 +1441,9 @@ void launch_tail_lu(hipStream_t st, int32_t m, T* S, int64_t lds, double limit, 
   }
 }
 template void launch_tail_lu<double>(hipStream_t, int32_t, double*, int64_t, double, int32_t*, double*, TailTimer*,
-                                     int32_t, bool);
+                                     int32_t, int);
 template void launch_tail_lu<float>(hipStream_t, int32_t, float*, int64_t, double, int32_t*, float*, TailTimer*,
-                                    int32_t, bool);
+                                    int32_t, int);
 
 // ---------------------------------------------------------------------------
 // Solve.

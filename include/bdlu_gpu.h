@@ -209,7 +209,11 @@ int bdlu_gpu_plan_dense_pivots(bdlu_ctx* ctx, uint64_t plan_id, int32_t* host_pi
 
 // Test hook: no-pivot dense LU of an m x m column-major matrix on the device
 // through the same kernels the tail factorization uses; A is overwritten by L\U.
-int bdlu_gpu_debug_dense_lu(bdlu_ctx* ctx, int32_t m, double* A, int precision);
+// lookahead selects the launch structure (0 separate diagonal LU launches,
+// 1 diagonal LU fused into each bulk update, 2 paired panels with K = 128 bulk
+// updates); trailing updates below small_tile_below rows use 64x64 tiles.
+int bdlu_gpu_debug_dense_lu(bdlu_ctx* ctx, int32_t m, double* A, int precision, int lookahead,
+                            int32_t small_tile_below);
 
 #ifdef __cplusplus
 }

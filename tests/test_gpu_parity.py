@@ -85,25 +85,38 @@ def test_cache_matrices_all_ones(gpu, changed, scale):
 
 
 def test_tail_lu_kernels_match_numpy(gpu):
-    # The no-pivot blocked LU used for dense Schur tails (MFMA trailing update),
-    # against a float64 numpy LU without pivoting on a diagonally dominant matrix.
-    _say("case: debug dense LU m=333")
+    """The no-pivot blocked LU used for dense Schur tails (MFMA trailing
+    updates), against a float64 numpy LU without pivoting on a diagonally
+    dominant matrix, in every launch structure: separate diagonal LU launches,
+    the diagonal LU fused into each bulk update, and paired panels with K = 128
+    bulk updates (with 128 tiles throughout, and with 64 tiles below 300 rows).
+    The structures apply the same MFMA sequence to every element, so their
+    factors must be identical bit for bit. m = 333 ends on an unpaired full
+    panel and a 13-column one; m = 704 is 11 full panels."""
     bd = gpu
     L = bd.lib()
     ctx = C.c_void_p(L.bdlu_solver_device_context(L.bdlu_solver_create()))
     rng = np.random.default_rng(5)
-    m = 333
-    A = rng.standard_normal((m, m)) + m * np.eye(m)
-    ref = A.copy()
-    for k in range(m - 1):
-        ref[k + 1:, k] /= ref[k, k]
-        ref[k + 1:, k + 1:] -= np.outer(ref[k + 1:, k], ref[k, k + 1:])
-    for prec, tol in ((0, 1e-12), (1, 2e-5)):
-        a = np.asfortranarray(A.copy())
-        assert L.gpu.bdlu_gpu_debug_dense_lu(ctx, m, a.ctypes.data_as(C.POINTER(C.c_double)), prec) == 0
-        err = np.max(np.abs(a - ref)) / np.max(np.abs(ref))
-        _say(f"  prec={prec} max rel err={err:.3e}")
-        assert err < tol
+    for m in (333, 704):
+        A = rng.standard_normal((m, m)) + m * np.eye(m)
+        ref = A.copy()
+        for k in range(m - 1):
+            ref[k + 1:, k] /= ref[k, k]
+            ref[k + 1:, k + 1:] -= np.outer(ref[k + 1:, k], ref[k, k + 1:])
+        for prec, tol in ((0, 1e-12), (1, 2e-5)):
+            first = None
+            for mode, small in ((0, 0), (1, 0), (2, 0), (2, 300)):
+                _say(f"case: debug dense LU m={m} prec={prec} structure={mode} small_tile_below={small}")
+                a = np.asfortranarray(A.copy())
+                assert L.gpu.bdlu_gpu_debug_dense_lu(ctx, m, a.ctypes.data_as(C.POINTER(C.c_double)), prec, mode,
+                                                     small) == 0
+                err = np.max(np.abs(a - ref)) / np.max(np.abs(ref))
+                _say(f"  max rel err={err:.3e}")
+                assert err < tol
+                if first is None:
+                    first = a
+                else:
+                    assert np.array_equal(a, first), "launch structures differ"
 
 
 def test_config0_fp64_matches_oracle(gpu, orc):
