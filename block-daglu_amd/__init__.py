@@ -147,6 +147,7 @@ GPU_SYMBOLS = [
     ("bdlu_gpu_plan_values", C.c_int, [_P, C.c_uint64, _F64P]),
     ("bdlu_gpu_stream", _P, [_P]),
     ("bdlu_gpu_plan_dense_pivots", C.c_int, [_P, C.c_uint64, _I32P]),
+    ("bdlu_gpu_set_side_streams", C.c_int, [_P, C.c_int]),
     ("bdlu_gpu_profile", C.c_int, [_P, C.c_int]),
     ("bdlu_gpu_profile_read", C.c_int, [_P, C.c_int, _F64P, C.POINTER(C.c_int64), _F64P, _F64P]),
     ("bdlu_gpu_debug_dense_lu", C.c_int, [_P, C.c_int32, _F64P, C.c_int, C.c_int, C.c_int32]),

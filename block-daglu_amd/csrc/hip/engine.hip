@@ -519,13 +519,18 @@ void upload_into(bdlu_ctx* c, const bdlu_plan_desc& d, Plan& P) {
     const int32_t ne = sd.a_ptr[sd.s];
     S.nL = sd.l_ptr[sd.k0];
     S.nU = sd.u_ptr[sd.s];
-    // Entries by local row (row scaling in a fixed order).
+    // Entries by local row for the row scaling, each row's entries in
+    // ascending local column order (the order klu_scale sums them in).
     std::vector<int32_t> rs_ptr(sd.s + 1, 0), rs_idx(ne);
     for (int32_t p = 0; p < ne; ++p) rs_ptr[sd.a_row[p] + 1]++;
     for (int32_t r = 0; r < sd.s; ++r) rs_ptr[r + 1] += rs_ptr[r];
     {
-      std::vector<int32_t> next(rs_ptr.begin(), rs_ptr.end() - 1);
-      for (int32_t p = 0; p < ne; ++p) rs_idx[next[sd.a_row[p]]++] = p;
+      std::vector<int32_t> next(rs_ptr.begin(), rs_ptr.end() - 1), pos_of_col(sd.s);
+      for (int32_t j = 0; j < sd.s; ++j) pos_of_col[sd.q[j]] = j;
+      for (int32_t c = 0; c < sd.s; ++c) {
+        const int32_t j = pos_of_col[c];
+        for (int32_t p = sd.a_ptr[j]; p < sd.a_ptr[j + 1]; ++p) rs_idx[next[sd.a_row[p]]++] = p;
+      }
     }
     std::vector<int32_t> tpos(sd.m);
     for (int32_t t = 0; t < sd.m; ++t) tpos[sd.tperm[t]] = t;
@@ -1510,6 +1515,14 @@ int bdlu_gpu_plan_dense_pivots(bdlu_ctx* c, uint64_t id, int32_t* host_piv) {
   } catch (const HipError& e) {
     return set_err(c, BDLU_E_BACKEND, e.msg);
   }
+  return BDLU_OK;
+}
+
+int bdlu_gpu_set_side_streams(bdlu_ctx* c, int enable) {
+  if (!c) return BDLU_E_INVALID;
+  std::lock_guard<std::mutex> g(c->mu);
+  c->fork = enable != 0;
+  for (auto& kv : c->plans) kv.second->drop_graphs();  // recapture with the new launch structure
   return BDLU_OK;
 }
 

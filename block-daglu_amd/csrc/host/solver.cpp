@@ -89,6 +89,7 @@ std::size_t cache_max() {
 }
 
 std::unique_ptr<Resident> lease(const PlanKey& k) {
+  if (!cache_enabled()) return nullptr;  // USERALGO_CACHE=0: every analyze() analyses its own values
   std::lock_guard<std::mutex> g(g_mu);
   for (auto it = g_free.begin(); it != g_free.end(); ++it)
     if ((*it)->key == k) {

@@ -131,10 +131,14 @@ bool analyze_sparse_block(int32_t s, const std::vector<int32_t>& rows, const std
   std::vector<double> rs(s, 1.0);
   if (out.scale) {
     std::fill(rs.begin(), rs.end(), 0.0);
-    if (opt.scale_kind == 2)
-      for (int32_t t = 0; t < ne; ++t) rs[rows[t]] = std::max(rs[rows[t]], std::abs(vals[t]));
-    else
-      for (int32_t t = 0; t < ne; ++t) rs[rows[t]] += std::abs(vals[t]);
+    // Each row's terms in ascending local column order, the order klu_scale
+    // visits the block's CSC (a sum's last bits depend on its order).
+    for (int32_t c = 0; c < s; ++c)
+      for (int32_t p = Ap[c]; p < Ap[c + 1]; ++p) {
+        const double a = std::abs(vals[Ae[p]]);
+        double& r = rs[Ai[p]];
+        r = opt.scale_kind == 2 ? std::max(r, a) : r + a;
+      }
     for (int32_t i = 0; i < s; ++i)
       if (rs[i] == 0.0) rs[i] = 1.0;
   }

@@ -182,6 +182,14 @@ int bdlu_gpu_solve_noalloc(bdlu_ctx* ctx, uint64_t plan_id, double* d_B, int32_t
 // Copies the plan's current canonical-order values (last refactor) to host memory.
 int bdlu_gpu_plan_values(bdlu_ctx* ctx, uint64_t plan_id, double* host_values);
 
+// Side streams of a refactor (no counterpart in the reference): with enable
+// = 1 (the default) the dense-block LU and the dense tails of different sparse
+// blocks run on the context's side streams, joined back into its stream; 0
+// keeps every launch on the one stream (the serving loop runs one context per
+// slot, and side streams of concurrent contexts would share hardware queues).
+// Results are identical either way.
+int bdlu_gpu_set_side_streams(bdlu_ctx* ctx, int enable);
+
 // The context's HIP stream (a hipStream_t), so callers can order or time their
 // own work around engine calls.
 void* bdlu_gpu_stream(bdlu_ctx* ctx);
