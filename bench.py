@@ -120,10 +120,10 @@ def traffic_from_counters(fetch_db: str, write_db: str) -> dict:
     out["bytes_per_launch_unscaled"] = 1024.0 * (out["FETCH_SIZE"]["mean_kib"] + out["WRITE_SIZE"]["mean_kib"])
     out["kernel"] = ROOFLINE_KERNEL
     out["commands"] = [
-        "rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmc_fetch -o run -- python3 bench.py --steps 2 --warmup 1 "
-        "--no-cpu-baseline --copy-calibration",
-        "rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pmc_write -o run -- python3 bench.py --steps 2 --warmup 1 "
-        "--no-cpu-baseline --copy-calibration",
+        "rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmc_fetch -o run -- python3 bench.py --config 1 --steps 2 "
+        "--warmup 1 --no-cpu-baseline --copy-calibration",
+        "rocprofv3 --pmc WRITE_SIZE -d gpurun_out/pmc_write -o run -- python3 bench.py --config 1 --steps 2 "
+        "--warmup 1 --no-cpu-baseline --copy-calibration",
         "python bench.py --traffic gpurun_out/pmc_fetch/run_results.db gpurun_out/pmc_write/run_results.db"]
     return out
 
