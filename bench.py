@@ -413,10 +413,31 @@ def bench_serving(bd, args):
     lam = None
     ncap = args.steps * REQS_PER_STEP
     nwarm = args.warmup * REQS_PER_STEP
+    dev = not args.host_buffers
+    if dev:
+        # Inputs resident in HBM before anything is timed: every value set in
+        # canonical order, every pattern's right-hand sides, and one solution
+        # region per request index (phases run one after another, and the
+        # indices inside a phase are distinct).
+        dvals, drhs, dwarm = [], [], []
+        for n, r, c, v, vals, rhs in pats:
+            order = bd.canonical_order(r, c)
+            assert len(np.unique(np.asarray(r, np.int64) * n + c)) == len(r)  # no duplicates: canonical = sorted
+            dvals.append([bd.DeviceArray.from_numpy(vs[order]) for vs in vals])
+            drhs.append(bd.DeviceArray.from_numpy(rhs.ravel(order="F")))
+            dwarm.append([bd.DeviceArray(n) for _ in range(args.slots)])
+        sizes = np.array([pats[req_pat[i]][0] * req_k[i] for i in range(nstream)], np.int64)
+        xoff = np.concatenate([[0], np.cumsum(sizes)])
+        xbuf = bd.DeviceArray(int(xoff[-1]))
+        log(f"serving: inputs resident in HBM ({xoff[-1] * 8 / 1e9:.2f} GB of solution space)")
 
     def submit(srv, pid, i):
         p = req_pat[i]
+        if dev:
+            return srv.submit_device(pid[p], dvals[p][req_val[i]].address, drhs[p].address,
+                                     xbuf.address + 8 * int(xoff[i]), int(req_k[i]))
         return srv.submit(pid[p], pats[p][4][req_val[i]], pats[p][5][:, :req_k[i]])
+
 
     # Plain FP64 sets the arrival rate; FP64 with FP64 refinement is the FP64
     # configuration that meets the 1e-12 residual gate (the reference's
@@ -434,7 +455,11 @@ def bench_serving(bd, args):
             pid.append(srv.add_pattern(p[1], p[2], p[3], p[0]))
         t_an = time.time() - t
         log(f"  {name}: 16 patterns registered (host analysis) in {t_an:.1f}s")
-        ids = [srv.submit(pid[p], pats[p][4][0], pats[p][5][:, :1]) for p in range(16) for _ in range(args.slots)]
+        if dev:
+            ids = [srv.submit_device(pid[p], dvals[p][0].address, drhs[p].address, dwarm[p][j].address, 1)
+                   for p in range(16) for j in range(args.slots)]
+        else:
+            ids = [srv.submit(pid[p], pats[p][4][0], pats[p][5][:, :1]) for p in range(16) for _ in range(args.slots)]
         for i in ids:
             srv.wait(i)
         ids = [submit(srv, pid, i) for i in range(nwarm)]
@@ -477,12 +502,15 @@ def bench_serving(bd, args):
                 time.sleep(0.0002)
             X, rr = srv.wait(oids[i])
             res.append(rr)
-            if i < 100:
+            if i < 100 and not dev:
                 xs.append(X)
             if (i + 1) % 500 == 0:
                 log(f"    {i + 1}/{nreq} done")
         th.join()
         wall = time.perf_counter() - t0
+        if dev:
+            xall = xbuf.numpy()
+            xs = [xall[xoff[i]:xoff[i + 1]].reshape(pats[req_pat[i]][0], req_k[i], order="F") for i in range(100)]
         keep[name] = xs
         lat = np.array([r["t_done_ms"] - r["t_submit_ms"] for r in res])
         qwait = np.array([r["t_start_ms"] - r["t_submit_ms"] for r in res])
@@ -670,6 +698,9 @@ def main():
                          "0-3 closed loop on one system")
     ap.add_argument("--requests", type=int, default=2000, help="open-loop serving stream length")
     ap.add_argument("--slots", type=int, default=4, help="serving slots")
+    ap.add_argument("--host-buffers", action="store_true",
+                    help="serving requests hand over host buffers (values, right-hand sides in, solutions out: "
+                         "the PCIe-inclusive rate); default: inputs resident in HBM")
     ap.add_argument("--no-closed-loop", action="store_true", help="serving stream only")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline sample per thread setting")
@@ -745,7 +776,9 @@ def main():
         "config": {"workload": f"serving stream: 16 patterns n=1e3..1e5 (circuit-like, giant block 10% of n), "
                                f"+-5% value perturbations, 1-8 RHS per request, {args.slots} slots, one 64 GB "
                                f"HBM budget; a step is a batch of {REQS_PER_STEP} requests queued at once "
-                               f"(closed loop); latency from {args.requests} Poisson arrivals at {lam:.1f}/s",
+                               f"(closed loop); latency from {args.requests} Poisson arrivals at {lam:.1f}/s; "
+                               + ("requests hand over host buffers (PCIe-inclusive)" if args.host_buffers else
+                                  "request values and right-hand sides resident in HBM, solutions written to HBM"),
                    "parallelism": "single GPU"},
         "p50_ms": f32["p50_ms"], "p99_ms": f32["p99_ms"], "latency": "arrival to result, open loop",
         "rel_residual": f32["max_rel_residual"], "backward_error": f32["max_backward_error"],

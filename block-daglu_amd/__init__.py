@@ -109,6 +109,7 @@ HOST_SYMBOLS = [
     ("bdlu_serve_add_pattern", C.c_int32, [_P, C.c_int64, _I32P, _I32P, _F64P, C.c_int32]),
     ("bdlu_serve_pattern_n", C.c_int32, [_P, C.c_int32]),
     ("bdlu_serve_submit", C.c_int64, [_P, C.c_int32, _F64P, C.c_int64, _F64P, C.c_int32]),
+    ("bdlu_serve_submit_device", C.c_int64, [_P, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]),
     ("bdlu_serve_wait", C.c_int, [_P, C.c_int64, _F64P, C.POINTER(_ServeResult)]),
     ("bdlu_host_analyze", _P, [C.c_int64, _I32P, _I32P, _F64P, C.c_int32, C.POINTER(C.c_int)]),
     ("bdlu_host_analyze_klu", _P, [C.c_int64, _I32P, _I32P, _F64P, C.c_int32, C.c_int, C.c_int, C.c_double,
@@ -148,6 +149,7 @@ GPU_SYMBOLS = [
     ("bdlu_gpu_stream", _P, [_P]),
     ("bdlu_gpu_plan_dense_pivots", C.c_int, [_P, C.c_uint64, _I32P]),
     ("bdlu_gpu_set_side_streams", C.c_int, [_P, C.c_int]),
+    ("bdlu_gpu_copy", C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_uint64]),
     ("bdlu_gpu_profile", C.c_int, [_P, C.c_int]),
     ("bdlu_gpu_profile_read", C.c_int, [_P, C.c_int, _F64P, C.POINTER(C.c_int64), _F64P, _F64P]),
     ("bdlu_gpu_debug_dense_lu", C.c_int, [_P, C.c_int32, _F64P, C.c_int, C.c_int, C.c_int32]),
@@ -501,15 +503,34 @@ class Server:
         self._pending[rid] = (self._n[pattern], k, b.ndim)
         return rid
 
+    def submit_device(self, pattern: int, d_vals: int, d_B: int, d_X: int, nrhs: int) -> int:
+        """A request whose inputs are already in device memory (addresses):
+        d_vals the pattern's canonical values, d_B / d_X n x nrhs column-major
+        right-hand sides / solutions. wait() then returns X = None."""
+        rid = self._L.bdlu_serve_submit_device(self._h, pattern, C.c_void_p(d_vals), C.c_void_p(d_B),
+                                               C.c_void_p(d_X), nrhs)
+        if rid < 0:
+            raise RuntimeError(f"submit_device failed: {SolverError(-rid).name}")
+        self._pending = getattr(self, "_pending", {})
+        self._pending[rid] = None
+        return rid
+
     def wait(self, request: int):
-        """-> (X, result dict). X has the shape of the submitted B."""
-        n, k, nd = self._pending.pop(request)
-        X = np.zeros((n, k), np.float64, order="F")
+        """-> (X, result dict). X has the shape of the submitted B (None for a
+        device-resident request: the solution is in its d_X)."""
+        info = self._pending.pop(request)
         res = _ServeResult()
-        self._L.bdlu_serve_wait(self._h, request, _ptr(X, _F64P), C.byref(res))
+        if info is None:
+            X = None
+            self._L.bdlu_serve_wait(self._h, request, None, C.byref(res))
+        else:
+            n, k, nd = info
+            X = np.zeros((n, k), np.float64, order="F")
+            self._L.bdlu_serve_wait(self._h, request, _ptr(X, _F64P), C.byref(res))
+            X = X[:, 0] if nd == 1 else X
         out = {f: getattr(res, f) for f, _ in _ServeResult._fields_}
         out["error"] = SolverError(out["error"])
-        return (X[:, 0] if nd == 1 else X), out
+        return X, out
 
 
 class HostAnalysis:

@@ -1518,6 +1518,19 @@ int bdlu_gpu_plan_dense_pivots(bdlu_ctx* c, uint64_t id, int32_t* host_piv) {
   return BDLU_OK;
 }
 
+int bdlu_gpu_copy(bdlu_ctx* c, void* dst, const void* src, uint64_t bytes) {
+  if (!c || (bytes && (!dst || !src))) return BDLU_E_INVALID;
+  std::lock_guard<std::mutex> g(c->mu);
+  try {
+    HIPCHECK(hipSetDevice(c->device));
+    if (bytes) HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  } catch (const HipError& e) {
+    return set_err(c, BDLU_E_BACKEND, e.msg);
+  }
+  return BDLU_OK;
+}
+
 int bdlu_gpu_set_side_streams(bdlu_ctx* c, int enable) {
   if (!c) return BDLU_E_INVALID;
   std::lock_guard<std::mutex> g(c->mu);

@@ -22,6 +22,8 @@ void serve_engine_destroy(ServeEngine* e);
 int32_t serve_engine_add_pattern(ServeEngine* e, const CooMatrix& m, int32_t dim);
 int32_t serve_engine_pattern_n(ServeEngine* e, int32_t p);
 int64_t serve_engine_submit(ServeEngine* e, int32_t p, const double* v, int64_t nv, const double* B, int32_t k);
+int64_t serve_engine_submit_device(ServeEngine* e, int32_t p, const double* d_v, const double* d_B, double* d_X,
+                                   int32_t k);
 bool serve_engine_wait(ServeEngine* e, int64_t id, double* X, ServeResult& res);
 SolverError serve_engine_last_error(ServeEngine* e);
 }  // namespace blockdaglu
@@ -353,6 +355,18 @@ int64_t bdlu_serve_submit(void* s, int32_t pattern, const double* vals, int64_t 
   auto* e = static_cast<blockdaglu::ServeEngine*>(s);
   try {
     const int64_t id = blockdaglu::serve_engine_submit(e, pattern, vals, nnz, B, nrhs);
+    return id >= 0 ? id : -static_cast<int64_t>(blockdaglu::serve_engine_last_error(e));
+  } catch (...) {
+    return -static_cast<int64_t>(BDLU_ERR_BACKEND_UNAVAILABLE);
+  }
+}
+
+int64_t bdlu_serve_submit_device(void* s, int32_t pattern, const double* d_vals, const double* d_B, double* d_X,
+                                 int32_t nrhs) {
+  if (!s || !d_vals || !d_B || !d_X) return -static_cast<int64_t>(BDLU_ERR_INVALID_RHS);
+  auto* e = static_cast<blockdaglu::ServeEngine*>(s);
+  try {
+    const int64_t id = blockdaglu::serve_engine_submit_device(e, pattern, d_vals, d_B, d_X, nrhs);
     return id >= 0 ? id : -static_cast<int64_t>(blockdaglu::serve_engine_last_error(e));
   } catch (...) {
     return -static_cast<int64_t>(BDLU_ERR_BACKEND_UNAVAILABLE);

@@ -49,6 +49,11 @@ struct Request {
   int32_t pattern = 0;
   std::vector<double> vals;  // canonical order
   std::vector<double> B;     // right-hand sides in, solutions out (column-major n x nrhs)
+  // Device-resident request (submit_device): canonical values, right-hand
+  // sides and the solution buffer in device memory, owned by the caller.
+  const double* d_vals = nullptr;
+  const double* d_B = nullptr;
+  double* d_X = nullptr;
   int32_t nrhs = 1;
   ServeResult res;
   bool done = false;
@@ -221,8 +226,38 @@ class ServeEngine {
     return id;
   }
 
+  // A request whose inputs are already in device memory: d_vals in canonical
+  // order (canon.nnz values), d_B and d_X n x nrhs column-major; the solution
+  // is written to d_X. The buffers must stay valid until wait() returns.
+  int64_t submit_device(int32_t pattern, const double* d_vals, const double* d_B, double* d_X, int32_t nrhs) {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      if (pattern < 0 || pattern >= static_cast<int32_t>(patterns_.size())) {
+        last_error_ = SolverError::InvalidIndex;
+        return -1;
+      }
+    }
+    if (!d_vals || !d_B || !d_X || nrhs < 1) {
+      last_error_ = SolverError::InvalidRightHandSide;
+      return -1;
+    }
+    auto r = std::make_unique<Request>();
+    r->pattern = pattern;
+    r->nrhs = nrhs;
+    r->d_vals = d_vals;
+    r->d_B = d_B;
+    r->d_X = d_X;
+    std::lock_guard<std::mutex> g(mu_);
+    r->res.t_submit_ms = now_ms();
+    const int64_t id = next_id_++;
+    queue_.push_back(r.get());
+    requests_.emplace(id, std::move(r));
+    cv_.notify_one();
+    return id;
+  }
+
   // Blocks until request `id` is done; its solutions go to X (raw pointer, n*nrhs)
-  // or Xv (resized), whichever is given.
+  // or Xv (resized), whichever is given (device requests: already in d_X).
   bool wait(int64_t id, double* X, std::vector<double>* Xv, ServeResult& out) {
     std::unique_lock<std::mutex> lk(mu_);
     auto it = requests_.find(id);
@@ -233,8 +268,10 @@ class ServeEngine {
     Request* r = it->second.get();
     done_cv_.wait(lk, [&] { return r->done; });
     out = r->res;
-    if (X) std::copy(r->B.begin(), r->B.end(), X);
-    if (Xv) *Xv = std::move(r->B);
+    if (!r->d_X) {
+      if (X) std::copy(r->B.begin(), r->B.end(), X);
+      if (Xv) *Xv = std::move(r->B);
+    }
     requests_.erase(it);
     return out.error == SolverError::None;
   }
@@ -374,9 +411,20 @@ class ServeEngine {
     }
     const int prec = precision_ == Precision::Fp32Ir ? BDLU_FP32 : BDLU_FP64;
     bdlu_factor_stats fs{};
-    int rc = bdlu_gpu_refactor(ctx, id, r.vals.data(), BDLU_MEM_HOST, prec, &fs);
+    const bool dev = r.d_vals != nullptr;
+    int rc = dev ? bdlu_gpu_refactor(ctx, id, r.d_vals, BDLU_MEM_DEVICE, prec, &fs)
+                 : bdlu_gpu_refactor(ctx, id, r.vals.data(), BDLU_MEM_HOST, prec, &fs);
     uint64_t use = id;
     uint64_t tmp_bytes = 0;
+    if (rc == BDLU_E_REPIVOT && dev) {
+      // The re-analysis runs on the host: fetch this request's values.
+      r.vals.resize(pt->canon.nnz());
+      if (bdlu_gpu_plan_values(ctx, id, r.vals.data()) != BDLU_OK) {
+        unuse(w, r.pattern);
+        r.res.error = SolverError::BackendUnavailable;
+        return;
+      }
+    }
     if (rc == BDLU_E_REPIVOT) {
       // This request's values fail the fixed pivot order: re-analyse a private
       // copy with its values (the registered plan stays as it is).
@@ -415,7 +463,12 @@ class ServeEngine {
       ir.tol = 1e-12;
       ir.fp64_fallback = precision_ == Precision::Fp32Ir ? 1 : 0;
       bdlu_solve_stats ss{};
-      rc = bdlu_gpu_solve(ctx, use, r.B.data(), r.nrhs, pt->n, BDLU_MEM_HOST, &ir, &ss);
+      if (dev) {
+        rc = bdlu_gpu_copy(ctx, r.d_X, r.d_B, sizeof(double) * static_cast<uint64_t>(pt->n) * r.nrhs);
+        if (rc == BDLU_OK) rc = bdlu_gpu_solve(ctx, use, r.d_X, r.nrhs, pt->n, BDLU_MEM_DEVICE, &ir, &ss);
+      } else {
+        rc = bdlu_gpu_solve(ctx, use, r.B.data(), r.nrhs, pt->n, BDLU_MEM_HOST, &ir, &ss);
+      }
       r.res.ir_iterations = ss.iterations;
       r.res.ir_fp64_refactor = ss.fp64_refactor;
       r.res.rel_residual = ss.rel_residual;
@@ -496,6 +549,10 @@ int64_t Server::submit(int32_t pattern, const std::vector<double>& values, const
                        int32_t nrhs) {
   return engine_->submit(pattern, values.data(), static_cast<int64_t>(values.size()), B.data(), nrhs);
 }
+int64_t Server::submit_device(int32_t pattern, const double* d_values, const double* d_B, double* d_X,
+                              int32_t nrhs) {
+  return engine_->submit_device(pattern, d_values, d_B, d_X, nrhs);
+}
 bool Server::wait(int64_t request, std::vector<double>& X, ServeResult& result) {
   return engine_->wait(request, nullptr, &X, result);
 }
@@ -510,6 +567,10 @@ int32_t serve_engine_add_pattern(ServeEngine* e, const CooMatrix& m, int32_t dim
 int32_t serve_engine_pattern_n(ServeEngine* e, int32_t p) { return e->pattern_n(p); }
 int64_t serve_engine_submit(ServeEngine* e, int32_t p, const double* v, int64_t nv, const double* B, int32_t k) {
   return e->submit(p, v, nv, B, k);
+}
+int64_t serve_engine_submit_device(ServeEngine* e, int32_t p, const double* d_v, const double* d_B, double* d_X,
+                                   int32_t k) {
+  return e->submit_device(p, d_v, d_B, d_X, k);
 }
 bool serve_engine_wait(ServeEngine* e, int64_t id, double* X, ServeResult& res) { return e->wait(id, X, nullptr, res); }
 SolverError serve_engine_last_error(ServeEngine* e) { return e->last_error(); }

@@ -179,6 +179,11 @@ int bdlu_gpu_solve(bdlu_ctx* ctx, uint64_t plan_id, double* B, int32_t nrhs, int
 int bdlu_gpu_solve_noalloc(bdlu_ctx* ctx, uint64_t plan_id, double* d_B, int32_t nrhs, int64_t ldb,
                            const bdlu_ir_opts* ir, bdlu_solve_stats* stats);
 
+// Device-to-device copy of `bytes` on the context's stream, complete on return
+// (no counterpart in the reference; the serving loop uses it to place a
+// device-resident request's right-hand sides in its solution buffer).
+int bdlu_gpu_copy(bdlu_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+
 // Copies the plan's current canonical-order values (last refactor) to host memory.
 int bdlu_gpu_plan_values(bdlu_ctx* ctx, uint64_t plan_id, double* host_values);
 

@@ -109,6 +109,12 @@ int32_t bdlu_serve_pattern_n(void* s, int32_t pattern);
 // values: nnz entries in the registration's entry order; B: n x nrhs column-major.
 // Returns a request id (>= 0) or -(SolverError).
 int64_t bdlu_serve_submit(void* s, int32_t pattern, const double* vals, int64_t nnz, const double* B, int32_t nrhs);
+// The same with inputs already in device memory: d_vals the registered
+// pattern's canonical values (sorted by (row, col), duplicates summed), d_B the
+// right-hand sides and d_X the solutions (n x nrhs column-major, device
+// memory, kept valid until bdlu_serve_wait returns; its X may be NULL).
+int64_t bdlu_serve_submit_device(void* s, int32_t pattern, const double* d_vals, const double* d_B, double* d_X,
+                                 int32_t nrhs);
 // Blocks until done; X (n x nrhs) receives the solutions. Returns a SolverError value.
 int bdlu_serve_wait(void* s, int64_t request, double* X, bdlu_serve_result* out);
 

@@ -61,6 +61,12 @@ class Server {
   // summed to zero at registration, or an entry the 1e-20 drop rule removed) is
   // a different sparsity pattern and is rejected with AnalysisFailed.
   int64_t submit(int32_t pattern, const std::vector<double>& values, const std::vector<double>& B, int32_t nrhs);
+  // The same with inputs already in device memory (HBM): d_values in canonical
+  // order (sorted by (row, col), duplicates summed: the registered pattern's
+  // canonical entries), d_B the right-hand sides and d_X the solutions, n x nrhs
+  // column-major. The caller keeps the three buffers valid until wait() returns;
+  // wait()'s X is left empty.
+  int64_t submit_device(int32_t pattern, const double* d_values, const double* d_B, double* d_X, int32_t nrhs);
   // Blocks until the request is done; X gets n x nrhs (column-major).
   bool wait(int64_t request, std::vector<double>& X, ServeResult& result);
 
