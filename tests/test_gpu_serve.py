@@ -189,3 +189,36 @@ def test_side_streams_match_one_stream(gpu, monkeypatch, large):
             assert np.array_equal(xs, x1), "side-stream refactor differs from one stream"
             assert rs["ir_iterations"] == r1["ir_iterations"]
     _say(f"  n={n} requests={len(reqs)} equal")
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_device_resident_requests_equal_host_ones(gpu, precision):
+    """Requests whose canonical values, right-hand sides and solution buffers
+    are in device memory (submit_device) give exactly the solutions and
+    statistics of the same requests handed over as host buffers."""
+    bd = gpu
+    prec = bd.Precision(precision)
+    _say(f"case: device-resident vs host-buffer requests precision={prec.name}")
+    pats = _patterns(bd)
+    reqs = _requests(bd, pats, 12, seed=21 + precision)
+    srv = bd.Server(slots=4, hbm_budget_gb=8.0, precision=prec)
+    pid = [srv.add_pattern(r, c, v, n) for (n, r, c, v) in pats]
+    host = [srv.wait(srv.submit(pid[p], vals, B)) for (p, vals, B) in reqs]
+    keep, ids = [], []
+    for p, vals, B in reqs:
+        n, r, c, _ = pats[p]
+        order = bd.canonical_order(r, c)
+        dv = bd.DeviceArray.from_numpy(vals[order])
+        dB = bd.DeviceArray.from_numpy(B.ravel(order="F"))
+        dX = bd.DeviceArray(B.size)
+        keep.append((dv, dB, dX))
+        ids.append(srv.submit_device(pid[p], dv.address, dB.address, dX.address, B.shape[1]))
+    dev = [srv.wait(i)[1] for i in ids]
+    srv.close()
+    for (p, vals, B), (xh, rh), rd, (_, dB, dX) in zip(reqs, host, dev, keep):
+        xd = dX.numpy().reshape(B.shape, order="F")
+        assert rh["error"] == bd.SolverError.None_ and rd["error"] == bd.SolverError.None_
+        assert np.array_equal(xd, xh.reshape(B.shape, order="F")), "device-resident request differs"
+        assert np.array_equal(dB.numpy(), B.ravel(order="F")), "right-hand sides overwritten"
+        assert rd["ir_iterations"] == rh["ir_iterations"] and rd["rel_residual"] == rh["rel_residual"]
+    _say(f"  {len(reqs)} requests equal")
