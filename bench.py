@@ -11,7 +11,8 @@ HBM budget, with symbolic state and factors resident. Reported:
                    factors + FP64 refinement to ||b-Ax||/||b|| < 1e-12; a step
                    is a batch of 100 requests
   p50/p99          arrival-to-result latency of 2000 Poisson arrivals at 0.8 x
-                   the FP64 capacity (same rate for both precisions)
+                   the smallest capacity of the three passes (the same rate for
+                   every pass)
   fp64_plain       the same for the plain FP64 GPU path (no refinement, the
                    reference's default; its residual misses the 1e-12 gate)
   fp64_ir          FP64 factors with FP64 refinement (the FP64 configuration
@@ -398,8 +399,8 @@ def bench_serving(bd, args):
       capacity   K x 100 requests queued at once (closed loop), timed from the
                  first submission to the last result: the refactor+solve/s value;
       open loop  the --requests stream with Poisson arrivals at a fixed rate
-                 (0.8 x the measured FP64 capacity, the same rate for both
-                 precisions): arrival-to-result p50/p99 latency.
+                 (0.8 x the smallest measured capacity, the same rate for every
+                 pass): arrival-to-result p50/p99 latency.
     Three passes: plain FP64, FP64 with FP64 refinement (the FP64 configuration
     that meets the 1e-12 residual gate) and FP32 factors with FP64 refinement.
     FP32+IR solutions of the first 100 open-loop requests are compared with the
@@ -442,12 +443,19 @@ def bench_serving(bd, args):
     # Plain FP64 sets the arrival rate; FP64 with FP64 refinement is the FP64
     # configuration that meets the 1e-12 residual gate (the reference's
     # refinement, USERALGO_REFINE_ITERS, here up to 10 steps).
-    for name, prec, refine in (("FP64", bd.Precision.FP64, None), ("FP64_IR", bd.Precision.FP64, "10"),
-                               ("FP32_IR", bd.Precision.FP32_IR, None)):
+    passes = (("FP64", bd.Precision.FP64, None), ("FP64_IR", bd.Precision.FP64, "10"),
+              ("FP32_IR", bd.Precision.FP32_IR, None))
+
+    def set_refine(refine):
         if refine is None:
             os.environ.pop("USERALGO_REFINE_ITERS", None)
         else:
             os.environ["USERALGO_REFINE_ITERS"] = refine
+
+    servers, caps = {}, {}
+    # Phase 1, per pass: registration, warm-up, closed-loop capacity.
+    for name, prec, refine in passes:
+        set_refine(refine)
         srv = bd.Server(slots=args.slots, hbm_budget_gb=64.0, precision=prec)
         t = time.time()
         pid = []
@@ -476,11 +484,18 @@ def bench_serving(bd, args):
                 step_ends.append(time.perf_counter() - t0)
         wall_cap = time.perf_counter() - t0
         cap = ncap / wall_cap
-        if lam is None:
-            lam = 0.8 * cap
         step_ms = np.diff(np.concatenate([[0.0], step_ends])) * 1e3
-        log(f"  {name}: closed-loop capacity {cap:.1f} req/s over {ncap} requests; open loop at {lam:.1f} req/s")
-        # open loop
+        log(f"  {name}: closed-loop capacity {cap:.1f} req/s over {ncap} requests")
+        servers[name] = (srv, pid)
+        caps[name] = (cap, wall_cap, step_ms, t_an)
+    # Phase 2: open loop, every pass at the same Poisson rate, 0.8 x the smallest
+    # capacity (so that no pass is offered more than it can serve).
+    lam = 0.8 * min(c[0] for c in caps.values())
+    for name, prec, refine in passes:
+        set_refine(refine)
+        srv, pid = servers[name]
+        cap, wall_cap, step_ms, t_an = caps[name]
+        log(f"  {name}: open loop at {lam:.1f} req/s")
         nreq = args.requests
         arrivals = np.cumsum(gaps[:nreq] / lam)
         oids = [None] * nreq
