@@ -713,23 +713,39 @@ constexpr int kNb = 64;
 // row-major (upper). Rows/columns at or past b are an identity pad. Each panel
 // keeps its pair (the tail solve applies them to its diagonal tiles).
 //
-// Three wavefronts, the inverses formed while the LU runs. The LU is one
-// dependent chain of 64 steps, so it runs on a single wave (wave 0) with no
-// barriers: lane i keeps row i in registers, and at step k every lane takes
-// pivot row k from lane k with readlane (scalar registers, compile-time lane)
-// and updates its own row. After step k, column k of the working matrix is
-// final in both factors (L(i,k) below the diagonal, U(i,k) on and above it);
-// each lane publishes its entry to a column-major LDS copy (one store per lane
-// and step). Wave 1 forms L11^{-1} column by column (lane j: x_i -= L(i,k) x_k
-// for i > k, k ascending, as forward substitution) and wave 2 forms U11^{-1}
-// row by row by bordering (lane i: U^{-1}(i,k) = -(sum_{m<k} U^{-1}(i,m)
-// U(m,k)) / U(k,k), k ascending), reading the published entries at
-// wave-uniform addresses (broadcast reads). The waves meet at a workgroup
-// barrier every 8 LU steps (straight-line code, no polling loops inside the
-// unrolled steps): wave 0 passes barrier t after step 8t-1, waves 1 and 2
-// before their steps 8(t-1), so they run one block of 8 steps behind the LU
-// with less work per step, and every wave executes the same barriers.
-constexpr int kDiagThreads = 3 * kNb;
+// Blocked LU of the 64x64 block in LDS, four 16-column steps, one workgroup of
+// four wavefronts. Step K (columns c0 = 16K .. c0+15):
+//   wave 0: the column panel's unblocked LU over all rows from c0 down (lane i
+//     keeps row i's 16 entries in registers; at step k every lane takes pivot
+//     row c0+k from lane c0+k with readlane and updates its own row), so the
+//     16x16 diagonal factors and L_IK = A_IK U_KK^{-1} for every block below
+//     come out of the same 16 dependent steps; column k of the panel is
+//     published to LDS after step k;
+//   wave 1 (lanes j >= c0+16): the row panel U_KJ = L_KK^{-1} A_KJ by forward
+//     substitution down column j, from the published columns;
+//   wave 3 (lanes j < 16): column j of the diagonal block's L_KK^{-1}, the same way;
+//   waves 1 and 3 run up to kDiagStep steps behind wave 0: every wave passes a
+//     workgroup barrier every kDiagStep steps (wave 0 after its steps, the
+//     others before theirs), so every wave executes the same barriers;
+//   wave 2, between the same barriers: U_{K-1,K-1}^{-1} (column by column,
+//     backward substitution) and whichever off-diagonal blocks of the inverses
+//     have become computable (below);
+//   then the trailing update A_IJ -= L_IK U_KJ for I, J > K on MFMA, 16x16x16
+//     block products spread over the four waves.
+// Pivot reciprocals use the hardware reciprocal with Newton steps.
+// The inverses' off-diagonal blocks follow from the diagonal ones:
+// X_IJ = -Linv_II sum_{M=J}^{I-1} L_IM X_MJ (lower) and Y_IJ = -Uinv_II
+// sum_{M=I+1}^{J} U_IM Y_MJ (upper), one MFMA chain each. Block row I of
+// L^{-1} and block column J of U^{-1} are computable once panel I (J) is done,
+// so wave 2 forms them during the next panel; only L^{-1}'s last block row and
+// U^{-1}'s last block column remain after the LU.
+constexpr int kDiagThreads = 4 * kNb;
+constexpr int kDiagLF = kNb + 1;  // LDS column length of F and G (column-major, padded)
+constexpr int kDiagStep = 4;      // panel steps between barriers
+template <typename T>
+constexpr std::size_t diag_lds_elems() {
+  return 2 * kNb * kDiagLF + 16 + 16 / sizeof(T);  // F, G, a 16-element sink, one int32 (status)
+}
 
 __device__ __forceinline__ float readlane_t(float v, int src) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), src));
@@ -739,189 +755,6 @@ __device__ __forceinline__ double readlane_t(double v, int src) {
   const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(u)), src);
   const uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(u >> 32)), src);
   return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
-}
-
-constexpr int kDiagLC = kNb + 4;  // Lc[k*68 + i] = LU(i, k); columns start on 16-byte boundaries
-constexpr int kDiagStep = 8;      // LU steps between barriers
-template <typename T>
-constexpr std::size_t diag_lds_elems() {
-  return kNb * kDiagLC + 16 / sizeof(T);  // + one int32 (status)
-}
-
-// Body of the diagonal LU + inverses for a workgroup of 192 or 256 threads
-// (wave 3, when present, only takes part in the barriers). fc: LDS of
-// diag_lds_elems<T>() elements, 16-byte aligned.
-template <typename T>
-__device__ __forceinline__ void diag_lu_inv_body(T* __restrict__ S, int64_t lds, int32_t p, int32_t b, double limit,
-                                                 int32_t* __restrict__ status, T* __restrict__ inv, T* fc) {
-  constexpr int LC = kDiagLC;
-  constexpr int kV = 16 / sizeof(T);  // elements per 16-byte store
-  using VT = T __attribute__((ext_vector_type(16 / sizeof(T))));
-  T* Lc = fc;
-  int32_t* st_sh = reinterpret_cast<int32_t*>(fc + kNb * LC);
-  const int lane = threadIdx.x & (kNb - 1);
-  const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-  if (threadIdx.x == 0) *st_sh = 0;
-  __syncthreads();
-  int32_t st = 0;
-  if (w == 0) {
-    const int i = lane;
-    T a[kNb];
-    const T* src = S + int64_t(p) * lds + p + (i < b ? i : 0);
-    if constexpr (sizeof(T) == 4) {
-      // Clamped addresses, unconditional loads, all in flight before the first
-      // use (a conditional load branches and waits; a shared destination
-      // register serializes the loads).
-#pragma unroll
-      for (int j = 0; j < kNb; ++j) a[j] = __builtin_nontemporal_load(src + int64_t(j < b ? j : 0) * lds);
-#pragma unroll
-      for (int j = 0; j < kNb; ++j) a[j] = (i < b && j < b) ? a[j] : (i == j ? T(1) : T(0));
-    } else {
-#pragma unroll
-      for (int j = 0; j < kNb; ++j) {
-        // Clamped address, unconditional load (a conditional load branches and waits).
-        const T v = src[int64_t(j < b ? j : 0) * lds];
-        a[j] = (i < b && j < b) ? v : (i == j ? T(1) : T(0));
-      }
-    }
-    // The next pivot's reciprocal is started as soon as that pivot is final
-    // (after its own column's update), so the division overlaps the rest of
-    // the step. FP32 reads the pivot row's entries with readlane in groups of
-    // 8 into distinct scalar registers before their FMAs, so the wait between a
-    // readlane and the VALU op that uses its result is paid once per group
-    // (FP64 keeps one readlane pair per FMA: the groups were slower inside the
-    // fused update kernel).
-    T akk = readlane_t(a[0], 0);
-    T rp = T(1) / akk;
-#pragma clang loop unroll(full)
-    for (int k = 0; k < kNb; ++k) {
-      if (akk == T(0)) st |= kStZeroPivot;
-      const bool below = i > k;
-      const T lik = a[k] * rp;
-      // Rows at or above k take the update with a zero multiplier, which leaves
-      // them unchanged (up to the sign of an exact zero) without a select per element.
-      const T m = below ? -lik : T(0);
-      if (k + 1 < kNb) {
-        a[k + 1] = tfma(m, readlane_t(a[k + 1], k), a[k + 1]);
-        akk = readlane_t(a[k + 1], k + 1);
-        rp = T(1) / akk;
-      }
-      if constexpr (sizeof(T) == 4) {
-#pragma clang loop unroll(full)
-        for (int g = 0; g < kNb / 8; ++g) {
-          T r[8];
-#pragma clang loop unroll(full)
-          for (int u = 0; u < 8; ++u) r[u] = (8 * g + u >= k + 2) ? readlane_t(a[8 * g + u], k) : T(0);
-#pragma clang loop unroll(full)
-          for (int u = 0; u < 8; ++u)
-            if (8 * g + u >= k + 2) a[8 * g + u] = tfma(m, r[u], a[8 * g + u]);
-        }
-      } else {
-#pragma unroll
-        for (int j = k + 2; j < kNb; ++j) a[j] = tfma(m, readlane_t(a[j], k), a[j]);
-      }
-      a[k] = below ? lik : a[k];
-      Lc[k * LC + i] = a[k];  // column k, final in both factors
-      if ((k + 1) % kDiagStep == 0) __syncthreads();
-    }
-    // |L| check on the finished factor (a NaN fails it).
-    T lmax = T(0);
-#pragma unroll
-    for (int j = 0; j < kNb; ++j) {
-      const T al = (j < i) ? tabs(a[j]) : T(0);
-      lmax = (al <= lmax || lmax != lmax) ? lmax : al;
-    }
-    if (!(lmax <= static_cast<T>(limit))) st |= kStCheck;
-    // The factor back to S, coalesced down each column.
-    if (i < b) {
-      T* dst = S + int64_t(p) * lds + p + i;
-#pragma unroll
-      for (int c = 0; c < kNb; ++c)
-        if (c < b) dst[int64_t(c) * lds] = a[c];
-    }
-  } else if (w == 1) {
-    // Column j of L11^{-1}. FP32 reads column k of L as 16-byte broadcast
-    // vectors; FP64 element by element (its vectors in flight would cost the
-    // fused update kernel a wave per SIMD in registers).
-    const int j = lane;
-    T x[kNb];
-#pragma unroll
-    for (int q = 0; q < kNb; ++q) x[q] = (q == j) ? T(1) : T(0);
-#pragma clang loop unroll(full)
-    for (int k = 0; k < kNb - 1; ++k) {
-      if (k % kDiagStep == 0) __syncthreads();
-      if constexpr (sizeof(T) == 4) {
-        const T xk = x[k];
-#pragma clang loop unroll(full)
-        for (int t = 0; t < kNb / kV; ++t) {
-          if (kV * t + kV - 1 <= k) continue;
-          const VT lv = *reinterpret_cast<const VT*>(Lc + k * LC + kV * t);
-#pragma clang loop unroll(full)
-          for (int e = 0; e < kV; ++e)
-            if (kV * t + e > k) x[kV * t + e] = tfma(-lv[e], xk, x[kV * t + e]);
-        }
-      } else {
-#pragma unroll
-        for (int q = k + 1; q < kNb; ++q) x[q] = tfma(-Lc[k * LC + q], x[k], x[q]);
-      }
-      __builtin_amdgcn_sched_barrier(0);  // one step's reads in flight at a time (registers)
-    }
-#pragma unroll
-    for (int q = 0; q < kNb; ++q) inv[q * kNb + j] = x[q];
-  } else if (w == 2) {
-    // Row i of U11^{-1}: entries left of the diagonal stay zero, so each sum
-    // runs over m = i..k-1 in effect (four partial sums, combined in a fixed order).
-    const int i = lane;
-    T z[kNb];
-#pragma unroll
-    for (int q = 0; q < kNb; ++q) z[q] = T(0);
-#pragma clang loop unroll(full)
-    for (int k = 0; k < kNb; ++k) {
-      if (k % kDiagStep == 0) __syncthreads();
-      T ps[4] = {T(0), T(0), T(0), T(0)};
-      if constexpr (sizeof(T) == 4) {
-#pragma clang loop unroll(full)
-        for (int t = 0; t < kNb / kV; ++t) {
-          if (kV * t >= k) continue;
-          const VT lv = *reinterpret_cast<const VT*>(Lc + k * LC + kV * t);
-#pragma clang loop unroll(full)
-          for (int e = 0; e < kV; ++e) {
-            const int mm = kV * t + e;
-            if (mm < k) ps[mm & 3] = tfma(z[mm], lv[e], ps[mm & 3]);
-          }
-        }
-      } else {
-#pragma unroll
-        for (int m = 0; m < k; ++m) ps[m & 3] = tfma(z[m], Lc[k * LC + m], ps[m & 3]);
-      }
-      const T sum = (ps[0] + ps[1]) + (ps[2] + ps[3]);
-      z[k] = (i < k ? -sum : (i == k ? T(1) : T(0))) / Lc[k * LC + k];
-      __builtin_amdgcn_sched_barrier(0);
-    }
-    const int j = i;
-    T* dst = inv + kNb * kNb + j * kNb;
-#pragma unroll
-    for (int q = 0; q < kNb; q += kV) {
-      VT v;
-#pragma unroll
-      for (int e = 0; e < kV; ++e) v[e] = z[q + e];
-      *reinterpret_cast<VT*>(dst + q) = v;
-    }
-  } else {
-#pragma unroll
-    for (int t = 0; t < kNb / kDiagStep; ++t) __syncthreads();
-  }
-  if (st) atomicOr(st_sh, st);
-  __syncthreads();
-  if (threadIdx.x == 0 && *st_sh) atomicOr(status, *st_sh);
-}
-
-template <typename T>
-__global__ __launch_bounds__(kDiagThreads) void k_diag_lu_inv(T* __restrict__ S, int64_t lds, int32_t p, int32_t b,
-                                                              double limit, int32_t* __restrict__ status,
-                                                              T* __restrict__ inv) {
-  __shared__ __attribute__((aligned(16))) T fc[diag_lds_elems<T>()];
-  diag_lu_inv_body<T>(S, lds, p, b, limit, status, inv, fc);
 }
 
 // Inputs of both forms: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16].
@@ -942,6 +775,249 @@ struct MfmaOps<float> {
   __device__ static v4 mma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
   __device__ static int row(int lk, int r) { return 4 * lk + r; }
 };
+
+// 1/x from the hardware reciprocal refined by Newton steps (one for FP32, two
+// for FP64): a few dependent instructions instead of the IEEE division
+// sequence, within an ulp or so of 1/x; 1/0 stays infinite.
+__device__ __forceinline__ float fast_rcp(float x) {
+  float r = __builtin_amdgcn_rcpf(x);
+  return __builtin_fmaf(__builtin_fmaf(-x, r, 1.0f), r, r);
+}
+__device__ __forceinline__ double fast_rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
+  return __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
+}
+
+// 16x16 block products on column-major LDS arrays (column length kDiagLF):
+// acc (the output layout of MfmaOps) += sign * X Y with X at rows xr, columns
+// xc of X0 and Y at rows yr, columns yc of Y0. MX / MY read a diagonal block
+// of G as one triangle: 0 the whole block, 1 the unit lower triangle (L^{-1}),
+// 2 the upper triangle with the diagonal (U^{-1}).
+template <typename T>
+__device__ __forceinline__ T tri_pick(T v, int r, int c, int mode) {
+  return mode == 0 ? v : mode == 1 ? (r > c ? v : (r == c ? T(1) : T(0))) : (r <= c ? v : T(0));
+}
+template <typename T, int MX = 0, int MY = 0>
+__device__ __forceinline__ void blk_mma(const T* X0, int xr, int xc, const T* Y0, int yr, int yc, T sign,
+                                        typename MfmaOps<T>::v4& acc, int li, int lk) {
+#pragma unroll
+  for (int k0 = 0; k0 < 16; k0 += 4) {
+    const T a = sign * tri_pick(X0[(xc + k0 + lk) * kDiagLF + xr + li], li, k0 + lk, MX);
+    const T b = tri_pick(Y0[(yc + li) * kDiagLF + yr + k0 + lk], k0 + lk, li, MY);
+    acc = MfmaOps<T>::mma(a, b, acc);
+  }
+}
+template <typename T>
+__device__ __forceinline__ void blk_load(const T* C0, int r0, int c0, typename MfmaOps<T>::v4& acc, int li, int lk) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) acc[r] = C0[(c0 + li) * kDiagLF + r0 + MfmaOps<T>::row(lk, r)];
+}
+template <typename T>
+__device__ __forceinline__ void blk_store(T* C0, int r0, int c0, const typename MfmaOps<T>::v4& acc, int li, int lk) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) C0[(c0 + li) * kDiagLF + r0 + MfmaOps<T>::row(lk, r)] = acc[r];
+}
+
+// Body of the diagonal LU + inverses for a workgroup of 256 threads. fc: LDS
+// of diag_lds_elems<T>() elements, 16-byte aligned.
+template <typename T>
+__device__ __forceinline__ void diag_lu_inv_body(T* __restrict__ S, int64_t lds, int32_t p, int32_t b, double limit,
+                                                 int32_t* __restrict__ status, T* __restrict__ inv, T* fc) {
+  using V = typename MfmaOps<T>::v4;
+  constexpr int LF = kDiagLF;
+  T* F = fc;                   // F[j*LF + i] = A(i, j), then the factors
+  // G: the inverses in one array, L^{-1} below the diagonal and U^{-1} on and
+  // above it (in the diagonal blocks too: L^{-1}'s unit diagonal is implied).
+  T* G = fc + kNb * LF;
+  T* sink = fc + 2 * kNb * LF;  // where lanes without a column of their own store theirs
+  int32_t* st_sh = reinterpret_cast<int32_t*>(fc + 2 * kNb * LF + 16);
+  const int tid = threadIdx.x;
+  const int lane = tid & (kNb - 1);
+  const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(tid >> 6));
+  const int li = lane & 15, lk = lane >> 4;
+  if (tid == 0) *st_sh = 0;
+  __syncthreads();
+  // The block into LDS, identity past b (coalesced down columns; clamped
+  // addresses, unconditional loads).
+#pragma unroll
+  for (int u = 0; u < kNb * kNb / kDiagThreads; ++u) {
+    const int e = u * kDiagThreads + tid;
+    const int i = e & (kNb - 1), j = e >> 6;
+    const T v = S[int64_t(p + (j < b ? j : 0)) * lds + p + (i < b ? i : 0)];
+    F[j * LF + i] = (i < b && j < b) ? v : (i == j ? T(1) : T(0));
+  }
+  __syncthreads();
+  // Pieces of the inverses (column j of U_bb^{-1}; block (I,J) of L^{-1} or of
+  // U^{-1} off the diagonal), run by wave 2 between the panel steps' barriers
+  // as soon as their inputs are final, the rest after the last panel.
+  auto u_leaf = [&](int bb) {
+    // Lane j < 16: column j of U_bb^{-1} by backward substitution, column
+    // oriented (each step one multiply on the critical path); every lane
+    // stores unconditionally, as in the panel.
+    const int b0 = 16 * bb;
+    const int j = lane;
+    T x[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) x[q] = q == j ? T(1) : T(0);
+#pragma unroll
+    for (int c = 15; c >= 0; --c) {
+      x[c] = x[c] * fast_rcp(F[(b0 + c) * LF + b0 + c]);
+#pragma unroll
+      for (int r = 0; r < c; ++r) x[r] = tfma(-F[(b0 + c) * LF + b0 + r], x[c], x[r]);
+      if (c % 2 == 0) __builtin_amdgcn_sched_barrier(0);  // LDS reads of 2 steps in flight at a time (registers)
+    }
+    T* col = j < 16 ? G + (b0 + j) * LF + b0 : sink;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) (q > j ? sink : col)[q] = x[q];  // the upper triangle, with the diagonal
+  };
+  auto l_item = [&](int I, int J) {  // X_IJ = -Linv_II sum_{M=J}^{I-1} L_IM X_MJ
+    V acc = V{T(0), T(0), T(0), T(0)};
+    blk_mma<T, 0, 1>(F, 16 * I, 16 * J, G, 16 * J, 16 * J, T(1), acc, li, lk);
+    for (int M = J + 1; M < I; ++M) blk_mma<T>(F, 16 * I, 16 * M, G, 16 * M, 16 * J, T(1), acc, li, lk);
+    blk_store<T>(G, 16 * I, 16 * J, acc, li, lk);
+    acc = V{T(0), T(0), T(0), T(0)};
+    blk_mma<T, 1, 0>(G, 16 * I, 16 * I, G, 16 * I, 16 * J, T(-1), acc, li, lk);
+    blk_store<T>(G, 16 * I, 16 * J, acc, li, lk);
+  };
+  auto u_item = [&](int I, int J) {  // Y_IJ = -Uinv_II sum_{M=I+1}^{J} U_IM Y_MJ
+    V acc = V{T(0), T(0), T(0), T(0)};
+    for (int M = I + 1; M < J; ++M) blk_mma<T>(F, 16 * I, 16 * M, G, 16 * M, 16 * J, T(1), acc, li, lk);
+    blk_mma<T, 0, 2>(F, 16 * I, 16 * J, G, 16 * J, 16 * J, T(1), acc, li, lk);
+    blk_store<T>(G, 16 * I, 16 * J, acc, li, lk);
+    acc = V{T(0), T(0), T(0), T(0)};
+    blk_mma<T, 2, 0>(G, 16 * I, 16 * I, G, 16 * I, 16 * J, T(-1), acc, li, lk);
+    blk_store<T>(G, 16 * I, 16 * J, acc, li, lk);
+  };
+  int32_t st = 0;
+#pragma unroll 1
+  for (int K = 0; K < 4; ++K) {
+    const int c0 = 16 * K;
+    if (w == 0) {
+      // Column panel: rows c0..63 of columns c0..c0+15 (lanes above c0 only
+      // follow along). The next pivot's reciprocal starts as soon as that
+      // pivot is final, so the division overlaps the rest of the step.
+      T a[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) a[q] = F[(c0 + q) * LF + lane];
+      T akk = readlane_t(a[0], c0);
+      T rp = fast_rcp(akk);
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        if (akk == T(0)) st |= kStZeroPivot;
+        const bool below = lane > c0 + k;
+        const T lik = a[k] * rp;
+        const T m = below ? -lik : T(0);
+        if (k + 1 < 16) {
+          a[k + 1] = tfma(m, readlane_t(a[k + 1], c0 + k), a[k + 1]);
+          akk = readlane_t(a[k + 1], c0 + k + 1);
+          rp = fast_rcp(akk);
+        }
+#pragma unroll
+        for (int j = k + 2; j < 16; ++j) a[j] = tfma(m, readlane_t(a[j], c0 + k), a[j]);
+        a[k] = below ? lik : a[k];
+        if (lane >= c0) F[(c0 + k) * LF + lane] = a[k];  // column k: final in both factors
+        if ((k + 1) % kDiagStep == 0) __syncthreads();
+      }
+    } else if (w == 1 || w == 3) {
+      // Wave 1, lane j >= c0+16: column j of the row panel; wave 3, lane j <
+      // 16: column j of L_KK^{-1}, from e_j (the identity is written to its
+      // place first, so both waves run the same code on different columns).
+      // Every lane stores its result unconditionally (lanes without a column
+      // into the sink): a store on some lanes only lets the compiler sink the
+      // arithmetic into that branch while every step's LDS reads stay above
+      // it, all live at once.
+      const int j = lane;
+      const bool mine = w == 1 ? j >= c0 + 16 : j < 16;
+      T* col = !mine ? sink : w == 1 ? F + j * LF + c0 : G + (c0 + j) * LF + c0;
+      T x[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const T v = F[(w == 1 ? j : c0) * LF + c0 + q];  // wave 3: any in-bounds address
+        x[q] = w == 1 ? v : (q == j ? T(1) : T(0));
+      }
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        if (k % kDiagStep == 0) __syncthreads();
+#pragma unroll
+        for (int q = k + 1; q < 16; ++q) x[q] = tfma(-F[(c0 + k) * LF + c0 + q], x[k], x[q]);
+      }
+      // Wave 3 keeps the strictly lower entries (q > j); the rest of the
+      // column of G belongs to U_KK^{-1}.
+#pragma unroll
+      for (int q = 0; q < 16; ++q) (w == 3 && q <= j ? sink : col)[q] = x[q];
+    } else {
+      // Wave 2, during panel K: U_{K-1}^{-1}, then the blocks of L^{-1}'s row
+      // K-1 and U^{-1}'s column K-1 (their inputs were final when panel K
+      // started; panel K writes none of what they read or write).
+#pragma unroll 1
+      for (int t = 0; t < 16 / kDiagStep; ++t) {
+        __syncthreads();
+        if (K >= 1 && t == 0) u_leaf(K - 1);
+        if (K >= 2 && t == 1)
+          for (int J = 0; J < K - 1; ++J) l_item(K - 1, J);
+        if (K >= 2 && t == 2)
+          for (int I = K - 2; I >= 0; --I) u_item(I, K - 1);
+      }
+    }
+    __syncthreads();
+    // Trailing update A_IJ -= L_IK U_KJ, I, J > K.
+    const int nr = 3 - K;
+    for (int pr = w; pr < nr * nr; pr += 4) {
+      const int I = K + 1 + pr % nr, J = K + 1 + pr / nr;
+      V acc;
+      blk_load<T>(F, 16 * I, 16 * J, acc, li, lk);
+      blk_mma<T>(F, 16 * I, c0, F, c0, 16 * J, T(-1), acc, li, lk);
+      blk_store<T>(F, 16 * I, 16 * J, acc, li, lk);
+    }
+    __syncthreads();
+  }
+  // L^{-1}'s last block row (waves 0, 1, 3) and U^{-1}'s last block column
+  // (wave 2, after U_33^{-1}).
+  if (w == 2) {
+    u_leaf(3);
+    for (int I = 2; I >= 0; --I) u_item(I, 3);
+  } else {
+    l_item(3, w == 3 ? 2 : w);
+  }
+  // |L| check on the finished factor (a NaN fails it) and the status, then the
+  // factor back to S (coalesced down columns) and the inverses, row-major, to
+  // inv, with no barrier after the stores.
+  T lmax = T(0);
+#pragma unroll
+  for (int u = 0; u < kNb * kNb / kDiagThreads; ++u) {
+    const int e = u * kDiagThreads + tid;
+    const int i = e & (kNb - 1), j = e >> 6;
+    const T al = i > j ? tabs(F[j * LF + i]) : T(0);
+    lmax = (al <= lmax || lmax != lmax) ? lmax : al;
+  }
+  if (!(lmax <= static_cast<T>(limit))) st |= kStCheck;
+  if (st) atomicOr(st_sh, st);
+  __syncthreads();
+  if (tid == 0 && *st_sh) atomicOr(status, *st_sh);
+#pragma unroll
+  for (int u = 0; u < kNb * kNb / kDiagThreads; ++u) {
+    const int e = u * kDiagThreads + tid;
+    const int i = e & (kNb - 1), j = e >> 6;
+    if (i < b && j < b) S[int64_t(p + j) * lds + p + i] = F[j * LF + i];
+  }
+#pragma unroll
+  for (int u = 0; u < kNb * kNb / kDiagThreads; ++u) {
+    const int e = u * kDiagThreads + tid;
+    const int i = e >> 6, j = e & (kNb - 1);
+    const T g = G[j * LF + i];
+    inv[e] = i > j ? g : (i == j ? T(1) : T(0));
+    inv[kNb * kNb + e] = i <= j ? g : T(0);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kDiagThreads) void k_diag_lu_inv(T* __restrict__ S, int64_t lds, int32_t p, int32_t b,
+                                                              double limit, int32_t* __restrict__ status,
+                                                              T* __restrict__ inv) {
+  __shared__ __attribute__((aligned(16))) T fc[diag_lds_elems<T>()];
+  diag_lu_inv_body<T>(S, lds, p, b, limit, status, inv, fc);
+}
 
 // One tile product job: C(MxN) op= A(MxK) B(KxN), K <= 64, element (i,j) of A
 // at A[i*a_r + j*a_c] (B likewise); C is column-major (c_r must be 1, column
