@@ -270,6 +270,10 @@ struct bdlu_ctx {
   // panels with K = 128 bulk updates and the next diagonal LU fused into them,
   // 1 one bulk update per panel, 0 the next diagonal LU in its own launch.
   int lookahead = 2;
+  // Largest persistent grid of the dense-block solve (BLOCKDAGLU_GPU_SEG_GRID_MAX):
+  // its workgroups wait for one another, so where several contexts share the
+  // GPU (serving) a smaller grid leaves room for the others' work.
+  int32_t seg_grid_max = 1024;
   std::vector<hipEvent_t> events;  // fork/join points of a refactor
   hipEvent_t event(std::size_t i) {
     while (events.size() <= i) {
@@ -724,6 +728,7 @@ void upload_into(bdlu_ctx* c, const bdlu_plan_desc& d, Plan& P) {
     }
   }
   flush();
+  P.sched.seg_grid = std::min(P.sched.seg_grid, c->seg_grid_max);
   P.nticket = static_cast<int32_t>(P.phases.size()) + 2 * d.nsparse;
   Work& W = P.work;
   W.tickets = A.get<int32_t>(P.nticket);
@@ -889,8 +894,11 @@ void autotune_factor(Plan& P, hipStream_t st, const std::function<void()>& launc
   P.sched.dense_lpt = pick.dense_lpt;
 }
 
-void autotune_solve(Plan& P, hipStream_t st, int32_t kt) {
-  std::vector<int32_t> grids{256, 512, 1024};
+void autotune_solve(Plan& P, hipStream_t st, int32_t kt, int32_t grid_max) {
+  std::vector<int32_t> grids;
+  for (int32_t g : {256, 512, 1024})
+    if (g <= grid_max) grids.push_back(g);
+  if (grids.empty()) grids.push_back(grid_max);
   auto run = [&] { block_solve_launches(P, P.work, st, P.work.b, P.work.x, kt); };
   run();  // warm-up
   float best = 1e30f;
@@ -1236,6 +1244,7 @@ int bdlu_gpu_create(int device, uint64_t hbm_budget_bytes, bdlu_ctx** out) {
   if (const char* e = std::getenv("BLOCKDAGLU_GPU_AUTOTUNE")) c->autotune = std::atoi(e) != 0;
   if (const char* e = std::getenv("BLOCKDAGLU_GPU_STREAMS")) c->fork = std::atoi(e) > 1;
   if (const char* e = std::getenv("BLOCKDAGLU_GPU_LOOKAHEAD")) c->lookahead = std::max(0, std::min(2, std::atoi(e)));
+  if (const char* e = std::getenv("BLOCKDAGLU_GPU_SEG_GRID_MAX")) c->seg_grid_max = std::max(1, std::atoi(e));
   bool ok = hipSetDevice(device) == hipSuccess &&
             hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
   if (ok) c->scratch = scratch_address(device);
@@ -1375,7 +1384,7 @@ int bdlu_gpu_solve(bdlu_ctx* c, uint64_t id, double* B, int32_t nrhs, int64_t ld
         HIPCHECK(hipMemcpyAsync(W.b + k * n, B + (c0 + k) * ldb, sizeof(double) * n, in, st));
       if (kt > kc) bdk::launch_fill_f64(st, W.b + kc * n, (kt - kc) * n, 0.0);
       if (c->autotune && !c->prof && !P.sched.solve_tuned) {
-        autotune_solve(P, st, kt);
+        autotune_solve(P, st, kt, c->seg_grid_max);
         P.sched.solve_tuned = true;
         P.drop_graphs();
       }
@@ -1536,6 +1545,18 @@ int bdlu_gpu_set_side_streams(bdlu_ctx* c, int enable) {
   std::lock_guard<std::mutex> g(c->mu);
   c->fork = enable != 0;
   for (auto& kv : c->plans) kv.second->drop_graphs();  // recapture with the new launch structure
+  return BDLU_OK;
+}
+
+int bdlu_gpu_set_solve_grid_max(bdlu_ctx* c, int32_t max_workgroups) {
+  if (!c || max_workgroups < 1) return BDLU_E_INVALID;
+  std::lock_guard<std::mutex> g(c->mu);
+  c->seg_grid_max = max_workgroups;
+  for (auto& kv : c->plans) {
+    Plan& P = *kv.second;
+    P.sched.seg_grid = std::min(P.sched.seg_grid, max_workgroups);
+    P.drop_graphs();  // recapture with the new grid
+  }
   return BDLU_OK;
 }
 

@@ -195,6 +195,14 @@ int bdlu_gpu_plan_values(bdlu_ctx* ctx, uint64_t plan_id, double* host_values);
 // Results are identical either way.
 int bdlu_gpu_set_side_streams(bdlu_ctx* ctx, int enable);
 
+// Largest persistent grid of the dense-block solve (no counterpart in the
+// reference; default 1024, or BLOCKDAGLU_GPU_SEG_GRID_MAX). The solve's
+// workgroups wait on one another's results, so where several contexts share
+// the GPU (the serving loop) a small grid leaves compute units to the other
+// contexts' work; alone, the autotuner picks among grids up to this size.
+// Results are identical for every grid.
+int bdlu_gpu_set_solve_grid_max(bdlu_ctx* ctx, int32_t max_workgroups);
+
 // The context's HIP stream (a hipStream_t), so callers can order or time their
 // own work around engine calls.
 void* bdlu_gpu_stream(bdlu_ctx* ctx);
