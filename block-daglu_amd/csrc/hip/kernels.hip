@@ -1279,13 +1279,14 @@ __global__ __launch_bounds__(256) void k_tail_update(TileJob<T> j0, TileJob<T> j
 // its four waves); workgroups from 1 on update the rest (jobs j0, j1), so the
 // diagonal work runs beside the bulk of the update from the start. The tile
 // and diagonal code share one LDS buffer.
-// FP32: two waves per SIMD, as for the plain update (the diagonal path would
-// otherwise raise the register count of the whole kernel).
+// FP32: three waves per SIMD with 64x64 tiles and two with 128x128 (the
+// diagonal path would otherwise raise the register count of the whole kernel;
+// three with 128x128 tiles would spill).
 // kK = 128: the jobs apply two consecutive panels at once (K = 128), which
 // reads and writes the trailing matrix once per two panels; each element still
 // gets the MFMA sequence of the two single-panel updates in turn.
 template <typename T, int kTile, int kK = kNb>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (kTile == 64 ? 3 : 2) : 1)))
 void k_tail_update_diag(TileJob<T> j0, TileJob<T> j1, TileJob<T> next, double limit,
                         int32_t* __restrict__ status, T* __restrict__ S, int64_t lds_s, int32_t q, int32_t nb,
                         T* __restrict__ inv) {
