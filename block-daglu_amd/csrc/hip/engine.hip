@@ -834,10 +834,10 @@ void block_solve_launches(Plan& P, const Work& W, hipStream_t st, const double* 
                                S.lr_vidx, Lx, nullptr, S.y, ldy);
       bdk::launch_sp_rows<T>(st, kt, S.m, S.tl_rows, S.lr_ptr, S.lr_col, S.lr_vidx, Lx, nullptr, S.y, ldy);
       // Tail: forward into ty, backward back into y (each output starts unpublished).
-      for (int32_t c = 0; c < kt; ++c) bdk::launch_fill_sentinel(st, S.ty + c * ldt, S.m);
+      bdk::launch_fill_sentinel_cols(st, S.ty, S.m, kt, ldt);
       bdk::launch_tail_trsv<T>(st, kt, S.m, Sm, S.ldt, inv, cpl, S.y + S.k0, ldy, S.ty, ldt, false, W.tickets + tix,
                                W.err);
-      for (int32_t c = 0; c < kt; ++c) bdk::launch_fill_sentinel(st, S.y + c * ldy + S.k0, S.m);
+      bdk::launch_fill_sentinel_cols(st, S.y + S.k0, S.m, kt, ldy);
       bdk::launch_tail_trsv<T>(st, kt, S.m, Sm, S.ldt, inv, cpl, S.ty, ldt, S.y + S.k0, ldy, true, W.tickets + tix + 1,
                                W.err);
       for (std::size_t l = 0; l + 1 < S.bl_ptr.size(); ++l)

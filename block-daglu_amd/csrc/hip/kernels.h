@@ -36,6 +36,7 @@ void launch_fill_f64(hipStream_t st, double* p, int64_t n, double v);
 void launch_fill_f32(hipStream_t st, float* p, int64_t n, float v);
 // Fills with the solve's "not yet published" pattern (see kernels.hip).
 void launch_fill_sentinel(hipStream_t st, double* p, int64_t n);
+void launch_fill_sentinel_cols(hipStream_t st, double* p, int32_t m, int32_t k, int64_t ld);  // k columns, stride ld
 
 // key == nullptr: partial pivoting (BLOCK route); else KLU's threshold rule
 // with tolerance tol and per-row tie keys (global-KLU route).

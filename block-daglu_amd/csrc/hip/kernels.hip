@@ -137,6 +137,19 @@ void launch_fill_f64(hipStream_t st, double* p, int64_t n, double v) {
 void launch_fill_f32(hipStream_t st, float* p, int64_t n, float v) {
   if (n > 0) hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, p, n, v);
 }
+// The unpublished mark in rows [0, m) of k columns p + c * ld (one launch).
+__global__ void k_fill_sentinel_cols(uint64_t* p, int32_t m, int32_t k, int64_t ld) {
+  const int64_t n = int64_t(m) * k;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < n; e += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t c = e / m;
+    p[c * ld + (e - c * m)] = kXSentinel;
+  }
+}
+void launch_fill_sentinel_cols(hipStream_t st, double* p, int32_t m, int32_t k, int64_t ld) {
+  if (m > 0 && k > 0)
+    hipLaunchKernelGGL(k_fill_sentinel_cols, dim3(grid_for(int64_t(m) * k, 256, 4096)), dim3(256), 0, st,
+                       reinterpret_cast<uint64_t*>(p), m, k, ld);
+}
 void launch_fill_sentinel(hipStream_t st, double* p, int64_t n) {
   if (n > 0)
     hipLaunchKernelGGL(k_fill<uint64_t>, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st,
