@@ -1168,7 +1168,7 @@ __device__ __forceinline__ void tile_gemm(const TileJob<T>& j0, const TileJob<T>
 // global loads are in flight while the MFMAs take the current one, and the C
 // loads overlap the first chunk's staging. Per element the MFMA sequence is
 // that of a single staged pass over K = 0..63. C does not alias A or B.
-constexpr int kUpdKc = 16;
+constexpr int kUpdKc = 32;
 template <int kTile, typename T>
 constexpr std::size_t update_lds_elems() { return 2 * 2 * kUpdKc * (kTile + 1); }
 
