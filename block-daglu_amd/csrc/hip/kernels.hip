@@ -9,6 +9,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <type_traits>
 
 namespace bdk {
@@ -194,40 +195,34 @@ __global__ __launch_bounds__(64) void k_dense_lu(int32_t nd, const int32_t* __re
   int32_t st = 0;
   const bool klu = key != nullptr;
   int32_t mykey = (klu && lane < s) ? key[starts[t] + lane] : lane;
+  // Half the butterfly group: the group (2 * half lanes) covers every lane below s.
+  const int half = s > 32 ? 32 : s > 16 ? 16 : s > 8 ? 8 : s > 4 ? 4 : s > 2 ? 2 : 1;
   for (int k = 0; k < s; ++k) {
-    T v = (lane >= k && lane < s) ? tabs(a[k * kLdsPad + lane]) : T(-1);
-    int idx = lane;
-    if (!klu) {
-      for (int off = 32; off > 0; off >>= 1) {
-        const T ov = __shfl_xor(v, off);
-        const int oi = __shfl_xor(idx, off);
-        if (ov > v || (ov == v && oi < idx)) {
-          v = ov;
-          idx = oi;
-        }
-      }
-    } else {
-      int kk = mykey;
-      for (int off = 32; off > 0; off >>= 1) {
-        const T ov = __shfl_xor(v, off);
-        const int oi = __shfl_xor(idx, off);
-        const int ok = __shfl_xor(kk, off);
-        if (ov > v || (ov == v && ok < kk)) {
-          v = ov;
-          idx = oi;
-          kk = ok;
-        }
-      }
-    }
-    const int32_t zero = uniform_from_lane0(!(v > T(0)) ? 1 : 0);
-    if (zero) {  // best == 0 (or NaN): the reference marks the block failed
+    const T v = (lane >= k && lane < s) ? tabs(a[k * kLdsPad + lane]) : T(-1);
+    // The largest candidate over the lanes below s (a butterfly of max over
+    // the smallest power-of-two group of lanes that holds them), then the
+    // first lane holding it (ballot), or with KLU's rule the smallest key
+    // among the lanes holding it. Rows are lanes (physical swaps), so the first
+    // lane is the first row position, as in the reference.
+    T vm = v;
+    for (int off = half; off > 0; off >>= 1) vm = fmax(vm, __shfl_xor(vm, off));
+    vm = __shfl(vm, 0);
+    if (uniform_from_lane0(!(vm > T(0)) ? 1 : 0)) {  // best == 0 (or NaN): the reference marks the block failed
       st |= kStZeroPivot;
       break;
     }
-    int p = idx;
-    // Diagonal preference (KLU's test: |diag| >= tol * largest); every lane
-    // holds the same largest value after the butterfly.
-    if (klu && tabs(a[k * kLdsPad + k]) >= static_cast<T>(tol) * v) p = k;
+    const bool top = v == vm;
+    int p;
+    if (!klu) {
+      p = __builtin_ctzll(__ballot(top));
+    } else {
+      int kk = top ? mykey : INT_MAX;
+      for (int off = half; off > 0; off >>= 1) kk = min(kk, __shfl_xor(kk, off));
+      kk = __shfl(kk, 0);
+      p = __builtin_ctzll(__ballot(top && mykey == kk));
+      // Diagonal preference (KLU's test: |diag| >= tol * largest).
+      if (tabs(a[k * kLdsPad + k]) >= static_cast<T>(tol) * vm) p = k;
+    }
     p = uniform_from_lane0(p);
     if (klu) {
       const int32_t kk_k = __shfl(mykey, k), kk_p = __shfl(mykey, p);
@@ -246,7 +241,21 @@ __global__ __launch_bounds__(64) void k_dense_lu(int32_t nd, const int32_t* __re
     if (lane > k && lane < s) {
       const T l = a[k * kLdsPad + lane] / akk;
       a[k * kLdsPad + lane] = l;
-      for (int j = k + 1; j < s; ++j) a[j * kLdsPad + lane] = a[j * kLdsPad + lane] - l * a[j * kLdsPad + k];
+      // Columns in groups of 8: a group's LDS reads issue together, then its
+      // updates and stores (one column at a time, each read waited for the
+      // previous store). Per element the same a - l * u.
+      int j = k + 1;
+      for (; j + 8 <= s; j += 8) {
+        T u[8], v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          u[q] = a[(j + q) * kLdsPad + k];
+          v[q] = a[(j + q) * kLdsPad + lane];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) a[(j + q) * kLdsPad + lane] = v[q] - l * u[q];
+      }
+      for (; j < s; ++j) a[j * kLdsPad + lane] = a[j * kLdsPad + lane] - l * a[j * kLdsPad + k];
     }
     __syncthreads();
   }
