@@ -119,6 +119,7 @@ struct Work {
   double* norms = nullptr;     // per column c: [c*9 + 0..5] residual norms
   int32_t* tickets = nullptr;  // the plan's nticket counters
   int32_t* err = nullptr;
+  double* irs = nullptr;  // refinement state of a loop run on the device (bdk::kIrStateLen)
 };
 
 struct Phase {
@@ -194,6 +195,7 @@ struct Plan {
     bool solve_tuned = false;
   } sched;
   int32_t* d_lpt = nullptr;  // dense blocks by size, largest first
+  int32_t ir_steps = 1;      // refinement steps of the last solve: the length of the next device-side loop
   // Captured launch sequences, replayed per call: the refactor per precision,
   // and the solve per (column count, vector pair). Lazily instantiated.
   hipGraphExec_t refactor_graph[2] = {nullptr, nullptr};
@@ -742,6 +744,7 @@ void upload_into(bdlu_ctx* c, const bdlu_plan_desc& d, Plan& P) {
   W.xperm = A.get<double>(nk);
   W.cw = A.get<double>(nk);
   W.partials = A.get<double>(512 * 6 * kMaxRhs);
+  W.irs = A.get<double>(bdk::kIrStateLen);
   P.norms = A.get<double>(kAnormSlot + 8);
   W.norms = P.norms;
   HIPCHECK(hipStreamSynchronize(st));
@@ -1133,8 +1136,10 @@ int refactor_impl(bdlu_ctx* c, Plan& P, const double* values, int where, int pre
 // halve the worst stop measure (FP32 factors too inaccurate for this matrix).
 // The plan's own work set replays captured graphs; any other set (the
 // no-allocation entry point's) is launched directly, without profiling.
-bool solve_chunk(bdlu_ctx* c, Plan& P, const Work& W, int32_t kt, int32_t kc, const bdlu_ir_opts* ir, double anorm,
-                 bool watch_stall, bdlu_solve_stats& cs) {
+// anorm is read only after the stream has been synchronized (the callers queue
+// its copy from the device just before).
+bool solve_chunk(bdlu_ctx* c, Plan& P, const Work& W, int32_t kt, int32_t kc, const bdlu_ir_opts* ir,
+                 const double& anorm, bool watch_stall, bdlu_solve_stats& cs) {
   hipStream_t st = c->stream;
   const bool own = &W == &P.work;
   const bool prof = own && c->prof;
@@ -1145,24 +1150,66 @@ bool solve_chunk(bdlu_ctx* c, Plan& P, const Work& W, int32_t kt, int32_t kc, co
   const int64_t n = P.n;
   const int32_t max_iters = ir ? ir->max_iters : 0;
   const bool want_stats = max_iters > 0 || (ir && ir->stop_rule < 0);
-  ProfEvents pev;
-  if (prof) pev.mark(BDLU_PROF_SOLVE, st);
-  solve(W.b, W.x);
-  if (prof) {
-    pev.mark(BDLU_PROF_SOLVE, st);
-    HIPCHECK(hipStreamSynchronize(st));
-    pev.collect(c);
-  }
   cs = bdlu_solve_stats{0, -1.0, -1.0, -1.0, 0};
-  if (!want_stats) return true;
   double h[9 * kMaxRhs];
-  auto eval = [&]() {
+  auto residual = [&]() {
     bdk::launch_residual(st, P.n, kt, P.csr_ptr, P.csr_col, P.vals, W.x, W.b, W.r, W.cw, n, P.res_is_long,
                          P.res_nlong, P.res_long_rows);
     bdk::launch_norms(st, P.n, kt, W.r, W.x, W.b, W.cw, n, W.partials, W.norms);
+  };
+  auto eval = [&]() {
+    residual();
     HIPCHECK(hipMemcpyAsync(h, W.norms, sizeof(double) * 9 * kt, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
   };
+  int32_t iters = 0;
+  bool converged = false, ended = false;
+  double prev = 0.0;
+  const bool device_loop = own && c->graphs && !prof && max_iters > 0;
+  if (device_loop) {
+    // The solve, the first evaluation and as many refinement steps as the last
+    // solve of this plan took, as one graph with the stop decisions taken on
+    // the device: steps after the loop has ended leave x alone (the gated
+    // update) and re-evaluate the same x, so the result is the host loop's.
+    // A loop still running at the graph's end goes on below.
+    const int32_t steps = std::min(max_iters, P.ir_steps);
+    const int32_t rule = ir->stop_rule;
+    int32_t tol_bits[2];
+    std::memcpy(tol_bits, &ir->tol, sizeof(tol_bits));
+    std::vector<int32_t> key{kt, 2, kc, steps, rule, max_iters, watch_stall ? 1 : 0, tol_bits[0], tol_bits[1]};
+    for (const SparseDev& S : P.sp) key.push_back(S.prec);
+    hipGraphExec_t& exec = P.solve_graph[key];
+    const double* anorm_d = P.norms + kAnormSlot;
+    run_captured(st, exec, true, [&] {
+      block_solve_launches(P, W, st, W.b, W.x, kt);
+      residual();
+      bdk::launch_ir_check(st, W.norms, kc, rule, ir->tol, anorm_d, max_iters, watch_stall, true, W.irs);
+      for (int32_t i = 0; i < steps; ++i) {
+        block_solve_launches(P, W, st, W.r, W.d, kt);
+        bdk::launch_axpy_gated(st, P.n, kt, W.d, W.x, n, W.irs);
+        residual();
+        bdk::launch_ir_check(st, W.norms, kc, rule, ir->tol, anorm_d, max_iters, watch_stall, false, W.irs);
+      }
+    });
+    double s[bdk::kIrStateLen];
+    HIPCHECK(hipMemcpyAsync(h, W.norms, sizeof(double) * 9 * kt, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(s, W.irs, sizeof(s), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    iters = static_cast<int32_t>(s[bdk::kIrIters]);
+    converged = s[bdk::kIrConverged] != 0.0;
+    prev = s[bdk::kIrWorst];
+    ended = s[bdk::kIrActive] == 0.0;
+  } else {
+    ProfEvents pev;
+    if (prof) pev.mark(BDLU_PROF_SOLVE, st);
+    solve(W.b, W.x);
+    if (prof) {
+      pev.mark(BDLU_PROF_SOLVE, st);
+      HIPCHECK(hipStreamSynchronize(st));
+      pev.collect(c);
+    }
+    if (!want_stats) return true;
+  }
   auto rel = [&](int k) {
     const double bn2 = std::sqrt(h[k * 9 + 3]);
     return bn2 > 0 ? std::sqrt(h[k * 9 + 0]) / bn2 : 0.0;
@@ -1187,11 +1234,12 @@ bool solve_chunk(bdlu_ctx* c, Plan& P, const Work& W, int32_t kt, int32_t kc, co
     }
     return true;
   };
-  eval();
-  int32_t iters = 0;
-  bool converged = max_iters <= 0 || met();
-  double prev = worst();
-  while (!converged && iters < max_iters) {
+  if (!ended && iters == 0) {
+    if (!device_loop) eval();
+    converged = max_iters <= 0 || met();
+    prev = worst();
+  }
+  while (!ended && !converged && iters < max_iters) {
     solve(W.r, W.d);
     bdk::launch_axpy(st, P.n, kt, W.d, W.x, n);
     ++iters;
@@ -1201,6 +1249,7 @@ bool solve_chunk(bdlu_ctx* c, Plan& P, const Work& W, int32_t kt, int32_t kc, co
     if (!converged && watch_stall && !(now <= 0.5 * prev)) break;
     prev = now;
   }
+  if (own && max_iters > 0) P.ir_steps = iters;
   // Worst right-hand side of the chunk.
   for (int k = 0; k < kc; ++k) {
     cs.rel_residual = std::max(cs.rel_residual, rel(k));

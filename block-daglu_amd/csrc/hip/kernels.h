@@ -165,5 +165,12 @@ void launch_norms(hipStream_t st, int32_t n, int32_t kc, const double* r, const 
 void launch_anorm(hipStream_t st, int32_t n, const int64_t* csr_ptr, const double* vals, double* partials,
                   double* out);
 void launch_axpy(hipStream_t st, int32_t n, int32_t kc, const double* d, double* x, int64_t ld);  // x += d
+// Refinement state of a loop run on the device (doubles): still running, steps
+// taken, converged, worst measure of the last evaluation, ended by a stall.
+constexpr int kIrActive = 0, kIrIters = 1, kIrConverged = 2, kIrWorst = 3, kIrStalled = 4, kIrStateLen = 8;
+void launch_axpy_gated(hipStream_t st, int32_t n, int32_t kc, const double* d, double* x, int64_t ld,
+                       const double* state);  // x += d while state[kIrActive] != 0
+void launch_ir_check(hipStream_t st, const double* norms, int32_t kc, int32_t rule, double tol, const double* anorm,
+                     int32_t max_iters, bool watch_stall, bool first, double* state);
 
 }  // namespace bdk

@@ -2250,4 +2250,75 @@ void launch_axpy(hipStream_t st, int32_t n, int32_t kc, const double* d, double*
   hipLaunchKernelGGL(k_axpy, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, n, kc, d, x, ld);
 }
 
+// x += d while the refinement state says the loop is still running.
+__global__ void k_axpy_gated(int32_t n, int32_t kc, const double* __restrict__ d, double* __restrict__ x, int64_t ld,
+                             const double* __restrict__ state) {
+  if (state[kIrActive] == 0.0) return;
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    for (int32_t c = 0; c < kc; ++c) x[c * ld + i] = x[c * ld + i] + d[c * ld + i];
+}
+void launch_axpy_gated(hipStream_t st, int32_t n, int32_t kc, const double* d, double* x, int64_t ld,
+                       const double* state) {
+  hipLaunchKernelGGL(k_axpy_gated, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, n, kc, d, x, ld, state);
+}
+
+// One stop decision of iterative refinement, taken on the device from the
+// norms of the last residual (h[c*9 + 0..5] per column, as k_norms_final
+// writes them) so that a whole refinement loop can run without a host round
+// trip. The measures, the stop rule (0: ||r||_2/||b||_2 < tol; 1: normwise
+// backward error ||r||_inf / (||A||_inf ||x||_inf + ||b||_inf) <= tol; 2:
+// componentwise backward error <= tol), columns with b = 0 counting as
+// converged, and the stall rule (a step that fails to halve the worst measure
+// ends the loop) are the host loop's, in the same double operations
+// (correctly rounded sqrt and division, no contraction), so both take the
+// same decisions. `first` evaluates the initial solution; later calls do
+// nothing once the loop has ended.
+__global__ void k_ir_check(const double* __restrict__ h, int32_t kc, int32_t rule, double tol,
+                           const double* __restrict__ anorm, int32_t max_iters, int32_t watch_stall, int32_t first,
+                           double* __restrict__ state) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0) return;
+  if (!first && state[kIrActive] == 0.0) return;
+  const double an = *anorm;
+  bool met = true;
+  double worst = 0.0;
+  for (int32_t k = 0; k < kc; ++k) {
+    const double* q = h + k * 9;
+    const double bn2 = sqrt(q[3]);
+    const double rel = bn2 > 0 ? sqrt(q[0]) / bn2 : 0.0;
+    const double den = an * q[2] + q[4];
+    const double berr = den > 0 ? q[1] / den : 0.0;
+    const double m = rule == 1 ? berr : rule == 2 ? q[5] : rel;
+    const bool stop = rule == 0 ? rel < tol : m <= tol;
+    if (q[3] > 0.0) {
+      if (!stop) met = false;
+      worst = worst < m ? m : worst;
+    }
+  }
+  if (first) {
+    const bool conv = max_iters <= 0 || met;
+    state[kIrIters] = 0.0;
+    state[kIrConverged] = conv ? 1.0 : 0.0;
+    state[kIrWorst] = worst;
+    state[kIrStalled] = 0.0;
+    state[kIrActive] = conv ? 0.0 : 1.0;
+    return;
+  }
+  const double iters = state[kIrIters] + 1.0;
+  state[kIrIters] = iters;
+  state[kIrConverged] = met ? 1.0 : 0.0;
+  if (!met && watch_stall && !(worst <= 0.5 * state[kIrWorst])) {
+    state[kIrStalled] = 1.0;
+    state[kIrActive] = 0.0;
+    return;
+  }
+  state[kIrWorst] = worst;
+  state[kIrActive] = (!met && iters < double(max_iters)) ? 1.0 : 0.0;
+}
+void launch_ir_check(hipStream_t st, const double* norms, int32_t kc, int32_t rule, double tol, const double* anorm,
+                     int32_t max_iters, bool watch_stall, bool first, double* state) {
+  hipLaunchKernelGGL(k_ir_check, dim3(1), dim3(64), 0, st, norms, kc, rule, tol, anorm, max_iters,
+                     watch_stall ? 1 : 0, first ? 1 : 0, state);
+}
+
 }  // namespace bdk
