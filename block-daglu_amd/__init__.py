@@ -185,6 +185,7 @@ def hip() -> C.CDLL:
                            ("hipMemcpy", [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
                            ("hipMemcpyAsync", [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]),
                            ("hipDeviceSynchronize", []), ("hipStreamSynchronize", [C.c_void_p]),
+                           ("hipStreamCreateWithFlags", [C.POINTER(C.c_void_p), C.c_uint]),
                            ("hipEventCreate", [C.POINTER(C.c_void_p)]), ("hipEventDestroy", [C.c_void_p]),
                            ("hipEventRecord", [C.c_void_p, C.c_void_p]), ("hipEventSynchronize", [C.c_void_p]),
                            ("hipEventElapsedTime", [C.POINTER(C.c_float), C.c_void_p, C.c_void_p])):
@@ -200,8 +201,25 @@ def _hipcheck(rc, what):
         raise RuntimeError(f"{what} failed with HIP error {rc}")
 
 
+_copy_stream = None
+
+
+def _copies() -> C.c_void_p:
+    """A non-blocking stream of this module for host<->device copies. Copies on
+    the legacy default stream (plain hipMemcpy) fail with
+    hipErrorStreamCaptureImplicit while any thread of the process captures a
+    graph, and the engine captures its launch sequences on first use."""
+    global _copy_stream
+    if _copy_stream is None:
+        st = C.c_void_p()
+        _hipcheck(hip().hipStreamCreateWithFlags(C.byref(st), 1), "hipStreamCreateWithFlags")
+        _copy_stream = st
+    return _copy_stream
+
+
 class DeviceArray:
-    """A float64 buffer in device memory (HBM) with host copies in and out."""
+    """A float64 buffer in device memory (HBM) with host copies in and out
+    (on a non-blocking stream of their own, complete when the call returns)."""
     H2D, D2H, D2D = 1, 2, 3
 
     def __init__(self, count: int):
@@ -219,7 +237,9 @@ class DeviceArray:
     def upload(self, a: np.ndarray) -> None:
         a = np.ascontiguousarray(a, dtype=np.float64)
         assert a.size <= self.count
-        _hipcheck(hip().hipMemcpy(self.ptr, a.ctypes.data, a.size * 8, self.H2D), "hipMemcpy")
+        st = _copies()
+        _hipcheck(hip().hipMemcpyAsync(self.ptr, a.ctypes.data, a.size * 8, self.H2D, st), "hipMemcpyAsync")
+        _hipcheck(hip().hipStreamSynchronize(st), "hipStreamSynchronize")
 
     def copy_from(self, other: "DeviceArray", stream: int = 0) -> None:
         assert other.count <= self.count
@@ -228,7 +248,9 @@ class DeviceArray:
 
     def numpy(self) -> np.ndarray:
         out = np.empty(self.count, np.float64)
-        _hipcheck(hip().hipMemcpy(out.ctypes.data, self.ptr, self.count * 8, self.D2H), "hipMemcpy")
+        st = _copies()
+        _hipcheck(hip().hipMemcpyAsync(out.ctypes.data, self.ptr, self.count * 8, self.D2H, st), "hipMemcpyAsync")
+        _hipcheck(hip().hipStreamSynchronize(st), "hipStreamSynchronize")
         return out
 
     @property

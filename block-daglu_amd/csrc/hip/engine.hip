@@ -195,7 +195,11 @@ struct Plan {
     bool solve_tuned = false;
   } sched;
   int32_t* d_lpt = nullptr;  // dense blocks by size, largest first
-  int32_t ir_steps = 1;      // refinement steps of the last solve: the length of the next device-side loop
+  // Refinement steps of the last four solves (-1: none yet); their minimum is
+  // the length of the next device-side loop, so that steps it runs past the
+  // stop (each a whole correction solve, gated off) stay rare.
+  int32_t ir_steps[4] = {-1, -1, -1, -1};
+  int32_t ir_next = 0;
   // Captured launch sequences, replayed per call: the refactor per precision,
   // and the solve per (column count, vector pair). Lazily instantiated.
   hipGraphExec_t refactor_graph[2] = {nullptr, nullptr};
@@ -1167,28 +1171,34 @@ bool solve_chunk(bdlu_ctx* c, Plan& P, const Work& W, int32_t kt, int32_t kc, co
   double prev = 0.0;
   const bool device_loop = own && c->graphs && !prof && max_iters > 0;
   if (device_loop) {
-    // The solve, the first evaluation and as many refinement steps as the last
-    // solve of this plan took, as one graph with the stop decisions taken on
-    // the device: steps after the loop has ended leave x alone (the gated
-    // update) and re-evaluate the same x, so the result is the host loop's.
-    // A loop still running at the graph's end goes on below.
-    const int32_t steps = std::min(max_iters, P.ir_steps);
+    // The solve, the first evaluation and as many refinement steps as the
+    // fewest of the plan's recent solves took, as one graph with the stop
+    // decisions taken on the device: steps after the loop has ended leave x
+    // alone (the gated update) and re-evaluate the same x, so the result is
+    // the host loop's. A loop still running at the graph's end goes on below.
+    int32_t steps = max_iters;
+    bool seen = false;
+    for (int32_t v : P.ir_steps)
+      if (v >= 0) steps = std::min(steps, v), seen = true;
+    if (!seen) steps = 0;
     const int32_t rule = ir->stop_rule;
     int32_t tol_bits[2];
     std::memcpy(tol_bits, &ir->tol, sizeof(tol_bits));
-    std::vector<int32_t> key{kt, 2, kc, steps, rule, max_iters, watch_stall ? 1 : 0, tol_bits[0], tol_bits[1]};
+    // The stop test runs over all kt columns: the padding columns have b = 0
+    // and count as converged, as they do for the host's test over kc.
+    std::vector<int32_t> key{kt, 2, steps, rule, max_iters, watch_stall ? 1 : 0, tol_bits[0], tol_bits[1]};
     for (const SparseDev& S : P.sp) key.push_back(S.prec);
     hipGraphExec_t& exec = P.solve_graph[key];
     const double* anorm_d = P.norms + kAnormSlot;
     run_captured(st, exec, true, [&] {
       block_solve_launches(P, W, st, W.b, W.x, kt);
       residual();
-      bdk::launch_ir_check(st, W.norms, kc, rule, ir->tol, anorm_d, max_iters, watch_stall, true, W.irs);
+      bdk::launch_ir_check(st, W.norms, kt, rule, ir->tol, anorm_d, max_iters, watch_stall, true, W.irs);
       for (int32_t i = 0; i < steps; ++i) {
         block_solve_launches(P, W, st, W.r, W.d, kt);
         bdk::launch_axpy_gated(st, P.n, kt, W.d, W.x, n, W.irs);
         residual();
-        bdk::launch_ir_check(st, W.norms, kc, rule, ir->tol, anorm_d, max_iters, watch_stall, false, W.irs);
+        bdk::launch_ir_check(st, W.norms, kt, rule, ir->tol, anorm_d, max_iters, watch_stall, false, W.irs);
       }
     });
     double s[bdk::kIrStateLen];
@@ -1249,7 +1259,10 @@ bool solve_chunk(bdlu_ctx* c, Plan& P, const Work& W, int32_t kt, int32_t kc, co
     if (!converged && watch_stall && !(now <= 0.5 * prev)) break;
     prev = now;
   }
-  if (own && max_iters > 0) P.ir_steps = iters;
+  if (own && max_iters > 0) {
+    P.ir_steps[P.ir_next] = iters;
+    P.ir_next = (P.ir_next + 1) % 4;
+  }
   // Worst right-hand side of the chunk.
   for (int k = 0; k < kc; ++k) {
     cs.rel_residual = std::max(cs.rel_residual, rel(k));

@@ -161,6 +161,11 @@ int bdlu_gpu_plan_schedule(bdlu_ctx* ctx, uint64_t plan_id, int32_t* out4);
 int bdlu_gpu_refactor(bdlu_ctx* ctx, uint64_t plan_id, const double* values, int where, int precision,
                       bdlu_factor_stats* stats);
 // B: n x nrhs column-major (leading dimension ldb), overwritten by X.
+// With refinement and graphs on, the solve and the refinement steps (as many
+// as the fewest of the plan's last four solves needed) replay as one graph
+// whose stop tests run on the device; a loop still running at its end
+// continues with one host stop test per step. Either way the same kernels run
+// and the results are the same bit for bit.
 int bdlu_gpu_solve(bdlu_ctx* ctx, uint64_t plan_id, double* B, int32_t nrhs, int64_t ldb, int where,
                    const bdlu_ir_opts* ir, bdlu_solve_stats* stats);
 
