@@ -150,6 +150,7 @@ GPU_SYMBOLS = [
     ("bdlu_gpu_plan_dense_pivots", C.c_int, [_P, C.c_uint64, _I32P]),
     ("bdlu_gpu_set_side_streams", C.c_int, [_P, C.c_int]),
     ("bdlu_gpu_set_solve_grid_max", C.c_int, [_P, C.c_int32]),
+    ("bdlu_gpu_set_device_refinement", C.c_int, [_P, C.c_int]),
     ("bdlu_gpu_copy", C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_uint64]),
     ("bdlu_gpu_profile", C.c_int, [_P, C.c_int]),
     ("bdlu_gpu_profile_read", C.c_int, [_P, C.c_int, _F64P, C.POINTER(C.c_int64), _F64P, _F64P]),

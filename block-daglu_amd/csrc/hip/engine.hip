@@ -257,6 +257,7 @@ struct bdlu_ctx {
   double fp64_growth = 1e8;
   bool prof = false;
   bool graphs = true;  // capture launch sequences into HIP graphs (BLOCKDAGLU_GPU_GRAPH=0 disables)
+  bool device_ir = true;  // refinement stop tests on the device (BLOCKDAGLU_GPU_IR_DEVICE=0 disables)
   bool autotune = true;  // time schedule candidates on a plan's first calls (BLOCKDAGLU_GPU_AUTOTUNE=0 disables)
   double prof_ms[BDLU_PROF_KINDS] = {0};
   int64_t prof_n[BDLU_PROF_KINDS] = {0};
@@ -1140,6 +1141,17 @@ int refactor_impl(bdlu_ctx* c, Plan& P, const double* values, int where, int pre
 // halve the worst stop measure (FP32 factors too inaccurate for this matrix).
 // The plan's own work set replays captured graphs; any other set (the
 // no-allocation entry point's) is launched directly, without profiling.
+// kc columns of n doubles between column-major arrays with leading dimensions
+// ldd and lds: one copy when both are contiguous, else one per column.
+void copy_columns(double* dst, int64_t ldd, const double* src, int64_t lds, int64_t n, int32_t kc, hipMemcpyKind kind,
+                  hipStream_t st) {
+  if (ldd == n && lds == n) {
+    HIPCHECK(hipMemcpyAsync(dst, src, sizeof(double) * n * kc, kind, st));
+    return;
+  }
+  for (int32_t k = 0; k < kc; ++k) HIPCHECK(hipMemcpyAsync(dst + k * ldd, src + k * lds, sizeof(double) * n, kind, st));
+}
+
 // anorm is read only after the stream has been synchronized (the callers queue
 // its copy from the device just before).
 bool solve_chunk(bdlu_ctx* c, Plan& P, const Work& W, int32_t kt, int32_t kc, const bdlu_ir_opts* ir,
@@ -1169,7 +1181,7 @@ bool solve_chunk(bdlu_ctx* c, Plan& P, const Work& W, int32_t kt, int32_t kc, co
   int32_t iters = 0;
   bool converged = false, ended = false;
   double prev = 0.0;
-  const bool device_loop = own && c->graphs && !prof && max_iters > 0;
+  const bool device_loop = own && c->graphs && c->device_ir && !prof && max_iters > 0;
   if (device_loop) {
     // The solve, the first evaluation and as many refinement steps as the
     // fewest of the plan's recent solves took, as one graph with the stop
@@ -1303,6 +1315,7 @@ int bdlu_gpu_create(int device, uint64_t hbm_budget_bytes, bdlu_ctx** out) {
   if (const char* e = std::getenv("BLOCKDAGLU_GPU_FP32_GROWTH")) c->fp32_growth = std::max(1.0, std::atof(e));
   if (const char* e = std::getenv("BLOCKDAGLU_GPU_REPIVOT_GROWTH")) c->fp64_growth = std::max(1.0, std::atof(e));
   if (const char* e = std::getenv("BLOCKDAGLU_GPU_GRAPH")) c->graphs = std::atoi(e) != 0;
+  if (const char* e = std::getenv("BLOCKDAGLU_GPU_IR_DEVICE")) c->device_ir = std::atoi(e) != 0;
   if (const char* e = std::getenv("BLOCKDAGLU_GPU_AUTOTUNE")) c->autotune = std::atoi(e) != 0;
   if (const char* e = std::getenv("BLOCKDAGLU_GPU_STREAMS")) c->fork = std::atoi(e) > 1;
   if (const char* e = std::getenv("BLOCKDAGLU_GPU_LOOKAHEAD")) c->lookahead = std::max(0, std::min(2, std::atoi(e)));
@@ -1442,8 +1455,7 @@ int bdlu_gpu_solve(bdlu_ctx* c, uint64_t id, double* B, int32_t nrhs, int64_t ld
     for (int32_t c0 = 0; c0 < nrhs; c0 += kMaxRhs) {
       const int32_t kc = std::min(kMaxRhs, nrhs - c0);
       const int32_t kt = kc <= 1 ? 1 : kc <= 2 ? 2 : kc <= 4 ? 4 : 8;
-      for (int32_t k = 0; k < kc; ++k)
-        HIPCHECK(hipMemcpyAsync(W.b + k * n, B + (c0 + k) * ldb, sizeof(double) * n, in, st));
+      copy_columns(W.b, n, B + c0 * ldb, ldb, n, kc, in, st);
       if (kt > kc) bdk::launch_fill_f64(st, W.b + kc * n, (kt - kc) * n, 0.0);
       if (c->autotune && !c->prof && !P.sched.solve_tuned) {
         autotune_solve(P, st, kt, c->seg_grid_max);
@@ -1468,8 +1480,7 @@ int bdlu_gpu_solve(bdlu_ctx* c, uint64_t id, double* B, int32_t nrhs, int64_t ld
       ss.backward_error = std::max(ss.backward_error, cs.backward_error);
       ss.componentwise_error = std::max(ss.componentwise_error, cs.componentwise_error);
       ss.iterations = std::max(ss.iterations, cs.iterations);
-      for (int32_t k = 0; k < kc; ++k)
-        HIPCHECK(hipMemcpyAsync(B + (c0 + k) * ldb, W.x + k * n, sizeof(double) * n, out, st));
+      copy_columns(B + c0 * ldb, ldb, W.x, n, n, kc, out, st);
     }
     int32_t herr = 0;
     HIPCHECK(hipMemcpyAsync(&herr, W.err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -1533,8 +1544,7 @@ int bdlu_gpu_solve_noalloc(bdlu_ctx* c, uint64_t id, double* d_B, int32_t nrhs, 
     for (int32_t c0 = 0; c0 < nrhs; c0 += kMaxRhs) {
       const int32_t kc = std::min(kMaxRhs, nrhs - c0);
       const int32_t kt = kc <= 1 ? 1 : kc <= 2 ? 2 : kc <= 4 ? 4 : 8;
-      for (int32_t k = 0; k < kc; ++k)
-        HIPCHECK(hipMemcpyAsync(W.b + k * n, d_B + (c0 + k) * ldb, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+      copy_columns(W.b, n, d_B + c0 * ldb, ldb, n, kc, hipMemcpyDeviceToDevice, st);
       if (kt > kc) bdk::launch_fill_f64(st, W.b + kc * n, (kt - kc) * n, 0.0);
       bdlu_solve_stats cs{};
       all_converged = solve_chunk(c, P, W, kt, kc, ir, anorm, false, cs) && all_converged;
@@ -1542,8 +1552,7 @@ int bdlu_gpu_solve_noalloc(bdlu_ctx* c, uint64_t id, double* d_B, int32_t nrhs, 
       ss.backward_error = std::max(ss.backward_error, cs.backward_error);
       ss.componentwise_error = std::max(ss.componentwise_error, cs.componentwise_error);
       ss.iterations = std::max(ss.iterations, cs.iterations);
-      for (int32_t k = 0; k < kc; ++k)
-        HIPCHECK(hipMemcpyAsync(d_B + (c0 + k) * ldb, W.x + k * n, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+      copy_columns(d_B + c0 * ldb, ldb, W.x, n, n, kc, hipMemcpyDeviceToDevice, st);
     }
     int32_t herr = 0;
     HIPCHECK(hipMemcpyAsync(&herr, W.err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -1619,6 +1628,13 @@ int bdlu_gpu_set_solve_grid_max(bdlu_ctx* c, int32_t max_workgroups) {
     P.sched.seg_grid = std::min(P.sched.seg_grid, max_workgroups);
     P.drop_graphs();  // recapture with the new grid
   }
+  return BDLU_OK;
+}
+
+int bdlu_gpu_set_device_refinement(bdlu_ctx* c, int enable) {
+  if (!c) return BDLU_E_INVALID;
+  std::lock_guard<std::mutex> g(c->mu);
+  c->device_ir = enable != 0;
   return BDLU_OK;
 }
 

@@ -2230,11 +2230,15 @@ __global__ __launch_bounds__(256) void k_anorm_partial(int32_t n, const int64_t*
   }
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
-__global__ void k_max_final(int32_t nb, const double* __restrict__ part, double* __restrict__ out) {
-  if (threadIdx.x != 0) return;
+// One wave: strided maxima per lane, then a butterfly (max is exact and
+// order-free, so the result does not depend on the reduction order).
+__global__ __launch_bounds__(64) void k_max_final(int32_t nb, const double* __restrict__ part,
+                                                  double* __restrict__ out) {
+  const int lane = threadIdx.x;
   double v = 0.0;
-  for (int32_t i = 0; i < nb; ++i) v = fmax(v, part[i]);
-  out[0] = v;
+  for (int32_t i = lane; i < nb; i += kWave) v = fmax(v, part[i]);
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  if (lane == 0) out[0] = v;
 }
 void launch_anorm(hipStream_t st, int32_t n, const int64_t* csr_ptr, const double* vals, double* partials,
                   double* out) {

@@ -208,6 +208,14 @@ int bdlu_gpu_set_side_streams(bdlu_ctx* ctx, int enable);
 // Results are identical for every grid.
 int bdlu_gpu_set_solve_grid_max(bdlu_ctx* ctx, int32_t max_workgroups);
 
+// Refinement stop tests on the device, the solve and its steps replayed as one
+// graph (enable = 1, the default, or BLOCKDAGLU_GPU_IR_DEVICE), or one host
+// stop test per step (0). No counterpart in the reference. The device form
+// saves a host round trip per step, which matters to a lone caller waiting on
+// its solve; where several contexts share the GPU (the serving loop) other
+// contexts fill those gaps anyway. Results are identical either way.
+int bdlu_gpu_set_device_refinement(bdlu_ctx* ctx, int enable);
+
 // The context's HIP stream (a hipStream_t), so callers can order or time their
 // own work around engine calls.
 void* bdlu_gpu_stream(bdlu_ctx* ctx);
