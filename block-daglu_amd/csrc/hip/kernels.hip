@@ -842,8 +842,10 @@ __device__ __forceinline__ void blk_store(T* C0, int r0, int c0, const typename 
 }
 
 // Body of the diagonal LU + inverses for a workgroup of 256 threads. fc: LDS
-// of diag_lds_elems<T>() elements, 16-byte aligned.
-template <typename T>
+// of diag_lds_elems<T>() elements, 16-byte aligned. kPreloaded: the block is
+// already in fc (F, identity past b, as the load below writes it), put there
+// by the tile update that produced it, so it is not read back from S.
+template <typename T, bool kPreloaded = false>
 __device__ __forceinline__ void diag_lu_inv_body(T* __restrict__ S, int64_t lds, int32_t p, int32_t b, double limit,
                                                  int32_t* __restrict__ status, T* __restrict__ inv, T* fc) {
   using V = typename MfmaOps<T>::v4;
@@ -859,15 +861,17 @@ __device__ __forceinline__ void diag_lu_inv_body(T* __restrict__ S, int64_t lds,
   const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(tid >> 6));
   const int li = lane & 15, lk = lane >> 4;
   if (tid == 0) *st_sh = 0;
-  __syncthreads();
-  // The block into LDS, identity past b (coalesced down columns; clamped
-  // addresses, unconditional loads).
+  if constexpr (!kPreloaded) {
+    __syncthreads();
+    // The block into LDS, identity past b (coalesced down columns; clamped
+    // addresses, unconditional loads).
 #pragma unroll
-  for (int u = 0; u < kNb * kNb / kDiagThreads; ++u) {
-    const int e = u * kDiagThreads + tid;
-    const int i = e & (kNb - 1), j = e >> 6;
-    const T v = S[int64_t(p + (j < b ? j : 0)) * lds + p + (i < b ? i : 0)];
-    F[j * LF + i] = (i < b && j < b) ? v : (i == j ? T(1) : T(0));
+    for (int u = 0; u < kNb * kNb / kDiagThreads; ++u) {
+      const int e = u * kDiagThreads + tid;
+      const int i = e & (kNb - 1), j = e >> 6;
+      const T v = S[int64_t(p + (j < b ? j : 0)) * lds + p + (i < b ? i : 0)];
+      F[j * LF + i] = (i < b && j < b) ? v : (i == j ? T(1) : T(0));
+    }
   }
   __syncthreads();
   // Pieces of the inverses (column j of U_bb^{-1}; block (I,J) of L^{-1} or of
@@ -1183,7 +1187,10 @@ constexpr std::size_t update_lds_elems() { return 2 * 2 * kUpdKc * (kTile + 1); 
 
 // kK: the largest K the tile takes (64, or 128 for two panels applied at once;
 // the chunks past J.K stage zeros).
-template <typename T, int kTile, bool kAllChunks = false, int kK = kNb>
+// kToDiag (one 64x64 tile at the job's origin): the result goes to lds in the
+// diagonal body's layout (column-major, leading dimension kDiagLF, identity
+// past M x N) instead of to C.
+template <typename T, int kTile, bool kAllChunks = false, int kK = kNb, bool kToDiag = false>
 __device__ __forceinline__ void tile_update(const TileJob<T>& j0, const TileJob<T>& j1, T* lds, int32_t bid) {
   using V = typename MfmaOps<T>::v4;
   constexpr int kLd = kTile + 1;                // LDS row length (padded against bank conflicts)
@@ -1275,6 +1282,21 @@ __device__ __forceinline__ void tile_update(const TileJob<T>& j0, const TileJob<
     if (ch + 1 < kChunks) lstore((ch + 1) & 1, kSets == 1 ? 0 : ch + 1, (ch + 1) * kUpdKc);
     __syncthreads();
   }
+  if constexpr (kToDiag) {
+    static_assert(kTile == kNb, "the diagonal block is one 64x64 tile");
+    // After the loop's last barrier every wave is done with the staging ring.
+#pragma unroll
+    for (int c = 0; c < kMB; ++c)
+#pragma unroll
+      for (int a = 0; a < kMB; ++a)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = wr + a * 16 + li;
+          const int j = wc + c * 16 + MfmaOps<T>::row(lk, r);
+          lds[j * kDiagLF + i] = (i < M && j < N) ? acc[c][a][r] : (i == j ? T(1) : T(0));
+        }
+    return;
+  }
 #pragma unroll
   for (int c = 0; c < kMB; ++c)
 #pragma unroll
@@ -1316,10 +1338,11 @@ void k_tail_update_diag(TileJob<T> j0, TileJob<T> j1, TileJob<T> next, double li
   constexpr std::size_t kDiagElems = diag_lds_elems<T>();
   __shared__ __attribute__((aligned(16))) T lds[kTileElems > kDiagElems ? kTileElems : kDiagElems];
   if (blockIdx.x == 0) {
-    // The tile ends on a barrier after its last LDS read; the diagonal body's
-    // first barrier makes the block's new values visible to its loads.
-    tile_update<T, 64, sizeof(T) == 8 && kK == kNb, kK>(next, next, lds, 0);
-    diag_lu_inv_body<T>(S, lds_s, q, nb, limit, status, inv, lds);
+    // The updated block goes straight into the diagonal body's LDS (the tile
+    // writes it after its last barrier; the body's first barrier makes it
+    // visible), not through S.
+    tile_update<T, 64, sizeof(T) == 8 && kK == kNb, kK, true>(next, next, lds, 0);
+    diag_lu_inv_body<T, true>(S, lds_s, q, nb, limit, status, inv, lds);
     return;
   }
   // FP64 64x64 tiles with K = 64: all K chunks' loads up front (two waves per
@@ -1438,14 +1461,15 @@ void tail_panel(hipStream_t st, T* S, int64_t lds, int32_t p, int32_t b, int32_t
 // workgroup 0 updating and factoring the diagonal block after them. Every
 // element gets the same MFMA sequence as with one update per panel, so the
 // factors are identical; the trailing matrix is read and written once per two
-// panels.
+// panels. The panel products use 64x64 tiles at every size: one side of each
+// product is 64 wide, so a 128x128 tile would leave half its MFMAs idle and
+// halve the workgroups (the per-element MFMA sequence is the same either way).
 template <typename T>
 void tail_pair(hipStream_t st, T* S, int64_t lds, int32_t p, int32_t rest, double limit, int32_t* status, T* inv,
                TailTimer* timer, int32_t small_tile_below) {
   const int64_t kInvPair = 2 * kNb * kNb;
   const int32_t b = kNb;
-  if (rest < small_tile_below) tail_panel<T, 64>(st, S, lds, p, b, rest, limit, status, inv, timer);
-  else tail_panel<T, 128>(st, S, lds, p, b, rest, limit, status, inv, timer);
+  tail_panel<T, 64>(st, S, lds, p, b, rest, limit, status, inv, timer);
   const int32_t q = p + b;   // panel s+1 (64 columns, since rest > 64)
   const int32_t n0 = rest - kNb;
   {
@@ -1463,8 +1487,7 @@ void tail_pair(hipStream_t st, T* S, int64_t lds, int32_t p, int32_t rest, doubl
   }
   // Panel s+1.
   const int32_t rest1 = n0;
-  if (rest1 < small_tile_below) tail_panel<T, 64>(st, S, lds, q, kNb, rest1, limit, status, inv + kInvPair, timer);
-  else tail_panel<T, 128>(st, S, lds, q, kNb, rest1, limit, status, inv + kInvPair, timer);
+  tail_panel<T, 64>(st, S, lds, q, kNb, rest1, limit, status, inv + kInvPair, timer);
   const int32_t q2 = q + kNb;                        // next diagonal block
   const int32_t nb2 = rest1 < kNb ? rest1 : kNb;     // its size
   const int32_t n02 = rest1 - nb2;
