@@ -387,13 +387,25 @@ __global__ __launch_bounds__(128) void k_dense_inv(int32_t nd, const int32_t* __
   }
   __syncthreads();
   if (w == 0) {
+    // M(i, j) = sum over k >= i of U^{-1}(i, k) x[k], k ascending for every i,
+    // eight rows at a time with k outermost, so eight sums advance together
+    // (the same operations in the same order per entry). In place: a group's
+    // rows are read by no later group.
 #pragma unroll
-    for (int i = 0; i < 64; ++i) {
-      if (i < s) {
-        T acc = T(0);
+    for (int i0 = 0; i0 < 64; i0 += 8) {
+      if (i0 < s) {
+        T acc[8];
 #pragma unroll
-        for (int k = i; k < 64; ++k) acc = tfma(ui[k * LC + i], x[k], acc);
-        x[i] = acc;
+        for (int u = 0; u < 8; ++u) acc[u] = T(0);
+#pragma unroll
+        for (int k = i0; k < 64; ++k) {
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (i0 + u <= k) acc[u] = tfma(ui[k * LC + i0 + u], x[k], acc[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (i0 + u < s) x[i0 + u] = acc[u];
       }
     }
     if (j < s) {
