@@ -339,13 +339,19 @@ void launch_row_scale(hipStream_t st, int32_t n, const int64_t* ptr, const doubl
 // L^{-1}(k,j), k ascending, in place for i ascending (row i of the product
 // reads only entries k >= i). Applying M in the solve replaces two chains of s
 // dependent substitution steps per block by one matrix-vector product.
+// Two waves per SIMD: with FP64 the column in registers (x[64]) would
+// otherwise take ~280 registers and one wave per SIMD; capped at 256 a few
+// values spill, and the kernel still runs 1.8x faster on many small blocks.
 template <typename T>
-__global__ __launch_bounds__(128) void k_dense_inv(int32_t nd, const int32_t* __restrict__ sizes,
-                                                   const int64_t* __restrict__ lu0, const T* __restrict__ lu,
-                                                   T* __restrict__ minv) {
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k_dense_inv(
+    int32_t nd, const int32_t* __restrict__ sizes, const int64_t* __restrict__ lu0, const T* __restrict__ lu,
+    T* __restrict__ minv) {
   constexpr int LC = 66;
+  // The factors; after the two substitutions U^{-1} replaces U on and above
+  // the diagonal (fc[k*66 + i] = U^{-1}(i, k), i <= k): wave 0 reads only the
+  // strictly lower part, and wave 1 has finished its reads of the upper part
+  // when it writes.
   __shared__ T fc[64 * LC];
-  __shared__ T ui[64 * LC];  // ui[k*66 + i] = U^{-1}(i, k)
   const int32_t t = static_cast<int32_t>(blockIdx.x);
   if (t >= nd) return;
   const int s = __builtin_amdgcn_readfirstlane(sizes[t]);
@@ -383,7 +389,8 @@ __global__ __launch_bounds__(128) void k_dense_inv(int32_t nd, const int32_t* __
       }
     }
 #pragma unroll
-    for (int i = 0; i < 64; ++i) ui[j * LC + i] = i <= j ? x[i] : T(0);
+    for (int i = 0; i < 64; ++i)
+      if (i <= j) fc[j * LC + i] = x[i];
   }
   __syncthreads();
   if (w == 0) {
@@ -401,7 +408,7 @@ __global__ __launch_bounds__(128) void k_dense_inv(int32_t nd, const int32_t* __
         for (int k = i0; k < 64; ++k) {
 #pragma unroll
           for (int u = 0; u < 8; ++u)
-            if (i0 + u <= k) acc[u] = tfma(ui[k * LC + i0 + u], x[k], acc[u]);
+            if (i0 + u <= k) acc[u] = tfma(fc[k * LC + i0 + u], x[k], acc[u]);
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u)
