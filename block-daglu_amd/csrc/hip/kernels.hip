@@ -54,6 +54,56 @@ __device__ __forceinline__ int32_t uniform_from_lane0(int32_t v) {
   return __builtin_amdgcn_readfirstlane(__shfl(v, 0));
 }
 
+// Cross-lane moves through DPP (the VALU's lane network, no LDS round trip as
+// with __shfl): kCtrl 0xb1 / 0x4e quad_perm [1,0,3,2] / [2,3,0,1], 0x124 /
+// 0x128 row_ror 4 / 8, 0x142 / 0x143 row_bcast 15 / 31. A lane whose source is
+// invalid (row 0 for row_bcast 15, rows 0-1 for row_bcast 31) keeps its value.
+template <int kCtrl>
+__device__ __forceinline__ int32_t dpp_i32(int32_t v) {
+  return __builtin_amdgcn_update_dpp(v, v, kCtrl, 0xF, 0xF, false);
+}
+template <int kCtrl>
+__device__ __forceinline__ float dpp_t(float v) {
+  return __builtin_bit_cast(float, dpp_i32<kCtrl>(__builtin_bit_cast(int32_t, v)));
+}
+template <int kCtrl>
+__device__ __forceinline__ double dpp_t(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = static_cast<uint32_t>(dpp_i32<kCtrl>(static_cast<int32_t>(static_cast<uint32_t>(u))));
+  const uint32_t hi = static_cast<uint32_t>(dpp_i32<kCtrl>(static_cast<int32_t>(static_cast<uint32_t>(u >> 32))));
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+__device__ __forceinline__ float lane63(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int32_t, v), 63));
+}
+__device__ __forceinline__ double lane63(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int32_t>(static_cast<uint32_t>(u)), 63);
+  const uint32_t hi = __builtin_amdgcn_readlane(static_cast<int32_t>(static_cast<uint32_t>(u >> 32)), 63);
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+// fmax over the wave's 64 lanes (NaN lanes ignored, as fmax does), returned
+// wave-uniform. Every pairing order gives the same value: max is exact.
+template <typename T>
+__device__ __forceinline__ T wave_fmax(T v) {
+  v = fmax(v, dpp_t<0xb1>(v));
+  v = fmax(v, dpp_t<0x4e>(v));
+  v = fmax(v, dpp_t<0x124>(v));
+  v = fmax(v, dpp_t<0x128>(v));
+  v = fmax(v, dpp_t<0x142>(v));
+  v = fmax(v, dpp_t<0x143>(v));
+  return lane63(v);
+}
+__device__ __forceinline__ int32_t wave_min_i32(int32_t v) {
+  v = min(v, dpp_i32<0xb1>(v));
+  v = min(v, dpp_i32<0x4e>(v));
+  v = min(v, dpp_i32<0x124>(v));
+  v = min(v, dpp_i32<0x128>(v));
+  v = min(v, dpp_i32<0x142>(v));
+  v = min(v, dpp_i32<0x143>(v));
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
 __device__ __forceinline__ int32_t load_relaxed(const int32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -195,18 +245,14 @@ __global__ __launch_bounds__(64) void k_dense_lu(int32_t nd, const int32_t* __re
   int32_t st = 0;
   const bool klu = key != nullptr;
   int32_t mykey = (klu && lane < s) ? key[starts[t] + lane] : lane;
-  // Half the butterfly group: the group (2 * half lanes) covers every lane below s.
-  const int half = s > 32 ? 32 : s > 16 ? 16 : s > 8 ? 8 : s > 4 ? 4 : s > 2 ? 2 : 1;
   for (int k = 0; k < s; ++k) {
     const T v = (lane >= k && lane < s) ? tabs(a[k * kLdsPad + lane]) : T(-1);
-    // The largest candidate over the lanes below s (a butterfly of max over
-    // the smallest power-of-two group of lanes that holds them), then the
-    // first lane holding it (ballot), or with KLU's rule the smallest key
-    // among the lanes holding it. Rows are lanes (physical swaps), so the first
-    // lane is the first row position, as in the reference.
-    T vm = v;
-    for (int off = half; off > 0; off >>= 1) vm = fmax(vm, __shfl_xor(vm, off));
-    vm = __shfl(vm, 0);
+    // The largest candidate (a max reduction over the wave through DPP; lanes
+    // outside k..s-1 hold -1), then the first lane holding it (ballot), or
+    // with KLU's rule the smallest key among the lanes holding it. Rows are
+    // lanes (physical swaps), so the first lane is the first row position, as
+    // in the reference.
+    const T vm = wave_fmax(v);
     if (uniform_from_lane0(!(vm > T(0)) ? 1 : 0)) {  // best == 0 (or NaN): the reference marks the block failed
       st |= kStZeroPivot;
       break;
@@ -216,16 +262,14 @@ __global__ __launch_bounds__(64) void k_dense_lu(int32_t nd, const int32_t* __re
     if (!klu) {
       p = __builtin_ctzll(__ballot(top));
     } else {
-      int kk = top ? mykey : INT_MAX;
-      for (int off = half; off > 0; off >>= 1) kk = min(kk, __shfl_xor(kk, off));
-      kk = __shfl(kk, 0);
+      const int kk = wave_min_i32(top ? mykey : INT_MAX);
       p = __builtin_ctzll(__ballot(top && mykey == kk));
       // Diagonal preference (KLU's test: |diag| >= tol * largest).
       if (tabs(a[k * kLdsPad + k]) >= static_cast<T>(tol) * vm) p = k;
     }
     p = uniform_from_lane0(p);
     if (klu) {
-      const int32_t kk_k = __shfl(mykey, k), kk_p = __shfl(mykey, p);
+      const int32_t kk_k = __builtin_amdgcn_readlane(mykey, k), kk_p = __builtin_amdgcn_readlane(mykey, p);
       mykey = (lane == k) ? kk_p : ((lane == p) ? kk_k : mykey);
     }
     mypiv = (lane == k) ? p : mypiv;
