@@ -51,7 +51,7 @@ __device__ __forceinline__ void store_release(int32_t* p, int32_t v) {
 // (even one that is equal in every lane) may be split so that lane 0 leaves
 // the loop while the others iterate on, which deadlocks a ticket loop.
 __device__ __forceinline__ int32_t uniform_from_lane0(int32_t v) {
-  return __builtin_amdgcn_readfirstlane(__shfl(v, 0));
+  return __builtin_amdgcn_readlane(v, 0);  // lane 0's register, into a scalar register
 }
 
 // Cross-lane moves through DPP (the VALU's lane network, no LDS round trip as
@@ -106,23 +106,6 @@ __device__ __forceinline__ int32_t wave_min_i32(int32_t v) {
 
 __device__ __forceinline__ int32_t load_relaxed(const int32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Bounded spin by the whole wavefront on one flag (every lane loads it, lane
-// 0's value decides, so the loop exit is uniform). The spin uses relaxed
-// coherent loads; one acquire fence follows once the flag is seen, so the
-// caches are invalidated once per dependency rather than once per poll.
-// Returns false if the flag never reaches `epoch`; valid schedules never come
-// near the bound.
-__device__ __forceinline__ bool wait_flag_uniform(const int32_t* flag, int32_t epoch) {
-  for (int32_t it = 0; it < (1 << 24); ++it) {
-    if (uniform_from_lane0(load_relaxed(flag)) == epoch) {
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      return true;
-    }
-    __builtin_amdgcn_s_sleep(4);
-  }
-  return false;
 }
 
 // Published-value protocol of the solves: an output vector starts filled with
