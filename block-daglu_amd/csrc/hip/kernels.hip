@@ -923,25 +923,37 @@ __device__ __forceinline__ void diag_lu_inv_body(T* __restrict__ S, int64_t lds,
   // Pieces of the inverses (column j of U_bb^{-1}; block (I,J) of L^{-1} or of
   // U^{-1} off the diagonal), run by wave 2 between the panel steps' barriers
   // as soon as their inputs are final, the rest after the last panel.
-  auto u_leaf = [&](int bb) {
-    // Lane j < 16: column j of U_bb^{-1} by backward substitution, column
-    // oriented (each step one multiply on the critical path); every lane
-    // stores unconditionally, as in the panel.
+  // Column j (lane j < 16) of U_bb^{-1} by backward substitution, column
+  // oriented (each step one multiply on the critical path), in two halves
+  // (steps 15..8, then 7..0 and the store) so that it can span two of the
+  // panel's barrier intervals; every lane stores unconditionally, as in the
+  // panel.
+  T ux[16];
+  auto u_leaf_a = [&](int bb) {
     const int b0 = 16 * bb;
-    const int j = lane;
-    T x[16];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) x[q] = q == j ? T(1) : T(0);
+    for (int q = 0; q < 16; ++q) ux[q] = q == lane ? T(1) : T(0);
 #pragma unroll
-    for (int c = 15; c >= 0; --c) {
-      x[c] = x[c] * fast_rcp(F[(b0 + c) * LF + b0 + c]);
+    for (int c = 15; c >= 8; --c) {
+      ux[c] = ux[c] * fast_rcp(F[(b0 + c) * LF + b0 + c]);
 #pragma unroll
-      for (int r = 0; r < c; ++r) x[r] = tfma(-F[(b0 + c) * LF + b0 + r], x[c], x[r]);
+      for (int r = 0; r < c; ++r) ux[r] = tfma(-F[(b0 + c) * LF + b0 + r], ux[c], ux[r]);
       if (c % 2 == 0) __builtin_amdgcn_sched_barrier(0);  // LDS reads of 2 steps in flight at a time (registers)
     }
+  };
+  auto u_leaf_b = [&](int bb) {
+    const int b0 = 16 * bb;
+#pragma unroll
+    for (int c = 7; c >= 0; --c) {
+      ux[c] = ux[c] * fast_rcp(F[(b0 + c) * LF + b0 + c]);
+#pragma unroll
+      for (int r = 0; r < c; ++r) ux[r] = tfma(-F[(b0 + c) * LF + b0 + r], ux[c], ux[r]);
+      if (c % 2 == 0) __builtin_amdgcn_sched_barrier(0);
+    }
+    const int j = lane;
     T* col = j < 16 ? G + (b0 + j) * LF + b0 : sink;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) (q > j ? sink : col)[q] = x[q];  // the upper triangle, with the diagonal
+    for (int q = 0; q < 16; ++q) (q > j ? sink : col)[q] = ux[q];  // the upper triangle, with the diagonal
   };
   auto l_item = [&](int I, int J) {  // X_IJ = -Linv_II sum_{M=J}^{I-1} L_IM X_MJ
     V acc = V{T(0), T(0), T(0), T(0)};
@@ -1020,16 +1032,34 @@ __device__ __forceinline__ void diag_lu_inv_body(T* __restrict__ S, int64_t lds,
       for (int q = 0; q < 16; ++q) (w == 3 && q <= j ? sink : col)[q] = x[q];
     } else {
       // Wave 2, during panel K: U_{K-1}^{-1}, then the blocks of L^{-1}'s row
-      // K-1 and U^{-1}'s column K-1 (their inputs were final when panel K
-      // started; panel K writes none of what they read or write).
+      // K-1 and of U^{-1}'s column K-1 (their inputs were final when panel K
+      // started; panel K writes none of what they read or write). FP64: one
+      // piece per interval between the panel's barriers, the first before the
+      // first barrier, U_{K-1}^{-1} over two (its steps outlast an interval:
+      // -4% on the diagonal LU); FP32 keeps three pieces after the first
+      // barrier (the spread schedule raised the fused FP32 kernel's registers
+      // and made it 8% slower).
 #pragma unroll 1
       for (int t = 0; t < 16 / kDiagStep; ++t) {
-        __syncthreads();
-        if (K >= 1 && t == 0) u_leaf(K - 1);
-        if (K >= 2 && t == 1)
-          for (int J = 0; J < K - 1; ++J) l_item(K - 1, J);
-        if (K >= 2 && t == 2)
-          for (int I = K - 2; I >= 0; --I) u_item(I, K - 1);
+        if constexpr (sizeof(T) == 8) {
+          if (K >= 1 && t == 0) u_leaf_a(K - 1);
+          if (K >= 1 && t == 1) u_leaf_b(K - 1);
+          if (K >= 2 && t == 2)
+            for (int J = 0; J < K - 1; ++J) l_item(K - 1, J);
+          if (K >= 2 && t == 3)
+            for (int I = K - 2; I >= 0; --I) u_item(I, K - 1);
+          __syncthreads();
+        } else {
+          __syncthreads();
+          if (K >= 1 && t == 0) {
+            u_leaf_a(K - 1);
+            u_leaf_b(K - 1);
+          }
+          if (K >= 2 && t == 1)
+            for (int J = 0; J < K - 1; ++J) l_item(K - 1, J);
+          if (K >= 2 && t == 2)
+            for (int I = K - 2; I >= 0; --I) u_item(I, K - 1);
+        }
       }
     }
     __syncthreads();
@@ -1047,7 +1077,8 @@ __device__ __forceinline__ void diag_lu_inv_body(T* __restrict__ S, int64_t lds,
   // L^{-1}'s last block row (waves 0, 1, 3) and U^{-1}'s last block column
   // (wave 2, after U_33^{-1}).
   if (w == 2) {
-    u_leaf(3);
+    u_leaf_a(3);
+    u_leaf_b(3);
     for (int I = 2; I >= 0; --I) u_item(I, 3);
   } else {
     l_item(3, w == 3 ? 2 : w);
