@@ -1033,12 +1033,12 @@ __device__ __forceinline__ void diag_lu_inv_body(T* __restrict__ S, int64_t lds,
     } else {
       // Wave 2, during panel K: U_{K-1}^{-1}, then the blocks of L^{-1}'s row
       // K-1 and of U^{-1}'s column K-1 (their inputs were final when panel K
-      // started; panel K writes none of what they read or write). FP64: one
-      // piece per interval between the panel's barriers, the first before the
-      // first barrier, U_{K-1}^{-1} over two (its steps outlast an interval:
-      // -4% on the diagonal LU); FP32 keeps three pieces after the first
-      // barrier (the spread schedule raised the fused FP32 kernel's registers
-      // and made it 8% slower).
+      // started; panel K writes none of what they read or write). One piece
+      // per interval between the panel's barriers, the first before the first
+      // barrier; FP64 spreads U_{K-1}^{-1} over two intervals (its steps
+      // outlast one: -4% on the diagonal LU). In FP32 that split raised the
+      // fused kernel's registers and made it 8% slower, so U_{K-1}^{-1} stays
+      // whole there.
 #pragma unroll 1
       for (int t = 0; t < 16 / kDiagStep; ++t) {
         if constexpr (sizeof(T) == 8) {
@@ -1050,7 +1050,6 @@ __device__ __forceinline__ void diag_lu_inv_body(T* __restrict__ S, int64_t lds,
             for (int I = K - 2; I >= 0; --I) u_item(I, K - 1);
           __syncthreads();
         } else {
-          __syncthreads();
           if (K >= 1 && t == 0) {
             u_leaf_a(K - 1);
             u_leaf_b(K - 1);
@@ -1059,6 +1058,7 @@ __device__ __forceinline__ void diag_lu_inv_body(T* __restrict__ S, int64_t lds,
             for (int J = 0; J < K - 1; ++J) l_item(K - 1, J);
           if (K >= 2 && t == 2)
             for (int I = K - 2; I >= 0; --I) u_item(I, K - 1);
+          __syncthreads();
         }
       }
     }
