@@ -169,6 +169,7 @@ struct Plan {
   double* rs = nullptr;
   double* svals = nullptr;
   int32_t* piv = nullptr;  // per row: block-local source row of the pivot gather
+  int32_t* pinv = nullptr; // per row: block-local position the gather moves it to (inverse of piv)
   int32_t* block_status = nullptr;  // per block (dense status)
   int32_t* prec_of_block = nullptr; // per block: 0 fp64 factors, 1 fp32
   std::vector<int32_t> h_prec;
@@ -503,6 +504,7 @@ void upload_into(bdlu_ctx* c, const bdlu_plan_desc& d, Plan& P) {
   P.minv64 = A.get<double>(lu);
   P.minv32 = A.get<float>(lu);
   P.piv = A.get<int32_t>(n);
+  P.pinv = A.get<int32_t>(n);
   P.block_status = A.get<int32_t>(nb);
   P.prec_of_block = A.get<int32_t>(nb);
   P.h_prec.assign(nb, 0);
@@ -825,7 +827,7 @@ void block_solve_launches(Plan& P, const Work& W, hipStream_t st, const double* 
     if (!ph.sparse) {
       // Each dense block's factors are read in the precision it was factored in.
       bdk::launch_dense_seg_solve(st, kt, ph.ntasks, ph.tasks, P.R, P.lu0_of_block, P.minv64, P.minv32,
-                                  P.prec_of_block, P.piv, P.off_ptr, P.off_col, P.off_val, W.tickets + tix,
+                                  P.prec_of_block, P.pinv, P.off_ptr, P.off_col, P.off_val, W.tickets + tix,
                                   W.bperm, W.xperm, ld, W.err, P.sched.seg_grid);
       ++tix;
       continue;
@@ -1001,11 +1003,11 @@ int refactor_impl(bdlu_ctx* c, Plan& P, const double* values, int where, int pre
     if (pe) pe->mark(BDLU_PROF_DENSE_LU, st);
     const int32_t* order = P.sched.dense_lpt ? P.d_lpt : nullptr;
     if (fp32) {
-      bdk::launch_dense_lu<float>(ds, P.ndense, P.d_size, P.d_start, P.d_val0, P.d_lu0, P.arena, P.lu32, P.piv,
+      bdk::launch_dense_lu<float>(ds, P.ndense, P.d_size, P.d_start, P.d_val0, P.d_lu0, P.arena, P.lu32, P.piv, P.pinv,
                                   P.block_status, P.d_block, order, P.dense_tol, P.dense_key);
       bdk::launch_dense_inv<float>(ds, P.ndense, P.d_size, P.d_lu0, P.lu32, P.minv32);
     } else {
-      bdk::launch_dense_lu<double>(ds, P.ndense, P.d_size, P.d_start, P.d_val0, P.d_lu0, P.arena, P.lu64, P.piv,
+      bdk::launch_dense_lu<double>(ds, P.ndense, P.d_size, P.d_start, P.d_val0, P.d_lu0, P.arena, P.lu64, P.piv, P.pinv,
                                    P.block_status, P.d_block, order, P.dense_tol, P.dense_key);
       bdk::launch_dense_inv<double>(ds, P.ndense, P.d_size, P.d_lu0, P.lu64, P.minv64);
     }
@@ -1092,7 +1094,7 @@ int refactor_impl(bdlu_ctx* c, Plan& P, const double* values, int where, int pre
     int64_t* dv0 = tmp.copy(v0, st);
     int64_t* dl0 = tmp.copy(l0, st);
     bdk::launch_dense_lu<double>(st, static_cast<int32_t>(redo.size()), dsz, dst_, dv0, dl0, P.arena, P.lu64,
-                                 P.piv, P.block_status, dbl, nullptr, P.dense_tol, P.dense_key);
+                                 P.piv, P.pinv, P.block_status, dbl, nullptr, P.dense_tol, P.dense_key);
     bdk::launch_dense_inv<double>(st, static_cast<int32_t>(redo.size()), dsz, dl0, P.lu64, P.minv64);
     // The FP64 redo writes its own statuses: a block that is singular in
     // FP64 (exact zero pivot) fails like any other dense block.

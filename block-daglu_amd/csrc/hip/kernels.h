@@ -42,7 +42,7 @@ void launch_fill_sentinel_cols(hipStream_t st, double* p, int32_t m, int32_t k, 
 // with tolerance tol and per-row tie keys (global-KLU route).
 template <typename T>
 void launch_dense_lu(hipStream_t st, int32_t nd, const int32_t* sizes, const int32_t* starts, const int64_t* val0,
-                     const int64_t* lu0, const double* arena, T* lu, int32_t* piv, int32_t* status,
+                     const int64_t* lu0, const double* arena, T* lu, int32_t* piv, int32_t* pinv, int32_t* status,
                      const int32_t* block_of, const int32_t* order = nullptr, double tol = 1.0,
                      const int32_t* key = nullptr);
 // KLU whole-row scaling: rs[i] = sum (kind 1) or max (kind 2) of |vals| in CSR
@@ -125,7 +125,7 @@ void launch_unpermute(hipStream_t st, int32_t n, int32_t k, const int32_t* Q, co
 void launch_dense_seg_solve(hipStream_t st, int32_t kc, int32_t ntasks, const int32_t* tasks, const int32_t* R,
                             const int64_t* lu0_of_block, const double* minv64, const float* minv32,
                             const int32_t* prec_of_block,
-                            const int32_t* gsrc, const int32_t* off_ptr, const int32_t* off_col,
+                            const int32_t* gpinv, const int32_t* off_ptr, const int32_t* off_col,
                             const double* off_val, int32_t* ticket, const double* bperm, double* xperm, int64_t ldv,
                             int32_t* err, int32_t grid = 1024);
 // Sparse block solve pieces; y is an s x kc work set in position order.

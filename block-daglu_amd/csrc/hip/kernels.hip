@@ -209,7 +209,7 @@ __global__ __launch_bounds__(64) void k_dense_lu(int32_t nd, const int32_t* __re
                                                  const int32_t* __restrict__ starts, const int64_t* __restrict__ val0,
                                                  const int64_t* __restrict__ lu0, const double* __restrict__ arena,
                                                  T* __restrict__ lu, int32_t* __restrict__ piv,
-                                                 int32_t* __restrict__ status, const int32_t* __restrict__ block_of,
+                                                 int32_t* __restrict__ pinv, int32_t* __restrict__ status, const int32_t* __restrict__ block_of,
                                                  const int32_t* __restrict__ order, double tol,
                                                  const int32_t* __restrict__ key) {
   __shared__ T a[64 * kLdsPad];
@@ -313,23 +313,26 @@ __global__ __launch_bounds__(64) void k_dense_lu(int32_t nd, const int32_t* __re
     const int32_t pk = __builtin_amdgcn_readlane(mypiv, k);
     srcrow = (srcrow == k) ? pk : ((srcrow == pk) ? k : srcrow);
   }
-  if (lane < s) piv[starts[t] + lane] = srcrow;
+  if (lane < s) {
+    piv[starts[t] + lane] = srcrow;
+    pinv[starts[t] + srcrow] = lane;  // the inverse: where the solve's gather puts row `srcrow`
+  }
   if (lane == 0) status[block_of[t]] = st;
 }
 
 template <typename T>
 void launch_dense_lu(hipStream_t st, int32_t nd, const int32_t* sizes, const int32_t* starts, const int64_t* val0,
-                     const int64_t* lu0, const double* arena, T* lu, int32_t* piv, int32_t* status,
+                     const int64_t* lu0, const double* arena, T* lu, int32_t* piv, int32_t* pinv, int32_t* status,
                      const int32_t* block_of, const int32_t* order, double tol, const int32_t* key) {
   if (nd <= 0) return;
-  hipLaunchKernelGGL(k_dense_lu<T>, dim3(nd), dim3(64), 0, st, nd, sizes, starts, val0, lu0, arena, lu, piv,
+  hipLaunchKernelGGL(k_dense_lu<T>, dim3(nd), dim3(64), 0, st, nd, sizes, starts, val0, lu0, arena, lu, piv, pinv,
                      status, block_of, order, tol, key);
 }
 template void launch_dense_lu<double>(hipStream_t, int32_t, const int32_t*, const int32_t*, const int64_t*,
-                                      const int64_t*, const double*, double*, int32_t*, int32_t*, const int32_t*,
+                                      const int64_t*, const double*, double*, int32_t*, int32_t*, int32_t*, const int32_t*,
                                       const int32_t*, double, const int32_t*);
 template void launch_dense_lu<float>(hipStream_t, int32_t, const int32_t*, const int32_t*, const int64_t*,
-                                     const int64_t*, const double*, float*, int32_t*, int32_t*, const int32_t*,
+                                     const int64_t*, const double*, float*, int32_t*, int32_t*, int32_t*, const int32_t*,
                                      const int32_t*, double, const int32_t*);
 
 // KLU row scaling over whole rows (klu_scale, scale 1 sums / 2 maxima of |a_ij|
@@ -1703,9 +1706,9 @@ __device__ __forceinline__ double readlane_f64(double v, int src) {
 // addresses. No masks and no per-step branches.
 template <int N, int K>
 __device__ __forceinline__ void block_inverse_product(const double (&rowm)[64], double* __restrict__ vb,
-                                                      double (&v)[K], int lane) {
+                                                      double (&v)[K], int at) {
 #pragma unroll
-  for (int c = 0; c < K; ++c) vb[c * 64 + lane] = v[c];
+  for (int c = 0; c < K; ++c) vb[c * 64 + at] = v[c];
   __syncthreads();
 #pragma unroll
   for (int c = 0; c < K; ++c) v[c] = 0.0;
@@ -1722,7 +1725,7 @@ template <int NW>
 __global__ __launch_bounds__(64 * NW) void k_dense_seg_solve(
     int32_t ntasks, const int32_t* __restrict__ tasks, const int32_t* __restrict__ R,
     const int64_t* __restrict__ lu0_of_block, const double* __restrict__ minv64, const float* __restrict__ minv32,
-    const int32_t* __restrict__ prec_of_block, const int32_t* __restrict__ gsrc, const int32_t* __restrict__ off_ptr,
+    const int32_t* __restrict__ prec_of_block, const int32_t* __restrict__ gpinv, const int32_t* __restrict__ off_ptr,
     const int32_t* __restrict__ off_col, const double* __restrict__ off_val, int32_t* __restrict__ ticket,
     const double* __restrict__ bperm, double* __restrict__ xperm, int64_t ldv, int32_t* __restrict__ err) {
   __shared__ double vb[NW * 64];
@@ -1746,7 +1749,7 @@ __global__ __launch_bounds__(64 * NW) void k_dense_seg_solve(
     const bool act = lane < s;
     const int32_t r = r0 + lane;
     // --- independent of other blocks ---
-    const int32_t src = act ? gsrc[r] : 0;
+    const int32_t at = act ? gpinv[r] : lane;  // this row's position after the pivot gather
     double acc[1] = {act ? bcol[r] : 0.0};
     const int32_t p0 = act ? off_ptr[r] : 0, p1 = act ? off_ptr[r + 1] : 0;
     int32_t oc[kPre];
@@ -1796,11 +1799,10 @@ __global__ __launch_bounds__(64 * NW) void k_dense_seg_solve(
     // Row gather (the block's pivoting), then x = M (P rhs) as a matrix-vector
     // product: every step reads the finished input vector, so there is no
     // dependency chain across the block.
-    acc[0] = __shfl(acc[0], src);
     if (!act) acc[0] = 0.0;
-    if (s <= 16) block_inverse_product<16, 1>(rowm, vbw, acc, lane);
-    else if (s <= 32) block_inverse_product<32, 1>(rowm, vbw, acc, lane);
-    else block_inverse_product<64, 1>(rowm, vbw, acc, lane);
+    if (s <= 16) block_inverse_product<16, 1>(rowm, vbw, acc, at);
+    else if (s <= 32) block_inverse_product<32, 1>(rowm, vbw, acc, at);
+    else block_inverse_product<64, 1>(rowm, vbw, acc, at);
     if (act) st_x_relaxed(xcol + r, acc[0]);
     __syncthreads();  // vb is rewritten by the next block; the next ticket is in its slot
     t = __builtin_amdgcn_readfirstlane(tk_sh[slot]);
@@ -1812,14 +1814,14 @@ __global__ __launch_bounds__(64 * NW) void k_dense_seg_solve(
 void launch_dense_seg_solve(hipStream_t st, int32_t kc, int32_t ntasks, const int32_t* tasks, const int32_t* R,
                             const int64_t* lu0_of_block, const double* minv64, const float* minv32,
                             const int32_t* prec_of_block,
-                            const int32_t* gsrc, const int32_t* off_ptr, const int32_t* off_col,
+                            const int32_t* gpinv, const int32_t* off_ptr, const int32_t* off_col,
                             const double* off_val, int32_t* ticket, const double* bperm,
                             double* xperm, int64_t ldv, int32_t* err, int32_t grid) {
   if (ntasks <= 0) return;
   const int g = ntasks < grid ? ntasks : grid;  // persistent workgroups, one wavefront per column
 #define BDLU_SEG(KK)                                                                                               \
   hipLaunchKernelGGL(k_dense_seg_solve<KK>, dim3(g), dim3(64 * KK), 0, st, ntasks, tasks, R, lu0_of_block, minv64, \
-                     minv32, prec_of_block, gsrc, off_ptr, off_col, off_val, ticket, bperm, xperm, ldv, err)
+                     minv32, prec_of_block, gpinv, off_ptr, off_col, off_val, ticket, bperm, xperm, ldv, err)
   switch (kc) {
     case 1: BDLU_SEG(1); break;
     case 2: BDLU_SEG(2); break;
