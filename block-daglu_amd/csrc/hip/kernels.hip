@@ -1709,7 +1709,11 @@ __device__ __forceinline__ void block_inverse_product(const double (&rowm)[64], 
                                                       double (&v)[K], int at) {
 #pragma unroll
   for (int c = 0; c < K; ++c) vb[c * 64 + at] = v[c];
-  __syncthreads();
+  // vb is this wavefront's own slice: a wave-level barrier orders the writes
+  // before the broadcast reads (the other columns' waves need not wait).
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
   for (int c = 0; c < K; ++c) v[c] = 0.0;
 #pragma unroll
